@@ -1,0 +1,286 @@
+"""convector_amd -- Convector's per-ray path on AMD MI355X (gfx950).
+
+Python mirror of the reference's Rust surface (ruuda/convector):
+
+    Mesh.load / Mesh.load_with_materials     wavefront.rs:79-152
+    Scene.from_meshes, Scene.camera          scene.rs:20-129
+    Scene.intersect_nearest (Bvh)            scene.rs:220-244, bvh.rs:589-669
+    Renderer.render_patch_u8 / accumulate_patch_f32 / new_buffer_f32 /
+    buffer_f32_into_render_buffer            renderer.rs:121-410
+
+Every call goes through the C ABI of include/convector.h into the HIP
+kernels of libconvector_amd.so; nothing here computes the path.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from typing import Dict, Optional, Sequence
+
+import numpy as np
+
+from . import _native
+from ._native import CvError, check, lib  # noqa: F401
+
+NO_HIT = _native.CV_NO_HIT
+RNG_PIXEL = _native.CV_RNG_PIXEL
+RNG_REF_PATCH = _native.CV_RNG_REF_PATCH
+PTR_DEVICE = _native.CV_PTR_DEVICE
+RENDER_COUNT = _native.CV_RENDER_COUNT
+RENDER_TIMING = _native.CV_RENDER_TIMING
+
+NODE_DTYPE = np.dtype([("min", "<f4", 3), ("max", "<f4", 3), ("index", "<u4"), ("len", "<u4")])
+HIT_DTYPE = np.dtype([("tri_id", "<u4"), ("t", "<f4"), ("u", "<f4"), ("v", "<f4")])
+ISECT_DTYPE = np.dtype([("position", "<f4", 3), ("normal", "<f4", 3), ("distance", "<f4"),
+                        ("material", "<u4"), ("tex_coords", "<f4", 2), ("tri_id", "<u4"),
+                        ("u", "<f4"), ("v", "<f4"), ("numi_aabb", "<u4"), ("numi_tri", "<u4"),
+                        ("reserved", "<u4")])
+
+
+def _p(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Mesh:
+    """A triangle mesh (wavefront.rs:18-28) held by the library."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    @classmethod
+    def load(cls, path: str) -> "Mesh":
+        return cls.load_with_materials(path, {})
+
+    @classmethod
+    def load_with_materials(cls, path: str, materials: Dict[str, int]) -> "Mesh":
+        entries = (_native.MaterialEntry * max(1, len(materials)))()
+        names = []
+        for i, (k, v) in enumerate(materials.items()):
+            names.append(k.encode())
+            entries[i].name = names[-1]
+            entries[i].material = int(v)
+        h = ctypes.c_void_p()
+        check(lib().cv_mesh_load_obj(path.encode(), ctypes.cast(entries, ctypes.c_void_p),
+                                     len(materials), ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_triangles(cls, tris: np.ndarray, materials: np.ndarray,
+                       uv: Optional[np.ndarray] = None) -> "Mesh":
+        t = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 9)
+        m = np.ascontiguousarray(materials, dtype=np.uint32).reshape(-1)
+        u = None if uv is None else np.ascontiguousarray(uv, dtype=np.float32).reshape(-1, 6)
+        h = ctypes.c_void_p()
+        check(lib().cv_mesh_from_triangles(_p(t), _p(u), _p(m), t.shape[0], ctypes.byref(h)))
+        return cls(h)
+
+    @property
+    def num_triangles(self) -> int:
+        return int(lib().cv_mesh_num_triangles(self._h))
+
+    def triangles(self):
+        n = self.num_triangles
+        v = np.zeros((n, 9), np.float32)
+        uv = np.zeros((n, 6), np.float32)
+        m = np.zeros(n, np.uint32)
+        check(lib().cv_mesh_copy_triangles(self._h, _p(v), _p(uv), _p(m)))
+        return v, uv, m
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().cv_mesh_destroy(h)
+            self._h = None
+
+
+class Camera:
+    """Camera of scene.rs:20-100; changes are pushed to the scene."""
+
+    def __init__(self, scene: "Scene"):
+        self._scene = scene
+        self.position = (0.0, 0.0, 0.0)
+        self.position_delta = (0.0, 0.0, 0.0)
+        self.orientation = (1.0, 0.0, 0.0, 0.0)
+        self.orientation_delta = (0.0, 0.0, 0.0, 0.0)
+        self.screen_distance = float(np.float32(1.0) / np.float32(math.sin(math.pi / 5.0)))
+        self._push()
+
+    def _push(self):
+        f = lambda x: np.ascontiguousarray(x, dtype=np.float32)  # noqa: E731
+        a, b, c, d = (f(self.position), f(self.position_delta), f(self.orientation),
+                      f(self.orientation_delta))
+        check(lib().cv_scene_set_camera(self._scene._h, _p(a), _p(b), _p(c), _p(d),
+                                        float(self.screen_distance)))
+
+    def set_position(self, position, delta=(0.0, 0.0, 0.0)):
+        self.position, self.position_delta = tuple(position), tuple(delta)
+        self._push()
+
+    def set_orientation(self, orientation, delta=(0.0, 0.0, 0.0, 0.0)):
+        self.orientation, self.orientation_delta = tuple(orientation), tuple(delta)
+        self._push()
+
+    def set_fov(self, fov: float):
+        self.screen_distance = float(np.float32(1.0) / np.float32(math.sin(fov / 2.0)))
+        self._push()
+
+    def set_screen_distance(self, d: float):
+        self.screen_distance = float(d)
+        self._push()
+
+
+class Scene:
+    """Scene + Bvh (scene.rs:102-245, bvh.rs:32-669) resident on the GPU."""
+
+    def __init__(self, handle):
+        self._h = handle
+        self.camera = Camera(self)
+
+    @staticmethod
+    def _opts(emission_override):
+        o = _native.SceneOpts()
+        o.builder = _native.CV_BUILDER_REFERENCE
+        if emission_override is not None:
+            o.has_emission_override = 1
+            for k in range(3):
+                o.emission_override[k] = float(emission_override[k])
+        return o
+
+    @classmethod
+    def from_meshes(cls, meshes: Sequence[Mesh], emission_override=None) -> "Scene":
+        arr = (ctypes.c_void_p * len(meshes))(*[m._h for m in meshes])
+        o = cls._opts(emission_override)
+        h = ctypes.c_void_p()
+        check(lib().cv_scene_create(ctypes.cast(arr, ctypes.c_void_p), len(meshes),
+                                    ctypes.byref(o), ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_meshes_with_bvh(cls, meshes: Sequence[Mesh], nodes: np.ndarray, order: np.ndarray,
+                             emission_override=None) -> "Scene":
+        arr = (ctypes.c_void_p * len(meshes))(*[m._h for m in meshes])
+        o = cls._opts(emission_override)
+        n = np.ascontiguousarray(nodes).view(NODE_DTYPE)
+        od = np.ascontiguousarray(order, dtype=np.uint32)
+        h = ctypes.c_void_p()
+        check(lib().cv_scene_create_with_bvh(ctypes.cast(arr, ctypes.c_void_p), len(meshes),
+                                             ctypes.byref(o), _p(n), len(n), _p(od),
+                                             ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_scene_data(cls, sd, with_bvh=None) -> "Scene":
+        """Scene from a convector_amd.scenes.SceneData (camera included)."""
+        mesh = Mesh.from_triangles(sd.tris, sd.materials, sd.uv)
+        if with_bvh is None:
+            s = cls.from_meshes([mesh], emission_override=sd.emission_override)
+        else:
+            s = cls.from_meshes_with_bvh([mesh], with_bvh[0], with_bvh[1],
+                                         emission_override=sd.emission_override)
+        s.camera.set_position(sd.camera_pos, sd.camera_dpos)
+        s.camera.set_orientation(sd.camera_q, sd.camera_dq)
+        s.camera.set_screen_distance(sd.screen_distance)
+        return s
+
+    def bvh_info(self) -> dict:
+        info = _native.BvhInfo()
+        check(lib().cv_scene_get_bvh_info(self._h, ctypes.byref(info)))
+        return {k: getattr(info, k) for k, _ in info._fields_}
+
+    def bvh(self):
+        info = self.bvh_info()
+        nodes = np.zeros(info["num_nodes"], NODE_DTYPE)
+        order = np.zeros(info["num_triangles"], np.uint32)
+        check(lib().cv_scene_copy_bvh(self._h, _p(nodes), _p(order)))
+        return nodes, order
+
+    def intersect_nearest(self, rays: np.ndarray, full: bool = False):
+        """rays: (n, 6) float32 origin+direction.  Returns the cv_hit array
+        (and the cv_isect array when full)."""
+        r = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 6)
+        n = r.shape[0]
+        hits = np.zeros(n, HIT_DTYPE)
+        isects = np.zeros(n, ISECT_DTYPE) if full else None
+        check(lib().cv_bvh_intersect_nearest(self._h, _p(r), n, _p(hits), _p(isects), 0))
+        return (hits, isects) if full else hits
+
+    def intersect_nearest_device(self, rays_ptr: int, n: int, hits_ptr: int, isects_ptr=None):
+        check(lib().cv_bvh_intersect_nearest(self._h, ctypes.c_void_p(rays_ptr), n,
+                                             ctypes.c_void_p(hits_ptr),
+                                             ctypes.c_void_p(isects_ptr) if isects_ptr else None,
+                                             PTR_DEVICE))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().cv_scene_destroy(h)
+            self._h = None
+
+
+class Renderer:
+    """Renderer (renderer.rs:16-130) bound to a GPU scene."""
+
+    def __init__(self, scene: Scene, width: int, height: int, rng_mode: int = RNG_PIXEL,
+                 max_bounces: int = 5):
+        self.scene = scene
+        self.width, self.height = int(width), int(height)
+        o = _native.RenderOpts(rng_mode, _native.CV_APPROX_PORTABLE, max_bounces)
+        h = ctypes.c_void_p()
+        check(lib().cv_renderer_create(scene._h, self.width, self.height, ctypes.byref(o),
+                                       ctypes.byref(h)))
+        self._h = h
+
+    def new_buffer_f32(self) -> np.ndarray:
+        """renderer.rs:366-374: (height/4)*(width/16) blocks of [8][3][8] f32."""
+        return np.zeros(self.width * self.height * 3, np.float32)
+
+    def new_bitmap(self) -> np.ndarray:
+        return np.zeros(self.width * self.height, np.uint32)
+
+    def render_patch_u8(self, bitmap, gbuffer, patch_width, x, y, frame_number):
+        check(lib().cv_renderer_render_patch_u8(self._h, _p(bitmap), _p(gbuffer), patch_width, x,
+                                                y, frame_number, 0))
+
+    def accumulate_patch_f32(self, hdr, gbuffer, patch_width, x, y, frame_number):
+        check(lib().cv_renderer_accumulate_patch_f32(self._h, _p(hdr), _p(gbuffer), patch_width,
+                                                     x, y, frame_number, 0))
+
+    def render_frame_u8(self, frame_number, gbuffer=True):
+        bitmap = self.new_bitmap()
+        gb = self.new_bitmap() if gbuffer else None
+        check(lib().cv_renderer_render_frame_u8(self._h, _p(bitmap), _p(gb), frame_number, 0))
+        return bitmap, gb
+
+    def accumulate_frames_f32(self, hdr, first_frame, n_frames, gbuffer=None, flags=0):
+        check(lib().cv_renderer_accumulate_frames_f32(self._h, _p(hdr), _p(gbuffer), first_frame,
+                                                      n_frames, flags))
+
+    def accumulate_frames_device(self, hdr_ptr: int, first_frame, n_frames, flags=0):
+        check(lib().cv_renderer_accumulate_frames_f32(self._h, ctypes.c_void_p(hdr_ptr), None,
+                                                      first_frame, n_frames,
+                                                      flags | PTR_DEVICE))
+
+    def buffer_f32_into_render_buffer(self, hdr, num_samples) -> np.ndarray:
+        bitmap = self.new_bitmap()
+        check(lib().cv_renderer_buffer_f32_into_bitmap(self._h, _p(hdr), _p(bitmap), num_samples,
+                                                       0))
+        return bitmap
+
+    def stats(self) -> dict:
+        s = _native.RenderStats()
+        check(lib().cv_renderer_get_stats(self._h, ctypes.byref(s)))
+        return {k: getattr(s, k) for k, _ in s._fields_}
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().cv_renderer_destroy(h)
+            self._h = None
+
+
+def device_synchronize():
+    check(lib().cv_device_synchronize())
+
+
+def version() -> str:
+    return lib().cv_version().decode()

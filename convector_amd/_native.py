@@ -1,0 +1,139 @@
+"""ctypes declarations of include/convector.h (libconvector_amd.so).
+
+The library is the product: there is no Python or CPU fallback.  Loading
+fails loudly when the built library is missing.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libconvector_amd.so")
+
+CV_OK = 0
+STATUS_NAMES = {
+    0: "CV_OK", 1: "CV_ERR_INVALID_ARGUMENT", 2: "CV_ERR_IO", 3: "CV_ERR_PARSE",
+    4: "CV_ERR_DEGENERATE_TRIANGLE", 5: "CV_ERR_UNKNOWN_MATERIAL",
+    6: "CV_ERR_NOT_MULTIPLE_OF_16", 7: "CV_ERR_LIGHT_COUNT", 8: "CV_ERR_BVH_NO_SPLIT",
+    9: "CV_ERR_NAN", 10: "CV_ERR_DEVICE", 11: "CV_ERR_OUT_OF_MEMORY", 12: "CV_ERR_NO_DEVICE",
+    13: "CV_ERR_OUT_OF_RANGE",
+}
+
+CV_PTR_DEVICE = 1
+CV_RENDER_COUNT = 2
+CV_RENDER_TIMING = 4
+CV_RNG_PIXEL = 0
+CV_RNG_REF_PATCH = 1
+CV_APPROX_PORTABLE = 0
+CV_BUILDER_REFERENCE = 0
+CV_NO_HIT = 0xFFFFFFFF
+
+# every function the header declares
+EXPORTS = [
+    "cv_mesh_load_obj", "cv_mesh_from_triangles", "cv_mesh_num_triangles",
+    "cv_mesh_copy_triangles", "cv_mesh_destroy", "cv_scene_create", "cv_scene_create_with_bvh",
+    "cv_scene_set_camera", "cv_scene_get_bvh_info", "cv_scene_copy_bvh", "cv_scene_destroy",
+    "cv_bvh_intersect_nearest", "cv_renderer_create", "cv_renderer_render_patch_u8",
+    "cv_renderer_accumulate_patch_f32", "cv_renderer_render_frame_u8",
+    "cv_renderer_accumulate_frames_f32", "cv_renderer_buffer_f32_into_bitmap",
+    "cv_renderer_get_stats", "cv_renderer_destroy", "cv_device_synchronize", "cv_last_error",
+    "cv_status_string", "cv_version",
+]
+
+
+class CvError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        self.status = status
+        self.name = STATUS_NAMES.get(status, str(status))
+        super().__init__(f"{self.name}: {message}")
+
+
+class MaterialEntry(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char_p), ("material", ctypes.c_uint32)]
+
+
+class SceneOpts(ctypes.Structure):
+    _fields_ = [("builder", ctypes.c_uint32), ("has_emission_override", ctypes.c_uint32),
+                ("emission_override", ctypes.c_float * 3)]
+
+
+class BvhInfo(ctypes.Structure):
+    _fields_ = [("num_nodes", ctypes.c_uint32), ("num_triangles", ctypes.c_uint32),
+                ("depth", ctypes.c_uint32), ("max_leaf", ctypes.c_uint32),
+                ("num_direct", ctypes.c_uint32), ("avg_area_ratio", ctypes.c_float),
+                ("avg_tris_per_leaf", ctypes.c_float), ("build_ms", ctypes.c_float)]
+
+
+class RenderOpts(ctypes.Structure):
+    _fields_ = [("rng_mode", ctypes.c_uint32), ("approx_mode", ctypes.c_uint32),
+                ("max_bounces", ctypes.c_uint32)]
+
+
+class RenderStats(ctypes.Structure):
+    _fields_ = [("paths", ctypes.c_uint64), ("extend_rays", ctypes.c_uint64),
+                ("extend_launches", ctypes.c_uint64), ("extend_ms", ctypes.c_double),
+                ("shade_ms", ctypes.c_double), ("generate_ms", ctypes.c_double),
+                ("total_ms", ctypes.c_double), ("sum_numi_aabb", ctypes.c_uint64),
+                ("sum_numi_tri", ctypes.c_uint64)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build the HIP extension first "
+                "(python -c 'import __graft_entry__; __graft_entry__.build()')")
+        L = ctypes.CDLL(LIB_PATH)
+        _declare(L)
+        _lib = L
+    return _lib
+
+
+def _declare(L):
+    P = ctypes.c_void_p
+    U32 = ctypes.c_uint32
+    SZ = ctypes.c_size_t
+    I = ctypes.c_int
+    F = ctypes.c_float
+    PP = ctypes.POINTER(ctypes.c_void_p)
+    sig = {
+        "cv_mesh_load_obj": (I, [ctypes.c_char_p, P, SZ, PP]),
+        "cv_mesh_from_triangles": (I, [P, P, P, SZ, PP]),
+        "cv_mesh_num_triangles": (SZ, [P]),
+        "cv_mesh_copy_triangles": (I, [P, P, P, P]),
+        "cv_mesh_destroy": (None, [P]),
+        "cv_scene_create": (I, [P, SZ, P, PP]),
+        "cv_scene_create_with_bvh": (I, [P, SZ, P, P, SZ, P, PP]),
+        "cv_scene_set_camera": (I, [P, P, P, P, P, F]),
+        "cv_scene_get_bvh_info": (I, [P, P]),
+        "cv_scene_copy_bvh": (I, [P, P, P]),
+        "cv_scene_destroy": (None, [P]),
+        "cv_bvh_intersect_nearest": (I, [P, P, SZ, P, P, U32]),
+        "cv_renderer_create": (I, [P, U32, U32, P, PP]),
+        "cv_renderer_render_patch_u8": (I, [P, P, P, U32, U32, U32, U32, U32]),
+        "cv_renderer_accumulate_patch_f32": (I, [P, P, P, U32, U32, U32, U32, U32]),
+        "cv_renderer_render_frame_u8": (I, [P, P, P, U32, U32]),
+        "cv_renderer_accumulate_frames_f32": (I, [P, P, P, U32, U32, U32]),
+        "cv_renderer_buffer_f32_into_bitmap": (I, [P, P, P, U32, U32]),
+        "cv_renderer_get_stats": (I, [P, P]),
+        "cv_renderer_destroy": (None, [P]),
+        "cv_device_synchronize": (I, []),
+        "cv_last_error": (ctypes.c_char_p, []),
+        "cv_status_string": (ctypes.c_char_p, [I]),
+        "cv_version": (ctypes.c_char_p, []),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+
+
+def check(status: int):
+    if status != CV_OK:
+        msg = lib().cv_last_error().decode("utf-8", "replace")
+        raise CvError(status, msg)

@@ -1,0 +1,130 @@
+// internal.hpp -- argument blocks and launch wrappers shared by the kernels
+// and the C-ABI host code.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "device_math.hpp"
+
+namespace cvk {
+
+constexpr int kMaxBounceSlots = 16;
+
+// Per-render control block in device memory: queue sizes and work cursors
+// per bounce, plus the optional traversal counters.
+struct Ctrl {
+  uint32_t count[kMaxBounceSlots];
+  uint32_t ext_cursor[kMaxBounceSlots];
+  uint32_t shade_cursor[kMaxBounceSlots];
+  unsigned long long sum_aabb;
+  unsigned long long sum_tri;
+  unsigned long long total_rays;
+};
+
+struct Queue {
+  float4* r0;     // origin.xyz, pixel index bits
+  float4* r1;     // direction.xyz, unused
+  float4* col;    // path throughput rgb
+  uint64_t* rng;  // PIXEL stream state
+};
+
+struct CameraParams {
+  float pos[3];
+  float dpos[3];
+  float q[4];
+  float dq[4];
+  float screen_distance;
+};
+
+struct GenerateArgs {
+  Ctrl* ctrl;
+  Queue q;
+  uint32_t width, height;
+  uint32_t rect_x, rect_y, rect_w, rect_h;
+  uint32_t frame;
+  CameraParams cam;
+  uint32_t* gbuffer;
+  uint32_t* bitmap;
+};
+
+struct ExtendArgs {
+  Ctrl* ctrl;
+  uint32_t bounce;
+  const float4* r0;
+  const float4* r1;
+  uint4* hits;
+  const DevNode* nodes;
+  const DevTri* tris;
+  uint2* spill;
+};
+
+struct ShadeArgs {
+  Ctrl* ctrl;
+  uint32_t bounce;
+  uint32_t max_bounces;
+  uint32_t width;
+  Queue in;
+  Queue out;
+  const uint4* hits;
+  const DevTri* tris;
+  const uint32_t* tri_mat;
+  const float* tri_uv;
+  const DevLight* lights;
+  uint32_t nlights;
+  uint32_t has_override;
+  float emission[3];
+  float* hdr;
+  uint32_t* bitmap;
+  uint32_t* gbuffer;
+};
+
+struct IntersectArgs {
+  const float* rays;
+  uint32_t n;
+  uint32_t* cursor;
+  const DevNode* nodes;
+  const DevTri* tris;
+  const uint32_t* tri_src;
+  const uint32_t* tri_mat;
+  const float* tri_uv;
+  uint2* spill;
+  uint32_t* hits;    // n x 4 u32 or null
+  uint32_t* isects;  // n x 16 u32 or null
+};
+
+hipError_t launch_intersect(const IntersectArgs& a, bool count, bool packed, int stack,
+                            uint32_t blocks, hipStream_t s);
+hipError_t launch_extend(const ExtendArgs& a, bool count, bool packed, int stack, uint32_t blocks,
+                         hipStream_t s);
+hipError_t launch_generate(const GenerateArgs& a, uint32_t blocks, hipStream_t s);
+hipError_t launch_shade(const ShadeArgs& a, uint32_t blocks, hipStream_t s);
+hipError_t launch_hdr_to_bitmap(const float* hdr, uint32_t* bitmap, uint32_t width,
+                                uint32_t height, uint32_t num_samples, hipStream_t s);
+hipError_t launch_prepare_tris(const float* v9, const uint32_t* order, uint32_t ntris, DevTri* out,
+                               hipStream_t s);
+hipError_t launch_prepare_lights(const DevTri* tris, const uint32_t* direct, uint32_t n,
+                                 DevLight* out, hipStream_t s);
+hipError_t launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t s);
+
+// ---------------------------------------------------------------- builder
+
+struct BuildResult {
+  uint32_t num_nodes;
+  uint32_t depth;
+  uint32_t max_leaf;
+  uint32_t leaves;
+  float avg_area_ratio;
+  float avg_tris_per_leaf;
+  float build_ms;
+  int error;  // 0, or a cv_status value
+};
+
+// Builds the reference BVH (bvh.rs:91-545) of ntris triangles (device v9,
+// source order) on the GPU.  nodes_out (device, 2*ntris DevNode) receives
+// the depth-first pair layout of crystallize (bvh.rs:389-428), order_out
+// (device, ntris) the BVH-position -> source-index permutation.
+hipError_t build_reference_bvh(const float* d_v9, uint32_t ntris, DevNode* nodes_out,
+                               uint32_t* order_out, BuildResult* res, hipStream_t s);
+
+}  // namespace cvk

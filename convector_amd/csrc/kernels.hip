@@ -1,0 +1,820 @@
+// kernels.hip -- hand-written gfx950 kernels of the per-ray path:
+// BVH traversal with triangle intersection (extend), camera rays
+// (generate), one-sample-MIS shading (shade) and the output conversions.
+//
+// One ray per lane of a wave64.  Paths live in HBM as structure-of-arrays
+// queues; every bounce is an extend launch and a shade launch that appends
+// the surviving paths to the next queue with one atomic per wave.  Extend and
+// shade are persistent: each wave fetches 64 queue entries at a time from a
+// device cursor, so no host round trip is needed between launches.
+#include "device_math.hpp"
+#include "internal.hpp"
+
+namespace cvk {
+
+// ---------------------------------------------------------------------------
+// Traversal (bvh.rs:589-657, aabb.rs:105-183, triangle.rs:59-148)
+// ---------------------------------------------------------------------------
+
+// Aabb::intersect with the reciprocals hoisted (aabb.rs:105-141).
+template <bool EXACT_NAN>
+__device__ __forceinline__ void slab(float4 a, float4 b, V3 o, V3 inv, float& tmin, float& tmax) {
+  float tx1 = (a.x - o.x) * inv.x, tx2 = (a.w - o.x) * inv.x;
+  float ty1 = (a.y - o.y) * inv.y, ty2 = (b.x - o.y) * inv.y;
+  float tz1 = (a.z - o.z) * inv.z, tz2 = (b.y - o.z) * inv.z;
+  if (EXACT_NAN) {  // 0 * inf gives the negative default NaN on x86
+    tx1 = x86nan(tx1);
+    tx2 = x86nan(tx2);
+    ty1 = x86nan(ty1);
+    ty2 = x86nan(ty2);
+    tz1 = x86nan(tz1);
+    tz2 = x86nan(tz2);
+  }
+  float txmin = xmin(tx1, tx2), txmax = xmax(tx1, tx2);
+  float tymin = xmin(ty1, ty2), tymax = xmax(ty1, ty2);
+  float tzmin = xmin(tz1, tz2), tzmax = xmax(tz1, tz2);
+  tmin = xmax(txmin, xmax(tymin, tzmin));
+  tmax = xmin(txmax, xmin(tymax, tzmax));
+}
+
+struct TravState {
+  float best_t;
+  uint32_t best;
+  float bu, bv;
+  uint32_t na, nt;
+};
+
+// Per-lane stack: S entries in LDS (stride 64 lanes, bank-conflict free),
+// deeper entries in a global spill area (stride = all persistent lanes).
+// Entry: x = node.index | node.len << 27, y = tmin bits.  PACKED == false
+// stores the node id instead and reads index/len at pop time.
+template <int S, bool PACKED>
+struct Stack {
+  uint2* lds;
+  uint2* spill;
+  uint32_t stride;
+  int sp;
+  __device__ __forceinline__ void push(uint32_t x, float tmin) {
+    uint2 e = make_uint2(x, f2u(tmin));
+    if (sp < S)
+      lds[sp * 64] = e;
+    else
+      spill[(size_t)(sp - S) * stride] = e;
+    sp++;
+  }
+  __device__ __forceinline__ uint2 pop() {
+    sp--;
+    return sp < S ? lds[sp * 64] : spill[(size_t)(sp - S) * stride];
+  }
+};
+
+template <bool EXACT_NAN, bool COUNT, int S, bool PACKED>
+__device__ __forceinline__ void visit_pair(const DevNode* __restrict__ nodes, uint32_t base, V3 o,
+                                           V3 inv, Stack<S, PACKED>& st) {
+  const float4* p = reinterpret_cast<const float4*>(nodes + base);
+  float4 a0 = p[0], b0 = p[1], a1 = p[2], b1 = p[3];
+  float tmin0, tmax0, tmin1, tmax1;
+  slab<EXACT_NAN>(a0, b0, o, inv, tmin0, tmax0);
+  slab<EXACT_NAN>(a1, b1, o, inv, tmin1, tmax1);
+  // any_masked: mask (tmax NLT tmin) and tmax's sign bit 0 (aabb.rs:155-158)
+  bool h0 = !(tmax0 < tmin0) && sgn(tmax0) == 0;
+  bool h1 = !(tmax1 < tmin1) && sgn(tmax1) == 0;
+  uint32_t x0, x1;
+  if (PACKED) {
+    x0 = f2u(b0.z) | (f2u(b0.w) << 27);
+    x1 = f2u(b1.z) | (f2u(b1.w) << 27);
+  } else {
+    x0 = base;
+    x1 = base + 1;
+  }
+  // should_try_before: push 0 first (so 1 pops first) when tmin0 - tmin1
+  // has sign bit 0 (bvh.rs:616-622)
+  if (sgn_x86_sub(tmin0, tmin1) == 0) {
+    if (h0) st.push(x0, tmin0);
+    if (h1) st.push(x1, tmin1);
+  } else {
+    if (h1) st.push(x1, tmin1);
+    if (h0) st.push(x0, tmin0);
+  }
+}
+
+// Triangle::intersect, keeping only what decides the pick (triangle.rs:59-127).
+__device__ __forceinline__ void test_tri(const DevTri* __restrict__ tris, uint32_t k, V3 o, V3 d,
+                                         TravState& ts) {
+  const float4* tp = reinterpret_cast<const float4*>(tris + k);
+  float4 P = tp[0], Q = tp[1], R = tp[2];
+  V3 v0 = mk3(P.x, P.y, P.z);
+  V3 n = mk3(P.w, Q.w, R.w);
+  V3 e1 = mk3(Q.x, Q.y, Q.z);
+  V3 e2 = mk3(R.x, R.y, R.z);
+  V3 fr = sub3(v0, o);
+  float denom = 1.0f / dot3(d, n);
+  float t = dot3(fr, n) * denom;
+  V3 c = cross3(d, fr);
+  float u = dot3(c, e2) * denom;
+  float v = dot3(c, e1) * denom;
+  float w = x86nan((1.0f - u) - v);
+  uint32_t mp = (f2u(t) | f2u(u)) | (f2u(v) | f2u(w));
+  if ((mp >> 31) == 0 && t < ts.best_t) {
+    ts.best_t = t;
+    ts.best = k;
+    ts.bu = u;
+    ts.bv = v;
+  }
+}
+
+template <bool EXACT_NAN, bool COUNT, int S, bool PACKED>
+__device__ __forceinline__ void traverse_impl(const DevNode* __restrict__ nodes,
+                                              const DevTri* __restrict__ tris, V3 o, V3 d, V3 inv,
+                                              Stack<S, PACKED>& st, TravState& ts) {
+  visit_pair<EXACT_NAN, COUNT>(nodes, 0u, o, inv, st);
+  while (st.sp > 0) {
+    uint2 e = st.pop();
+    // is_further_away_than (aabb.rs:161-170)
+    if (!(u2f(e.y) < ts.best_t)) continue;
+    uint32_t idx, len;
+    if (PACKED) {
+      idx = e.x & 0x07ffffffu;
+      len = e.x >> 27;
+    } else {
+      float4 nb = reinterpret_cast<const float4*>(nodes + e.x)[1];
+      idx = f2u(nb.z);
+      len = f2u(nb.w);
+    }
+    if (len == 0) {
+      if (COUNT) ts.na += 2;
+      visit_pair<EXACT_NAN, COUNT>(nodes, idx, o, inv, st);
+    } else {
+      for (uint32_t k = idx; k < idx + len; k++) {
+        test_tri(tris, k, o, d, ts);
+        if (COUNT) ts.nt++;
+      }
+    }
+  }
+}
+
+// Nearest hit of one ray seeded with the far-away sky hit at 1e5
+// (scene.rs:220-230).
+template <bool COUNT, int S, bool PACKED>
+__device__ __forceinline__ TravState traverse(const DevNode* __restrict__ nodes,
+                                              const DevTri* __restrict__ tris, V3 o, V3 d,
+                                              uint2* lds, uint2* spill, uint32_t stride,
+                                              bool valid) {
+  TravState ts;
+  ts.best_t = 1.0e5f;
+  ts.best = kNoHit;
+  ts.bu = ts.bv = 0.0f;
+  ts.na = 2;
+  ts.nt = 0;
+  V3 inv = mk3(rcp(d.x), rcp(d.y), rcp(d.z));
+  Stack<S, PACKED> st{lds, spill, stride, 0};
+  bool exact = !(isfinite(inv.x) && isfinite(inv.y) && isfinite(inv.z));
+  if (!valid) return ts;
+  if (__builtin_expect(__any(exact), 0))
+    traverse_impl<true, COUNT>(nodes, tris, o, d, inv, st, ts);
+  else
+    traverse_impl<false, COUNT>(nodes, tris, o, d, inv, st, ts);
+  return ts;
+}
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// Wave-wide work fetch: lane 0 takes 64 entries from the cursor.
+__device__ __forceinline__ uint32_t fetch64(uint32_t* cursor) {
+  uint32_t base = 0;
+  if (lane_id() == 0) base = atomicAdd(cursor, 64u);
+  return __shfl(base, 0);
+}
+
+// ---------------------------------------------------------------------------
+// cv_bvh_intersect_nearest
+// ---------------------------------------------------------------------------
+
+template <bool COUNT, int S, bool PACKED>
+__global__ __launch_bounds__(64) void k_intersect(IntersectArgs a) {
+  __shared__ uint2 lds[S * 64];
+  const uint32_t lane = lane_id();
+  const uint32_t gtid = blockIdx.x * 64 + lane;
+  const uint32_t stride = gridDim.x * 64;
+  for (;;) {
+    uint32_t base = fetch64(a.cursor);
+    if (base >= a.n) break;
+    uint32_t i = base + lane;
+    bool valid = i < a.n;
+    V3 o = mk3(0, 0, 0), d = mk3(1, 0, 0);
+    if (valid) {
+      const float* r = a.rays + 6 * (size_t)i;
+      o = mk3(r[0], r[1], r[2]);
+      d = mk3(r[3], r[4], r[5]);
+    }
+    TravState ts = traverse<COUNT, S, PACKED>(a.nodes, a.tris, o, d, lds + lane,
+                                              a.spill + gtid, stride, valid);
+    if (!valid) continue;
+    uint32_t src = ts.best == kNoHit ? kNoHit : a.tri_src[ts.best];
+    if (a.hits) {
+      reinterpret_cast<uint4*>(a.hits)[i] =
+          make_uint4(src, f2u(ts.best_t), f2u(ts.bu), f2u(ts.bv));
+    }
+    if (a.isects) {
+      // The deferred MIntersection fields (triangle.rs:129-142, scene.rs:221-228).
+      V3 pos, nrm;
+      uint32_t mat;
+      float tu = 0.0f, tv = 0.0f;
+      if (ts.best == kNoHit) {
+        pos = muladd3(d, 1.0e5f, o);
+        nrm = d;
+        mat = kMatSky;
+      } else {
+        const float4* tp = reinterpret_cast<const float4*>(a.tris + ts.best);
+        V3 n = mk3(tp[0].w, tp[1].w, tp[2].w);
+        pos = muladd3(d, ts.best_t, o);
+        nrm = normalized3(n);
+        mat = a.tri_mat[ts.best];
+        const float* uv = a.tri_uv + 6 * (size_t)ts.best;
+        float w = (1.0f - ts.bu) - ts.bv;
+        tu = fmaf(uv[0], w, fmaf(uv[2], ts.bv, uv[4] * ts.bu));
+        tv = fmaf(uv[1], w, fmaf(uv[3], ts.bv, uv[5] * ts.bu));
+      }
+      uint32_t* out = a.isects + 16 * (size_t)i;
+      out[0] = f2u(pos.x);
+      out[1] = f2u(pos.y);
+      out[2] = f2u(pos.z);
+      out[3] = f2u(nrm.x);
+      out[4] = f2u(nrm.y);
+      out[5] = f2u(nrm.z);
+      out[6] = f2u(ts.best_t);
+      out[7] = mat;
+      out[8] = f2u(tu);
+      out[9] = f2u(tv);
+      out[10] = src;
+      out[11] = f2u(ts.bu);
+      out[12] = f2u(ts.bv);
+      out[13] = ts.na;
+      out[14] = ts.nt;
+      out[15] = 0;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Path state helpers
+// ---------------------------------------------------------------------------
+
+// PIXEL stream: splitmix64 premix, xorshift64 (13, 7, 17), high 32 bits.
+__device__ __forceinline__ uint64_t splitmix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+__device__ __forceinline__ uint64_t pixel_seed(uint32_t pixel, uint32_t frame) {
+  uint64_t s = splitmix64((uint64_t)pixel ^ ((uint64_t)frame << 32));
+  return s ? s : 0x9E3779B97F4A7C15ull;
+}
+__device__ __forceinline__ uint32_t rng_next(uint64_t& s) {
+  uint64_t x = s;
+  x ^= x << 13;
+  x ^= x >> 7;
+  x ^= x << 17;
+  s = x;
+  return (uint32_t)(x >> 32);
+}
+// Rng conversions (random.rs:104-136)
+__device__ __forceinline__ float u32_unit(uint32_t u) {
+  return fmaf((float)(int32_t)u, 0.5f / -2147483648.0f, 0.5f);
+}
+__device__ __forceinline__ float u32_biunit(uint32_t u) {
+  return (float)(int32_t)u * (1.0f / -2147483648.0f);
+}
+__device__ __forceinline__ float u32_angle(uint32_t u) {
+  return (float)(int32_t)u * (3.14159274f / -2147483648.0f);
+}
+
+// Polynomial sine and cosine (simd.rs:190-233).
+__device__ __forceinline__ float sin_poly(float x) {
+  float x2 = x * x;
+  float x3 = x * x2;
+  float x5 = x3 * x2;
+  return fmaf(x5, 0.00533738081738135917f, fmaf(x3, -0.152175915294f, x * 0.982006669405f));
+}
+__device__ __forceinline__ float cos_poly(float x) {
+  float x2 = x * x;
+  float x4 = x2 * x2;
+  float x6 = x2 * x4;
+  return fmaf(x6, -0.000966231179636657107f,
+              fmaf(x4, 0.0392596924214f, fmaf(x2, -0.496000299455f, 1.0f)));
+}
+
+// MVector3::rotate_hemisphere (vector3.rs:219-251)
+__device__ __forceinline__ V3 rotate_hemisphere(V3 v, V3 normal) {
+  uint32_t flip = f2u(normal.z);
+  V3 n = pick3(normal, neg3(normal), flip);
+  float rz = rcp(1.0f + n.z);
+  float c = (n.x * n.y) * rz;
+  float x = fmaf(v.x, fmaf(n.y, n.y * rz, n.z), -fmaf(v.y, c, -(v.z * n.x)));
+  float y = fmaf(-v.x, c, fmaf(v.y, fmaf(n.x, n.x * rz, n.z), v.z * n.y));
+  float z = fmaf(-v.x, n.x, fmaf(-v.y, n.y, v.z * n.z));
+  V3 r = mk3(x, y, z);
+  return pick3(r, neg3(r), flip);
+}
+
+// MQuaternion::interpolate + quaternion::rotate (quaternion.rs:62-153)
+__device__ __forceinline__ V3 camera_dir(const CameraParams& cam, float x, float y, float t) {
+  float a = fmaf(cam.dq[0], t, cam.q[0]);
+  float b = fmaf(cam.dq[1], t, cam.q[1]);
+  float c = fmaf(cam.dq[2], t, cam.q[2]);
+  float d = fmaf(cam.dq[3], t, cam.q[3]);
+  float ns = fmaf(a, a, b * b) + fmaf(c, c, d * d);
+  float rnorm = 1.0f / sqrtf(ns);
+  float qa = a * rnorm, qb = b * rnorm, qc = c * rnorm, qd = d * rnorm;
+  V3 v = normalized3(mk3(x, y, -cam.screen_distance));
+  float pa = (qb * v.x + qc * v.y) + qd * v.z;
+  float pb = (((qb * v.x - qc * v.y) + qd * v.z) + (qa - qb) * v.x) + (qc - qd) * (v.y + v.z);
+  float pc = (qd * v.x + qa * v.y) - qb * v.z;
+  float pd = (((qd * v.x - qa * v.y) - qb * v.z) - (qc + qd) * v.x) + (qa + qb) * (v.y + v.z);
+  float rb = (((((pa + pb) * (qa + qb) - (pc - pd) * (qc + qd)) - pa * qa) - pb * qb) + pc * qc) -
+             pd * qd;
+  float rc = ((pc * qa - pd * qb) + pa * qc) + pb * qd;
+  float rd = (((((pc + pd) * (qa + qb) + (pa - pb) * (qc + qd)) - pc * qa) - pd * qb) - pa * qc) +
+             pb * qd;
+  return mk3(rb, rc, rd);
+}
+
+// sky_intensity (material.rs:205-215)
+__device__ __forceinline__ V3 sky_intensity(V3 dir) {
+  float dd = fmaf(dot3(dir, mk3(0.0f, 0.0f, 1.0f)), 0.5f, 0.5f);
+  return muladd3(mk3(dd, dd * dd, dd * (dd * dd)), 2.0f, mk3(0.5f, 0.5f, 0.5f));
+}
+
+// cvtps2dq: round to nearest even, NaN/out of range -> 0x80000000.
+__device__ __forceinline__ uint32_t cvt_rne(float x) {
+  if (!(x >= -2147483648.0f && x < 2147483648.0f)) return 0x80000000u;
+  return (uint32_t)(int32_t)rintf(x);
+}
+
+// store_pixels_color_16x4 (renderer.rs:239-256)
+__device__ __forceinline__ uint32_t pack_color(V3 c) {
+  V3 x2 = scale3(c, 2.0f);
+  uint32_t r = cvt_rne(xmin(1.0f, x2.x) * 255.0f);
+  uint32_t g = cvt_rne(xmin(1.0f, x2.y) * 255.0f) << 8;
+  uint32_t b = cvt_rne(xmin(1.0f, x2.z) * 255.0f) << 16;
+  return (r | g) | b;
+}
+// store_pixels_gbuffer_16x4 (renderer.rs:260-286)
+__device__ __forceinline__ uint32_t pack_gbuffer(uint32_t tex_index, float tu, float tv, float fr) {
+  uint32_t r = cvt_rne(tu * 255.0f) & 0xffu;
+  uint32_t g = (cvt_rne(tv * 255.0f) & 0xffu) << 8;
+  uint32_t b = cvt_rne(fr * 255.0f) << 16;
+  uint32_t a = tex_index << 24;
+  return (r | g) | (b | a);
+}
+
+// Offset of a pixel in the 16x4-block HDR layout (renderer.rs:154-206,356-358).
+__device__ __forceinline__ size_t hdr_offset(uint32_t px, uint32_t py, uint32_t width) {
+  uint32_t dx = px & 15u, dy = py & 3u;
+  uint32_t g = (dx >> 2) * 2 + (dy >> 1);
+  uint32_t l = (dx & 3u) + 4 * (dy & 1u);
+  size_t blk = (size_t)(py >> 2) * (width >> 4) + (px >> 4);
+  return blk * 192 + g * 24 + l;
+}
+
+// ---------------------------------------------------------------------------
+// generate: camera rays (renderer.rs:171-206,413-416, scene.rs:81-99)
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_generate(GenerateArgs a) {
+  const uint32_t npix = a.rect_w * a.rect_h;
+  if (blockIdx.x == 0 && threadIdx.x < kMaxBounceSlots) {
+    a.ctrl->count[threadIdx.x] = threadIdx.x == 0 ? npix : 0u;
+    a.ctrl->ext_cursor[threadIdx.x] = 0u;
+    a.ctrl->shade_cursor[threadIdx.x] = 0u;
+  }
+  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < npix; p += gridDim.x * blockDim.x) {
+    uint32_t px = a.rect_x + p % a.rect_w;
+    uint32_t py = a.rect_y + p / a.rect_w;
+    uint32_t pixel = py * a.width + px;
+    uint32_t xb = px & ~15u, yb = py & ~3u, dx = px & 15u, dy = py & 3u;
+    uint32_t g = (dx >> 2) * 2 + (dy >> 1);
+    uint32_t l = (dx & 3u) + 4 * (dy & 1u);
+    const float scale = 2.0f / (float)a.width;
+    const float cx = (float)xb - (float)a.width * 0.5f;
+    const float cy = (float)yb - (float)a.height * 0.5f;
+    float bx = scale * ((float)(l & 3u) + cx);
+    float by = scale * ((float)(l >> 2) + cy);
+    float xs = bx, ys = by;
+    uint32_t gx = g >> 1;
+    if (gx == 1) xs = bx + 4.0f * scale;
+    if (gx == 2) xs = bx + 8.0f * scale;
+    if (gx == 3) xs = bx + 12.0f * scale;
+    if (g & 1u) ys = by + 2.0f * scale;
+    uint64_t st = pixel_seed(pixel, a.frame);
+    float x = fmaf(u32_unit(rng_next(st)), scale, xs);
+    float y = fmaf(u32_unit(rng_next(st)), scale, ys);
+    float t = u32_unit(rng_next(st));
+    V3 o = muladd3(mk3(a.cam.dpos[0], a.cam.dpos[1], a.cam.dpos[2]), t,
+                   mk3(a.cam.pos[0], a.cam.pos[1], a.cam.pos[2]));
+    V3 d = camera_dir(a.cam, x, y, t);
+    a.q.r0[p] = make_float4(o.x, o.y, o.z, u2f(pixel));
+    a.q.r1[p] = make_float4(d.x, d.y, d.z, 0.0f);
+    a.q.col[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    a.q.rng[p] = st;
+    if (a.gbuffer) a.gbuffer[pixel] = 0u;
+    if (a.bitmap) a.bitmap[pixel] = 0u;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// extend: nearest hit of every queued ray
+// ---------------------------------------------------------------------------
+
+template <bool COUNT, int S, bool PACKED>
+__global__ __launch_bounds__(64) void k_extend(ExtendArgs a) {
+  __shared__ uint2 lds[S * 64];
+  const uint32_t lane = lane_id();
+  const uint32_t gtid = blockIdx.x * 64 + lane;
+  const uint32_t stride = gridDim.x * 64;
+  const uint32_t n = a.ctrl->count[a.bounce];
+  if (blockIdx.x == 0 && lane == 0) atomicAdd(&a.ctrl->total_rays, (unsigned long long)n);
+  uint64_t na = 0, nt = 0;
+  for (;;) {
+    uint32_t base = fetch64(&a.ctrl->ext_cursor[a.bounce]);
+    if (base >= n) break;
+    uint32_t i = base + lane;
+    bool valid = i < n;
+    V3 o = mk3(0, 0, 0), d = mk3(1, 0, 0);
+    if (valid) {
+      float4 r0 = a.r0[i], r1 = a.r1[i];
+      o = mk3(r0.x, r0.y, r0.z);
+      d = mk3(r1.x, r1.y, r1.z);
+    }
+    TravState ts = traverse<COUNT, S, PACKED>(a.nodes, a.tris, o, d, lds + lane,
+                                              a.spill + gtid, stride, valid);
+    if (valid) {
+      a.hits[i] = make_uint4(ts.best, f2u(ts.best_t), f2u(ts.bu), f2u(ts.bv));
+      if (COUNT) {
+        na += ts.na;
+        nt += ts.nt;
+      }
+    }
+  }
+  if (COUNT) {
+    for (int off = 32; off > 0; off >>= 1) {
+      na += __shfl_down(na, off);
+      nt += __shfl_down(nt, off);
+    }
+    if (lane == 0) {
+      atomicAdd(&a.ctrl->sum_aabb, (unsigned long long)na);
+      atomicAdd(&a.ctrl->sum_tri, (unsigned long long)nt);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// shade: continue_path (material.rs:327-411) and the loop of render_pixels
+// (renderer.rs:413-468), one pixel per lane.
+// ---------------------------------------------------------------------------
+
+struct ShadeOut {
+  V3 o, d, cm;
+  float fresnel;
+};
+
+__device__ __forceinline__ ShadeOut continue_path(const ShadeArgs& a, V3 ray_d, V3 pos, V3 normal,
+                                                  uint32_t mat, uint64_t& st,
+                                                  bool ignore_fresnel) {
+  const uint32_t d_angle = rng_next(st), d_unit = rng_next(st);
+  const uint32_t d_bits = rng_next(st), d_u = rng_next(st), d_v = rng_next(st);
+  const uint32_t d_rr = rng_next(st);
+  // continue_path_brdf (material.rs:228-242), sample_hemisphere_vector (random.rs:140-159)
+  float phi = u32_angle(d_angle), r_sqr = u32_unit(d_unit);
+  float rr_ = sqrtf(r_sqr);
+  V3 dir_z = mk3(sin_poly(phi) * rr_, cos_poly(phi) * rr_, sqrtf(1.0f - r_sqr));
+  V3 dir_b = rotate_hemisphere(dir_z, normal);
+  V3 org_b = muladd3(dir_b, 1.0e-4f, pos);
+  // continue_path_direct_sample + Scene::get_direct_sample (scene.rs:143-202)
+  const DevLight& L = a.lights[d_bits >> 29];
+  float su = u32_unit(d_u), sv = u32_unit(d_v);
+  uint32_t sm = f2u((1.0f - su) - sv);
+  float uu = pickf(su, 1.0f - su, sm);
+  float vv = pickf(sv, 1.0f - sv, sm);
+  V3 e1 = mk3(L.e1[0], L.e1[1], L.e1[2]);
+  V3 e2 = mk3(L.e2[0], L.e2[1], L.e2[2]);
+  V3 lv0 = mk3(L.v0[0], L.v0[1], L.v0[2]);
+  V3 p = muladd3(e2, vv, negmuladd3(e1, uu, lv0));
+  V3 dir_d = normalized3(sub3(p, pos));
+  V3 org_d = muladd3(dir_d, 1.0e-4f, pos);
+  float rr = fmaf(normal.z, 0.8f, u32_biunit(d_rr));
+  uint32_t rrm = f2u(rr);
+  V3 no = pick3(org_b, org_d, rrm);
+  V3 nd = pick3(dir_b, dir_d, rrm);
+  // pd_brdf (material.rs:245-250)
+  float pd_b = xmax(dot3(normal, nd), 0.0f) * (1.0f / 3.14159274f);
+  // pd_direct_sample (material.rs:281-320) over every light in BVH order
+  float total = 0.0f;
+  for (uint32_t k = 0; k < a.nlights; k++) {
+    const DevLight& M = a.lights[k];
+    V3 m0 = mk3(M.v0[0], M.v0[1], M.v0[2]);
+    V3 mn = mk3(M.n[0], M.n[1], M.n[2]);
+    V3 me1 = mk3(M.e1[0], M.e1[1], M.e1[2]);
+    V3 me2 = mk3(M.e2[0], M.e2[1], M.e2[2]);
+    V3 fr = sub3(m0, no);
+    float denom = rcp(dot3(nd, mn));
+    float t = x86nan(dot3(fr, mn) * denom);
+    V3 c = cross3(nd, fr);
+    float u = dot3(c, me2) * denom;
+    float v = dot3(c, me1) * denom;
+    uint32_t mask_uv = geq(u + v, 1.0f);
+    uint32_t mask = (f2u(u) | f2u(v)) | (f2u(t) | mask_uv);
+    float dist_sqr = t * t;
+    float dot_em = absb(dot3(mk3(M.normal[0], M.normal[1], M.normal[2]), nd)) + 0.0001f;
+    float pd = dist_sqr * rcp(M.area * dot_em);
+    total = pickf(total + pd, total, mask);
+  }
+  float pd_d = total * (1.0f / (float)a.nlights);
+  float weight_denom = (pd_b + pd_d) + 0.01f;
+  float p_brdf = fmaf(normal.z, 0.4f, 0.5f);
+  float p_direct = fmaf(-normal.z, 0.4f, 0.5f);
+  float pp = pickf(p_brdf, p_direct, rrm);
+  float modulation = rcp(weight_denom * pp);
+  float cos_theta = xmax(dot3(normal, nd), 0.0f);
+  // microfacet_brdf (material.rs:415-450): ray_in = the new ray, ray_out = the old
+  int32_t mi = (int32_t)mat;
+  const float k255 = 1.0f / 255.0f;
+  V3 color = mk3((float)(mi & 0xff) * k255, (float)((mi >> 8) & 0xff) * k255,
+                 (float)((mi >> 16) & 0xff) * k255);
+  int32_t gloss = (mi >> 26) & 7;
+  V3 h = normalized3(sub3(nd, ray_d));
+  // microfacet_normal_dist (material.rs:475-513)
+  const float pi = 3.14159274f;
+  float c1 = dot3(h, normal);
+  float c2 = c1 * c1;
+  float c4 = c2 * c2;
+  float c8 = c4 * c4;
+  float c16 = c8 * c8;
+  float dterm;
+  switch (gloss) {
+    case 0: dterm = 0.5f / pi; break;
+    case 1: dterm = xmax(c1, 0.0f) * (1.0f / pi); break;
+    case 2: dterm = c2 * (1.5f / pi); break;
+    case 3: dterm = c4 * (2.5f / pi); break;
+    case 4: dterm = c8 * (4.5f / pi); break;
+    default: dterm = c16 * (8.5f / pi); break;
+  }
+  // microfacet_fresnel (material.rs:453-469)
+  V3 r1 = sub3(mk3(1.0f, 1.0f, 1.0f), color);
+  float ct = 1.0f - absb(dot3(h, nd));
+  float ct2 = ct * ct;
+  float ct3 = ct2 * ct;
+  float ct5 = absb(ct2 * ct3);
+  V3 fc = muladd3(r1, ct5, color);
+  if (ignore_fresnel && (((uint32_t)mi << 6) | ((uint32_t)mi << 7)) >> 31) fc = mk3(1.0f, 1.0f, 1.0f);
+  V3 brdf = scale3(fc, dterm);
+  V3 cm = scale3(brdf, modulation * cos_theta);
+  ShadeOut r;
+  r.cm = mk3(xmin(cm.x, 2.0f), xmin(cm.y, 2.0f), xmin(cm.z, 2.0f));
+  r.o = no;
+  r.d = nd;
+  r.fresnel = ct5;
+  return r;
+}
+
+__global__ __launch_bounds__(64) void k_shade(ShadeArgs a) {
+  const uint32_t lane = lane_id();
+  const uint32_t n = a.ctrl->count[a.bounce];
+  const bool last = a.bounce + 1 >= a.max_bounces;
+  for (;;) {
+    uint32_t base = fetch64(&a.ctrl->shade_cursor[a.bounce]);
+    if (base >= n) break;
+    uint32_t i = base + lane;
+    bool valid = i < n;
+    bool keep = false;
+    float4 r0 = make_float4(0, 0, 0, 0), r1 = r0, col = r0;
+    uint64_t st = 0;
+    ShadeOut so;
+    if (valid) {
+      r0 = a.in.r0[i];
+      r1 = a.in.r1[i];
+      col = a.in.col[i];
+      st = a.in.rng[i];
+      uint4 hit = a.hits[i];
+      V3 o = mk3(r0.x, r0.y, r0.z), d = mk3(r1.x, r1.y, r1.z);
+      V3 color = mk3(col.x, col.y, col.z);
+      uint32_t pixel = f2u(r0.w);
+      uint32_t px = pixel % a.width, py = pixel / a.width;
+      float t = u2f(hit.y);
+      V3 pos, normal;
+      uint32_t mat;
+      float tu = 0.0f, tv = 0.0f;
+      if (hit.x == kNoHit) {
+        pos = muladd3(d, 1.0e5f, o);
+        normal = d;
+        mat = kMatSky;
+      } else {
+        const float4* tp = reinterpret_cast<const float4*>(a.tris + hit.x);
+        V3 nn = mk3(tp[0].w, tp[1].w, tp[2].w);
+        pos = muladd3(d, t, o);
+        normal = normalized3(nn);
+        mat = a.tri_mat[hit.x];
+        if (a.bounce == 0 && a.gbuffer) {
+          const float* uv = a.tri_uv + 6 * (size_t)hit.x;
+          float bu = u2f(hit.z), bv = u2f(hit.w);
+          float w = (1.0f - bu) - bv;
+          tu = fmaf(uv[0], w, fmaf(uv[2], bv, uv[4] * bu));
+          tv = fmaf(uv[1], w, fmaf(uv[3], bv, uv[5] * bu));
+        }
+      }
+      if (mat >> 31) {
+        // The path found an emitter: colour x emission (renderer.rs:452-460).
+        V3 em = (a.has_override && hit.x != kNoHit)
+                    ? mk3(a.emission[0], a.emission[1], a.emission[2])
+                    : sky_intensity(d);
+        V3 c = mulc3(color, em);
+        if (a.hdr) {
+          float* h = a.hdr + hdr_offset(px, py, a.width);
+          h[0] += c.x;
+          h[8] += c.y;
+          h[16] += c.z;
+        }
+        if (a.bitmap) a.bitmap[pixel] = pack_color(c);
+      } else if (!last || (a.bounce == 0 && a.gbuffer)) {
+        so = continue_path(a, d, pos, normal, mat, st, a.bounce == 0);
+        if (a.bounce == 0 && a.gbuffer) {
+          uint32_t tex_index = ((int32_t)mat >> 24) & 3;
+          a.gbuffer[pixel] = pack_gbuffer(tex_index, tu, tv, so.fresnel);
+        }
+        if (!last) {
+          keep = true;
+          so.cm = mulc3(color, so.cm);
+        }
+      }
+    }
+    // wave-aggregated append to the next queue
+    uint64_t ballot = __ballot(keep);
+    if (ballot == 0) continue;
+    uint32_t cnt = __popcll(ballot);
+    uint32_t leader = __ffsll((unsigned long long)ballot) - 1;
+    uint32_t slot0 = 0;
+    if (lane == leader) slot0 = atomicAdd(&a.ctrl->count[a.bounce + 1], cnt);
+    slot0 = __shfl(slot0, leader);
+    if (keep) {
+      uint32_t j = slot0 + __popcll(ballot & ((1ull << lane) - 1ull));
+      a.out.r0[j] = make_float4(so.o.x, so.o.y, so.o.z, r0.w);
+      a.out.r1[j] = make_float4(so.d.x, so.d.y, so.d.z, 0.0f);
+      a.out.col[j] = make_float4(so.cm.x, so.cm.y, so.cm.z, 0.0f);
+      a.out.rng[j] = st;
+    }
+  }
+}
+
+// buffer_f32_into_render_buffer (renderer.rs:378-410)
+__global__ __launch_bounds__(256) void k_hdr_to_bitmap(const float* hdr, uint32_t* bitmap,
+                                                       uint32_t width, uint32_t height,
+                                                       uint32_t num_samples) {
+  const float factor = 1.0f / (float)num_samples;
+  uint32_t npix = width * height;
+  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < npix; p += gridDim.x * blockDim.x) {
+    uint32_t px = p % width, py = p / width;
+    const float* h = hdr + hdr_offset(px, py, width);
+    bitmap[p] = pack_color(mk3(h[0] * factor, h[8] * factor, h[16] * factor));
+  }
+}
+
+// Per-triangle device geometry (DevTri) and light data (DevLight) with the
+// reference's operations (triangle.rs:65-80,152-165).
+__global__ __launch_bounds__(256) void k_prepare_tris(const float* v9, const uint32_t* order,
+                                                      uint32_t ntris, DevTri* out) {
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < ntris; k += gridDim.x * blockDim.x) {
+    const float* t = v9 + 9 * (size_t)order[k];
+    V3 v0 = mk3(t[0], t[1], t[2]), v1 = mk3(t[3], t[4], t[5]), v2 = mk3(t[6], t[7], t[8]);
+    V3 e1 = sub3(v0, v2);
+    V3 e2 = sub3(v1, v0);
+    V3 n = cross3(e1, e2);
+    out[k].p = make_float4(v0.x, v0.y, v0.z, n.x);
+    out[k].q = make_float4(e1.x, e1.y, e1.z, n.y);
+    out[k].r = make_float4(e2.x, e2.y, e2.z, n.z);
+  }
+}
+
+__global__ void k_prepare_lights(const DevTri* tris, const uint32_t* direct, uint32_t n,
+                                 DevLight* out) {
+  uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const DevTri& T = tris[direct[k]];
+  V3 v0 = mk3(T.p.x, T.p.y, T.p.z);
+  V3 e1 = mk3(T.q.x, T.q.y, T.q.z);
+  V3 e2 = mk3(T.r.x, T.r.y, T.r.z);
+  V3 nn = mk3(T.p.w, T.q.w, T.r.w);
+  float rnorm = rsqrt(dot3(nn, nn));
+  float area = 0.5f * rcp(rnorm);
+  V3 nrm = scale3(nn, rnorm);
+  DevLight L;
+  L.v0[0] = v0.x; L.v0[1] = v0.y; L.v0[2] = v0.z;
+  L.e1[0] = e1.x; L.e1[1] = e1.y; L.e1[2] = e1.z;
+  L.e2[0] = e2.x; L.e2[1] = e2.y; L.e2[2] = e2.z;
+  L.n[0] = nn.x; L.n[1] = nn.y; L.n[2] = nn.z;
+  L.normal[0] = nrm.x; L.normal[1] = nrm.y; L.normal[2] = nrm.z;
+  L.area = area;
+  out[k] = L;
+}
+
+__global__ void k_zero_u32(uint32_t* p, uint32_t n) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+}  // namespace cvk
+
+// ---------------------------------------------------------------------------
+// Host launch wrappers
+// ---------------------------------------------------------------------------
+
+namespace cvk {
+
+#define CVK_STACK_VARIANTS(X) X(8) X(16) X(24) X(32)
+
+template <bool COUNT, int S>
+static hipError_t launch_intersect_s(const IntersectArgs& a, bool packed, uint32_t blocks,
+                                     hipStream_t s) {
+  if (packed)
+    hipLaunchKernelGGL((k_intersect<COUNT, S, true>), dim3(blocks), dim3(64), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_intersect<COUNT, S, false>), dim3(blocks), dim3(64), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_intersect(const IntersectArgs& a, bool count, bool packed, int stack,
+                            uint32_t blocks, hipStream_t s) {
+#define CASE(S)                                                                \
+  if (stack == S)                                                              \
+    return count ? launch_intersect_s<true, S>(a, packed, blocks, s)           \
+                 : launch_intersect_s<false, S>(a, packed, blocks, s);
+  CVK_STACK_VARIANTS(CASE)
+#undef CASE
+  return hipErrorInvalidValue;
+}
+
+template <bool COUNT, int S>
+static hipError_t launch_extend_s(const ExtendArgs& a, bool packed, uint32_t blocks,
+                                  hipStream_t s) {
+  if (packed)
+    hipLaunchKernelGGL((k_extend<COUNT, S, true>), dim3(blocks), dim3(64), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_extend<COUNT, S, false>), dim3(blocks), dim3(64), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_extend(const ExtendArgs& a, bool count, bool packed, int stack, uint32_t blocks,
+                         hipStream_t s) {
+#define CASE(S)                                                              \
+  if (stack == S)                                                            \
+    return count ? launch_extend_s<true, S>(a, packed, blocks, s)            \
+                 : launch_extend_s<false, S>(a, packed, blocks, s);
+  CVK_STACK_VARIANTS(CASE)
+#undef CASE
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_generate(const GenerateArgs& a, uint32_t blocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_generate, dim3(blocks), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_shade(const ShadeArgs& a, uint32_t blocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_shade, dim3(blocks), dim3(64), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_hdr_to_bitmap(const float* hdr, uint32_t* bitmap, uint32_t width,
+                                uint32_t height, uint32_t num_samples, hipStream_t s) {
+  uint32_t npix = width * height;
+  uint32_t blocks = (npix + 255) / 256;
+  if (blocks > 65535) blocks = 65535;
+  hipLaunchKernelGGL(k_hdr_to_bitmap, dim3(blocks), dim3(256), 0, s, hdr, bitmap, width, height,
+                     num_samples);
+  return hipGetLastError();
+}
+
+hipError_t launch_prepare_tris(const float* v9, const uint32_t* order, uint32_t ntris, DevTri* out,
+                               hipStream_t s) {
+  uint32_t blocks = (ntris + 255) / 256;
+  if (blocks > 65535) blocks = 65535;
+  if (blocks == 0) blocks = 1;
+  hipLaunchKernelGGL(k_prepare_tris, dim3(blocks), dim3(256), 0, s, v9, order, ntris, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_prepare_lights(const DevTri* tris, const uint32_t* direct, uint32_t n,
+                                 DevLight* out, hipStream_t s) {
+  uint32_t blocks = (n + 63) / 64;
+  if (blocks == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_prepare_lights, dim3(blocks), dim3(64), 0, s, tris, direct, n, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_zero_u32, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
+  return hipGetLastError();
+}
+
+}  // namespace cvk

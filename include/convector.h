@@ -1,0 +1,251 @@
+/*
+ * convector.h -- C ABI of the MI355X (gfx950) per-ray path of Convector.
+ *
+ * The reference (ruuda/convector) is a Rust crate with no FFI; these entry
+ * points are what a Rust crate would bind with `extern "C"` to replace its
+ * hot path.  Each declaration names the reference item it stands in for.
+ *
+ * Conventions
+ *  - Every reference panic/assert on the path becomes a cv_status; the text
+ *    of the last error of the calling thread is in cv_last_error().
+ *  - Handles are opaque.  cv_scene and cv_renderer calls that render are
+ *    serialised internally, so one handle may be shared between threads
+ *    (the reference shares `&Renderer` between its patch workers).
+ *  - Buffers are caller owned.  With CV_PTR_DEVICE in `flags` the pointers
+ *    are device pointers (hipMalloc'd); otherwise host pointers.
+ *  - Materials are the reference's 32-bit encoding (material.rs:10-30).
+ *  - Triangles are 9 floats v0,v1,v2; texture coordinates 6 floats
+ *    uv0,uv1,uv2.  Triangle ids are source indices in load order: faces in
+ *    file order after fan triangulation, meshes concatenated
+ *    (wavefront.rs:125-140, bvh.rs:547-568).
+ *  - Barycentrics follow triangle.rs:109-134: u weights v2, v weights v1,
+ *    w = 1 - u - v weights v0.
+ */
+#ifndef CONVECTOR_H
+#define CONVECTOR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum cv_status {
+    CV_OK = 0,
+    CV_ERR_INVALID_ARGUMENT = 1,
+    CV_ERR_IO = 2,                  /* FileBuffer::open(..).expect, wavefront.rs:86 */
+    CV_ERR_PARSE = 3,               /* from_str(..).unwrap / missing index, wavefront.rs:46-53,100 */
+    CV_ERR_DEGENERATE_TRIANGLE = 4, /* assert_nondegenerate panic, wavefront.rs:30-46 */
+    CV_ERR_UNKNOWN_MATERIAL = 5,    /* usemtl lookup panic, wavefront.rs:116-123 */
+    CV_ERR_NOT_MULTIPLE_OF_16 = 6,  /* assert_eq!(w & 15, 0), renderer.rs:46-47,313,346 */
+    CV_ERR_LIGHT_COUNT = 7,         /* 8 direct-sample triangles needed, scene.rs:147-160 */
+    CV_ERR_BVH_NO_SPLIT = 8,        /* assert!(!is_first) / root split, bvh.rs:303,512 */
+    CV_ERR_NAN = 9,                 /* partial_cmp(..).unwrap() on NaN, bvh.rs:186 */
+    CV_ERR_DEVICE = 10,             /* HIP runtime failure */
+    CV_ERR_OUT_OF_MEMORY = 11,
+    CV_ERR_NO_DEVICE = 12,
+    CV_ERR_OUT_OF_RANGE = 13        /* patch outside the frame */
+} cv_status;
+
+/* ---------------------------------------------------------------- meshes */
+
+typedef struct cv_mesh cv_mesh;
+
+/* One entry of the name -> material dictionary of
+ * Mesh::load_with_materials (wavefront.rs:83-85). */
+typedef struct cv_material_entry {
+    const char* name;
+    uint32_t material;
+} cv_material_entry;
+
+/* Mesh::load / Mesh::load_with_materials (wavefront.rs:79-152): `v`, `vt`,
+ * `usemtl` and `f` lines, convex n-gons fan triangulated, degenerate
+ * triangles rejected, default material SMaterial::white().
+ * table may be NULL (n = 0) for Mesh::load. */
+cv_status cv_mesh_load_obj(const char* path, const cv_material_entry* table, size_t n,
+                           cv_mesh** out);
+/* Builds a mesh from a triangle soup (no reference counterpart; the
+ * generator's path).  uv may be NULL.  Degenerate triangles are rejected as
+ * in wavefront.rs:30-46. */
+cv_status cv_mesh_from_triangles(const float* v9, const float* uv6, const uint32_t* materials,
+                                 size_t ntris, cv_mesh** out);
+size_t cv_mesh_num_triangles(const cv_mesh* mesh);
+/* Copies the triangles out (any pointer may be NULL). */
+cv_status cv_mesh_copy_triangles(const cv_mesh* mesh, float* v9, float* uv6, uint32_t* materials);
+void cv_mesh_destroy(cv_mesh* mesh);
+
+/* ---------------------------------------------------------------- scenes */
+
+typedef struct cv_scene cv_scene;
+
+/* The reference's BvhNode (bvh.rs:20-30): 32 bytes, siblings in pairs.
+ * len == 0: internal node, children at index, index+1.
+ * len > 0: leaf holding triangles [index, index+len) of the BVH order. */
+typedef struct cv_bvh_node {
+    float min[3];
+    float max[3];
+    uint32_t index;
+    uint32_t len;
+} cv_bvh_node;
+
+#define CV_BUILDER_REFERENCE 0u /* 64 bins, TreeSAH, sorted fallback: bvh.rs:91-545 */
+
+typedef struct cv_scene_opts {
+    uint32_t builder;               /* CV_BUILDER_REFERENCE */
+    uint32_t has_emission_override; /* 0: emissive hits give sky_intensity(dir) */
+    float emission_override[3];     /* radiance of emissive triangle hits when set */
+} cv_scene_opts;
+
+typedef struct cv_bvh_info {
+    uint32_t num_nodes;       /* nodes array length (pairs, bvh.rs:514-533) */
+    uint32_t num_triangles;
+    uint32_t depth;           /* levels below the implicit root */
+    uint32_t max_leaf;        /* largest leaf */
+    uint32_t num_direct;      /* Scene::direct_sample length (scene.rs:114-129) */
+    float avg_area_ratio;     /* bvh.rs:536-544 */
+    float avg_tris_per_leaf;
+    float build_ms;           /* device time of the GPU build */
+} cv_bvh_info;
+
+/* Scene::from_meshes + Bvh::from_meshes + Bvh::build (scene.rs:114-129,
+ * bvh.rs:487-568): concatenates the meshes, builds the reference BVH on the
+ * GPU and uploads the scene; the camera starts as Camera::new (scene.rs:33-42).
+ * opts may be NULL. */
+cv_status cv_scene_create(const cv_mesh* const* meshes, size_t n, const cv_scene_opts* opts,
+                          cv_scene** out);
+/* Same, with a BVH given by the caller (a serialised tree in the reference
+ * layout plus the BVH-order -> source-id permutation) instead of building. */
+cv_status cv_scene_create_with_bvh(const cv_mesh* const* meshes, size_t n,
+                                   const cv_scene_opts* opts, const cv_bvh_node* nodes,
+                                   size_t num_nodes, const uint32_t* order, cv_scene** out);
+/* Camera::set_position + set_orientation (scene.rs:47-58) and the screen
+ * distance of Camera::set_fov (scene.rs:61-63).  q = (a, b, c, d). */
+cv_status cv_scene_set_camera(cv_scene* scene, const float pos[3], const float dpos[3],
+                              const float q[4], const float dq[4], float screen_distance);
+/* Bvh::print_stats / Scene::print_stats as numbers (bvh.rs:570-583). */
+cv_status cv_scene_get_bvh_info(const cv_scene* scene, cv_bvh_info* info);
+/* Copies the BVH out: nodes (num_nodes) and order (num_triangles). */
+cv_status cv_scene_copy_bvh(const cv_scene* scene, cv_bvh_node* nodes, uint32_t* order);
+void cv_scene_destroy(cv_scene* scene);
+
+/* -------------------------------------------------------------- traversal */
+
+typedef struct cv_ray {
+    float origin[3];
+    float direction[3];
+} cv_ray;
+
+#define CV_NO_HIT 0xffffffffu
+
+typedef struct cv_hit {
+    uint32_t tri_id; /* source triangle id or CV_NO_HIT (sky, distance 1e5) */
+    float t;
+    float u;
+    float v;
+} cv_hit;
+
+/* MIntersection (ray.rs:35-50) plus the debug counters of
+ * Bvh::intersect_debug (bvh.rs:604-669). */
+typedef struct cv_isect {
+    float position[3];
+    float normal[3];
+    float distance;
+    uint32_t material;
+    float tex_coords[2];
+    uint32_t tri_id;
+    float u;
+    float v;
+    uint32_t numi_aabb;
+    uint32_t numi_tri;
+    uint32_t reserved;
+} cv_isect;
+
+#define CV_PTR_DEVICE 1u /* buffers are device pointers */
+
+/* Scene::intersect_nearest / Bvh::intersect_nearest / intersect_debug
+ * (scene.rs:220-244, bvh.rs:589-669), batched, one ray per GPU lane.  Each
+ * ray is traversed as the reference traverses a packet holding that ray in
+ * every lane.  hits and isects may each be NULL. */
+cv_status cv_bvh_intersect_nearest(const cv_scene* scene, const cv_ray* rays, size_t n,
+                                   cv_hit* hits, cv_isect* isects, uint32_t flags);
+
+/* --------------------------------------------------------------- renderer */
+
+typedef struct cv_renderer cv_renderer;
+
+#define CV_RNG_PIXEL 0u     /* xorshift64 per pixel, seed splitmix64(pixel ^ frame << 32) */
+#define CV_RNG_REF_PATCH 1u /* the reference Rng per 32x32 patch (random.rs, renderer.rs:316) */
+#define CV_APPROX_PORTABLE 0u /* recip_fast = 1/x, rsqrt = 1/sqrt(x), correctly rounded */
+
+typedef struct cv_render_opts {
+    uint32_t rng_mode;    /* CV_RNG_PIXEL */
+    uint32_t approx_mode; /* CV_APPROX_PORTABLE */
+    uint32_t max_bounces; /* 5 (renderer.rs:422); 0 selects 5 */
+} cv_render_opts;
+
+typedef struct cv_render_stats {
+    uint64_t paths;            /* camera paths traced by the last call */
+    uint64_t extend_rays;      /* intersect_nearest queries of the last call */
+    uint64_t extend_launches;  /* traversal kernel launches of the last call */
+    double extend_ms;          /* summed device time of those launches */
+    double shade_ms;           /* summed device time of the shading launches */
+    double generate_ms;        /* summed device time of the camera-ray launches */
+    double total_ms;           /* device time from first to last launch */
+    uint64_t sum_numi_aabb;    /* with CV_RENDER_COUNT: summed box tests */
+    uint64_t sum_numi_tri;     /* with CV_RENDER_COUNT: summed triangle tests */
+} cv_render_stats;
+
+#define CV_RENDER_COUNT 2u /* count box/triangle tests (slower) */
+#define CV_RENDER_TIMING 4u /* record HIP events around every launch */
+
+/* Renderer::new (renderer.rs:121-130).  The renderer keeps a reference to
+ * the scene (which must outlive it).  width and height must be multiples
+ * of 16 (renderer.rs:45-47).  Fails with CV_ERR_LIGHT_COUNT when the scene
+ * has fewer than 8 direct-sample triangles.  opts may be NULL. */
+cv_status cv_renderer_create(cv_scene* scene, uint32_t width, uint32_t height,
+                             const cv_render_opts* opts, cv_renderer** out);
+/* Renderer::render_patch_u8 (renderer.rs:306-327): renders the
+ * patch_width x patch_width square at (x, y) (bottom-left) of frame
+ * `frame` into the RGBA bitmap and gbuffer (width*height u32 each, row y
+ * at offset y*width, y = 0 the bottom row). */
+cv_status cv_renderer_render_patch_u8(cv_renderer* r, uint32_t* bitmap, uint32_t* gbuffer,
+                                      uint32_t patch_width, uint32_t x, uint32_t y,
+                                      uint32_t frame, uint32_t flags);
+/* Renderer::accumulate_patch_f32 (renderer.rs:339-362): adds the patch's
+ * colours to the HDR buffer in the reference's 16x4 block layout
+ * ((height/4)*(width/16) blocks of [8 groups][3 channels][8 lanes] f32). */
+cv_status cv_renderer_accumulate_patch_f32(cv_renderer* r, float* hdr, uint32_t* gbuffer,
+                                           uint32_t patch_width, uint32_t x, uint32_t y,
+                                           uint32_t frame, uint32_t flags);
+/* The whole frame at once: every patch of main.rs:174-203 in one launch
+ * sequence.  gbuffer may be NULL. */
+cv_status cv_renderer_render_frame_u8(cv_renderer* r, uint32_t* bitmap, uint32_t* gbuffer,
+                                      uint32_t frame, uint32_t flags);
+/* Accumulates frames first_frame .. first_frame+n_frames-1 into hdr
+ * (block layout) in frame order; gbuffer (may be NULL) receives the last
+ * frame's gbuffer. */
+cv_status cv_renderer_accumulate_frames_f32(cv_renderer* r, float* hdr, uint32_t* gbuffer,
+                                            uint32_t first_frame, uint32_t n_frames,
+                                            uint32_t flags);
+/* Renderer::buffer_f32_into_render_buffer (renderer.rs:378-410). */
+cv_status cv_renderer_buffer_f32_into_bitmap(cv_renderer* r, const float* hdr, uint32_t* bitmap,
+                                             uint32_t num_samples, uint32_t flags);
+/* Counters and device times of the last render call. */
+cv_status cv_renderer_get_stats(const cv_renderer* r, cv_render_stats* stats);
+void cv_renderer_destroy(cv_renderer* r);
+
+/* ----------------------------------------------------------------- device */
+
+/* Waits for all work of this library on the current device. */
+cv_status cv_device_synchronize(void);
+/* Text of the last error raised on the calling thread ("" if none). */
+const char* cv_last_error(void);
+const char* cv_status_string(cv_status s);
+/* Library version string. */
+const char* cv_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CONVECTOR_H */
