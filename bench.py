@@ -1,0 +1,237 @@
+#!/usr/bin/env python3
+"""Benchmark of the per-ray path on one MI355X.
+
+Workload (the large incoherent configuration): the closed box with its
+2x2-quad light, an icosphere subdivided 8 times (1,310,720 triangles)
+displaced by 4-octave value noise of amplitude 0.08, and 20,000 free
+triangles -- 1,330,746 triangles -- rendered at 1920x1080 with 16 samples
+per pixel accumulated, 5 bounces, one-sample MIS direct lighting, FP32.
+
+One step = one full 16-spp accumulation (frames 1..16, 33,177,600 camera
+paths) into a device-resident HDR buffer, with the scene and BVH already in
+HBM.  The metric is extend queries (nearest-hit traversals) per second.
+
+JSON line fields beyond the contract:
+  roofline     the traversal (extend) kernel: algorithmic bytes per launch
+               (32 B per box test + 36 B per triangle test, the oracle's
+               counters, identical on the GPU) over its HIP-event time
+  valu         the same kernel against the FP32 VALU issue peak
+               (23 ops per box test + 48 per triangle test)
+  cpu_baseline the CPU oracle (reference semantics: per-patch Rng,
+               host rcpps/rsqrtps) on a bounded strided sample of patches
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+VALU_PEAK_TOPS = 39.3  # 256 CU x 4 SIMD x 16 lanes x 2.4 GHz, non-packed FP32 lane-ops
+W, H, SPP = 1920, 1080, 16
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+class DeviceBuffer:
+    """A hipMalloc'd buffer (device memory without torch)."""
+
+    def __init__(self, nbytes):
+        self.hip = ctypes.CDLL("libamdhip64.so")
+        self.ptr = ctypes.c_void_p()
+        rc = self.hip.hipMalloc(ctypes.byref(self.ptr), ctypes.c_size_t(nbytes))
+        if rc != 0:
+            raise RuntimeError("hipMalloc failed: %d" % rc)
+        self.nbytes = nbytes
+
+    def zero(self):
+        rc = self.hip.hipMemset(self.ptr, 0, ctypes.c_size_t(self.nbytes))
+        if rc != 0:
+            raise RuntimeError("hipMemset failed: %d" % rc)
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            self.hip.hipFree(self.ptr)
+
+
+def cpu_baseline(sd, gpu_scene, args):
+    """Times the oracle (reference semantics) on a strided sample of the
+    frame's 32x32 patches with the host cores."""
+    from oracle import oracle as O
+
+    cores = min(args.cpu_threads, os.cpu_count() or 1)
+    t0 = time.perf_counter()
+    osc = O.Scene.from_scene_data(sd, approx=O.APPROX_HOST_X86)
+    build_s = time.perf_counter() - t0
+    log("cpu oracle BVH build %.2f s" % build_s)
+    # identical tree check on the side (the GPU build restates this one)
+    gn, go = gpu_scene.bvh()
+    on, oo = osc.bvh()
+    same = len(gn) == len(on) and gn.tobytes() == on.tobytes() and bool((go == oo).all())
+    hp = 1088  # the reference's renderer needs heights divisible by 16
+    npy = (hp + 31) // 32
+    npatch = (W // 32) * npy
+    stride = 37  # coprime with the patch count, spreads the sample over the frame
+    order = [(k * stride) % npatch for k in range(npatch)]
+    done, q, frame = 0, 0, 1
+    t_used = 0.0
+    chunk = max(4 * cores, 64)
+    # whole frames while the time budget lasts (frame f = the f-th sample per
+    # pixel), then stop at a chunk boundary
+    while t_used < args.cpu_seconds and frame <= SPP:
+        k = done % npatch
+        ids = order[k:k + chunk]
+        t1 = time.perf_counter()
+        qq, _, _ = osc.sample_patches(W, hp, frame, ids, rng_mode=O.RNG_REF_PATCH, nthreads=cores)
+        t_used += time.perf_counter() - t1
+        q += qq
+        done += len(ids)
+        if done % npatch == 0:
+            frame += 1
+    paths = done * 32 * 32
+    return {
+        "value": q / t_used / 1e6,
+        "unit": "Mrays/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": "%d patch renders (32x32, %d per frame, stride-%d order) of frames 1..%d at "
+                  "1920x1088, reference Rng per patch, host rcpps/rsqrtps, %d threads; %d paths, "
+                  "%d extend queries in %.1f s" % (done, npatch, stride, frame, cores, paths, q,
+                                                   t_used),
+        "paths_per_s": paths / t_used,
+        "bvh_build_s": build_s,
+        "bvh_identical_to_gpu": same,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--subdivisions", type=int, default=8)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every box of this pool has a single MI355X and the "
+              "path is built for one GPU (see the sharding notes); --gpus %d is not run."
+              % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import convector_amd as cv
+    from convector_amd import scenes
+
+    t0 = time.perf_counter()
+    sd = scenes.config1(subdivisions=args.subdivisions)
+    log("scene %s: %d triangles generated in %.1f s" % (sd.name, sd.ntris, time.perf_counter() - t0))
+    t0 = time.perf_counter()
+    scene = cv.Scene.from_scene_data(sd)
+    info = scene.bvh_info()
+    log("GPU BVH: %d nodes, depth %d, device build %.1f ms (call %.2f s)"
+        % (info["num_nodes"], info["depth"], info["build_ms"], time.perf_counter() - t0))
+    r = cv.Renderer(scene, W, H)
+    hdr = DeviceBuffer(W * H * 3 * 4)
+    hdr.zero()
+
+    def step(flags):
+        r.accumulate_frames_device(hdr.ptr.value, 1, SPP, flags)
+        return r.stats()
+
+    # counting pass: box/triangle tests of exactly the work one step does
+    cnt = step(cv.RENDER_COUNT)
+    sum_b, sum_t, rays_per_step = cnt["sum_numi_aabb"], cnt["sum_numi_tri"], cnt["extend_rays"]
+    log("per step: %d extend queries, %.2f box / %.2f triangle tests per query"
+        % (rays_per_step, sum_b / rays_per_step, sum_t / rays_per_step))
+    for _ in range(args.warmup):
+        step(cv.RENDER_TIMING)
+    cv.device_synchronize()
+    ext_ms = ext_launch = rays = 0
+    tot_gpu_ms = 0.0
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        s = step(cv.RENDER_TIMING)
+        ext_ms += s["extend_ms"]
+        ext_launch += s["extend_launches"]
+        rays += s["extend_rays"]
+        tot_gpu_ms += s["total_ms"]
+        log("step %d: %.1f ms device, %d rays" % (k, s["total_ms"], s["extend_rays"]))
+    cv.device_synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = wall * 1e3 / args.steps
+    value = rays / wall / 1e6
+    paths = W * H * SPP * args.steps
+    # roofline of the traversal kernel
+    bytes_per_step = 32.0 * sum_b + 36.0 * sum_t
+    ops_per_step = 23.0 * sum_b + 48.0 * sum_t
+    launches_per_step = ext_launch / args.steps
+    avg_launch_ms = ext_ms / ext_launch
+    achieved_gbs = (bytes_per_step / launches_per_step) / (avg_launch_ms * 1e-3) / 1e9
+    achieved_tops = (ops_per_step / launches_per_step) / (avg_launch_ms * 1e-3) / 1e12
+    out = {
+        "metric": "Mrays/s vs host AVX",
+        "value": round(value, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "box + light + icosphere-%d (4-octave value noise 0.08) + 20,000 free "
+                        "triangles = %d triangles, %dx%d, %d spp accumulated, 5 bounces, "
+                        "one-sample MIS, PIXEL rng, portable rcp/rsqrt"
+                        % (args.subdivisions, sd.ntris, W, H, SPP),
+            "paths_per_step": W * H * SPP,
+            "extend_queries_per_step": rays_per_step,
+        },
+        "paths_per_s": round(paths / wall, 1),
+        "mpaths_per_s": round(paths / wall / 1e6, 2),
+        "gpu_device_ms_per_step": round(tot_gpu_ms / args.steps, 3),
+        "bvh_build_ms": round(info["build_ms"], 2),
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_extend (BVH traversal + triangle intersection)",
+            "achieved": round(achieved_gbs, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved_gbs / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "bytes_per_launch": round(bytes_per_step / launches_per_step),
+            "avg_launch_ms": round(avg_launch_ms, 4),
+        },
+        "valu": {
+            "achieved": round(achieved_tops, 3),
+            "peak": VALU_PEAK_TOPS,
+            "unit": "T lane-ops/s",
+            "frac": round(achieved_tops / VALU_PEAK_TOPS, 4),
+        },
+    }
+    if not args.no_cpu:
+        try:
+            cb = cpu_baseline(sd, scene, args)
+            out["cpu_baseline"] = cb
+            out["gpu_over_cpu"] = round(value / cb["value"], 1)
+        except Exception as e:  # the GPU figure stands without it
+            log("cpu baseline failed: %r" % (e,))
+            out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

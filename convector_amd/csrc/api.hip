@@ -683,8 +683,11 @@ cv_status cv_renderer_create(cv_scene* s, uint32_t width, uint32_t height,
                              const cv_render_opts* opts, cv_renderer** out) {
   if (!s || !out) return fail(CV_ERR_INVALID_ARGUMENT, "null argument");
   *out = nullptr;
-  if ((width & 15) || (height & 15) || width == 0 || height == 0)
-    return fail(CV_ERR_NOT_MULTIPLE_OF_16, "width and height must be multiples of 16");
+  // The 16x4-block HDR layout needs width % 16 == 0 and height % 4 == 0
+  // (renderer.rs:366-374, 384-385); the reference's RenderBuffer also
+  // rounds heights to 16, which the per-pixel GPU path does not need.
+  if ((width & 15) || (height & 3) || width == 0 || height == 0)
+    return fail(CV_ERR_NOT_MULTIPLE_OF_16, "width must be a multiple of 16 and height of 4");
   if (s->ndirect < 8)
     return fail(CV_ERR_LIGHT_COUNT, "the scene has " + std::to_string(s->ndirect) +
                                         " direct-sample triangles; direct sampling needs 8");

@@ -200,8 +200,10 @@ typedef struct cv_render_stats {
 #define CV_RENDER_TIMING 4u /* record HIP events around every launch */
 
 /* Renderer::new (renderer.rs:121-130).  The renderer keeps a reference to
- * the scene (which must outlive it).  width and height must be multiples
- * of 16 (renderer.rs:45-47).  Fails with CV_ERR_LIGHT_COUNT when the scene
+ * the scene (which must outlive it).  width must be a multiple of 16 and
+ * height of 4: the 16x4-block HDR layout (renderer.rs:366-385).  The
+ * reference's RenderBuffer further asks for heights divisible by 16
+ * (renderer.rs:45-47); patch calls keep that rule.  Fails with CV_ERR_LIGHT_COUNT when the scene
  * has fewer than 8 direct-sample triangles.  opts may be NULL. */
 cv_status cv_renderer_create(cv_scene* scene, uint32_t width, uint32_t height,
                              const cv_render_opts* opts, cv_renderer** out);

@@ -1909,3 +1909,77 @@ int or_frame_work(const or_scene* s, uint32_t width, uint32_t height, uint32_t f
     *sum_tri = j.wc.sum_tri;
     return st;
 }
+
+/* A bounded CPU-baseline sample: renders the listed 32x32 patches (indices
+ * into the column-major patch order of main.rs:174-203) of one frame with
+ * nthreads workers and returns the work done. */
+typedef struct {
+    const or_scene* s;
+    uint32_t width, height, frame;
+    int rng_mode;
+    const uint32_t* ids;
+    uint32_t nids;
+    float* hdr;
+    volatile uint32_t next;
+    pthread_mutex_t lock;
+    work_counts wc;
+    int status;
+} sample_job;
+
+static void* sample_worker(void* arg) {
+    sample_job* j = (sample_job*)arg;
+    work_counts local = {0, 0, 0};
+    uint32_t npy = (j->height + 31) / 32;
+    for (;;) {
+        pthread_mutex_lock(&j->lock);
+        uint32_t k = j->next++;
+        pthread_mutex_unlock(&j->lock);
+        if (k >= j->nids) break;
+        uint32_t p = j->ids[k];
+        uint32_t x = (p / npy) * 32, y = (p % npy) * 32;
+        uint32_t pw = (x + 32 <= j->width && y + 32 <= j->height) ? 32 : 16;
+        uint32_t xe = x + 32 <= j->width ? x + 32 : j->width;
+        uint32_t ye = y + 32 <= j->height ? y + 32 : j->height;
+        for (uint32_t yy = y; yy < ye; yy += pw)
+            for (uint32_t xx = x; xx < xe; xx += pw) {
+                int st = patch_impl(j->s, j->width, j->height, NULL, j->hdr, NULL, pw, xx, yy,
+                                    j->frame, j->rng_mode, &local);
+                if (st) j->status = st;
+            }
+    }
+    pthread_mutex_lock(&j->lock);
+    j->wc.queries += local.queries;
+    j->wc.sum_aabb += local.sum_aabb;
+    j->wc.sum_tri += local.sum_tri;
+    pthread_mutex_unlock(&j->lock);
+    return NULL;
+}
+
+int or_sample_patches(const or_scene* s, uint32_t width, uint32_t height, uint32_t frame,
+                      int rng_mode, int nthreads, const uint32_t* patch_ids, uint32_t n_ids,
+                      uint64_t* out3) {
+    if ((width & 15) || (height & 15)) return -10;
+    if (s->ndirect < 8) return -12;
+    sample_job j;
+    memset(&j, 0, sizeof(j));
+    j.s = s;
+    j.width = width;
+    j.height = height;
+    j.frame = frame;
+    j.rng_mode = rng_mode;
+    j.ids = patch_ids;
+    j.nids = n_ids;
+    j.hdr = (float*)calloc((size_t)width * height * 3, sizeof(float));
+    pthread_mutex_init(&j.lock, NULL);
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 256) nthreads = 256;
+    pthread_t th[256];
+    for (int t = 0; t < nthreads; t++) pthread_create(&th[t], NULL, sample_worker, &j);
+    for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+    pthread_mutex_destroy(&j.lock);
+    free(j.hdr);
+    out3[0] = j.wc.queries;
+    out3[1] = j.wc.sum_aabb;
+    out3[2] = j.wc.sum_tri;
+    return j.status;
+}

@@ -159,6 +159,12 @@ int or_render_frame_u8(const or_scene* s, uint32_t width, uint32_t height, uint3
 int or_frame_work(const or_scene* s, uint32_t width, uint32_t height, uint32_t frame, int rng_mode,
                   int nthreads, uint64_t* n_queries, uint64_t* sum_aabb, uint64_t* sum_tri);
 
+/* CPU-baseline sample: renders the listed patches (column-major patch index)
+ * of one frame; out3 = {extend queries, box tests, triangle tests}. */
+int or_sample_patches(const or_scene* s, uint32_t width, uint32_t height, uint32_t frame,
+                      int rng_mode, int nthreads, const uint32_t* patch_ids, uint32_t n_ids,
+                      uint64_t* out3);
+
 /* Converts between the 16x4-block HDR layout and row-major RGB float. */
 void or_hdr_blocks_to_rows(uint32_t width, uint32_t height, const float* hdr, float* rows);
 

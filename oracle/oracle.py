@@ -95,6 +95,7 @@ def _declare(L):
         "or_render_frame_u8": (I32, [P, U32, U32, P, P, U32, I32, I32]),
         "or_frame_work": (I32, [P, U32, U32, U32, I32, I32, P, P, P]),
         "or_hdr_blocks_to_rows": (None, [U32, U32, P, P]),
+        "or_sample_patches": (I32, [P, U32, U32, U32, I32, I32, P, U32, P]),
         "or_avx_available": (I32, []),
         "or_avx_accumulate_frames": (I32, [P, U32, U32, P, U32, U32, U32, U32, I32]),
     }
@@ -263,6 +264,16 @@ class Scene:
         if st:
             raise RuntimeError("oracle frame_work status %d" % st)
         return int(q[0]), int(a[0]), int(t[0])
+
+    def sample_patches(self, width, height, frame, patch_ids, rng_mode=RNG_REF_PATCH,
+                       nthreads=8):
+        ids = u32(patch_ids)
+        out = np.zeros(3, dtype=np.uint64)
+        st = lib().or_sample_patches(self._h, width, height, frame, rng_mode, nthreads, ptr(ids),
+                                     len(ids), ptr(out))
+        if st:
+            raise RuntimeError("oracle sample_patches status %d" % st)
+        return int(out[0]), int(out[1]), int(out[2])
 
     def avx_accumulate_frames(self, width, height, first_frame, n_frames, patch_begin=0,
                               patch_end=None, nthreads=8, hdr=None):
