@@ -1,0 +1,152 @@
+"""GPU nearest-hit queries (k_intersect) against the oracle's single-ray
+traversal: hit ids, t/u/v, every MIntersection field and the debug
+counters bit-identical; ids against FP64 brute force within the tolerance
+the reference path allows; special rays (axis-aligned, origins on box
+planes) that exercise x86 NaN and signed-zero rules."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from convector_amd import scenes
+from helpers import bits, indoor_scene, model_scene
+
+pytestmark = pytest.mark.gpu
+
+
+def _check_exact(hits, isects, ref):
+    assert (hits["tri_id"] == ref["tri"]).all()
+    assert (bits(hits["t"]) == bits(ref["tuv"][:, 0])).all()
+    assert (bits(hits["u"]) == bits(ref["tuv"][:, 1])).all()
+    assert (bits(hits["v"]) == bits(ref["tuv"][:, 2])).all()
+    if isects is not None:
+        o = ref["isect"]
+        assert (bits(isects["position"]) == bits(o[:, 0:3])).all()
+        assert (bits(isects["normal"]) == bits(o[:, 3:6])).all()
+        assert (bits(isects["distance"]) == bits(o[:, 6])).all()
+        assert (isects["material"] == bits(o[:, 7])).all()
+        assert (bits(isects["tex_coords"]) == bits(o[:, 8:10])).all()
+        assert (isects["numi_aabb"] == ref["numi_aabb"]).all()
+        assert (isects["numi_tri"] == ref["numi_tri"]).all()
+
+
+def _ray_families(center, extent, seed):
+    c = np.asarray(center, np.float64)
+    fams = [
+        scenes.rays_inward(4096, seed, radius=3 * extent, target_scale=0.5 * extent, center=c),
+        scenes.rays_inward(4096, seed + 1, radius=3 * extent, coherent=True,
+                           target_scale=0.5 * extent, center=c),
+        scenes.rays_inward(2048, seed + 2, radius=0.3 * extent, target_scale=0.9 * extent,
+                           center=c),
+    ]
+    # axis-aligned rays: zero direction components give infinite reciprocals
+    # and 0 * inf in the slab test (x86 default NaN rules)
+    rng = np.random.default_rng(seed)
+    o = c + rng.uniform(-1.5, 1.5, (3000, 3)) * extent
+    d = np.zeros((3000, 3))
+    axis = rng.integers(0, 3, 3000)
+    d[np.arange(3000), axis] = rng.choice([-1.0, 1.0], 3000)
+    fams.append(np.concatenate([o, d], 1).astype(np.float32))
+    return np.concatenate(fams).astype(np.float32)
+
+
+@pytest.mark.parametrize("name", ["suzanne", "stanford_bunny", "indoor"])
+def test_models_exact(cv, oracle, models_dir, name):
+    sd = model_scene(models_dir, name) if name != "indoor" else indoor_scene(models_dir)
+    print("traversal case", name, sd.ntris, flush=True)
+    osc = oracle.Scene.from_scene_data(sd)
+    gsc = cv.Scene.from_scene_data(sd)
+    lo, hi = sd.tris.reshape(-1, 3).min(0), sd.tris.reshape(-1, 3).max(0)
+    rays = _ray_families((lo + hi) / 2, float((hi - lo).max()), 3)
+    hits, isects = gsc.intersect_nearest(rays, full=True)
+    ref = osc.intersect(rays, isect=True)
+    _check_exact(hits, isects, ref)
+    assert (hits["tri_id"] != cv.NO_HIT).mean() > 0.2
+
+
+def test_axis_rays_on_box_planes(cv, oracle):
+    """Rays starting exactly on node-box planes with zero direction
+    components: -0.0 / +0.0 and NaN choices of minps/maxps decide the pushes."""
+    sd = scenes.config0(subdivisions=3)
+    osc = oracle.Scene.from_scene_data(sd)
+    gsc = cv.Scene.from_scene_data(sd)
+    nodes, _ = osc.bvh()
+    rng = np.random.default_rng(12)
+    pick = rng.integers(0, len(nodes), 4000)
+    o = np.where(rng.random((4000, 3)) < 0.5, nodes["min"][pick], nodes["max"][pick])
+    d = np.zeros((4000, 3), np.float32)
+    ax = rng.integers(0, 3, 4000)
+    d[np.arange(4000), ax] = rng.choice([-1.0, 1.0], 4000)
+    rays = np.concatenate([o, d], 1).astype(np.float32)
+    hits, isects = gsc.intersect_nearest(rays, full=True)
+    _check_exact(hits, isects, osc.intersect(rays, isect=True))
+
+
+def test_config0_camera_and_bounce_rays(cv, oracle):
+    sd = scenes.config0()
+    osc = oracle.Scene.from_scene_data(sd)
+    gsc = cv.Scene.from_scene_data(sd)
+    rays = np.concatenate([scenes.rays_inward(20000, 21, radius=0.9, target_scale=0.45),
+                           scenes.rays_inward(20000, 22, radius=0.95, target_scale=0.95)])
+    hits, isects = gsc.intersect_nearest(rays, full=True)
+    ref = osc.intersect(rays, isect=True)
+    _check_exact(hits, isects, ref)
+    # ids against FP64 brute force: >= 99.99% equal, the rest only at ties
+    sub = rays[:4000]
+    b = osc.intersect_brute_f64(sub)
+    mism = np.nonzero(hits["tri_id"][:4000] != b["tri"])[0]
+    assert len(mism) <= 1
+    both = hits["tri_id"][:4000] == b["tri"]
+    t64 = b["tuv"][both, 0]
+    assert (np.abs(hits["t"][:4000][both] - t64) <= 1e-5 * np.abs(t64)).all()
+
+
+def test_config1_sample_exact(cv, oracle):
+    sd = scenes.config1(subdivisions=7, n_free=5000)
+    osc = oracle.Scene.from_scene_data(sd)
+    gsc = cv.Scene.from_scene_data(sd)
+    rays = np.concatenate([scenes.rays_inward(30000, 31, radius=0.95, target_scale=0.45),
+                           scenes.rays_inward(30000, 32, radius=0.9, target_scale=0.95)])
+    hits, isects = gsc.intersect_nearest(rays, full=True)
+    _check_exact(hits, isects, osc.intersect(rays, isect=True))
+
+
+def test_imported_bvh(cv, oracle, models_dir):
+    """cv_scene_create_with_bvh: traversal of a caller-provided tree."""
+    sd = model_scene(models_dir, "suzanne")
+    osc = oracle.Scene.from_scene_data(sd)
+    nodes, order = osc.bvh()
+    gsc = cv.Scene.from_scene_data(sd, with_bvh=(nodes, order))
+    rays = scenes.rays_inward(5000, 41, radius=5, target_scale=1.0)
+    hits, isects = gsc.intersect_nearest(rays, full=True)
+    _check_exact(hits, isects, osc.intersect(rays, isect=True))
+
+
+def test_sizes_and_device_pointers(cv, oracle, models_dir):
+    sd = model_scene(models_dir, "suzanne")
+    osc = oracle.Scene.from_scene_data(sd)
+    gsc = cv.Scene.from_scene_data(sd)
+    for n in (0, 1, 63, 65, 1000):
+        rays = scenes.rays_inward(max(n, 1), 50 + n, radius=5, target_scale=1.0)[:n]
+        hits = gsc.intersect_nearest(rays)
+        if n:
+            ref = osc.intersect(rays)
+            assert (hits["tri_id"] == ref["tri"]).all()
+    # CV_PTR_DEVICE with hipMalloc'd buffers
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = 4096
+    rays = scenes.rays_inward(n, 60, radius=5, target_scale=1.0)
+    dr, dh = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(dr), ctypes.c_size_t(rays.nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(dh), ctypes.c_size_t(16 * n)) == 0
+    try:
+        assert hip.hipMemcpy(dr, rays.ctypes.data_as(ctypes.c_void_p),
+                             ctypes.c_size_t(rays.nbytes), 1) == 0
+        gsc.intersect_nearest_device(dr.value, n, dh.value)
+        out = np.zeros(n, cv.HIT_DTYPE)
+        assert hip.hipMemcpy(out.ctypes.data_as(ctypes.c_void_p), dh, ctypes.c_size_t(16 * n),
+                             2) == 0
+        assert (out["tri_id"] == osc.intersect(rays)["tri"]).all()
+    finally:
+        hip.hipFree(dr)
+        hip.hipFree(dh)
