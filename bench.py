@@ -123,6 +123,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--subdivisions", type=int, default=8)
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="device-to-device copy of this many MiB before the steps, so that "
+                         "rocprofv3 --pmc counters can be scaled by a copy of known size")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box of this pool has a single MI355X and the "
@@ -132,6 +135,15 @@ def main():
 
     import convector_amd as cv
     from convector_amd import scenes
+
+    if args.copy_calibration_mb:
+        nb = args.copy_calibration_mb << 20
+        src, dst = DeviceBuffer(nb), DeviceBuffer(nb)
+        src.zero()
+        rc = src.hip.hipMemcpy(dst.ptr, src.ptr, ctypes.c_size_t(nb), 3)  # device to device
+        src.hip.hipDeviceSynchronize()
+        log("copy calibration: %d MiB device-to-device (rc %d)" % (args.copy_calibration_mb, rc))
+        del src, dst
 
     t0 = time.perf_counter()
     sd = scenes.config1(subdivisions=args.subdivisions)

@@ -182,6 +182,17 @@ class Scene:
         s.camera.set_screen_distance(sd.screen_distance)
         return s
 
+    def set_tuning(self, stack_entries=0, extend_waves_per_cu=0, refill_threshold=0,
+                   fetch_batches=0):
+        """Launch shape of the traversal (the autotuner's knobs)."""
+        t = _native.Tuning(stack_entries, extend_waves_per_cu, refill_threshold, fetch_batches)
+        check(lib().cv_scene_set_tuning(self._h, ctypes.byref(t)))
+
+    def tuning(self) -> dict:
+        t = _native.Tuning()
+        check(lib().cv_scene_get_tuning(self._h, ctypes.byref(t)))
+        return {k: getattr(t, k) for k, _ in t._fields_}
+
     def bvh_info(self) -> dict:
         info = _native.BvhInfo()
         check(lib().cv_scene_get_bvh_info(self._h, ctypes.byref(info)))

@@ -33,7 +33,7 @@ CV_NO_HIT = 0xFFFFFFFF
 EXPORTS = [
     "cv_mesh_load_obj", "cv_mesh_from_triangles", "cv_mesh_num_triangles",
     "cv_mesh_copy_triangles", "cv_mesh_destroy", "cv_scene_create", "cv_scene_create_with_bvh",
-    "cv_scene_set_camera", "cv_scene_get_bvh_info", "cv_scene_copy_bvh", "cv_scene_destroy",
+    "cv_scene_set_camera", "cv_scene_set_tuning", "cv_scene_get_tuning", "cv_scene_get_bvh_info", "cv_scene_copy_bvh", "cv_scene_destroy",
     "cv_bvh_intersect_nearest", "cv_renderer_create", "cv_renderer_render_patch_u8",
     "cv_renderer_accumulate_patch_f32", "cv_renderer_render_frame_u8",
     "cv_renderer_accumulate_frames_f32", "cv_renderer_buffer_f32_into_bitmap",
@@ -63,6 +63,11 @@ class BvhInfo(ctypes.Structure):
                 ("depth", ctypes.c_uint32), ("max_leaf", ctypes.c_uint32),
                 ("num_direct", ctypes.c_uint32), ("avg_area_ratio", ctypes.c_float),
                 ("avg_tris_per_leaf", ctypes.c_float), ("build_ms", ctypes.c_float)]
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [("stack_entries", ctypes.c_uint32), ("extend_waves_per_cu", ctypes.c_uint32),
+                ("refill_threshold", ctypes.c_uint32), ("fetch_batches", ctypes.c_uint32)]
 
 
 class RenderOpts(ctypes.Structure):
@@ -110,6 +115,8 @@ def _declare(L):
         "cv_scene_create": (I, [P, SZ, P, PP]),
         "cv_scene_create_with_bvh": (I, [P, SZ, P, P, SZ, P, PP]),
         "cv_scene_set_camera": (I, [P, P, P, P, P, F]),
+        "cv_scene_set_tuning": (I, [P, P]),
+        "cv_scene_get_tuning": (I, [P, P]),
         "cv_scene_get_bvh_info": (I, [P, P]),
         "cv_scene_copy_bvh": (I, [P, P, P]),
         "cv_scene_destroy": (None, [P]),

@@ -149,6 +149,8 @@ struct cv_scene {
   float emission[3] = {0, 0, 0};
   DevBuf<uint2> spill;
   uint32_t persistent_blocks = 0;
+  uint32_t refill = 16;
+  uint32_t fetch_batches = 1;
   DevBuf<uint32_t> cursor;
   std::mutex mu;
 };
@@ -430,6 +432,26 @@ void cv_mesh_destroy(cv_mesh* mesh) { delete mesh; }
 
 // ------------------------------------------------------------------ scenes
 
+// Stack depth and persistent grid sizes; the spill area holds the entries
+// beyond the LDS part of every lane's stack (depth + 2 entries suffice: a pop
+// pushes at most two children).
+static cv_status scene_apply_tuning(cv_scene* s, const cv_tuning& t) {
+  if (t.stack_entries && t.stack_entries != 8 && t.stack_entries != 16 && t.stack_entries != 24 &&
+      t.stack_entries != 32)
+    return fail(CV_ERR_INVALID_ARGUMENT, "stack_entries must be 8, 16, 24 or 32");
+  if (t.extend_waves_per_cu > 64 || t.refill_threshold > 65 || t.fetch_batches > 64)
+    return fail(CV_ERR_INVALID_ARGUMENT, "tuning value out of range");
+  if (t.stack_entries) s->stack = (int)t.stack_entries;
+  if (t.extend_waves_per_cu) s->persistent_blocks = (uint32_t)g_num_cus * t.extend_waves_per_cu;
+  if (t.refill_threshold) s->refill = t.refill_threshold;
+  if (t.fetch_batches) s->fetch_batches = t.fetch_batches;
+  uint32_t need = s->info.depth + 4;
+  size_t spill_entries = need > (uint32_t)s->stack ? (size_t)(need - s->stack) : 0;
+  HIPTRY(hipStreamSynchronize(s->stream), "sync");
+  HIPTRY(s->spill.alloc(spill_entries * s->persistent_blocks * 64 + 1), "alloc spill");
+  return CV_OK;
+}
+
 static cv_status scene_finish(cv_scene* s, const std::vector<float>& v9,
                               const std::vector<float>& uv6, const std::vector<uint32_t>& mat,
                               const cv_bvh_node* nodes_in, size_t nnodes_in,
@@ -519,13 +541,9 @@ static cv_status scene_finish(cv_scene* s, const std::vector<float>& v9,
   // traversal configuration: packed stack entries need index < 2^27 and
   // leaves of at most 31 triangles
   s->packed = s->info.max_leaf <= 31 && N < (1u << 27) && s->info.num_nodes < (1u << 27);
-  s->stack = 16;
-  s->persistent_blocks = (uint32_t)g_num_cus * 16;
-  uint32_t need = s->info.depth + 4;
-  size_t spill_entries = need > (uint32_t)s->stack ? (size_t)(need - s->stack) : 0;
-  HIPTRY(s->spill.alloc(spill_entries * s->persistent_blocks * 64 + 1), "alloc spill");
   HIPTRY(s->cursor.alloc(4), "alloc");
-  return CV_OK;
+  cv_tuning t{8, 32, 32, 1};
+  return scene_apply_tuning(s, t);
 }
 
 static cv_status scene_create_impl(const cv_mesh* const* meshes, size_t n,
@@ -591,6 +609,21 @@ cv_status cv_scene_set_camera(cv_scene* s, const float pos[3], const float dpos[
   memcpy(s->cam.q, q, 16);
   memcpy(s->cam.dq, dq, 16);
   s->cam.screen_distance = screen_distance;
+  return CV_OK;
+}
+
+cv_status cv_scene_set_tuning(cv_scene* s, const cv_tuning* t) {
+  if (!s || !t) return fail(CV_ERR_INVALID_ARGUMENT, "null argument");
+  std::lock_guard<std::mutex> g(s->mu);
+  return scene_apply_tuning(s, *t);
+}
+
+cv_status cv_scene_get_tuning(const cv_scene* s, cv_tuning* t) {
+  if (!s || !t) return fail(CV_ERR_INVALID_ARGUMENT, "null argument");
+  t->stack_entries = (uint32_t)s->stack;
+  t->extend_waves_per_cu = s->persistent_blocks / (uint32_t)g_num_cus;
+  t->refill_threshold = s->refill;
+  t->fetch_batches = s->fetch_batches;
   return CV_OK;
 }
 
@@ -803,6 +836,8 @@ cv_status render_rect(cv_renderer* r, const Rect& rc, uint32_t frame, float* hdr
     x.nodes = s->nodes.p;
     x.tris = s->tris.p;
     x.spill = s->spill.p;
+    x.batches = s->fetch_batches;
+    x.refill = s->refill <= 64 ? s->refill : 0u;
     ep = ev_pair(1);
     if (ep) (void)hipEventRecord(ep->first, st);
     HIPTRY(cvk::launch_extend(x, count, s->packed, s->stack, s->persistent_blocks, st), "extend");
@@ -827,7 +862,7 @@ cv_status render_rect(cv_renderer* r, const Rect& rc, uint32_t frame, float* hdr
     sh.gbuffer = gbuffer_dev;
     ep = ev_pair(2);
     if (ep) (void)hipEventRecord(ep->first, st);
-    HIPTRY(cvk::launch_shade(sh, (uint32_t)g_num_cus * 8, st), "shade");
+    HIPTRY(cvk::launch_shade(sh, (uint32_t)npix, st), "shade");
     if (ep) (void)hipEventRecord(ep->second, st);
   }
   return CV_OK;

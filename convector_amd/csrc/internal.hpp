@@ -10,6 +10,7 @@
 namespace cvk {
 
 constexpr int kMaxBounceSlots = 16;
+constexpr uint32_t kShadeBlock = 256;
 
 // Per-render control block in device memory: queue sizes and work cursors
 // per bounce, plus the optional traversal counters.
@@ -57,6 +58,8 @@ struct ExtendArgs {
   const DevNode* nodes;
   const DevTri* tris;
   uint2* spill;
+  uint32_t batches;  // 64-ray batches per cursor fetch (batch mode)
+  uint32_t refill;   // > 0: lane refill when this many lanes are idle
 };
 
 struct ShadeArgs {
@@ -98,7 +101,7 @@ hipError_t launch_intersect(const IntersectArgs& a, bool count, bool packed, int
 hipError_t launch_extend(const ExtendArgs& a, bool count, bool packed, int stack, uint32_t blocks,
                          hipStream_t s);
 hipError_t launch_generate(const GenerateArgs& a, uint32_t blocks, hipStream_t s);
-hipError_t launch_shade(const ShadeArgs& a, uint32_t blocks, hipStream_t s);
+hipError_t launch_shade(const ShadeArgs& a, uint32_t max_paths, hipStream_t s);
 hipError_t launch_hdr_to_bitmap(const float* hdr, uint32_t* bitmap, uint32_t width,
                                 uint32_t height, uint32_t num_samples, hipStream_t s);
 hipError_t launch_prepare_tris(const float* v9, const uint32_t* order, uint32_t ntris, DevTri* out,

@@ -47,15 +47,29 @@ struct TravState {
 // Per-lane stack: S entries in LDS (stride 64 lanes, bank-conflict free),
 // deeper entries in a global spill area (stride = all persistent lanes).
 // Entry: x = node.index | node.len << 27, y = tmin bits.  PACKED == false
-// stores the node id instead and reads index/len at pop time.
+// stores the node id instead and reads index/len at pop time.  The LDS and
+// the spill area are addressed through separately typed pointers so that
+// the compiler emits ds_read_b64 / ds_write_b64 for the common case.
+// Stack entries are 64-bit words (x in the low half, tmin bits in the high
+// half) behind address-space-qualified pointers: LDS accesses compile to
+// ds_read_b64 / ds_write_b64 and spill accesses to global loads and stores.
+typedef __attribute__((address_space(3))) unsigned long long lds_u64;
+typedef __attribute__((address_space(1))) unsigned long long glb_u64;
+
+__device__ __forceinline__ unsigned long long pack_entry(uint32_t x, float tmin) {
+  return (unsigned long long)x | ((unsigned long long)f2u(tmin) << 32);
+}
+
 template <int S, bool PACKED>
 struct Stack {
-  uint2* lds;
-  uint2* spill;
+  lds_u64* lds;
+  glb_u64* spill;
   uint32_t stride;
   int sp;
+  __device__ __forceinline__ Stack(uint2* l, uint2* g, uint32_t str, int p)
+      : lds((lds_u64*)l), spill((glb_u64*)g), stride(str), sp(p) {}
   __device__ __forceinline__ void push(uint32_t x, float tmin) {
-    uint2 e = make_uint2(x, f2u(tmin));
+    unsigned long long e = pack_entry(x, tmin);
     if (sp < S)
       lds[sp * 64] = e;
     else
@@ -64,7 +78,28 @@ struct Stack {
   }
   __device__ __forceinline__ uint2 pop() {
     sp--;
-    return sp < S ? lds[sp * 64] : spill[(size_t)(sp - S) * stride];
+    unsigned long long e;
+    if (sp < S)
+      e = lds[sp * 64];
+    else
+      e = spill[(size_t)(sp - S) * stride];
+    return make_uint2((uint32_t)e, (uint32_t)(e >> 32));
+  }
+  // Pushes `first` then `second` where their flags are set, in that order.
+  // With room in LDS both slots are written unconditionally and the stack
+  // pointer advances by the number of pushes (unused slots are overwritten
+  // later), which avoids divergent branches.
+  __device__ __forceinline__ void push2(bool hf, uint32_t xf, float tf, bool hs, uint32_t xs,
+                                        float ts) {
+    if (sp + 2 <= S) {
+      unsigned long long ef = pack_entry(xf, tf), es = pack_entry(xs, ts);
+      lds[sp * 64] = hf ? ef : es;
+      lds[(sp + 1) * 64] = es;
+      sp += (int)hf + (int)hs;
+    } else {
+      if (hf) push(xf, tf);
+      if (hs) push(xs, ts);
+    }
   }
 };
 
@@ -88,14 +123,19 @@ __device__ __forceinline__ void visit_pair(const DevNode* __restrict__ nodes, ui
     x1 = base + 1;
   }
   // should_try_before: push 0 first (so 1 pops first) when tmin0 - tmin1
-  // has sign bit 0 (bvh.rs:616-622)
-  if (sgn_x86_sub(tmin0, tmin1) == 0) {
-    if (h0) st.push(x0, tmin0);
-    if (h1) st.push(x1, tmin1);
+  // has sign bit 0 (bvh.rs:616-622).  Without NaN inputs the difference is
+  // NaN only for inf - inf, whose x86 result is the negative default NaN.
+  uint32_t s01;
+  if (EXACT_NAN) {
+    s01 = sgn_x86_sub(tmin0, tmin1);
   } else {
-    if (h1) st.push(x1, tmin1);
-    if (h0) st.push(x0, tmin0);
+    float r = tmin0 - tmin1;
+    s01 = sgn(r) | (uint32_t)(r != r);
   }
+  if (s01 == 0)
+    st.push2(h0, x0, tmin0, h1, x1, tmin1);
+  else
+    st.push2(h1, x1, tmin1, h0, x0, tmin0);
 }
 
 // Triangle::intersect, keeping only what decides the pick (triangle.rs:59-127).
@@ -167,7 +207,7 @@ __device__ __forceinline__ TravState traverse(const DevNode* __restrict__ nodes,
   ts.na = 2;
   ts.nt = 0;
   V3 inv = mk3(rcp(d.x), rcp(d.y), rcp(d.z));
-  Stack<S, PACKED> st{lds, spill, stride, 0};
+  Stack<S, PACKED> st(lds, spill, stride, 0);
   bool exact = !(isfinite(inv.x) && isfinite(inv.y) && isfinite(inv.z));
   if (!valid) return ts;
   if (__builtin_expect(__any(exact), 0))
@@ -179,10 +219,10 @@ __device__ __forceinline__ TravState traverse(const DevNode* __restrict__ nodes,
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
-// Wave-wide work fetch: lane 0 takes 64 entries from the cursor.
-__device__ __forceinline__ uint32_t fetch64(uint32_t* cursor) {
+// Wave-wide work fetch: lane 0 takes `count` entries from the cursor.
+__device__ __forceinline__ uint32_t fetch(uint32_t* cursor, uint32_t count) {
   uint32_t base = 0;
-  if (lane_id() == 0) base = atomicAdd(cursor, 64u);
+  if (lane_id() == 0) base = atomicAdd(cursor, count);
   return __shfl(base, 0);
 }
 
@@ -197,7 +237,7 @@ __global__ __launch_bounds__(64) void k_intersect(IntersectArgs a) {
   const uint32_t gtid = blockIdx.x * 64 + lane;
   const uint32_t stride = gridDim.x * 64;
   for (;;) {
-    uint32_t base = fetch64(a.cursor);
+    uint32_t base = fetch(a.cursor, 64u);
     if (base >= a.n) break;
     uint32_t i = base + lane;
     bool valid = i < a.n;
@@ -436,8 +476,12 @@ __global__ __launch_bounds__(64) void k_extend(ExtendArgs a) {
   const uint32_t n = a.ctrl->count[a.bounce];
   if (blockIdx.x == 0 && lane == 0) atomicAdd(&a.ctrl->total_rays, (unsigned long long)n);
   uint64_t na = 0, nt = 0;
+  const uint32_t chunk = 64u * a.batches;
   for (;;) {
-    uint32_t base = fetch64(&a.ctrl->ext_cursor[a.bounce]);
+    uint32_t cbase = fetch(&a.ctrl->ext_cursor[a.bounce], chunk);
+    if (cbase >= n) break;
+    for (uint32_t b = 0; b < a.batches; b++) {
+    uint32_t base = cbase + 64u * b;
     if (base >= n) break;
     uint32_t i = base + lane;
     bool valid = i < n;
@@ -454,6 +498,116 @@ __global__ __launch_bounds__(64) void k_extend(ExtendArgs a) {
       if (COUNT) {
         na += ts.na;
         nt += ts.nt;
+      }
+    }
+    }
+  }
+  if (COUNT) {
+    for (int off = 32; off > 0; off >>= 1) {
+      na += __shfl_down(na, off);
+      nt += __shfl_down(nt, off);
+    }
+    if (lane == 0) {
+      atomicAdd(&a.ctrl->sum_aabb, (unsigned long long)na);
+      atomicAdd(&a.ctrl->sum_tri, (unsigned long long)nt);
+    }
+  }
+}
+
+// Traversal with lane refill: a lane whose ray is finished takes a new ray
+// from the queue while the other lanes continue, so a wave stays full until
+// the queue is empty.  A wave refills when at least `refill` lanes are idle
+// (one atomic for all of them).  Every ray is traversed exactly as by
+// traverse(): the root pair is an entry (index 0, len 0, tmin -inf) on the
+// fresh stack.
+template <bool EXACT_NAN, bool COUNT, int S, bool PACKED>
+__device__ __forceinline__ void trav_step(const DevNode* __restrict__ nodes,
+                                          const DevTri* __restrict__ tris, V3 o, V3 d, V3 inv,
+                                          Stack<S, PACKED>& st, TravState& ts) {
+  uint2 e = st.pop();
+  if (!(u2f(e.y) < ts.best_t)) return;
+  uint32_t idx, len;
+  if (PACKED) {
+    idx = e.x & 0x07ffffffu;
+    len = e.x >> 27;
+  } else if (e.x == 0xffffffffu) {
+    idx = 0;
+    len = 0;
+  } else {
+    float4 nb = reinterpret_cast<const float4*>(nodes + e.x)[1];
+    idx = f2u(nb.z);
+    len = f2u(nb.w);
+  }
+  if (len == 0) {
+    if (COUNT) ts.na += 2;
+    visit_pair<EXACT_NAN, COUNT>(nodes, idx, o, inv, st);
+  } else {
+    for (uint32_t k = idx; k < idx + len; k++) {
+      test_tri(tris, k, o, d, ts);
+      if (COUNT) ts.nt++;
+    }
+  }
+}
+
+template <bool COUNT, int S, bool PACKED>
+__global__ __launch_bounds__(64) void k_extend_refill(ExtendArgs a) {
+  __shared__ uint2 lds[S * 64];
+  const uint32_t lane = lane_id();
+  const uint32_t gtid = blockIdx.x * 64 + lane;
+  const uint32_t stride = gridDim.x * 64;
+  const uint32_t n = a.ctrl->count[a.bounce];
+  if (blockIdx.x == 0 && lane == 0) atomicAdd(&a.ctrl->total_rays, (unsigned long long)n);
+  uint32_t* cursor = &a.ctrl->ext_cursor[a.bounce];
+  const uint64_t lt = (1ull << lane) - 1ull;
+  Stack<S, PACKED> st(lds + lane, a.spill + gtid, stride, 0);
+  TravState ts;
+  V3 o = mk3(0, 0, 0), d = mk3(1, 0, 0), inv = mk3(1, 0, 0);
+  uint32_t ray = 0;
+  bool active = false, exact = false, exhausted = false;
+  uint64_t na = 0, nt = 0;
+  for (;;) {
+    const uint64_t idle = __ballot(!active);
+    const uint32_t nidle = __popcll(idle);
+    if (!exhausted && (nidle >= a.refill || nidle == 64)) {
+      uint32_t base = fetch(cursor, nidle);
+      if (base + nidle >= n) exhausted = true;
+      if (!active) {
+        uint32_t i = base + __popcll(idle & lt);
+        if (i < n) {
+          float4 r0 = a.r0[i], r1 = a.r1[i];
+          o = mk3(r0.x, r0.y, r0.z);
+          d = mk3(r1.x, r1.y, r1.z);
+          inv = mk3(rcp(d.x), rcp(d.y), rcp(d.z));
+          exact = !(isfinite(inv.x) && isfinite(inv.y) && isfinite(inv.z));
+          ts.best_t = 1.0e5f;
+          ts.best = kNoHit;
+          ts.bu = ts.bv = 0.0f;
+          ts.na = 0;
+          ts.nt = 0;
+          st.sp = 0;
+          st.push(PACKED ? 0u : 0xffffffffu, -INFINITY);
+          ray = i;
+          active = true;
+        }
+      }
+    }
+    if (!__any(active)) {
+      if (exhausted) break;
+      continue;
+    }
+    const bool wave_exact = __any(active && exact);
+    if (active) {
+      if (wave_exact)
+        trav_step<true, COUNT>(a.nodes, a.tris, o, d, inv, st, ts);
+      else
+        trav_step<false, COUNT>(a.nodes, a.tris, o, d, inv, st, ts);
+      if (st.sp == 0) {
+        a.hits[ray] = make_uint4(ts.best, f2u(ts.best_t), f2u(ts.bu), f2u(ts.bv));
+        if (COUNT) {
+          na += ts.na;
+          nt += ts.nt;
+        }
+        active = false;
       }
     }
   }
@@ -578,91 +732,96 @@ __device__ __forceinline__ ShadeOut continue_path(const ShadeArgs& a, V3 ray_d, 
   return r;
 }
 
-__global__ __launch_bounds__(64) void k_shade(ShadeArgs a) {
-  const uint32_t lane = lane_id();
+// One path per thread; blocks beyond the queue exit at once.  Survivors are
+// appended with one atomic per block: ballots per wave, a prefix over the
+// block's waves in LDS.
+__global__ __launch_bounds__(kShadeBlock) void k_shade(ShadeArgs a) {
+  __shared__ uint32_t wcount[kShadeBlock / 64];
+  __shared__ uint32_t block_base;
+  const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
   const uint32_t n = a.ctrl->count[a.bounce];
+  if (blockIdx.x * kShadeBlock >= n) return;
   const bool last = a.bounce + 1 >= a.max_bounces;
-  for (;;) {
-    uint32_t base = fetch64(&a.ctrl->shade_cursor[a.bounce]);
-    if (base >= n) break;
-    uint32_t i = base + lane;
-    bool valid = i < n;
-    bool keep = false;
-    float4 r0 = make_float4(0, 0, 0, 0), r1 = r0, col = r0;
-    uint64_t st = 0;
-    ShadeOut so;
-    if (valid) {
-      r0 = a.in.r0[i];
-      r1 = a.in.r1[i];
-      col = a.in.col[i];
-      st = a.in.rng[i];
-      uint4 hit = a.hits[i];
-      V3 o = mk3(r0.x, r0.y, r0.z), d = mk3(r1.x, r1.y, r1.z);
-      V3 color = mk3(col.x, col.y, col.z);
-      uint32_t pixel = f2u(r0.w);
-      uint32_t px = pixel % a.width, py = pixel / a.width;
-      float t = u2f(hit.y);
-      V3 pos, normal;
-      uint32_t mat;
-      float tu = 0.0f, tv = 0.0f;
-      if (hit.x == kNoHit) {
-        pos = muladd3(d, 1.0e5f, o);
-        normal = d;
-        mat = kMatSky;
-      } else {
-        const float4* tp = reinterpret_cast<const float4*>(a.tris + hit.x);
-        V3 nn = mk3(tp[0].w, tp[1].w, tp[2].w);
-        pos = muladd3(d, t, o);
-        normal = normalized3(nn);
-        mat = a.tri_mat[hit.x];
-        if (a.bounce == 0 && a.gbuffer) {
-          const float* uv = a.tri_uv + 6 * (size_t)hit.x;
-          float bu = u2f(hit.z), bv = u2f(hit.w);
-          float w = (1.0f - bu) - bv;
-          tu = fmaf(uv[0], w, fmaf(uv[2], bv, uv[4] * bu));
-          tv = fmaf(uv[1], w, fmaf(uv[3], bv, uv[5] * bu));
-        }
-      }
-      if (mat >> 31) {
-        // The path found an emitter: colour x emission (renderer.rs:452-460).
-        V3 em = (a.has_override && hit.x != kNoHit)
-                    ? mk3(a.emission[0], a.emission[1], a.emission[2])
-                    : sky_intensity(d);
-        V3 c = mulc3(color, em);
-        if (a.hdr) {
-          float* h = a.hdr + hdr_offset(px, py, a.width);
-          h[0] += c.x;
-          h[8] += c.y;
-          h[16] += c.z;
-        }
-        if (a.bitmap) a.bitmap[pixel] = pack_color(c);
-      } else if (!last || (a.bounce == 0 && a.gbuffer)) {
-        so = continue_path(a, d, pos, normal, mat, st, a.bounce == 0);
-        if (a.bounce == 0 && a.gbuffer) {
-          uint32_t tex_index = ((int32_t)mat >> 24) & 3;
-          a.gbuffer[pixel] = pack_gbuffer(tex_index, tu, tv, so.fresnel);
-        }
-        if (!last) {
-          keep = true;
-          so.cm = mulc3(color, so.cm);
-        }
+  const uint32_t i = blockIdx.x * kShadeBlock + threadIdx.x;
+  const bool valid = i < n;
+  bool keep = false;
+  float4 r0 = make_float4(0, 0, 0, 0);
+  uint64_t st = 0;
+  ShadeOut so;
+  if (valid) {
+    r0 = a.in.r0[i];
+    float4 r1 = a.in.r1[i];
+    float4 col = a.in.col[i];
+    st = a.in.rng[i];
+    uint4 hit = a.hits[i];
+    V3 o = mk3(r0.x, r0.y, r0.z), d = mk3(r1.x, r1.y, r1.z);
+    V3 color = mk3(col.x, col.y, col.z);
+    uint32_t pixel = f2u(r0.w);
+    uint32_t px = pixel % a.width, py = pixel / a.width;
+    float t = u2f(hit.y);
+    V3 pos, normal;
+    uint32_t mat;
+    float tu = 0.0f, tv = 0.0f;
+    if (hit.x == kNoHit) {
+      pos = muladd3(d, 1.0e5f, o);
+      normal = d;
+      mat = kMatSky;
+    } else {
+      const float4* tp = reinterpret_cast<const float4*>(a.tris + hit.x);
+      V3 nn = mk3(tp[0].w, tp[1].w, tp[2].w);
+      pos = muladd3(d, t, o);
+      normal = normalized3(nn);
+      mat = a.tri_mat[hit.x];
+      if (a.bounce == 0 && a.gbuffer) {
+        const float* uv = a.tri_uv + 6 * (size_t)hit.x;
+        float bu = u2f(hit.z), bv = u2f(hit.w);
+        float w = (1.0f - bu) - bv;
+        tu = fmaf(uv[0], w, fmaf(uv[2], bv, uv[4] * bu));
+        tv = fmaf(uv[1], w, fmaf(uv[3], bv, uv[5] * bu));
       }
     }
-    // wave-aggregated append to the next queue
-    uint64_t ballot = __ballot(keep);
-    if (ballot == 0) continue;
-    uint32_t cnt = __popcll(ballot);
-    uint32_t leader = __ffsll((unsigned long long)ballot) - 1;
-    uint32_t slot0 = 0;
-    if (lane == leader) slot0 = atomicAdd(&a.ctrl->count[a.bounce + 1], cnt);
-    slot0 = __shfl(slot0, leader);
-    if (keep) {
-      uint32_t j = slot0 + __popcll(ballot & ((1ull << lane) - 1ull));
-      a.out.r0[j] = make_float4(so.o.x, so.o.y, so.o.z, r0.w);
-      a.out.r1[j] = make_float4(so.d.x, so.d.y, so.d.z, 0.0f);
-      a.out.col[j] = make_float4(so.cm.x, so.cm.y, so.cm.z, 0.0f);
-      a.out.rng[j] = st;
+    if (mat >> 31) {
+      // The path found an emitter: colour x emission (renderer.rs:452-460).
+      V3 em = (a.has_override && hit.x != kNoHit)
+                  ? mk3(a.emission[0], a.emission[1], a.emission[2])
+                  : sky_intensity(d);
+      V3 c = mulc3(color, em);
+      if (a.hdr) {
+        float* h = a.hdr + hdr_offset(px, py, a.width);
+        h[0] += c.x;
+        h[8] += c.y;
+        h[16] += c.z;
+      }
+      if (a.bitmap) a.bitmap[pixel] = pack_color(c);
+    } else if (!last || (a.bounce == 0 && a.gbuffer)) {
+      so = continue_path(a, d, pos, normal, mat, st, a.bounce == 0);
+      if (a.bounce == 0 && a.gbuffer) {
+        uint32_t tex_index = ((int32_t)mat >> 24) & 3;
+        a.gbuffer[pixel] = pack_gbuffer(tex_index, tu, tv, so.fresnel);
+      }
+      if (!last) {
+        keep = true;
+        so.cm = mulc3(color, so.cm);
+      }
     }
+  }
+  const uint64_t ballot = __ballot(keep);
+  if (lane == 0) wcount[wave] = __popcll(ballot);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t tot = 0;
+    for (uint32_t w = 0; w < kShadeBlock / 64; w++) tot += wcount[w];
+    block_base = tot ? atomicAdd(&a.ctrl->count[a.bounce + 1], tot) : 0u;
+  }
+  __syncthreads();
+  if (keep) {
+    uint32_t j = block_base;
+    for (uint32_t w = 0; w < wave; w++) j += wcount[w];
+    j += __popcll(ballot & ((1ull << lane) - 1ull));
+    a.out.r0[j] = make_float4(so.o.x, so.o.y, so.o.z, r0.w);
+    a.out.r1[j] = make_float4(so.d.x, so.d.y, so.d.z, 0.0f);
+    a.out.col[j] = make_float4(so.cm.x, so.cm.y, so.cm.z, 0.0f);
+    a.out.rng[j] = st;
   }
 }
 
@@ -756,10 +915,16 @@ hipError_t launch_intersect(const IntersectArgs& a, bool count, bool packed, int
 template <bool COUNT, int S>
 static hipError_t launch_extend_s(const ExtendArgs& a, bool packed, uint32_t blocks,
                                   hipStream_t s) {
-  if (packed)
+  if (a.refill) {
+    if (packed)
+      hipLaunchKernelGGL((k_extend_refill<COUNT, S, true>), dim3(blocks), dim3(64), 0, s, a);
+    else
+      hipLaunchKernelGGL((k_extend_refill<COUNT, S, false>), dim3(blocks), dim3(64), 0, s, a);
+  } else if (packed) {
     hipLaunchKernelGGL((k_extend<COUNT, S, true>), dim3(blocks), dim3(64), 0, s, a);
-  else
+  } else {
     hipLaunchKernelGGL((k_extend<COUNT, S, false>), dim3(blocks), dim3(64), 0, s, a);
+  }
   return hipGetLastError();
 }
 
@@ -779,8 +944,10 @@ hipError_t launch_generate(const GenerateArgs& a, uint32_t blocks, hipStream_t s
   return hipGetLastError();
 }
 
-hipError_t launch_shade(const ShadeArgs& a, uint32_t blocks, hipStream_t s) {
-  hipLaunchKernelGGL(k_shade, dim3(blocks), dim3(64), 0, s, a);
+hipError_t launch_shade(const ShadeArgs& a, uint32_t max_paths, hipStream_t s) {
+  uint32_t blocks = (max_paths + kShadeBlock - 1) / kShadeBlock;
+  if (blocks == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_shade, dim3(blocks), dim3(kShadeBlock), 0, s, a);
   return hipGetLastError();
 }
 

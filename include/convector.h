@@ -125,6 +125,18 @@ cv_status cv_scene_set_camera(cv_scene* scene, const float pos[3], const float d
                               const float q[4], const float dq[4], float screen_distance);
 /* Bvh::print_stats / Scene::print_stats as numbers (bvh.rs:570-583). */
 cv_status cv_scene_get_bvh_info(const cv_scene* scene, cv_bvh_info* info);
+/* Launch shape of the kernels, the knobs of the autotuner (no reference
+ * counterpart; the reference tunes its patch size by hand, main.rs:91-98).
+ * Zero fields keep their current value. */
+typedef struct cv_tuning {
+    uint32_t stack_entries;       /* per-lane LDS stack entries: 8, 16, 24 or 32 */
+    uint32_t extend_waves_per_cu; /* persistent traversal waves per CU */
+    uint32_t refill_threshold;    /* 1..64: idle lanes of a wave take new rays when at least
+                                     this many are idle; 65: whole 64-ray batches */
+    uint32_t fetch_batches;       /* batch mode: 64-ray batches per fetch */
+} cv_tuning;
+cv_status cv_scene_set_tuning(cv_scene* scene, const cv_tuning* tuning);
+cv_status cv_scene_get_tuning(const cv_scene* scene, cv_tuning* tuning);
 /* Copies the BVH out: nodes (num_nodes) and order (num_triangles). */
 cv_status cv_scene_copy_bvh(const cv_scene* scene, cv_bvh_node* nodes, uint32_t* order);
 void cv_scene_destroy(cv_scene* scene);

@@ -141,3 +141,25 @@ def test_full_size_properties(cv):
         r.accumulate_frames_f32(c, f, 1)
     assert a.tobytes() == c.tobytes()
     assert 0.005 < float(a.mean()) / 16 < 1.0
+
+
+def test_tuning_variants_exact(cv, oracle):
+    """Launch shapes change only the schedule: every stack size, refill
+    threshold (lane refill and whole-batch fetch) and fetch width renders
+    the same bits."""
+    sd = scenes.config0(subdivisions=4)
+    g, o = _pair(cv, oracle, sd)
+    W, H, n = 160, 96, 2
+    ohdr, ogb = o.accumulate_frames(W, H, 1, n, oracle.RNG_PIXEL, 8)
+    for t in (dict(stack_entries=8, refill_threshold=1), dict(stack_entries=16, refill_threshold=32),
+              dict(stack_entries=24, refill_threshold=65, fetch_batches=1),
+              dict(stack_entries=32, refill_threshold=65, fetch_batches=4),
+              dict(stack_entries=8, extend_waves_per_cu=1, refill_threshold=64)):
+        print("tuning case", t, flush=True)
+        g.set_tuning(**t)
+        r = cv.Renderer(g, W, H)
+        hdr = r.new_buffer_f32()
+        gb = np.zeros(W * H, np.uint32)
+        r.accumulate_frames_f32(hdr, 1, n, gbuffer=gb)
+        assert hdr.tobytes() == ohdr.tobytes()
+        assert (gb == ogb).all()

@@ -150,3 +150,32 @@ def test_sizes_and_device_pointers(cv, oracle, models_dir):
     finally:
         hip.hipFree(dr)
         hip.hipFree(dh)
+
+
+def test_stack_sizes_exact(cv, oracle, models_dir):
+    """Every per-lane LDS stack size gives the same hits: deep trees spill
+    the stack to global memory at 8 entries, never at 32."""
+    sd = model_scene(models_dir, "stanford_dragon")
+    osc = oracle.Scene.from_scene_data(sd)
+    gsc = cv.Scene.from_scene_data(sd)
+    assert gsc.bvh_info()["depth"] > 8
+    lo, hi = sd.tris.reshape(-1, 3).min(0), sd.tris.reshape(-1, 3).max(0)
+    rays = _ray_families((lo + hi) / 2, float((hi - lo).max()), 70)
+    ref = osc.intersect(rays, isect=True)
+    for stack in (8, 16, 24, 32):
+        print("stack case", stack, sd.ntris, flush=True)
+        gsc.set_tuning(stack_entries=stack)
+        assert gsc.tuning()["stack_entries"] == stack
+        hits, isects = gsc.intersect_nearest(rays, full=True)
+        _check_exact(hits, isects, ref)
+
+
+def test_tuning_validation(cv, models_dir):
+    gsc = cv.Scene.from_scene_data(model_scene(models_dir, "suzanne"))
+    before = gsc.tuning()
+    for bad in (dict(stack_entries=12), dict(extend_waves_per_cu=65),
+                dict(refill_threshold=66), dict(fetch_batches=65)):
+        with pytest.raises(cv.CvError) as e:
+            gsc.set_tuning(**bad)
+        assert e.value.name == "CV_ERR_INVALID_ARGUMENT"
+    assert gsc.tuning() == before
