@@ -16,7 +16,9 @@ namespace cvk {
 // Traversal (bvh.rs:589-657, aabb.rs:105-183, triangle.rs:59-148)
 // ---------------------------------------------------------------------------
 
-// Aabb::intersect with the reciprocals hoisted (aabb.rs:105-141).
+// Aabb::intersect with the reciprocals hoisted (aabb.rs:105-141), with the
+// minps / maxps selection rules of x86 (the second operand on NaN and on
+// +-0 ties).
 template <bool EXACT_NAN>
 __device__ __forceinline__ void slab(float4 a, float4 b, V3 o, V3 inv, float& tmin, float& tmax) {
   float tx1 = (a.x - o.x) * inv.x, tx2 = (a.w - o.x) * inv.x;
@@ -37,6 +39,19 @@ __device__ __forceinline__ void slab(float4 a, float4 b, V3 o, V3 inv, float& tm
   tmax = xmin(txmax, xmin(tymax, tzmax));
 }
 
+// The same slab test with the hardware v_min / v_max (v_min3 / v_max3).
+// Valid only for NaN-free slab distances (finite origin and reciprocal):
+// then every min / max chain has the same value as the x86 chain and can
+// differ from it only in the sign of a zero result.
+__device__ __forceinline__ void slab_hw(float4 a, float4 b, V3 o, V3 inv, float& tmin,
+                                        float& tmax) {
+  float tx1 = (a.x - o.x) * inv.x, tx2 = (a.w - o.x) * inv.x;
+  float ty1 = (a.y - o.y) * inv.y, ty2 = (b.x - o.y) * inv.y;
+  float tz1 = (a.z - o.z) * inv.z, tz2 = (b.y - o.z) * inv.z;
+  tmin = fmaxf(fmaxf(fminf(tx1, tx2), fminf(ty1, ty2)), fminf(tz1, tz2));
+  tmax = fminf(fminf(fmaxf(tx1, tx2), fmaxf(ty1, ty2)), fmaxf(tz1, tz2));
+}
+
 struct TravState {
   float best_t;
   uint32_t best;
@@ -44,22 +59,23 @@ struct TravState {
   uint32_t na, nt;
 };
 
-// Per-lane stack: S entries in LDS (stride 64 lanes, bank-conflict free),
-// deeper entries in a global spill area (stride = all persistent lanes).
-// Entry: x = node.index | node.len << 27, y = tmin bits.  PACKED == false
-// stores the node id instead and reads index/len at pop time.  The LDS and
-// the spill area are addressed through separately typed pointers so that
-// the compiler emits ds_read_b64 / ds_write_b64 for the common case.
-// Stack entries are 64-bit words (x in the low half, tmin bits in the high
-// half) behind address-space-qualified pointers: LDS accesses compile to
-// ds_read_b64 / ds_write_b64 and spill accesses to global loads and stores.
+// Stack entries are 64-bit words: x = node.index | node.len << 27 (PACKED)
+// or the node id (PACKED == false, index/len read at pop time) in the low
+// half, the tmin bits in the high half.
 typedef __attribute__((address_space(3))) unsigned long long lds_u64;
 typedef __attribute__((address_space(1))) unsigned long long glb_u64;
+typedef unsigned long long entry_t;
 
-__device__ __forceinline__ unsigned long long pack_entry(uint32_t x, float tmin) {
-  return (unsigned long long)x | ((unsigned long long)f2u(tmin) << 32);
+__device__ __forceinline__ entry_t pack_entry(uint32_t x, float tmin) {
+  return (entry_t)x | ((entry_t)f2u(tmin) << 32);
 }
+__device__ __forceinline__ uint32_t entry_x(entry_t e) { return (uint32_t)e; }
+__device__ __forceinline__ float entry_tmin(entry_t e) { return u2f((uint32_t)(e >> 32)); }
 
+// Per-lane stack: S entries in LDS (stride 64 lanes, bank-conflict free),
+// deeper entries in a global spill area (stride = all persistent lanes).
+// The pointers are address-space qualified so that LDS accesses compile to
+// ds_read_b64 / ds_write_b64 and spill accesses to global loads / stores.
 template <int S, bool PACKED>
 struct Stack {
   lds_u64* lds;
@@ -68,52 +84,58 @@ struct Stack {
   int sp;
   __device__ __forceinline__ Stack(uint2* l, uint2* g, uint32_t str, int p)
       : lds((lds_u64*)l), spill((glb_u64*)g), stride(str), sp(p) {}
-  __device__ __forceinline__ void push(uint32_t x, float tmin) {
-    unsigned long long e = pack_entry(x, tmin);
-    if (sp < S)
+  // Pushes e when p is set.  Below S the slot at sp is written either way
+  // (it is above the top of the stack, so an unused write is harmless).
+  __device__ __forceinline__ void push_if(bool p, entry_t e) {
+    if (sp < S) {
       lds[sp * 64] = e;
-    else
+      sp += (int)p;
+    } else if (p) {
       spill[(size_t)(sp - S) * stride] = e;
-    sp++;
-  }
-  __device__ __forceinline__ uint2 pop() {
-    sp--;
-    unsigned long long e;
-    if (sp < S)
-      e = lds[sp * 64];
-    else
-      e = spill[(size_t)(sp - S) * stride];
-    return make_uint2((uint32_t)e, (uint32_t)(e >> 32));
-  }
-  // Pushes `first` then `second` where their flags are set, in that order.
-  // With room in LDS both slots are written unconditionally and the stack
-  // pointer advances by the number of pushes (unused slots are overwritten
-  // later), which avoids divergent branches.
-  __device__ __forceinline__ void push2(bool hf, uint32_t xf, float tf, bool hs, uint32_t xs,
-                                        float ts) {
-    if (sp + 2 <= S) {
-      unsigned long long ef = pack_entry(xf, tf), es = pack_entry(xs, ts);
-      lds[sp * 64] = hf ? ef : es;
-      lds[(sp + 1) * 64] = es;
-      sp += (int)hf + (int)hs;
-    } else {
-      if (hf) push(xf, tf);
-      if (hs) push(xs, ts);
+      sp++;
     }
+  }
+  __device__ __forceinline__ entry_t pop() {
+    sp--;
+    if (sp < S) return lds[sp * 64];
+    return spill[(size_t)(sp - S) * stride];
   }
 };
 
-template <bool EXACT_NAN, bool COUNT, int S, bool PACKED>
+// Both children of an inner node (bvh.rs:604-631).  The reference pushes the
+// hit children, the one that should be tried later first, and the next loop
+// iteration pops the last push.  Here the last push stays in registers as
+// `cur` (have = true) and only the other child goes to the stack, which
+// visits the nodes in exactly the reference's order.
+template <bool EXACT_NAN, int S, bool PACKED>
 __device__ __forceinline__ void visit_pair(const DevNode* __restrict__ nodes, uint32_t base, V3 o,
-                                           V3 inv, Stack<S, PACKED>& st) {
+                                           V3 inv, Stack<S, PACKED>& st, entry_t& cur,
+                                           bool& have) {
   const float4* p = reinterpret_cast<const float4*>(nodes + base);
   float4 a0 = p[0], b0 = p[1], a1 = p[2], b1 = p[3];
   float tmin0, tmax0, tmin1, tmax1;
-  slab<EXACT_NAN>(a0, b0, o, inv, tmin0, tmax0);
-  slab<EXACT_NAN>(a1, b1, o, inv, tmin1, tmax1);
+  uint32_t s01;
+  if (EXACT_NAN) {
+    slab<true>(a0, b0, o, inv, tmin0, tmax0);
+    slab<true>(a1, b1, o, inv, tmin1, tmax1);
+    s01 = sgn_x86_sub(tmin0, tmin1);
+  } else {
+    slab_hw(a0, b0, o, inv, tmin0, tmax0);
+    slab_hw(a1, b1, o, inv, tmin1, tmax1);
+    // Zero results are where the signs of the hardware min / max may differ
+    // from minps / maxps; the sign of tmax decides the hit and the signs of
+    // two zero tmins decide the order.  Redo those with the x86 chains.
+    const bool zero = (tmax0 == 0.0f) | (tmax1 == 0.0f) | ((tmin0 == 0.0f) & (tmin1 == 0.0f));
+    if (__builtin_expect(zero, 0)) {
+      slab<false>(a0, b0, o, inv, tmin0, tmax0);
+      slab<false>(a1, b1, o, inv, tmin1, tmax1);
+    }
+    // sign bit of tmin0 - tmin1; inf - inf is the negative default NaN
+    s01 = neg_or_nan(tmin0 - tmin1);
+  }
   // any_masked: mask (tmax NLT tmin) and tmax's sign bit 0 (aabb.rs:155-158)
-  bool h0 = !(tmax0 < tmin0) && sgn(tmax0) == 0;
-  bool h1 = !(tmax1 < tmin1) && sgn(tmax1) == 0;
+  const bool h0 = !(tmax0 < tmin0) && sgn(tmax0) == 0;
+  const bool h1 = !(tmax1 < tmin1) && sgn(tmax1) == 0;
   uint32_t x0, x1;
   if (PACKED) {
     x0 = f2u(b0.z) | (f2u(b0.w) << 27);
@@ -122,20 +144,15 @@ __device__ __forceinline__ void visit_pair(const DevNode* __restrict__ nodes, ui
     x0 = base;
     x1 = base + 1;
   }
-  // should_try_before: push 0 first (so 1 pops first) when tmin0 - tmin1
-  // has sign bit 0 (bvh.rs:616-622).  Without NaN inputs the difference is
-  // NaN only for inf - inf, whose x86 result is the negative default NaN.
-  uint32_t s01;
-  if (EXACT_NAN) {
-    s01 = sgn_x86_sub(tmin0, tmin1);
-  } else {
-    float r = tmin0 - tmin1;
-    s01 = sgn(r) | (uint32_t)(r != r);
-  }
-  if (s01 == 0)
-    st.push2(h0, x0, tmin0, h1, x1, tmin1);
-  else
-    st.push2(h1, x1, tmin1, h0, x0, tmin0);
+  // should_try_before (bvh.rs:616-622): with s01 == 0 child 0 is pushed
+  // first and child 1 is popped first.
+  const entry_t e0 = pack_entry(x0, tmin0), e1 = pack_entry(x1, tmin1);
+  const bool swap = s01 != 0;
+  const entry_t ef = swap ? e1 : e0, es = swap ? e0 : e1;
+  const bool hf = swap ? h1 : h0, hs = swap ? h0 : h1;
+  st.push_if(hf && hs, ef);
+  cur = hs ? es : ef;
+  have = hf || hs;
 }
 
 // Triangle::intersect, keeping only what decides the pick (triangle.rs:59-127).
@@ -163,34 +180,56 @@ __device__ __forceinline__ void test_tri(const DevTri* __restrict__ tris, uint32
   }
 }
 
+// One iteration of the traversal loop (bvh.rs:597-651): take the next
+// entry (the one kept in registers, else the top of the stack), skip it when
+// it is further away than the best hit (is_further_away_than,
+// aabb.rs:161-170), then visit an inner node's children or test a leaf's
+// triangles.  The ray is finished when have == false and the stack is empty.
+// The root is the entry (index 0, len 0, tmin -inf), so that its pair is
+// visited first and counted like every other pair.
 template <bool EXACT_NAN, bool COUNT, int S, bool PACKED>
-__device__ __forceinline__ void traverse_impl(const DevNode* __restrict__ nodes,
-                                              const DevTri* __restrict__ tris, V3 o, V3 d, V3 inv,
-                                              Stack<S, PACKED>& st, TravState& ts) {
-  visit_pair<EXACT_NAN, COUNT>(nodes, 0u, o, inv, st);
-  while (st.sp > 0) {
-    uint2 e = st.pop();
-    // is_further_away_than (aabb.rs:161-170)
-    if (!(u2f(e.y) < ts.best_t)) continue;
-    uint32_t idx, len;
-    if (PACKED) {
-      idx = e.x & 0x07ffffffu;
-      len = e.x >> 27;
-    } else {
-      float4 nb = reinterpret_cast<const float4*>(nodes + e.x)[1];
-      idx = f2u(nb.z);
-      len = f2u(nb.w);
-    }
-    if (len == 0) {
-      if (COUNT) ts.na += 2;
-      visit_pair<EXACT_NAN, COUNT>(nodes, idx, o, inv, st);
-    } else {
-      for (uint32_t k = idx; k < idx + len; k++) {
-        test_tri(tris, k, o, d, ts);
-        if (COUNT) ts.nt++;
-      }
+__device__ __forceinline__ void trav_step(const DevNode* __restrict__ nodes,
+                                          const DevTri* __restrict__ tris, V3 o, V3 d, V3 inv,
+                                          Stack<S, PACKED>& st, TravState& ts, entry_t& cur,
+                                          bool& have) {
+  entry_t e = cur;
+  if (!have) e = st.pop();
+  have = false;
+  if (!(entry_tmin(e) < ts.best_t)) return;
+  const uint32_t x = entry_x(e);
+  uint32_t idx, len;
+  if (PACKED) {
+    idx = x & 0x07ffffffu;
+    len = x >> 27;
+  } else if (x == 0xffffffffu) {
+    idx = 0;
+    len = 0;
+  } else {
+    float4 nb = reinterpret_cast<const float4*>(nodes + x)[1];
+    idx = f2u(nb.z);
+    len = f2u(nb.w);
+  }
+  if (len == 0) {
+    if (COUNT) ts.na += 2;
+    visit_pair<EXACT_NAN>(nodes, idx, o, inv, st, cur, have);
+  } else {
+    for (uint32_t k = idx; k < idx + len; k++) {
+      test_tri(tris, k, o, d, ts);
+      if (COUNT) ts.nt++;
     }
   }
+}
+
+template <bool PACKED>
+__device__ __forceinline__ entry_t root_entry() {
+  return pack_entry(PACKED ? 0u : 0xffffffffu, -INFINITY);
+}
+
+// A lane needs the exact NaN rules when a slab distance can be NaN: a
+// non-finite origin or reciprocal (0 * inf, inf - inf).
+__device__ __forceinline__ bool needs_exact_nan(V3 o, V3 inv) {
+  return !(isfinite(inv.x) && isfinite(inv.y) && isfinite(inv.z) && isfinite(o.x) &&
+           isfinite(o.y) && isfinite(o.z));
 }
 
 // Nearest hit of one ray seeded with the far-away sky hit at 1e5
@@ -204,16 +243,19 @@ __device__ __forceinline__ TravState traverse(const DevNode* __restrict__ nodes,
   ts.best_t = 1.0e5f;
   ts.best = kNoHit;
   ts.bu = ts.bv = 0.0f;
-  ts.na = 2;
+  ts.na = 0;
   ts.nt = 0;
   V3 inv = mk3(rcp(d.x), rcp(d.y), rcp(d.z));
   Stack<S, PACKED> st(lds, spill, stride, 0);
-  bool exact = !(isfinite(inv.x) && isfinite(inv.y) && isfinite(inv.z));
+  const bool exact = needs_exact_nan(o, inv);
   if (!valid) return ts;
-  if (__builtin_expect(__any(exact), 0))
-    traverse_impl<true, COUNT>(nodes, tris, o, d, inv, st, ts);
-  else
-    traverse_impl<false, COUNT>(nodes, tris, o, d, inv, st, ts);
+  entry_t cur = root_entry<PACKED>();
+  bool have = true;
+  if (__builtin_expect(__any(exact), 0)) {
+    while (have || st.sp > 0) trav_step<true, COUNT>(nodes, tris, o, d, inv, st, ts, cur, have);
+  } else {
+    while (have || st.sp > 0) trav_step<false, COUNT>(nodes, tris, o, d, inv, st, ts, cur, have);
+  }
   return ts;
 }
 
@@ -518,37 +560,7 @@ __global__ __launch_bounds__(64) void k_extend(ExtendArgs a) {
 // from the queue while the other lanes continue, so a wave stays full until
 // the queue is empty.  A wave refills when at least `refill` lanes are idle
 // (one atomic for all of them).  Every ray is traversed exactly as by
-// traverse(): the root pair is an entry (index 0, len 0, tmin -inf) on the
-// fresh stack.
-template <bool EXACT_NAN, bool COUNT, int S, bool PACKED>
-__device__ __forceinline__ void trav_step(const DevNode* __restrict__ nodes,
-                                          const DevTri* __restrict__ tris, V3 o, V3 d, V3 inv,
-                                          Stack<S, PACKED>& st, TravState& ts) {
-  uint2 e = st.pop();
-  if (!(u2f(e.y) < ts.best_t)) return;
-  uint32_t idx, len;
-  if (PACKED) {
-    idx = e.x & 0x07ffffffu;
-    len = e.x >> 27;
-  } else if (e.x == 0xffffffffu) {
-    idx = 0;
-    len = 0;
-  } else {
-    float4 nb = reinterpret_cast<const float4*>(nodes + e.x)[1];
-    idx = f2u(nb.z);
-    len = f2u(nb.w);
-  }
-  if (len == 0) {
-    if (COUNT) ts.na += 2;
-    visit_pair<EXACT_NAN, COUNT>(nodes, idx, o, inv, st);
-  } else {
-    for (uint32_t k = idx; k < idx + len; k++) {
-      test_tri(tris, k, o, d, ts);
-      if (COUNT) ts.nt++;
-    }
-  }
-}
-
+// traverse(), one trav_step per loop iteration.
 template <bool COUNT, int S, bool PACKED>
 __global__ __launch_bounds__(64) void k_extend_refill(ExtendArgs a) {
   __shared__ uint2 lds[S * 64];
@@ -563,6 +575,8 @@ __global__ __launch_bounds__(64) void k_extend_refill(ExtendArgs a) {
   TravState ts;
   V3 o = mk3(0, 0, 0), d = mk3(1, 0, 0), inv = mk3(1, 0, 0);
   uint32_t ray = 0;
+  entry_t cur = 0;
+  bool have = false;
   bool active = false, exact = false, exhausted = false;
   uint64_t na = 0, nt = 0;
   for (;;) {
@@ -578,14 +592,15 @@ __global__ __launch_bounds__(64) void k_extend_refill(ExtendArgs a) {
           o = mk3(r0.x, r0.y, r0.z);
           d = mk3(r1.x, r1.y, r1.z);
           inv = mk3(rcp(d.x), rcp(d.y), rcp(d.z));
-          exact = !(isfinite(inv.x) && isfinite(inv.y) && isfinite(inv.z));
+          exact = needs_exact_nan(o, inv);
           ts.best_t = 1.0e5f;
           ts.best = kNoHit;
           ts.bu = ts.bv = 0.0f;
           ts.na = 0;
           ts.nt = 0;
           st.sp = 0;
-          st.push(PACKED ? 0u : 0xffffffffu, -INFINITY);
+          cur = root_entry<PACKED>();
+          have = true;
           ray = i;
           active = true;
         }
@@ -598,10 +613,10 @@ __global__ __launch_bounds__(64) void k_extend_refill(ExtendArgs a) {
     const bool wave_exact = __any(active && exact);
     if (active) {
       if (wave_exact)
-        trav_step<true, COUNT>(a.nodes, a.tris, o, d, inv, st, ts);
+        trav_step<true, COUNT>(a.nodes, a.tris, o, d, inv, st, ts, cur, have);
       else
-        trav_step<false, COUNT>(a.nodes, a.tris, o, d, inv, st, ts);
-      if (st.sp == 0) {
+        trav_step<false, COUNT>(a.nodes, a.tris, o, d, inv, st, ts, cur, have);
+      if (!have && st.sp == 0) {
         a.hits[ray] = make_uint4(ts.best, f2u(ts.best_t), f2u(ts.bu), f2u(ts.bv));
         if (COUNT) {
           na += ts.na;
