@@ -7,6 +7,8 @@ Python mirror of the reference's Rust surface (ruuda/convector):
     Scene.intersect_nearest (Bvh)            scene.rs:220-244, bvh.rs:589-669
     Renderer.render_patch_u8 / accumulate_patch_f32 / new_buffer_f32 /
     buffer_f32_into_render_buffer            renderer.rs:121-410
+    Batch.render                             the patch loop of main.rs:174-214,
+                                             many jobs per launch
 
 Every call goes through the C ABI of include/convector.h into the HIP
 kernels of libconvector_amd.so; nothing here computes the path.
@@ -28,6 +30,7 @@ RNG_REF_PATCH = _native.CV_RNG_REF_PATCH
 PTR_DEVICE = _native.CV_PTR_DEVICE
 RENDER_COUNT = _native.CV_RENDER_COUNT
 RENDER_TIMING = _native.CV_RENDER_TIMING
+RENDER_GRAPH = _native.CV_RENDER_GRAPH
 
 NODE_DTYPE = np.dtype([("min", "<f4", 3), ("max", "<f4", 3), ("index", "<u4"), ("len", "<u4")])
 HIT_DTYPE = np.dtype([("tri_id", "<u4"), ("t", "<f4"), ("u", "<f4"), ("v", "<f4")])
@@ -286,6 +289,68 @@ class Renderer:
         h = getattr(self, "_h", None)
         if h:
             lib().cv_renderer_destroy(h)
+            self._h = None
+
+
+class Job:
+    """One render job of a batch: frames first_frame .. first_frame +
+    num_frames - 1 of `scene` at width x height, accumulated into hdr (the
+    16x4-block layout; a host array, or a device pointer with PTR_DEVICE)."""
+
+    def __init__(self, scene: Scene, width: int, height: int, first_frame: int = 1,
+                 num_frames: int = 1, hdr=None, gbuffer=None):
+        self.scene = scene
+        self.width, self.height = int(width), int(height)
+        self.first_frame, self.num_frames = int(first_frame), int(num_frames)
+        self.hdr = np.zeros(self.width * self.height * 3, np.float32) if hdr is None else hdr
+        self.gbuffer = gbuffer
+
+    def _ptr(self, a):
+        if a is None:
+            return None
+        if isinstance(a, int):
+            return a
+        return a.ctypes.data
+
+
+class Batch:
+    """Many render jobs in shared launches (cv_batch)."""
+
+    def __init__(self, max_paths_per_step: int = 0, max_bounces: int = 5):
+        o = _native.RenderOpts(RNG_PIXEL, _native.CV_APPROX_PORTABLE, max_bounces)
+        h = ctypes.c_void_p()
+        check(lib().cv_batch_create(ctypes.byref(o), max_paths_per_step, ctypes.byref(h)))
+        self._h = h
+
+    def set_tuning(self, stack_entries=0, extend_waves_per_cu=0, refill_threshold=0,
+                   fetch_batches=0):
+        t = _native.Tuning(stack_entries, extend_waves_per_cu, refill_threshold, fetch_batches)
+        check(lib().cv_batch_set_tuning(self._h, ctypes.byref(t)))
+
+    def tuning(self) -> dict:
+        t = _native.Tuning()
+        check(lib().cv_batch_get_tuning(self._h, ctypes.byref(t)))
+        return {k: getattr(t, k) for k, _ in t._fields_}
+
+    def render(self, jobs: Sequence[Job], flags: int = 0):
+        arr = (_native.Job * max(1, len(jobs)))()
+        for i, j in enumerate(jobs):
+            arr[i].scene = j.scene._h
+            arr[i].width, arr[i].height = j.width, j.height
+            arr[i].first_frame, arr[i].num_frames = j.first_frame, j.num_frames
+            arr[i].hdr = j._ptr(j.hdr)
+            arr[i].gbuffer = j._ptr(j.gbuffer)
+        check(lib().cv_batch_render(self._h, arr, len(jobs), flags))
+
+    def stats(self) -> dict:
+        s = _native.RenderStats()
+        check(lib().cv_batch_get_stats(self._h, ctypes.byref(s)))
+        return {k: getattr(s, k) for k, _ in s._fields_}
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().cv_batch_destroy(h)
             self._h = None
 
 

@@ -23,6 +23,7 @@ STATUS_NAMES = {
 CV_PTR_DEVICE = 1
 CV_RENDER_COUNT = 2
 CV_RENDER_TIMING = 4
+CV_RENDER_GRAPH = 8
 CV_RNG_PIXEL = 0
 CV_RNG_REF_PATCH = 1
 CV_APPROX_PORTABLE = 0
@@ -37,8 +38,9 @@ EXPORTS = [
     "cv_bvh_intersect_nearest", "cv_renderer_create", "cv_renderer_render_patch_u8",
     "cv_renderer_accumulate_patch_f32", "cv_renderer_render_frame_u8",
     "cv_renderer_accumulate_frames_f32", "cv_renderer_buffer_f32_into_bitmap",
-    "cv_renderer_get_stats", "cv_renderer_destroy", "cv_device_synchronize", "cv_last_error",
-    "cv_status_string", "cv_version",
+    "cv_renderer_get_stats", "cv_renderer_destroy", "cv_batch_create", "cv_batch_set_tuning",
+    "cv_batch_get_tuning", "cv_batch_render", "cv_batch_get_stats", "cv_batch_destroy",
+    "cv_device_synchronize", "cv_last_error", "cv_status_string", "cv_version",
 ]
 
 
@@ -73,6 +75,13 @@ class Tuning(ctypes.Structure):
 class RenderOpts(ctypes.Structure):
     _fields_ = [("rng_mode", ctypes.c_uint32), ("approx_mode", ctypes.c_uint32),
                 ("max_bounces", ctypes.c_uint32)]
+
+
+class Job(ctypes.Structure):
+    _fields_ = [("scene", ctypes.c_void_p), ("width", ctypes.c_uint32),
+                ("height", ctypes.c_uint32), ("first_frame", ctypes.c_uint32),
+                ("num_frames", ctypes.c_uint32), ("hdr", ctypes.c_void_p),
+                ("gbuffer", ctypes.c_void_p)]
 
 
 class RenderStats(ctypes.Structure):
@@ -129,6 +138,12 @@ def _declare(L):
         "cv_renderer_buffer_f32_into_bitmap": (I, [P, P, P, U32, U32]),
         "cv_renderer_get_stats": (I, [P, P]),
         "cv_renderer_destroy": (None, [P]),
+        "cv_batch_create": (I, [P, ctypes.c_uint64, PP]),
+        "cv_batch_set_tuning": (I, [P, P]),
+        "cv_batch_get_tuning": (I, [P, P]),
+        "cv_batch_render": (I, [P, P, SZ, U32]),
+        "cv_batch_get_stats": (I, [P, P]),
+        "cv_batch_destroy": (None, [P]),
         "cv_device_synchronize": (I, []),
         "cv_last_error": (ctypes.c_char_p, []),
         "cv_status_string": (ctypes.c_char_p, [I]),

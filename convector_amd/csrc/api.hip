@@ -1,8 +1,8 @@
-// api.cpp -- the C ABI of include/convector.h: meshes, scenes (GPU BVH
-// build + device upload), batched nearest-hit queries and the wavefront
-// renderer.  Every compute step runs in the kernels of kernels.hip and
-// build.hip; this file only parses input, owns device memory and orders
-// launches on one HIP stream per object.
+// api.hip -- the C ABI of include/convector.h for meshes, scenes (GPU BVH
+// build + device upload) and batched nearest-hit queries (the renderer and
+// batches are in render.hip).  Every compute step runs in the kernels of
+// kernels.hip and build.hip; this file only parses input, owns device memory
+// and orders launches on one HIP stream per object.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdio.h>
@@ -15,15 +15,23 @@
 #include <vector>
 
 #include "../../include/convector.h"
+#include "host.hpp"
 #include "internal.hpp"
 
 using cvk::DevLight;
 using cvk::DevNode;
 using cvk::DevTri;
+using cvh::DevBuf;
+using cvh::fail;
+using cvh::grid_for;
+using cvh::hip_fail;
+
+namespace cvh {
 
 namespace {
-
 thread_local std::string g_last_error;
+uint32_t g_num_cus = 0;
+}  // namespace
 
 cv_status fail(cv_status s, const std::string& msg) {
   g_last_error = msg;
@@ -35,14 +43,6 @@ cv_status hip_fail(hipError_t e, const char* what) {
   return fail(e == hipErrorOutOfMemory ? CV_ERR_OUT_OF_MEMORY : CV_ERR_DEVICE, m);
 }
 
-#define HIPTRY(x, what)                   \
-  do {                                    \
-    hipError_t e_ = (x);                  \
-    if (e_ != hipSuccess) return hip_fail(e_, what); \
-  } while (0)
-
-int g_num_cus = 0;
-
 cv_status ensure_device() {
   if (g_num_cus > 0) return CV_OK;
   int n = 0;
@@ -52,29 +52,57 @@ cv_status ensure_device() {
   HIPTRY(hipGetDevice(&dev), "hipGetDevice");
   hipDeviceProp_t prop;
   HIPTRY(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
-  g_num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+  g_num_cus = prop.multiProcessorCount > 0 ? (uint32_t)prop.multiProcessorCount : 256u;
   return CV_OK;
 }
 
-template <typename T>
-struct DevBuf {
-  T* p = nullptr;
-  size_t n = 0;
-  DevBuf() = default;
-  DevBuf(const DevBuf&) = delete;
-  DevBuf& operator=(const DevBuf&) = delete;
-  ~DevBuf() { reset(); }
-  void reset() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    n = 0;
-  }
-  hipError_t alloc(size_t count) {
-    reset();
-    n = count;
-    return hipMalloc((void**)&p, (count ? count : 1) * sizeof(T));
-  }
-};
+uint32_t num_cus() { return g_num_cus ? g_num_cus : 256u; }
+
+const char* last_error() { return g_last_error.c_str(); }
+
+cv_status validate_tuning(const cv_tuning& t) {
+  if (t.stack_entries && t.stack_entries != 8 && t.stack_entries != 16 && t.stack_entries != 24 &&
+      t.stack_entries != 32)
+    return fail(CV_ERR_INVALID_ARGUMENT, "stack_entries must be 8, 16, 24 or 32");
+  if (t.extend_waves_per_cu > 64 || t.refill_threshold > 65 || t.fetch_batches > 64)
+    return fail(CV_ERR_INVALID_ARGUMENT, "tuning value out of range");
+  return CV_OK;
+}
+
+void apply_tuning(LaunchCfg& c, const cv_tuning& t) {
+  if (t.stack_entries) c.stack = (int)t.stack_entries;
+  if (t.extend_waves_per_cu) c.blocks = num_cus() * t.extend_waves_per_cu;
+  if (t.refill_threshold) c.refill = t.refill_threshold;
+  if (t.fetch_batches) c.fetch_batches = t.fetch_batches;
+}
+
+cv_tuning tuning_of(const LaunchCfg& c) {
+  cv_tuning t{};
+  t.stack_entries = (uint32_t)c.stack;
+  t.extend_waves_per_cu = c.blocks / num_cus();
+  t.refill_threshold = c.refill;
+  t.fetch_batches = c.fetch_batches;
+  return t;
+}
+
+cvk::JobDesc job_for_scene(cv_scene* s) {
+  cvk::JobDesc j{};
+  j.nodes = s->nodes.p;
+  j.tris = s->tris.p;
+  j.tri_mat = s->tri_mat.p;
+  j.tri_uv = s->tri_uv.p;
+  j.lights = s->lights.p;
+  j.nlights = s->ndirect;
+  j.has_override = s->has_override ? 1u : 0u;
+  memcpy(j.emission, s->emission, 12);
+  std::lock_guard<std::mutex> g(s->mu);
+  j.cam = s->cam;
+  return j;
+}
+
+}  // namespace cvh
+
+namespace {
 
 // ----------------------------------------------------------- kernels local
 
@@ -113,64 +141,7 @@ __global__ __launch_bounds__(1024) void k_direct_list(const uint32_t* mat, uint3
   if (threadIdx.x == 0) *count = base;
 }
 
-uint32_t grid_for(size_t n, uint32_t block, uint32_t cap = 1u << 20) {
-  size_t b = (n + block - 1) / block;
-  if (b < 1) b = 1;
-  if (b > cap) b = cap;
-  return (uint32_t)b;
-}
-
 }  // namespace
-
-// ------------------------------------------------------------------ objects
-
-struct cv_mesh {
-  std::vector<float> v9;
-  std::vector<float> uv6;
-  std::vector<uint32_t> mat;
-  size_t ntris() const { return mat.size(); }
-};
-
-struct cv_scene {
-  hipStream_t stream = nullptr;
-  uint32_t ntris = 0;
-  DevBuf<DevNode> nodes;
-  DevBuf<DevTri> tris;
-  DevBuf<uint32_t> tri_mat;
-  DevBuf<float> tri_uv;
-  DevBuf<uint32_t> tri_src;
-  DevBuf<uint32_t> direct;
-  uint32_t ndirect = 0;
-  cv_bvh_info info{};
-  bool packed = true;
-  int stack = 16;
-  cvk::CameraParams cam{};
-  bool has_override = false;
-  float emission[3] = {0, 0, 0};
-  DevBuf<uint2> spill;
-  uint32_t persistent_blocks = 0;
-  uint32_t refill = 16;
-  uint32_t fetch_batches = 1;
-  DevBuf<uint32_t> cursor;
-  std::mutex mu;
-};
-
-struct cv_renderer {
-  cv_scene* scene = nullptr;
-  uint32_t width = 0, height = 0;
-  cv_render_opts opts{};
-  hipStream_t stream = nullptr;
-  DevBuf<float4> r0[2], r1[2], col[2];
-  DevBuf<uint64_t> rng[2];
-  DevBuf<uint4> hits;
-  DevBuf<cvk::Ctrl> ctrl;
-  DevBuf<DevLight> lights;
-  DevBuf<float> hdr;
-  DevBuf<uint32_t> bitmap, gbuffer;
-  cv_render_stats stats{};
-  std::vector<hipEvent_t> events;
-  std::mutex mu;
-};
 
 // -------------------------------------------------------------- OBJ loader
 
@@ -225,7 +196,7 @@ void split_ws(const std::string& line, std::vector<std::string>& out) {
 
 extern "C" {
 
-const char* cv_last_error(void) { return g_last_error.c_str(); }
+const char* cv_last_error(void) { return cvh::last_error(); }
 
 const char* cv_version(void) { return "convector-mi355x 0.1 (gfx950)"; }
 
@@ -436,19 +407,13 @@ void cv_mesh_destroy(cv_mesh* mesh) { delete mesh; }
 // beyond the LDS part of every lane's stack (depth + 2 entries suffice: a pop
 // pushes at most two children).
 static cv_status scene_apply_tuning(cv_scene* s, const cv_tuning& t) {
-  if (t.stack_entries && t.stack_entries != 8 && t.stack_entries != 16 && t.stack_entries != 24 &&
-      t.stack_entries != 32)
-    return fail(CV_ERR_INVALID_ARGUMENT, "stack_entries must be 8, 16, 24 or 32");
-  if (t.extend_waves_per_cu > 64 || t.refill_threshold > 65 || t.fetch_batches > 64)
-    return fail(CV_ERR_INVALID_ARGUMENT, "tuning value out of range");
-  if (t.stack_entries) s->stack = (int)t.stack_entries;
-  if (t.extend_waves_per_cu) s->persistent_blocks = (uint32_t)g_num_cus * t.extend_waves_per_cu;
-  if (t.refill_threshold) s->refill = t.refill_threshold;
-  if (t.fetch_batches) s->fetch_batches = t.fetch_batches;
+  cv_status v = cvh::validate_tuning(t);
+  if (v != CV_OK) return v;
+  cvh::apply_tuning(s->cfg, t);
   uint32_t need = s->info.depth + 4;
-  size_t spill_entries = need > (uint32_t)s->stack ? (size_t)(need - s->stack) : 0;
+  size_t spill_entries = need > (uint32_t)s->cfg.stack ? (size_t)(need - s->cfg.stack) : 0;
   HIPTRY(hipStreamSynchronize(s->stream), "sync");
-  HIPTRY(s->spill.alloc(spill_entries * s->persistent_blocks * 64 + 1), "alloc spill");
+  HIPTRY(s->spill.alloc(spill_entries * s->cfg.blocks * 64 + 1), "alloc spill");
   return CV_OK;
 }
 
@@ -538,6 +503,10 @@ static cv_status scene_finish(cv_scene* s, const std::vector<float>& v9,
   HIPTRY(hipMemcpyAsync(&s->ndirect, s->direct.p + N, 4, hipMemcpyDeviceToHost, st), "download");
   HIPTRY(hipStreamSynchronize(st), "sync");
   s->info.num_direct = s->ndirect;
+  HIPTRY(s->lights.alloc(s->ndirect), "alloc");
+  HIPTRY(cvk::launch_prepare_lights(s->tris.p, s->direct.p, s->ndirect, s->lights.p, st),
+         "prepare lights");
+  HIPTRY(hipStreamSynchronize(st), "sync");
   // traversal configuration: packed stack entries need index < 2^27 and
   // leaves of at most 31 triangles
   s->packed = s->info.max_leaf <= 31 && N < (1u << 27) && s->info.num_nodes < (1u << 27);
@@ -551,7 +520,7 @@ static cv_status scene_create_impl(const cv_mesh* const* meshes, size_t n,
                                    size_t nnodes, const uint32_t* order, cv_scene** out) {
   if (!out || (n && !meshes)) return fail(CV_ERR_INVALID_ARGUMENT, "null argument");
   *out = nullptr;
-  cv_status ds = ensure_device();
+  cv_status ds = cvh::ensure_device();
   if (ds != CV_OK) return ds;
   if (opts && opts->builder != CV_BUILDER_REFERENCE)
     return fail(CV_ERR_INVALID_ARGUMENT, "unknown builder");
@@ -620,10 +589,7 @@ cv_status cv_scene_set_tuning(cv_scene* s, const cv_tuning* t) {
 
 cv_status cv_scene_get_tuning(const cv_scene* s, cv_tuning* t) {
   if (!s || !t) return fail(CV_ERR_INVALID_ARGUMENT, "null argument");
-  t->stack_entries = (uint32_t)s->stack;
-  t->extend_waves_per_cu = s->persistent_blocks / (uint32_t)g_num_cus;
-  t->refill_threshold = s->refill;
-  t->fetch_batches = s->fetch_batches;
+  *t = cvh::tuning_of(s->cfg);
   return CV_OK;
 }
 
@@ -652,6 +618,7 @@ void cv_scene_destroy(cv_scene* s) {
   s->tri_uv.reset();
   s->tri_src.reset();
   s->direct.reset();
+  s->lights.reset();
   s->spill.reset();
   s->cursor.reset();
   if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -700,7 +667,7 @@ cv_status cv_bvh_intersect_nearest(const cv_scene* sc, const cv_ray* rays, size_
   a.hits = hp;
   a.isects = ip;
   HIPTRY(cvk::launch_zero_u32(s->cursor.p, 1, st), "zero");
-  HIPTRY(cvk::launch_intersect(a, isects != nullptr, s->packed, s->stack, s->persistent_blocks, st),
+  HIPTRY(cvk::launch_intersect(a, isects != nullptr, s->packed, s->cfg.stack, s->cfg.blocks, st),
          "intersect");
   if (!dev) {
     if (hits) HIPTRY(hipMemcpyAsync(hits, hp, 16 * n, hipMemcpyDeviceToHost, st), "download");
@@ -710,328 +677,8 @@ cv_status cv_bvh_intersect_nearest(const cv_scene* sc, const cv_ray* rays, size_
   return CV_OK;
 }
 
-// ---------------------------------------------------------------- renderer
-
-cv_status cv_renderer_create(cv_scene* s, uint32_t width, uint32_t height,
-                             const cv_render_opts* opts, cv_renderer** out) {
-  if (!s || !out) return fail(CV_ERR_INVALID_ARGUMENT, "null argument");
-  *out = nullptr;
-  // The 16x4-block HDR layout needs width % 16 == 0 and height % 4 == 0
-  // (renderer.rs:366-374, 384-385); the reference's RenderBuffer also
-  // rounds heights to 16, which the per-pixel GPU path does not need.
-  if ((width & 15) || (height & 3) || width == 0 || height == 0)
-    return fail(CV_ERR_NOT_MULTIPLE_OF_16, "width must be a multiple of 16 and height of 4");
-  if (s->ndirect < 8)
-    return fail(CV_ERR_LIGHT_COUNT, "the scene has " + std::to_string(s->ndirect) +
-                                        " direct-sample triangles; direct sampling needs 8");
-  cv_render_opts o{CV_RNG_PIXEL, CV_APPROX_PORTABLE, 5};
-  if (opts) o = *opts;
-  if (o.max_bounces == 0) o.max_bounces = 5;
-  if (o.max_bounces >= (uint32_t)cvk::kMaxBounceSlots)
-    return fail(CV_ERR_INVALID_ARGUMENT, "max_bounces too large");
-  if (o.rng_mode != CV_RNG_PIXEL)
-    return fail(CV_ERR_INVALID_ARGUMENT, "the GPU renderer implements CV_RNG_PIXEL");
-  if (o.approx_mode != CV_APPROX_PORTABLE)
-    return fail(CV_ERR_INVALID_ARGUMENT, "the GPU renderer implements CV_APPROX_PORTABLE");
-  cv_renderer* r = new cv_renderer();
-  r->scene = s;
-  r->width = width;
-  r->height = height;
-  r->opts = o;
-  auto bail = [&](cv_status st) {
-    cv_renderer_destroy(r);
-    return st;
-  };
-  hipError_t e = hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking);
-  if (e != hipSuccess) return bail(hip_fail(e, "hipStreamCreate"));
-  const size_t npix = (size_t)width * height;
-  for (int k = 0; k < 2; k++) {
-    if ((e = r->r0[k].alloc(npix)) || (e = r->r1[k].alloc(npix)) || (e = r->col[k].alloc(npix)) ||
-        (e = r->rng[k].alloc(npix)))
-      return bail(hip_fail(e, "alloc queues"));
-  }
-  if ((e = r->hits.alloc(npix)) || (e = r->ctrl.alloc(1)) || (e = r->lights.alloc(s->ndirect)) ||
-      (e = r->hdr.alloc(npix * 3)) || (e = r->bitmap.alloc(npix)) || (e = r->gbuffer.alloc(npix)))
-    return bail(hip_fail(e, "alloc buffers"));
-  if ((e = cvk::launch_prepare_lights(s->tris.p, s->direct.p, s->ndirect, r->lights.p, r->stream)))
-    return bail(hip_fail(e, "prepare lights"));
-  if ((e = hipStreamSynchronize(r->stream))) return bail(hip_fail(e, "sync"));
-  *out = r;
-  return CV_OK;
-}
-
-void cv_renderer_destroy(cv_renderer* r) {
-  if (!r) return;
-  if (r->stream) (void)hipStreamSynchronize(r->stream);
-  for (hipEvent_t ev : r->events) (void)hipEventDestroy(ev);
-  for (int k = 0; k < 2; k++) {
-    r->r0[k].reset();
-    r->r1[k].reset();
-    r->col[k].reset();
-    r->rng[k].reset();
-  }
-  r->hits.reset();
-  r->ctrl.reset();
-  r->lights.reset();
-  r->hdr.reset();
-  r->bitmap.reset();
-  r->gbuffer.reset();
-  if (r->stream) (void)hipStreamDestroy(r->stream);
-  delete r;
-}
-
-namespace {
-
-struct Rect {
-  uint32_t x, y, w, h;
-};
-
-// One frame over a rectangle: generate, then extend + shade per bounce.
-cv_status render_rect(cv_renderer* r, const Rect& rc, uint32_t frame, float* hdr_dev,
-                      uint32_t* bitmap_dev, uint32_t* gbuffer_dev, uint32_t flags,
-                      std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>>* evs) {
-  cv_scene* s = r->scene;
-  hipStream_t st = r->stream;
-  auto ev_pair = [&](int kind) -> std::pair<hipEvent_t, hipEvent_t>* {
-    if (!evs) return nullptr;
-    hipEvent_t a, b;
-    if (hipEventCreate(&a) != hipSuccess) return nullptr;
-    if (hipEventCreate(&b) != hipSuccess) {
-      (void)hipEventDestroy(a);
-      return nullptr;
-    }
-    evs->push_back({kind, {a, b}});
-    return &evs->back().second;
-  };
-  cvk::GenerateArgs g{};
-  g.ctrl = r->ctrl.p;
-  g.q = cvk::Queue{r->r0[0].p, r->r1[0].p, r->col[0].p, r->rng[0].p};
-  g.width = r->width;
-  g.height = r->height;
-  g.rect_x = rc.x;
-  g.rect_y = rc.y;
-  g.rect_w = rc.w;
-  g.rect_h = rc.h;
-  g.frame = frame;
-  {
-    std::lock_guard<std::mutex> lk(s->mu);
-    g.cam = s->cam;
-  }
-  g.gbuffer = gbuffer_dev;
-  g.bitmap = bitmap_dev;
-  const size_t npix = (size_t)rc.w * rc.h;
-  auto* ep = ev_pair(0);
-  if (ep) (void)hipEventRecord(ep->first, st);
-  HIPTRY(cvk::launch_generate(g, grid_for(npix, 256, 8192), st), "generate");
-  if (ep) (void)hipEventRecord(ep->second, st);
-  const bool count = flags & CV_RENDER_COUNT;
-  for (uint32_t b = 0; b < r->opts.max_bounces; b++) {
-    const int cur = b & 1, nxt = cur ^ 1;
-    cvk::ExtendArgs x{};
-    x.ctrl = r->ctrl.p;
-    x.bounce = b;
-    x.r0 = r->r0[cur].p;
-    x.r1 = r->r1[cur].p;
-    x.hits = r->hits.p;
-    x.nodes = s->nodes.p;
-    x.tris = s->tris.p;
-    x.spill = s->spill.p;
-    x.batches = s->fetch_batches;
-    x.refill = s->refill <= 64 ? s->refill : 0u;
-    ep = ev_pair(1);
-    if (ep) (void)hipEventRecord(ep->first, st);
-    HIPTRY(cvk::launch_extend(x, count, s->packed, s->stack, s->persistent_blocks, st), "extend");
-    if (ep) (void)hipEventRecord(ep->second, st);
-    cvk::ShadeArgs sh{};
-    sh.ctrl = r->ctrl.p;
-    sh.bounce = b;
-    sh.max_bounces = r->opts.max_bounces;
-    sh.width = r->width;
-    sh.in = cvk::Queue{r->r0[cur].p, r->r1[cur].p, r->col[cur].p, r->rng[cur].p};
-    sh.out = cvk::Queue{r->r0[nxt].p, r->r1[nxt].p, r->col[nxt].p, r->rng[nxt].p};
-    sh.hits = r->hits.p;
-    sh.tris = s->tris.p;
-    sh.tri_mat = s->tri_mat.p;
-    sh.tri_uv = s->tri_uv.p;
-    sh.lights = r->lights.p;
-    sh.nlights = s->ndirect;
-    sh.has_override = s->has_override ? 1u : 0u;
-    memcpy(sh.emission, s->emission, 12);
-    sh.hdr = hdr_dev;
-    sh.bitmap = bitmap_dev;
-    sh.gbuffer = gbuffer_dev;
-    ep = ev_pair(2);
-    if (ep) (void)hipEventRecord(ep->first, st);
-    HIPTRY(cvk::launch_shade(sh, (uint32_t)npix, st), "shade");
-    if (ep) (void)hipEventRecord(ep->second, st);
-  }
-  return CV_OK;
-}
-
-cv_status run_frames(cv_renderer* r, const Rect& rc, uint32_t first, uint32_t nframes, float* hdr,
-                     uint32_t* bitmap, uint32_t* gbuffer, uint32_t flags, bool hdr_mode) {
-  hipStream_t st = r->stream;
-  const bool dev = flags & CV_PTR_DEVICE;
-  const uint32_t W = r->width;
-  // HDR block region of the rectangle (rc.x, rc.w multiples of 16; rc.y, rc.h of 4)
-  const size_t blk_row_bytes = (size_t)(W / 16) * 768;
-  const size_t rect_blk_off = (size_t)(rc.y / 4) * blk_row_bytes + (size_t)(rc.x / 16) * 768;
-  const size_t rect_blk_w = (size_t)(rc.w / 16) * 768;
-  float* hdr_dev = nullptr;
-  if (hdr_mode) {
-    hdr_dev = dev ? hdr : r->hdr.p;
-    if (!dev)
-      HIPTRY(hipMemcpy2DAsync((char*)r->hdr.p + rect_blk_off, blk_row_bytes,
-                              (const char*)hdr + rect_blk_off, blk_row_bytes, rect_blk_w, rc.h / 4,
-                              hipMemcpyHostToDevice, st),
-             "upload hdr");
-  }
-  uint32_t* bm_dev = nullptr;
-  if (bitmap) bm_dev = dev ? bitmap : r->bitmap.p;
-  uint32_t* gb_dev = nullptr;
-  if (gbuffer) gb_dev = dev ? gbuffer : r->gbuffer.p;
-  HIPTRY(cvk::launch_zero_u32(reinterpret_cast<uint32_t*>(&r->ctrl.p->sum_aabb), 6, st), "zero");
-  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> evs;
-  const bool timing = flags & CV_RENDER_TIMING;
-  hipEvent_t t0, t1;
-  HIPTRY(hipEventCreate(&t0), "event");
-  HIPTRY(hipEventCreate(&t1), "event");
-  HIPTRY(hipEventRecord(t0, st), "event");
-  cv_status status = CV_OK;
-  for (uint32_t f = 0; f < nframes && status == CV_OK; f++)
-    status = render_rect(r, rc, first + f, hdr_dev, bm_dev, gb_dev, flags, timing ? &evs : nullptr);
-  (void)hipEventRecord(t1, st);
-  if (status == CV_OK && !dev) {
-    const size_t row = (size_t)W * 4;
-    const size_t off = ((size_t)rc.y * W + rc.x) * 4;
-    if (hdr_mode)
-      status = hipMemcpy2DAsync((char*)hdr + rect_blk_off, blk_row_bytes,
-                                (const char*)r->hdr.p + rect_blk_off, blk_row_bytes, rect_blk_w,
-                                rc.h / 4, hipMemcpyDeviceToHost, st) == hipSuccess
-                   ? CV_OK
-                   : fail(CV_ERR_DEVICE, "download hdr");
-    if (status == CV_OK && bitmap)
-      status = hipMemcpy2DAsync((char*)bitmap + off, row, (const char*)r->bitmap.p + off, row,
-                                (size_t)rc.w * 4, rc.h, hipMemcpyDeviceToHost, st) == hipSuccess
-                   ? CV_OK
-                   : fail(CV_ERR_DEVICE, "download bitmap");
-    if (status == CV_OK && gbuffer)
-      status = hipMemcpy2DAsync((char*)gbuffer + off, row, (const char*)r->gbuffer.p + off, row,
-                                (size_t)rc.w * 4, rc.h, hipMemcpyDeviceToHost, st) == hipSuccess
-                   ? CV_OK
-                   : fail(CV_ERR_DEVICE, "download gbuffer");
-  }
-  hipError_t se = hipStreamSynchronize(st);
-  if (status == CV_OK && se != hipSuccess) status = hip_fail(se, "render");
-  cv_render_stats stt{};
-  stt.paths = (uint64_t)rc.w * rc.h * nframes;
-  float ms = 0.0f;
-  (void)hipEventElapsedTime(&ms, t0, t1);
-  stt.total_ms = ms;
-  for (auto& e : evs) {
-    float m = 0.0f;
-    (void)hipEventElapsedTime(&m, e.second.first, e.second.second);
-    if (e.first == 0) stt.generate_ms += m;
-    if (e.first == 1) {
-      stt.extend_ms += m;
-      stt.extend_launches++;
-    }
-    if (e.first == 2) stt.shade_ms += m;
-    (void)hipEventDestroy(e.second.first);
-    (void)hipEventDestroy(e.second.second);
-  }
-  (void)hipEventDestroy(t0);
-  (void)hipEventDestroy(t1);
-  if (!timing) stt.extend_launches = (uint64_t)nframes * r->opts.max_bounces;
-  if (status == CV_OK) {
-    cvk::Ctrl hc;
-    if (hipMemcpy(&hc, r->ctrl.p, sizeof hc, hipMemcpyDeviceToHost) == hipSuccess) {
-      stt.sum_numi_aabb = hc.sum_aabb;
-      stt.sum_numi_tri = hc.sum_tri;
-      stt.extend_rays = hc.total_rays;
-    }
-  }
-  r->stats = stt;
-  return status;
-}
-
-}  // namespace
-
-static cv_status patch_args(cv_renderer* r, uint32_t pw, uint32_t x, uint32_t y) {
-  if (!r) return fail(CV_ERR_INVALID_ARGUMENT, "null renderer");
-  if ((pw & 15) || pw == 0) return fail(CV_ERR_NOT_MULTIPLE_OF_16, "patch width must be a multiple of 16");
-  if ((x & 15) || (y & 15)) return fail(CV_ERR_NOT_MULTIPLE_OF_16, "patch origin must be a multiple of 16");
-  if (x + pw > r->width || y + pw > r->height) return fail(CV_ERR_OUT_OF_RANGE, "patch outside the frame");
-  return CV_OK;
-}
-
-cv_status cv_renderer_render_patch_u8(cv_renderer* r, uint32_t* bitmap, uint32_t* gbuffer,
-                                      uint32_t patch_width, uint32_t x, uint32_t y,
-                                      uint32_t frame, uint32_t flags) {
-  cv_status st = patch_args(r, patch_width, x, y);
-  if (st != CV_OK) return st;
-  if (!bitmap) return fail(CV_ERR_INVALID_ARGUMENT, "null bitmap");
-  std::lock_guard<std::mutex> g(r->mu);
-  return run_frames(r, Rect{x, y, patch_width, patch_width}, frame, 1, nullptr, bitmap, gbuffer,
-                    flags, false);
-}
-
-cv_status cv_renderer_accumulate_patch_f32(cv_renderer* r, float* hdr, uint32_t* gbuffer,
-                                           uint32_t patch_width, uint32_t x, uint32_t y,
-                                           uint32_t frame, uint32_t flags) {
-  cv_status st = patch_args(r, patch_width, x, y);
-  if (st != CV_OK) return st;
-  if (!hdr) return fail(CV_ERR_INVALID_ARGUMENT, "null hdr");
-  std::lock_guard<std::mutex> g(r->mu);
-  return run_frames(r, Rect{x, y, patch_width, patch_width}, frame, 1, hdr, nullptr, gbuffer,
-                    flags, true);
-}
-
-cv_status cv_renderer_render_frame_u8(cv_renderer* r, uint32_t* bitmap, uint32_t* gbuffer,
-                                      uint32_t frame, uint32_t flags) {
-  if (!r || !bitmap) return fail(CV_ERR_INVALID_ARGUMENT, "null argument");
-  std::lock_guard<std::mutex> g(r->mu);
-  return run_frames(r, Rect{0, 0, r->width, r->height}, frame, 1, nullptr, bitmap, gbuffer, flags,
-                    false);
-}
-
-cv_status cv_renderer_accumulate_frames_f32(cv_renderer* r, float* hdr, uint32_t* gbuffer,
-                                            uint32_t first_frame, uint32_t n_frames,
-                                            uint32_t flags) {
-  if (!r || !hdr) return fail(CV_ERR_INVALID_ARGUMENT, "null argument");
-  std::lock_guard<std::mutex> g(r->mu);
-  return run_frames(r, Rect{0, 0, r->width, r->height}, first_frame, n_frames, hdr, nullptr,
-                    gbuffer, flags, true);
-}
-
-cv_status cv_renderer_buffer_f32_into_bitmap(cv_renderer* r, const float* hdr, uint32_t* bitmap,
-                                             uint32_t num_samples, uint32_t flags) {
-  if (!r || !hdr || !bitmap) return fail(CV_ERR_INVALID_ARGUMENT, "null argument");
-  std::lock_guard<std::mutex> g(r->mu);
-  hipStream_t st = r->stream;
-  const bool dev = flags & CV_PTR_DEVICE;
-  const size_t npix = (size_t)r->width * r->height;
-  const float* hp = hdr;
-  uint32_t* bp = bitmap;
-  if (!dev) {
-    HIPTRY(hipMemcpyAsync(r->hdr.p, hdr, npix * 12, hipMemcpyHostToDevice, st), "upload");
-    hp = r->hdr.p;
-    bp = r->bitmap.p;
-  }
-  HIPTRY(cvk::launch_hdr_to_bitmap(hp, bp, r->width, r->height, num_samples, st), "convert");
-  if (!dev) HIPTRY(hipMemcpyAsync(bitmap, bp, npix * 4, hipMemcpyDeviceToHost, st), "download");
-  HIPTRY(hipStreamSynchronize(st), "sync");
-  return CV_OK;
-}
-
-cv_status cv_renderer_get_stats(const cv_renderer* r, cv_render_stats* stats) {
-  if (!r || !stats) return fail(CV_ERR_INVALID_ARGUMENT, "null argument");
-  *stats = r->stats;
-  return CV_OK;
-}
-
 cv_status cv_device_synchronize(void) {
-  cv_status ds = ensure_device();
+  cv_status ds = cvh::ensure_device();
   if (ds != CV_OK) return ds;
   HIPTRY(hipDeviceSynchronize(), "hipDeviceSynchronize");
   return CV_OK;

@@ -12,9 +12,12 @@ namespace cvk {
 constexpr int kMaxBounceSlots = 16;
 constexpr uint32_t kShadeBlock = 256;
 
-// Per-render control block in device memory: queue sizes and work cursors
+// Per-step control block in device memory: the step's path count and job
+// count (written by the host before each step), queue sizes and work cursors
 // per bounce, plus the optional traversal counters.
 struct Ctrl {
+  uint32_t step_paths;
+  uint32_t step_jobs;
   uint32_t count[kMaxBounceSlots];
   uint32_t ext_cursor[kMaxBounceSlots];
   uint32_t shade_cursor[kMaxBounceSlots];
@@ -23,9 +26,10 @@ struct Ctrl {
   unsigned long long total_rays;
 };
 
+// Paths in HBM as structure of arrays; every field is one 16-B or 8-B load.
 struct Queue {
   float4* r0;     // origin.xyz, pixel index bits
-  float4* r1;     // direction.xyz, unused
+  float4* r1;     // direction.xyz, job index bits
   float4* col;    // path throughput rgb
   uint64_t* rng;  // PIXEL stream state
 };
@@ -38,15 +42,33 @@ struct CameraParams {
   float screen_distance;
 };
 
-struct GenerateArgs {
-  Ctrl* ctrl;
-  Queue q;
+// One render job of a step: a resident scene, an image rectangle, a frame
+// number and the outputs.  A step renders one frame of each of its jobs, so
+// that a pixel receives at most one path per step and accumulation keeps
+// frame order.  Jobs are sorted by path_base.
+struct JobDesc {
+  const DevNode* nodes;
+  const DevTri* tris;
+  const uint32_t* tri_mat;
+  const float* tri_uv;
+  const DevLight* lights;
+  float* hdr;         // 16x4-block HDR accumulation, or null
+  uint32_t* bitmap;   // RGBA8 rows, or null
+  uint32_t* gbuffer;  // gbuffer rows, or null
+  CameraParams cam;
+  float emission[3];
+  uint32_t nlights;
+  uint32_t has_override;
   uint32_t width, height;
   uint32_t rect_x, rect_y, rect_w, rect_h;
   uint32_t frame;
-  CameraParams cam;
-  uint32_t* gbuffer;
-  uint32_t* bitmap;
+  uint32_t path_base;  // index of the job's first path in the step
+};
+
+struct GenerateArgs {
+  Ctrl* ctrl;
+  Queue q;
+  const JobDesc* jobs;
 };
 
 struct ExtendArgs {
@@ -55,8 +77,9 @@ struct ExtendArgs {
   const float4* r0;
   const float4* r1;
   uint4* hits;
-  const DevNode* nodes;
+  const DevNode* nodes;  // the scene when every job of the step shares one
   const DevTri* tris;
+  const JobDesc* jobs;   // per-ray scenes otherwise (job index in r1.w)
   uint2* spill;
   uint32_t batches;  // 64-ray batches per cursor fetch (batch mode)
   uint32_t refill;   // > 0: lane refill when this many lanes are idle
@@ -66,20 +89,10 @@ struct ShadeArgs {
   Ctrl* ctrl;
   uint32_t bounce;
   uint32_t max_bounces;
-  uint32_t width;
   Queue in;
   Queue out;
   const uint4* hits;
-  const DevTri* tris;
-  const uint32_t* tri_mat;
-  const float* tri_uv;
-  const DevLight* lights;
-  uint32_t nlights;
-  uint32_t has_override;
-  float emission[3];
-  float* hdr;
-  uint32_t* bitmap;
-  uint32_t* gbuffer;
+  const JobDesc* jobs;
 };
 
 struct IntersectArgs {
@@ -98,10 +111,13 @@ struct IntersectArgs {
 
 hipError_t launch_intersect(const IntersectArgs& a, bool count, bool packed, int stack,
                             uint32_t blocks, hipStream_t s);
-hipError_t launch_extend(const ExtendArgs& a, bool count, bool packed, int stack, uint32_t blocks,
-                         hipStream_t s);
+// multi: rays of the step belong to more than one scene (per-ray node and
+// triangle pointers from the job table).
+hipError_t launch_extend(const ExtendArgs& a, bool count, bool packed, bool multi, int stack,
+                         uint32_t blocks, hipStream_t s);
 hipError_t launch_generate(const GenerateArgs& a, uint32_t blocks, hipStream_t s);
-hipError_t launch_shade(const ShadeArgs& a, uint32_t max_paths, hipStream_t s);
+// multi: the step has more than one job.
+hipError_t launch_shade(const ShadeArgs& a, bool multi, uint32_t max_paths, hipStream_t s);
 hipError_t launch_hdr_to_bitmap(const float* hdr, uint32_t* bitmap, uint32_t width,
                                 uint32_t height, uint32_t num_samples, hipStream_t s);
 hipError_t launch_prepare_tris(const float* v9, const uint32_t* order, uint32_t ntris, DevTri* out,

@@ -4,9 +4,10 @@
 //
 // One ray per lane of a wave64.  Paths live in HBM as structure-of-arrays
 // queues; every bounce is an extend launch and a shade launch that appends
-// the surviving paths to the next queue with one atomic per wave.  Extend and
-// shade are persistent: each wave fetches 64 queue entries at a time from a
-// device cursor, so no host round trip is needed between launches.
+// the surviving paths to the next queue with one atomic per block.  Extend
+// is persistent: its waves take rays from a device cursor, so no host round
+// trip is needed between launches.  A step renders one frame of every job
+// in a device job table; the paths of all jobs share the queues.
 #include "device_math.hpp"
 #include "internal.hpp"
 
@@ -464,23 +465,40 @@ __device__ __forceinline__ size_t hdr_offset(uint32_t px, uint32_t py, uint32_t 
 // generate: camera rays (renderer.rs:171-206,413-416, scene.rs:81-99)
 // ---------------------------------------------------------------------------
 
+// Job of path p: the last job whose path_base <= p (jobs sorted by
+// path_base, the first at 0).
+__device__ __forceinline__ uint32_t job_of(const JobDesc* jobs, uint32_t njobs, uint32_t p) {
+  uint32_t lo = 0, hi = njobs;
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (jobs[mid].path_base <= p)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
 __global__ __launch_bounds__(256) void k_generate(GenerateArgs a) {
-  const uint32_t npix = a.rect_w * a.rect_h;
+  const uint32_t npaths = a.ctrl->step_paths, njobs = a.ctrl->step_jobs;
   if (blockIdx.x == 0 && threadIdx.x < kMaxBounceSlots) {
-    a.ctrl->count[threadIdx.x] = threadIdx.x == 0 ? npix : 0u;
+    a.ctrl->count[threadIdx.x] = threadIdx.x == 0 ? npaths : 0u;
     a.ctrl->ext_cursor[threadIdx.x] = 0u;
     a.ctrl->shade_cursor[threadIdx.x] = 0u;
   }
-  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < npix; p += gridDim.x * blockDim.x) {
-    uint32_t px = a.rect_x + p % a.rect_w;
-    uint32_t py = a.rect_y + p / a.rect_w;
-    uint32_t pixel = py * a.width + px;
+  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < npaths; p += gridDim.x * blockDim.x) {
+    const uint32_t j = job_of(a.jobs, njobs, p);
+    const JobDesc& J = a.jobs[j];
+    const uint32_t k = p - J.path_base;
+    uint32_t px = J.rect_x + k % J.rect_w;
+    uint32_t py = J.rect_y + k / J.rect_w;
+    uint32_t pixel = py * J.width + px;
     uint32_t xb = px & ~15u, yb = py & ~3u, dx = px & 15u, dy = py & 3u;
     uint32_t g = (dx >> 2) * 2 + (dy >> 1);
     uint32_t l = (dx & 3u) + 4 * (dy & 1u);
-    const float scale = 2.0f / (float)a.width;
-    const float cx = (float)xb - (float)a.width * 0.5f;
-    const float cy = (float)yb - (float)a.height * 0.5f;
+    const float scale = 2.0f / (float)J.width;
+    const float cx = (float)xb - (float)J.width * 0.5f;
+    const float cy = (float)yb - (float)J.height * 0.5f;
     float bx = scale * ((float)(l & 3u) + cx);
     float by = scale * ((float)(l >> 2) + cy);
     float xs = bx, ys = by;
@@ -489,19 +507,19 @@ __global__ __launch_bounds__(256) void k_generate(GenerateArgs a) {
     if (gx == 2) xs = bx + 8.0f * scale;
     if (gx == 3) xs = bx + 12.0f * scale;
     if (g & 1u) ys = by + 2.0f * scale;
-    uint64_t st = pixel_seed(pixel, a.frame);
+    uint64_t st = pixel_seed(pixel, J.frame);
     float x = fmaf(u32_unit(rng_next(st)), scale, xs);
     float y = fmaf(u32_unit(rng_next(st)), scale, ys);
     float t = u32_unit(rng_next(st));
-    V3 o = muladd3(mk3(a.cam.dpos[0], a.cam.dpos[1], a.cam.dpos[2]), t,
-                   mk3(a.cam.pos[0], a.cam.pos[1], a.cam.pos[2]));
-    V3 d = camera_dir(a.cam, x, y, t);
+    V3 o = muladd3(mk3(J.cam.dpos[0], J.cam.dpos[1], J.cam.dpos[2]), t,
+                   mk3(J.cam.pos[0], J.cam.pos[1], J.cam.pos[2]));
+    V3 d = camera_dir(J.cam, x, y, t);
     a.q.r0[p] = make_float4(o.x, o.y, o.z, u2f(pixel));
-    a.q.r1[p] = make_float4(d.x, d.y, d.z, 0.0f);
+    a.q.r1[p] = make_float4(d.x, d.y, d.z, u2f(j));
     a.q.col[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
     a.q.rng[p] = st;
-    if (a.gbuffer) a.gbuffer[pixel] = 0u;
-    if (a.bitmap) a.bitmap[pixel] = 0u;
+    if (J.gbuffer) J.gbuffer[pixel] = 0u;
+    if (J.bitmap) J.bitmap[pixel] = 0u;
   }
 }
 
@@ -561,7 +579,7 @@ __global__ __launch_bounds__(64) void k_extend(ExtendArgs a) {
 // the queue is empty.  A wave refills when at least `refill` lanes are idle
 // (one atomic for all of them).  Every ray is traversed exactly as by
 // traverse(), one trav_step per loop iteration.
-template <bool COUNT, int S, bool PACKED>
+template <bool COUNT, int S, bool PACKED, bool MULTI>
 __global__ __launch_bounds__(64) void k_extend_refill(ExtendArgs a) {
   __shared__ uint2 lds[S * 64];
   const uint32_t lane = lane_id();
@@ -574,6 +592,8 @@ __global__ __launch_bounds__(64) void k_extend_refill(ExtendArgs a) {
   Stack<S, PACKED> st(lds + lane, a.spill + gtid, stride, 0);
   TravState ts;
   V3 o = mk3(0, 0, 0), d = mk3(1, 0, 0), inv = mk3(1, 0, 0);
+  const DevNode* nodes = a.nodes;
+  const DevTri* tris = a.tris;
   uint32_t ray = 0;
   entry_t cur = 0;
   bool have = false;
@@ -591,6 +611,11 @@ __global__ __launch_bounds__(64) void k_extend_refill(ExtendArgs a) {
           float4 r0 = a.r0[i], r1 = a.r1[i];
           o = mk3(r0.x, r0.y, r0.z);
           d = mk3(r1.x, r1.y, r1.z);
+          if (MULTI) {
+            const JobDesc& J = a.jobs[f2u(r1.w)];
+            nodes = J.nodes;
+            tris = J.tris;
+          }
           inv = mk3(rcp(d.x), rcp(d.y), rcp(d.z));
           exact = needs_exact_nan(o, inv);
           ts.best_t = 1.0e5f;
@@ -613,9 +638,9 @@ __global__ __launch_bounds__(64) void k_extend_refill(ExtendArgs a) {
     const bool wave_exact = __any(active && exact);
     if (active) {
       if (wave_exact)
-        trav_step<true, COUNT>(a.nodes, a.tris, o, d, inv, st, ts, cur, have);
+        trav_step<true, COUNT>(nodes, tris, o, d, inv, st, ts, cur, have);
       else
-        trav_step<false, COUNT>(a.nodes, a.tris, o, d, inv, st, ts, cur, have);
+        trav_step<false, COUNT>(nodes, tris, o, d, inv, st, ts, cur, have);
       if (!have && st.sp == 0) {
         a.hits[ray] = make_uint4(ts.best, f2u(ts.best_t), f2u(ts.bu), f2u(ts.bv));
         if (COUNT) {
@@ -648,7 +673,7 @@ struct ShadeOut {
   float fresnel;
 };
 
-__device__ __forceinline__ ShadeOut continue_path(const ShadeArgs& a, V3 ray_d, V3 pos, V3 normal,
+__device__ __forceinline__ ShadeOut continue_path(const JobDesc& J, V3 ray_d, V3 pos, V3 normal,
                                                   uint32_t mat, uint64_t& st,
                                                   bool ignore_fresnel) {
   const uint32_t d_angle = rng_next(st), d_unit = rng_next(st);
@@ -661,7 +686,7 @@ __device__ __forceinline__ ShadeOut continue_path(const ShadeArgs& a, V3 ray_d, 
   V3 dir_b = rotate_hemisphere(dir_z, normal);
   V3 org_b = muladd3(dir_b, 1.0e-4f, pos);
   // continue_path_direct_sample + Scene::get_direct_sample (scene.rs:143-202)
-  const DevLight& L = a.lights[d_bits >> 29];
+  const DevLight& L = J.lights[d_bits >> 29];
   float su = u32_unit(d_u), sv = u32_unit(d_v);
   uint32_t sm = f2u((1.0f - su) - sv);
   float uu = pickf(su, 1.0f - su, sm);
@@ -680,8 +705,9 @@ __device__ __forceinline__ ShadeOut continue_path(const ShadeArgs& a, V3 ray_d, 
   float pd_b = xmax(dot3(normal, nd), 0.0f) * (1.0f / 3.14159274f);
   // pd_direct_sample (material.rs:281-320) over every light in BVH order
   float total = 0.0f;
-  for (uint32_t k = 0; k < a.nlights; k++) {
-    const DevLight& M = a.lights[k];
+  const uint32_t nlights = J.nlights;
+  for (uint32_t k = 0; k < nlights; k++) {
+    const DevLight& M = J.lights[k];
     V3 m0 = mk3(M.v0[0], M.v0[1], M.v0[2]);
     V3 mn = mk3(M.n[0], M.n[1], M.n[2]);
     V3 me1 = mk3(M.e1[0], M.e1[1], M.e1[2]);
@@ -699,7 +725,7 @@ __device__ __forceinline__ ShadeOut continue_path(const ShadeArgs& a, V3 ray_d, 
     float pd = dist_sqr * rcp(M.area * dot_em);
     total = pickf(total + pd, total, mask);
   }
-  float pd_d = total * (1.0f / (float)a.nlights);
+  float pd_d = total * (1.0f / (float)nlights);
   float weight_denom = (pd_b + pd_d) + 0.01f;
   float p_brdf = fmaf(normal.z, 0.4f, 0.5f);
   float p_direct = fmaf(-normal.z, 0.4f, 0.5f);
@@ -749,7 +775,8 @@ __device__ __forceinline__ ShadeOut continue_path(const ShadeArgs& a, V3 ray_d, 
 
 // One path per thread; blocks beyond the queue exit at once.  Survivors are
 // appended with one atomic per block: ballots per wave, a prefix over the
-// block's waves in LDS.
+// block's waves in LDS.  MULTI == false: every path belongs to job 0.
+template <bool MULTI>
 __global__ __launch_bounds__(kShadeBlock) void k_shade(ShadeArgs a) {
   __shared__ uint32_t wcount[kShadeBlock / 64];
   __shared__ uint32_t block_base;
@@ -760,35 +787,38 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(ShadeArgs a) {
   const uint32_t i = blockIdx.x * kShadeBlock + threadIdx.x;
   const bool valid = i < n;
   bool keep = false;
-  float4 r0 = make_float4(0, 0, 0, 0);
+  float4 r0 = make_float4(0, 0, 0, 0), r1 = make_float4(0, 0, 0, 0);
   uint64_t st = 0;
   ShadeOut so;
   if (valid) {
     r0 = a.in.r0[i];
-    float4 r1 = a.in.r1[i];
+    r1 = a.in.r1[i];
     float4 col = a.in.col[i];
     st = a.in.rng[i];
     uint4 hit = a.hits[i];
+    const JobDesc& J = a.jobs[MULTI ? f2u(r1.w) : 0u];
     V3 o = mk3(r0.x, r0.y, r0.z), d = mk3(r1.x, r1.y, r1.z);
     V3 color = mk3(col.x, col.y, col.z);
     uint32_t pixel = f2u(r0.w);
-    uint32_t px = pixel % a.width, py = pixel / a.width;
+    const uint32_t width = J.width;
+    uint32_t px = pixel % width, py = pixel / width;
     float t = u2f(hit.y);
     V3 pos, normal;
     uint32_t mat;
     float tu = 0.0f, tv = 0.0f;
+    uint32_t* gbuffer = a.bounce == 0 ? J.gbuffer : nullptr;
     if (hit.x == kNoHit) {
       pos = muladd3(d, 1.0e5f, o);
       normal = d;
       mat = kMatSky;
     } else {
-      const float4* tp = reinterpret_cast<const float4*>(a.tris + hit.x);
+      const float4* tp = reinterpret_cast<const float4*>(J.tris + hit.x);
       V3 nn = mk3(tp[0].w, tp[1].w, tp[2].w);
       pos = muladd3(d, t, o);
       normal = normalized3(nn);
-      mat = a.tri_mat[hit.x];
-      if (a.bounce == 0 && a.gbuffer) {
-        const float* uv = a.tri_uv + 6 * (size_t)hit.x;
+      mat = J.tri_mat[hit.x];
+      if (gbuffer) {
+        const float* uv = J.tri_uv + 6 * (size_t)hit.x;
         float bu = u2f(hit.z), bv = u2f(hit.w);
         float w = (1.0f - bu) - bv;
         tu = fmaf(uv[0], w, fmaf(uv[2], bv, uv[4] * bu));
@@ -797,22 +827,22 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(ShadeArgs a) {
     }
     if (mat >> 31) {
       // The path found an emitter: colour x emission (renderer.rs:452-460).
-      V3 em = (a.has_override && hit.x != kNoHit)
-                  ? mk3(a.emission[0], a.emission[1], a.emission[2])
+      V3 em = (J.has_override && hit.x != kNoHit)
+                  ? mk3(J.emission[0], J.emission[1], J.emission[2])
                   : sky_intensity(d);
       V3 c = mulc3(color, em);
-      if (a.hdr) {
-        float* h = a.hdr + hdr_offset(px, py, a.width);
+      if (J.hdr) {
+        float* h = J.hdr + hdr_offset(px, py, width);
         h[0] += c.x;
         h[8] += c.y;
         h[16] += c.z;
       }
-      if (a.bitmap) a.bitmap[pixel] = pack_color(c);
-    } else if (!last || (a.bounce == 0 && a.gbuffer)) {
-      so = continue_path(a, d, pos, normal, mat, st, a.bounce == 0);
-      if (a.bounce == 0 && a.gbuffer) {
+      if (J.bitmap) J.bitmap[pixel] = pack_color(c);
+    } else if (!last || gbuffer) {
+      so = continue_path(J, d, pos, normal, mat, st, a.bounce == 0);
+      if (gbuffer) {
         uint32_t tex_index = ((int32_t)mat >> 24) & 3;
-        a.gbuffer[pixel] = pack_gbuffer(tex_index, tu, tv, so.fresnel);
+        gbuffer[pixel] = pack_gbuffer(tex_index, tu, tv, so.fresnel);
       }
       if (!last) {
         keep = true;
@@ -834,7 +864,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(ShadeArgs a) {
     for (uint32_t w = 0; w < wave; w++) j += wcount[w];
     j += __popcll(ballot & ((1ull << lane) - 1ull));
     a.out.r0[j] = make_float4(so.o.x, so.o.y, so.o.z, r0.w);
-    a.out.r1[j] = make_float4(so.d.x, so.d.y, so.d.z, 0.0f);
+    a.out.r1[j] = make_float4(so.d.x, so.d.y, so.d.z, r1.w);
     a.out.col[j] = make_float4(so.cm.x, so.cm.y, so.cm.z, 0.0f);
     a.out.rng[j] = st;
   }
@@ -928,13 +958,19 @@ hipError_t launch_intersect(const IntersectArgs& a, bool count, bool packed, int
 }
 
 template <bool COUNT, int S>
-static hipError_t launch_extend_s(const ExtendArgs& a, bool packed, uint32_t blocks,
+static hipError_t launch_extend_s(const ExtendArgs& a, bool packed, bool multi, uint32_t blocks,
                                   hipStream_t s) {
-  if (a.refill) {
+  if (multi) {
     if (packed)
-      hipLaunchKernelGGL((k_extend_refill<COUNT, S, true>), dim3(blocks), dim3(64), 0, s, a);
+      hipLaunchKernelGGL((k_extend_refill<COUNT, S, true, true>), dim3(blocks), dim3(64), 0, s, a);
     else
-      hipLaunchKernelGGL((k_extend_refill<COUNT, S, false>), dim3(blocks), dim3(64), 0, s, a);
+      hipLaunchKernelGGL((k_extend_refill<COUNT, S, false, true>), dim3(blocks), dim3(64), 0, s, a);
+  } else if (a.refill) {
+    if (packed)
+      hipLaunchKernelGGL((k_extend_refill<COUNT, S, true, false>), dim3(blocks), dim3(64), 0, s, a);
+    else
+      hipLaunchKernelGGL((k_extend_refill<COUNT, S, false, false>), dim3(blocks), dim3(64), 0, s,
+                         a);
   } else if (packed) {
     hipLaunchKernelGGL((k_extend<COUNT, S, true>), dim3(blocks), dim3(64), 0, s, a);
   } else {
@@ -943,12 +979,13 @@ static hipError_t launch_extend_s(const ExtendArgs& a, bool packed, uint32_t blo
   return hipGetLastError();
 }
 
-hipError_t launch_extend(const ExtendArgs& a, bool count, bool packed, int stack, uint32_t blocks,
-                         hipStream_t s) {
-#define CASE(S)                                                              \
-  if (stack == S)                                                            \
-    return count ? launch_extend_s<true, S>(a, packed, blocks, s)            \
-                 : launch_extend_s<false, S>(a, packed, blocks, s);
+hipError_t launch_extend(const ExtendArgs& a, bool count, bool packed, bool multi, int stack,
+                         uint32_t blocks, hipStream_t s) {
+  if (multi && !a.refill) return hipErrorInvalidValue;
+#define CASE(S)                                                                     \
+  if (stack == S)                                                                   \
+    return count ? launch_extend_s<true, S>(a, packed, multi, blocks, s)            \
+                 : launch_extend_s<false, S>(a, packed, multi, blocks, s);
   CVK_STACK_VARIANTS(CASE)
 #undef CASE
   return hipErrorInvalidValue;
@@ -959,10 +996,13 @@ hipError_t launch_generate(const GenerateArgs& a, uint32_t blocks, hipStream_t s
   return hipGetLastError();
 }
 
-hipError_t launch_shade(const ShadeArgs& a, uint32_t max_paths, hipStream_t s) {
+hipError_t launch_shade(const ShadeArgs& a, bool multi, uint32_t max_paths, hipStream_t s) {
   uint32_t blocks = (max_paths + kShadeBlock - 1) / kShadeBlock;
   if (blocks == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_shade, dim3(blocks), dim3(kShadeBlock), 0, s, a);
+  if (multi)
+    hipLaunchKernelGGL(k_shade<true>, dim3(blocks), dim3(kShadeBlock), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_shade<false>, dim3(blocks), dim3(kShadeBlock), 0, s, a);
   return hipGetLastError();
 }
 

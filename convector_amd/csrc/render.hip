@@ -1,0 +1,765 @@
+// render.hip -- the wavefront pipeline and the two C-ABI objects built on it:
+// cv_renderer (the reference's Renderer: one job per step, a frame or a
+// patch) and cv_batch (many jobs of different scenes, sizes and sample
+// counts sharing every launch).
+//
+// A step renders one frame of each of its jobs: generate, then per bounce
+// extend + shade over queues that hold the paths of all jobs.  The step's
+// job table and path count are copied to the device before its launches, so
+// the launch sequence of a step is the same for every step and can be
+// captured once as a HIP graph and replayed.
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/convector.h"
+#include "host.hpp"
+#include "internal.hpp"
+
+using cvh::DevBuf;
+using cvh::fail;
+using cvh::HostBuf;
+using cvh::hip_fail;
+
+namespace {
+
+constexpr uint32_t kRingSlots = 8;
+
+struct StepPlan {
+  std::vector<cvk::JobDesc> jobs;  // path_base filled, sorted
+  uint32_t paths = 0;
+};
+
+struct RunOpts {
+  cvh::LaunchCfg cfg;
+  uint32_t max_bounces = 5;
+  uint32_t flags = 0;
+  bool packed = true;
+  const cvk::DevNode* nodes = nullptr;  // shared scene, or null for per-job scenes
+  const cvk::DevTri* tris = nullptr;
+  bool multi_job = false;
+  uint32_t max_depth = 0;
+  bool graph = false;
+};
+
+struct GraphKey {
+  int stack = 0;
+  uint32_t blocks = 0, refill = 0, fetch_batches = 0, max_bounces = 0;
+  bool packed = false, count = false, multi_job = false;
+  const void* nodes = nullptr;
+  const void* tris = nullptr;
+  size_t capacity = 0;
+  bool operator==(const GraphKey& o) const {
+    return stack == o.stack && blocks == o.blocks && refill == o.refill &&
+           fetch_batches == o.fetch_batches && max_bounces == o.max_bounces &&
+           packed == o.packed && count == o.count && multi_job == o.multi_job &&
+           nodes == o.nodes && tris == o.tris && capacity == o.capacity;
+  }
+};
+
+// Header of a step upload, copied over the first words of Ctrl.
+struct StepHeader {
+  uint32_t paths;
+  uint32_t jobs;
+};
+
+}  // namespace
+
+// Queues, control block, job table and the staging of one pipeline.  Owned
+// by a renderer or a batch; used by one call at a time (the owner's mutex).
+struct Pipeline {
+  hipStream_t stream = nullptr;
+  size_t capacity = 0;  // paths per step the queues hold
+  DevBuf<float4> r0[2], r1[2], col[2];
+  DevBuf<uint64_t> rng[2];
+  DevBuf<uint4> hits;
+  DevBuf<cvk::Ctrl> ctrl;
+  DevBuf<cvk::JobDesc> jobs[kRingSlots];
+  size_t max_jobs = 0;
+  HostBuf<StepHeader> head[kRingSlots];
+  HostBuf<cvk::JobDesc> stage[kRingSlots];
+  hipEvent_t slot_done[kRingSlots] = {};
+  uint32_t next_slot = 0;
+  DevBuf<uint2> spill;
+  // one captured step per job-table slot (the slot's table address is part
+  // of the recorded launches)
+  hipGraphExec_t graphs[kRingSlots] = {};
+  hipGraph_t graph_defs[kRingSlots] = {};
+  GraphKey keys[kRingSlots];
+
+  ~Pipeline() { release(); }
+
+  void release() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    drop_graphs();
+    for (uint32_t k = 0; k < kRingSlots; k++) {
+      if (slot_done[k]) (void)hipEventDestroy(slot_done[k]);
+      slot_done[k] = nullptr;
+      jobs[k].reset();
+      head[k].reset();
+      stage[k].reset();
+    }
+    for (int k = 0; k < 2; k++) {
+      r0[k].reset();
+      r1[k].reset();
+      col[k].reset();
+      rng[k].reset();
+    }
+    hits.reset();
+    ctrl.reset();
+    spill.reset();
+    if (stream) (void)hipStreamDestroy(stream);
+    stream = nullptr;
+  }
+
+  void drop_graphs() {
+    for (uint32_t k = 0; k < kRingSlots; k++) {
+      if (graphs[k]) (void)hipGraphExecDestroy(graphs[k]);
+      if (graph_defs[k]) (void)hipGraphDestroy(graph_defs[k]);
+      graphs[k] = nullptr;
+      graph_defs[k] = nullptr;
+    }
+  }
+
+  cv_status init() {
+    HIPTRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+    HIPTRY(ctrl.alloc(1), "alloc ctrl");
+    for (uint32_t k = 0; k < kRingSlots; k++) {
+      HIPTRY(hipEventCreateWithFlags(&slot_done[k], hipEventDisableTiming), "event");
+      HIPTRY(head[k].reserve(1), "pinned alloc");
+    }
+    return CV_OK;
+  }
+
+  cv_status reserve(size_t paths, size_t njobs) {
+    if (paths > capacity) {
+      HIPTRY(hipStreamSynchronize(stream), "sync");
+      drop_graphs();
+      for (int k = 0; k < 2; k++) {
+        HIPTRY(r0[k].alloc(paths), "alloc queues");
+        HIPTRY(r1[k].alloc(paths), "alloc queues");
+        HIPTRY(col[k].alloc(paths), "alloc queues");
+        HIPTRY(rng[k].alloc(paths), "alloc queues");
+      }
+      HIPTRY(hits.alloc(paths), "alloc hits");
+      capacity = paths;
+    }
+    if (njobs > max_jobs) {
+      HIPTRY(hipStreamSynchronize(stream), "sync");
+      drop_graphs();
+      for (uint32_t k = 0; k < kRingSlots; k++) {
+        HIPTRY(jobs[k].alloc(njobs), "alloc job table");
+        HIPTRY(stage[k].reserve(njobs), "pinned alloc");
+      }
+      max_jobs = njobs;
+    }
+    return CV_OK;
+  }
+
+  cv_status reserve_spill(const cvh::LaunchCfg& cfg, uint32_t max_depth) {
+    uint32_t need = max_depth + 4;
+    size_t per_lane = need > (uint32_t)cfg.stack ? (size_t)(need - cfg.stack) : 0;
+    size_t entries = per_lane * cfg.blocks * 64 + 1;
+    if (spill.n < entries) {
+      HIPTRY(hipStreamSynchronize(stream), "sync");
+      drop_graphs();
+      HIPTRY(spill.alloc(entries), "alloc spill");
+    }
+    return CV_OK;
+  }
+};
+
+namespace {
+
+typedef std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> EventList;
+
+// The launches of one step (generate, then extend + shade per bounce) for
+// job-table slot `slot`.  With graph == true every grid is sized for the
+// pipeline's capacity, so that the recorded launches serve any step.
+cv_status launch_step(Pipeline& p, const RunOpts& o, uint32_t slot, uint32_t paths, bool graph,
+                      EventList* evs) {
+  hipStream_t st = p.stream;
+  auto ev_pair = [&](int kind) -> std::pair<hipEvent_t, hipEvent_t>* {
+    if (!evs) return nullptr;
+    hipEvent_t a, b;
+    if (hipEventCreate(&a) != hipSuccess) return nullptr;
+    if (hipEventCreate(&b) != hipSuccess) {
+      (void)hipEventDestroy(a);
+      return nullptr;
+    }
+    evs->push_back({kind, {a, b}});
+    return &evs->back().second;
+  };
+  const size_t grid_paths = graph ? p.capacity : paths;
+  const cvk::JobDesc* jobs = p.jobs[slot].p;
+  cvk::GenerateArgs g{};
+  g.ctrl = p.ctrl.p;
+  g.q = cvk::Queue{p.r0[0].p, p.r1[0].p, p.col[0].p, p.rng[0].p};
+  g.jobs = jobs;
+  auto* ep = ev_pair(0);
+  if (ep) (void)hipEventRecord(ep->first, st);
+  HIPTRY(cvk::launch_generate(g, cvh::grid_for(grid_paths, 256, 8192), st), "generate");
+  if (ep) (void)hipEventRecord(ep->second, st);
+  const bool count = o.flags & CV_RENDER_COUNT;
+  const bool multi_scene = o.nodes == nullptr;
+  for (uint32_t b = 0; b < o.max_bounces; b++) {
+    const int cur = b & 1, nxt = cur ^ 1;
+    cvk::ExtendArgs x{};
+    x.ctrl = p.ctrl.p;
+    x.bounce = b;
+    x.r0 = p.r0[cur].p;
+    x.r1 = p.r1[cur].p;
+    x.hits = p.hits.p;
+    x.nodes = o.nodes;
+    x.tris = o.tris;
+    x.jobs = jobs;
+    x.spill = p.spill.p;
+    x.batches = o.cfg.fetch_batches;
+    x.refill = o.cfg.refill <= 64 ? o.cfg.refill : (multi_scene ? 64u : 0u);
+    ep = ev_pair(1);
+    if (ep) (void)hipEventRecord(ep->first, st);
+    HIPTRY(cvk::launch_extend(x, count, o.packed, multi_scene, o.cfg.stack, o.cfg.blocks, st),
+           "extend");
+    if (ep) (void)hipEventRecord(ep->second, st);
+    cvk::ShadeArgs sh{};
+    sh.ctrl = p.ctrl.p;
+    sh.bounce = b;
+    sh.max_bounces = o.max_bounces;
+    sh.in = cvk::Queue{p.r0[cur].p, p.r1[cur].p, p.col[cur].p, p.rng[cur].p};
+    sh.out = cvk::Queue{p.r0[nxt].p, p.r1[nxt].p, p.col[nxt].p, p.rng[nxt].p};
+    sh.hits = p.hits.p;
+    sh.jobs = jobs;
+    ep = ev_pair(2);
+    if (ep) (void)hipEventRecord(ep->first, st);
+    HIPTRY(cvk::launch_shade(sh, o.multi_job, (uint32_t)grid_paths, st), "shade");
+    if (ep) (void)hipEventRecord(ep->second, st);
+  }
+  return CV_OK;
+}
+
+// Runs the steps in order on the pipeline's stream and waits for them.
+cv_status pipeline_run(Pipeline& p, const std::vector<StepPlan>& steps, const RunOpts& o,
+                       cv_render_stats* out) {
+  size_t max_paths = 0, max_jobs = 0;
+  for (const StepPlan& s : steps) {
+    max_paths = std::max<size_t>(max_paths, s.paths);
+    max_jobs = std::max(max_jobs, s.jobs.size());
+  }
+  cv_status cs = p.reserve(max_paths, max_jobs);
+  if (cs != CV_OK) return cs;
+  cs = p.reserve_spill(o.cfg, o.max_depth);
+  if (cs != CV_OK) return cs;
+  hipStream_t st = p.stream;
+  const bool timing = o.flags & CV_RENDER_TIMING;
+  const bool use_graph = o.graph && !timing;
+  HIPTRY(cvk::launch_zero_u32(reinterpret_cast<uint32_t*>(&p.ctrl.p->sum_aabb), 6, st), "zero");
+  GraphKey key;
+  key.stack = o.cfg.stack;
+  key.blocks = o.cfg.blocks;
+  key.refill = o.cfg.refill;
+  key.fetch_batches = o.cfg.fetch_batches;
+  key.max_bounces = o.max_bounces;
+  key.packed = o.packed;
+  key.count = o.flags & CV_RENDER_COUNT;
+  key.multi_job = o.multi_job;
+  key.nodes = o.nodes;
+  key.tris = o.tris;
+  key.capacity = p.capacity;
+  EventList evs;
+  hipEvent_t t0, t1;
+  HIPTRY(hipEventCreate(&t0), "event");
+  HIPTRY(hipEventCreate(&t1), "event");
+  HIPTRY(hipEventRecord(t0, st), "event");
+  cv_status status = CV_OK;
+  for (const StepPlan& s : steps) {
+    if (s.paths == 0) continue;
+    const uint32_t slot = p.next_slot;
+    p.next_slot = (p.next_slot + 1) % kRingSlots;
+    // the slot's previous upload must have been read before it is rewritten
+    if (hipEventSynchronize(p.slot_done[slot]) != hipSuccess) {
+      status = fail(CV_ERR_DEVICE, "event");
+      break;
+    }
+    p.head[slot].p[0] = StepHeader{s.paths, (uint32_t)s.jobs.size()};
+    memcpy(p.stage[slot].p, s.jobs.data(), s.jobs.size() * sizeof(cvk::JobDesc));
+    if (hipMemcpyAsync(p.ctrl.p, p.head[slot].p, sizeof(StepHeader), hipMemcpyHostToDevice, st) ||
+        hipMemcpyAsync(p.jobs[slot].p, p.stage[slot].p, s.jobs.size() * sizeof(cvk::JobDesc),
+                       hipMemcpyHostToDevice, st) ||
+        hipEventRecord(p.slot_done[slot], st)) {
+      status = fail(CV_ERR_DEVICE, "step upload");
+      break;
+    }
+    if (use_graph) {
+      if (!p.graphs[slot] || !(p.keys[slot] == key)) {
+        if (p.graphs[slot]) (void)hipGraphExecDestroy(p.graphs[slot]);
+        if (p.graph_defs[slot]) (void)hipGraphDestroy(p.graph_defs[slot]);
+        p.graphs[slot] = nullptr;
+        p.graph_defs[slot] = nullptr;
+        HIPTRY(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed), "graph capture");
+        cv_status ls = launch_step(p, o, slot, s.paths, true, nullptr);
+        hipGraph_t gdef = nullptr;
+        hipError_t ce = hipStreamEndCapture(st, &gdef);
+        if (ls != CV_OK) return ls;
+        HIPTRY(ce, "graph capture");
+        p.graph_defs[slot] = gdef;
+        HIPTRY(hipGraphInstantiate(&p.graphs[slot], gdef, nullptr, nullptr, 0), "graph instantiate");
+        p.keys[slot] = key;
+      }
+      HIPTRY(hipGraphLaunch(p.graphs[slot], st), "graph launch");
+    } else {
+      status = launch_step(p, o, slot, s.paths, false, timing ? &evs : nullptr);
+      if (status != CV_OK) break;
+    }
+  }
+  (void)hipEventRecord(t1, st);
+  hipError_t se = hipStreamSynchronize(st);
+  if (status == CV_OK && se != hipSuccess) status = hip_fail(se, "render");
+  cv_render_stats stt{};
+  uint32_t nsteps = 0;
+  for (const StepPlan& s : steps) {
+    stt.paths += s.paths;
+    nsteps += s.paths ? 1 : 0;
+  }
+  float ms = 0.0f;
+  (void)hipEventElapsedTime(&ms, t0, t1);
+  stt.total_ms = ms;
+  for (auto& e : evs) {
+    float m = 0.0f;
+    (void)hipEventElapsedTime(&m, e.second.first, e.second.second);
+    if (e.first == 0) stt.generate_ms += m;
+    if (e.first == 1) {
+      stt.extend_ms += m;
+      stt.extend_launches++;
+    }
+    if (e.first == 2) stt.shade_ms += m;
+    (void)hipEventDestroy(e.second.first);
+    (void)hipEventDestroy(e.second.second);
+  }
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  if (!timing) stt.extend_launches = (uint64_t)nsteps * o.max_bounces;
+  if (status == CV_OK) {
+    cvk::Ctrl hc;
+    if (hipMemcpy(&hc, p.ctrl.p, sizeof hc, hipMemcpyDeviceToHost) == hipSuccess) {
+      stt.sum_numi_aabb = hc.sum_aabb;
+      stt.sum_numi_tri = hc.sum_tri;
+      stt.extend_rays = hc.total_rays;
+    }
+  }
+  if (out) *out = stt;
+  return status;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- renderer
+
+struct cv_renderer {
+  cv_scene* scene = nullptr;
+  uint32_t width = 0, height = 0;
+  cv_render_opts opts{};
+  Pipeline pipe;
+  DevBuf<float> hdr;
+  DevBuf<uint32_t> bitmap, gbuffer;
+  cv_render_stats stats{};
+  std::mutex mu;
+};
+
+namespace {
+
+cv_status check_opts(const cv_render_opts* opts, cv_render_opts* o) {
+  *o = cv_render_opts{CV_RNG_PIXEL, CV_APPROX_PORTABLE, 5};
+  if (opts) *o = *opts;
+  if (o->max_bounces == 0) o->max_bounces = 5;
+  if (o->max_bounces >= (uint32_t)cvk::kMaxBounceSlots)
+    return fail(CV_ERR_INVALID_ARGUMENT, "max_bounces too large");
+  if (o->rng_mode != CV_RNG_PIXEL)
+    return fail(CV_ERR_INVALID_ARGUMENT, "the GPU renderer implements CV_RNG_PIXEL");
+  if (o->approx_mode != CV_APPROX_PORTABLE)
+    return fail(CV_ERR_INVALID_ARGUMENT, "the GPU renderer implements CV_APPROX_PORTABLE");
+  return CV_OK;
+}
+
+cv_status check_frame(const cv_scene* s, uint32_t width, uint32_t height) {
+  // The 16x4-block HDR layout needs width % 16 == 0 and height % 4 == 0
+  // (renderer.rs:366-374, 384-385); the reference's RenderBuffer also
+  // rounds heights to 16, which the per-pixel GPU path does not need.
+  if ((width & 15) || (height & 3) || width == 0 || height == 0)
+    return fail(CV_ERR_NOT_MULTIPLE_OF_16, "width must be a multiple of 16 and height of 4");
+  if ((uint64_t)width * height >= (1ull << 31)) return fail(CV_ERR_INVALID_ARGUMENT, "frame too large");
+  if (s->ndirect < 8)
+    return fail(CV_ERR_LIGHT_COUNT, "the scene has " + std::to_string(s->ndirect) +
+                                        " direct-sample triangles; direct sampling needs 8");
+  return CV_OK;
+}
+
+struct Rect {
+  uint32_t x, y, w, h;
+};
+
+cv_status run_frames(cv_renderer* r, const Rect& rc, uint32_t first, uint32_t nframes, float* hdr,
+                     uint32_t* bitmap, uint32_t* gbuffer, uint32_t flags, bool hdr_mode) {
+  hipStream_t st = r->pipe.stream;
+  const bool dev = flags & CV_PTR_DEVICE;
+  const uint32_t W = r->width;
+  // HDR block region of the rectangle (rc.x, rc.w multiples of 16; rc.y, rc.h of 4)
+  const size_t blk_row_bytes = (size_t)(W / 16) * 768;
+  const size_t rect_blk_off = (size_t)(rc.y / 4) * blk_row_bytes + (size_t)(rc.x / 16) * 768;
+  const size_t rect_blk_w = (size_t)(rc.w / 16) * 768;
+  float* hdr_dev = nullptr;
+  if (hdr_mode) {
+    hdr_dev = dev ? hdr : r->hdr.p;
+    if (!dev)
+      HIPTRY(hipMemcpy2DAsync((char*)r->hdr.p + rect_blk_off, blk_row_bytes,
+                              (const char*)hdr + rect_blk_off, blk_row_bytes, rect_blk_w, rc.h / 4,
+                              hipMemcpyHostToDevice, st),
+             "upload hdr");
+  }
+  uint32_t* bm_dev = nullptr;
+  if (bitmap) bm_dev = dev ? bitmap : r->bitmap.p;
+  uint32_t* gb_dev = nullptr;
+  if (gbuffer) gb_dev = dev ? gbuffer : r->gbuffer.p;
+  cv_scene* s = r->scene;
+  cvk::JobDesc base = cvh::job_for_scene(s);
+  base.width = r->width;
+  base.height = r->height;
+  base.rect_x = rc.x;
+  base.rect_y = rc.y;
+  base.rect_w = rc.w;
+  base.rect_h = rc.h;
+  base.hdr = hdr_dev;
+  base.bitmap = bm_dev;
+  base.path_base = 0;
+  std::vector<StepPlan> steps(nframes);
+  for (uint32_t f = 0; f < nframes; f++) {
+    cvk::JobDesc j = base;
+    j.frame = first + f;
+    // every frame rewrites the gbuffer; the last one is what remains
+    j.gbuffer = f + 1 == nframes ? gb_dev : nullptr;
+    steps[f].jobs.push_back(j);
+    steps[f].paths = rc.w * rc.h;
+  }
+  RunOpts o;
+  o.cfg = s->cfg;
+  o.max_bounces = r->opts.max_bounces;
+  o.flags = flags;
+  o.packed = s->packed;
+  o.nodes = s->nodes.p;
+  o.tris = s->tris.p;
+  o.multi_job = false;
+  o.max_depth = s->info.depth;
+  o.graph = flags & CV_RENDER_GRAPH;
+  cv_status status = pipeline_run(r->pipe, steps, o, &r->stats);
+  if (status == CV_OK && !dev) {
+    const size_t row = (size_t)W * 4;
+    const size_t off = ((size_t)rc.y * W + rc.x) * 4;
+    if (hdr_mode)
+      status = hipMemcpy2DAsync((char*)hdr + rect_blk_off, blk_row_bytes,
+                                (const char*)r->hdr.p + rect_blk_off, blk_row_bytes, rect_blk_w,
+                                rc.h / 4, hipMemcpyDeviceToHost, st) == hipSuccess
+                   ? CV_OK
+                   : fail(CV_ERR_DEVICE, "download hdr");
+    if (status == CV_OK && bitmap)
+      status = hipMemcpy2DAsync((char*)bitmap + off, row, (const char*)r->bitmap.p + off, row,
+                                (size_t)rc.w * 4, rc.h, hipMemcpyDeviceToHost, st) == hipSuccess
+                   ? CV_OK
+                   : fail(CV_ERR_DEVICE, "download bitmap");
+    if (status == CV_OK && gbuffer)
+      status = hipMemcpy2DAsync((char*)gbuffer + off, row, (const char*)r->gbuffer.p + off, row,
+                                (size_t)rc.w * 4, rc.h, hipMemcpyDeviceToHost, st) == hipSuccess
+                   ? CV_OK
+                   : fail(CV_ERR_DEVICE, "download gbuffer");
+    hipError_t se = hipStreamSynchronize(st);
+    if (status == CV_OK && se != hipSuccess) status = hip_fail(se, "download");
+  }
+  return status;
+}
+
+cv_status patch_args(cv_renderer* r, uint32_t pw, uint32_t x, uint32_t y) {
+  if (!r) return fail(CV_ERR_INVALID_ARGUMENT, "null renderer");
+  if ((pw & 15) || pw == 0) return fail(CV_ERR_NOT_MULTIPLE_OF_16, "patch width must be a multiple of 16");
+  if ((x & 15) || (y & 15)) return fail(CV_ERR_NOT_MULTIPLE_OF_16, "patch origin must be a multiple of 16");
+  if (x + pw > r->width || y + pw > r->height) return fail(CV_ERR_OUT_OF_RANGE, "patch outside the frame");
+  return CV_OK;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ batches
+
+struct cv_batch {
+  cv_render_opts opts{};
+  cvh::LaunchCfg cfg;
+  uint64_t max_paths_per_step = 0;
+  Pipeline pipe;
+  cv_render_stats stats{};
+  uint32_t last_steps = 0;
+  std::mutex mu;
+};
+
+extern "C" {
+
+cv_status cv_renderer_create(cv_scene* s, uint32_t width, uint32_t height,
+                             const cv_render_opts* opts, cv_renderer** out) {
+  if (!s || !out) return fail(CV_ERR_INVALID_ARGUMENT, "null argument");
+  *out = nullptr;
+  cv_status cs = check_frame(s, width, height);
+  if (cs != CV_OK) return cs;
+  cv_render_opts o;
+  cs = check_opts(opts, &o);
+  if (cs != CV_OK) return cs;
+  cv_renderer* r = new cv_renderer();
+  r->scene = s;
+  r->width = width;
+  r->height = height;
+  r->opts = o;
+  auto bail = [&](cv_status st) {
+    cv_renderer_destroy(r);
+    return st;
+  };
+  if ((cs = r->pipe.init()) != CV_OK) return bail(cs);
+  const size_t npix = (size_t)width * height;
+  if ((cs = r->pipe.reserve(npix, 1)) != CV_OK) return bail(cs);
+  hipError_t e;
+  if ((e = r->hdr.alloc(npix * 3)) || (e = r->bitmap.alloc(npix)) || (e = r->gbuffer.alloc(npix)))
+    return bail(hip_fail(e, "alloc buffers"));
+  *out = r;
+  return CV_OK;
+}
+
+void cv_renderer_destroy(cv_renderer* r) {
+  if (!r) return;
+  r->pipe.release();
+  r->hdr.reset();
+  r->bitmap.reset();
+  r->gbuffer.reset();
+  delete r;
+}
+
+cv_status cv_renderer_render_patch_u8(cv_renderer* r, uint32_t* bitmap, uint32_t* gbuffer,
+                                      uint32_t patch_width, uint32_t x, uint32_t y,
+                                      uint32_t frame, uint32_t flags) {
+  cv_status st = patch_args(r, patch_width, x, y);
+  if (st != CV_OK) return st;
+  if (!bitmap) return fail(CV_ERR_INVALID_ARGUMENT, "null bitmap");
+  std::lock_guard<std::mutex> g(r->mu);
+  return run_frames(r, Rect{x, y, patch_width, patch_width}, frame, 1, nullptr, bitmap, gbuffer,
+                    flags, false);
+}
+
+cv_status cv_renderer_accumulate_patch_f32(cv_renderer* r, float* hdr, uint32_t* gbuffer,
+                                           uint32_t patch_width, uint32_t x, uint32_t y,
+                                           uint32_t frame, uint32_t flags) {
+  cv_status st = patch_args(r, patch_width, x, y);
+  if (st != CV_OK) return st;
+  if (!hdr) return fail(CV_ERR_INVALID_ARGUMENT, "null hdr");
+  std::lock_guard<std::mutex> g(r->mu);
+  return run_frames(r, Rect{x, y, patch_width, patch_width}, frame, 1, hdr, nullptr, gbuffer,
+                    flags, true);
+}
+
+cv_status cv_renderer_render_frame_u8(cv_renderer* r, uint32_t* bitmap, uint32_t* gbuffer,
+                                      uint32_t frame, uint32_t flags) {
+  if (!r || !bitmap) return fail(CV_ERR_INVALID_ARGUMENT, "null argument");
+  std::lock_guard<std::mutex> g(r->mu);
+  return run_frames(r, Rect{0, 0, r->width, r->height}, frame, 1, nullptr, bitmap, gbuffer, flags,
+                    false);
+}
+
+cv_status cv_renderer_accumulate_frames_f32(cv_renderer* r, float* hdr, uint32_t* gbuffer,
+                                            uint32_t first_frame, uint32_t n_frames,
+                                            uint32_t flags) {
+  if (!r || !hdr) return fail(CV_ERR_INVALID_ARGUMENT, "null argument");
+  std::lock_guard<std::mutex> g(r->mu);
+  return run_frames(r, Rect{0, 0, r->width, r->height}, first_frame, n_frames, hdr, nullptr,
+                    gbuffer, flags, true);
+}
+
+cv_status cv_renderer_buffer_f32_into_bitmap(cv_renderer* r, const float* hdr, uint32_t* bitmap,
+                                             uint32_t num_samples, uint32_t flags) {
+  if (!r || !hdr || !bitmap) return fail(CV_ERR_INVALID_ARGUMENT, "null argument");
+  std::lock_guard<std::mutex> g(r->mu);
+  hipStream_t st = r->pipe.stream;
+  const bool dev = flags & CV_PTR_DEVICE;
+  const size_t npix = (size_t)r->width * r->height;
+  const float* hp = hdr;
+  uint32_t* bp = bitmap;
+  if (!dev) {
+    HIPTRY(hipMemcpyAsync(r->hdr.p, hdr, npix * 12, hipMemcpyHostToDevice, st), "upload");
+    hp = r->hdr.p;
+    bp = r->bitmap.p;
+  }
+  HIPTRY(cvk::launch_hdr_to_bitmap(hp, bp, r->width, r->height, num_samples, st), "convert");
+  if (!dev) HIPTRY(hipMemcpyAsync(bitmap, bp, npix * 4, hipMemcpyDeviceToHost, st), "download");
+  HIPTRY(hipStreamSynchronize(st), "sync");
+  return CV_OK;
+}
+
+cv_status cv_renderer_get_stats(const cv_renderer* r, cv_render_stats* stats) {
+  if (!r || !stats) return fail(CV_ERR_INVALID_ARGUMENT, "null argument");
+  *stats = r->stats;
+  return CV_OK;
+}
+
+// ------------------------------------------------------------------ batches
+
+cv_status cv_batch_create(const cv_render_opts* opts, uint64_t max_paths_per_step,
+                          cv_batch** out) {
+  if (!out) return fail(CV_ERR_INVALID_ARGUMENT, "null argument");
+  *out = nullptr;
+  cv_status cs = cvh::ensure_device();
+  if (cs != CV_OK) return cs;
+  cv_render_opts o;
+  if ((cs = check_opts(opts, &o)) != CV_OK) return cs;
+  if (max_paths_per_step >= (1ull << 31))
+    return fail(CV_ERR_INVALID_ARGUMENT, "max_paths_per_step must be below 2^31");
+  cv_batch* b = new cv_batch();
+  b->opts = o;
+  b->max_paths_per_step = max_paths_per_step ? max_paths_per_step : (1ull << 24);
+  cv_tuning t{8, 32, 32, 1};
+  cvh::apply_tuning(b->cfg, t);
+  if ((cs = b->pipe.init()) != CV_OK) {
+    cv_batch_destroy(b);
+    return cs;
+  }
+  *out = b;
+  return CV_OK;
+}
+
+void cv_batch_destroy(cv_batch* b) {
+  if (!b) return;
+  b->pipe.release();
+  delete b;
+}
+
+cv_status cv_batch_set_tuning(cv_batch* b, const cv_tuning* t) {
+  if (!b || !t) return fail(CV_ERR_INVALID_ARGUMENT, "null argument");
+  cv_status v = cvh::validate_tuning(*t);
+  if (v != CV_OK) return v;
+  std::lock_guard<std::mutex> g(b->mu);
+  cvh::apply_tuning(b->cfg, *t);
+  return CV_OK;
+}
+
+cv_status cv_batch_get_tuning(const cv_batch* b, cv_tuning* t) {
+  if (!b || !t) return fail(CV_ERR_INVALID_ARGUMENT, "null argument");
+  *t = cvh::tuning_of(b->cfg);
+  return CV_OK;
+}
+
+cv_status cv_batch_render(cv_batch* b, const cv_job* jobs, size_t n, uint32_t flags) {
+  if (!b || (n && !jobs)) return fail(CV_ERR_INVALID_ARGUMENT, "null argument");
+  if (n == 0) return CV_OK;
+  if (n > 65536) return fail(CV_ERR_INVALID_ARGUMENT, "too many jobs");
+  std::lock_guard<std::mutex> g(b->mu);
+  const bool dev = flags & CV_PTR_DEVICE;
+  uint32_t max_frames = 0, max_depth = 0;
+  size_t largest = 0, total = 0;
+  bool packed = true, one_scene = true;
+  for (size_t i = 0; i < n; i++) {
+    const cv_job& J = jobs[i];
+    if (!J.scene || !J.hdr) return fail(CV_ERR_INVALID_ARGUMENT, "job without scene or hdr");
+    cv_status cs = check_frame(J.scene, J.width, J.height);
+    if (cs != CV_OK) return cs;
+    if (J.num_frames == 0) return fail(CV_ERR_INVALID_ARGUMENT, "job with zero frames");
+    max_frames = std::max(max_frames, J.num_frames);
+    max_depth = std::max(max_depth, J.scene->info.depth);
+    largest = std::max(largest, (size_t)J.width * J.height);
+    total += (size_t)J.width * J.height;
+    packed = packed && J.scene->packed;
+    one_scene = one_scene && J.scene == jobs[0].scene;
+  }
+  // queue capacity: the first step's pixels, within the bound (a job larger
+  // than the bound still fits)
+  const size_t cap = std::max<size_t>(largest, std::min<size_t>(b->max_paths_per_step, total));
+  if (cap >= (1ull << 31)) return fail(CV_ERR_INVALID_ARGUMENT, "step too large");
+  // host outputs: device staging per job
+  std::vector<DevBuf<float>> d_hdr(dev ? 0 : n);
+  std::vector<DevBuf<uint32_t>> d_gb(dev ? 0 : n);
+  hipStream_t st = b->pipe.stream;
+  std::vector<cvk::JobDesc> base(n);
+  for (size_t i = 0; i < n; i++) {
+    const cv_job& J = jobs[i];
+    const size_t npix = (size_t)J.width * J.height;
+    cvk::JobDesc d = cvh::job_for_scene(J.scene);
+    d.width = J.width;
+    d.height = J.height;
+    d.rect_x = d.rect_y = 0;
+    d.rect_w = J.width;
+    d.rect_h = J.height;
+    d.hdr = J.hdr;
+    d.gbuffer = J.gbuffer;
+    d.bitmap = nullptr;
+    if (!dev) {
+      HIPTRY(d_hdr[i].alloc(npix * 3), "alloc hdr");
+      HIPTRY(hipMemcpyAsync(d_hdr[i].p, J.hdr, npix * 12, hipMemcpyHostToDevice, st), "upload hdr");
+      d.hdr = d_hdr[i].p;
+      if (J.gbuffer) {
+        HIPTRY(d_gb[i].alloc(npix), "alloc gbuffer");
+        d.gbuffer = d_gb[i].p;
+      }
+    }
+    base[i] = d;
+  }
+  // step s renders frame first_frame + s of every job with more than s
+  // frames; a step that would exceed the queue capacity is split by jobs
+  std::vector<StepPlan> steps;
+  bool multi_job = false;
+  for (uint32_t f = 0; f < max_frames; f++) {
+    StepPlan cur;
+    for (size_t i = 0; i < n; i++) {
+      if (jobs[i].num_frames <= f) continue;
+      const uint32_t npix = jobs[i].width * jobs[i].height;
+      if (cur.paths + (size_t)npix > cap) {
+        steps.push_back(std::move(cur));
+        cur = StepPlan();
+      }
+      cvk::JobDesc d = base[i];
+      d.frame = jobs[i].first_frame + f;
+      d.gbuffer = f + 1 == jobs[i].num_frames ? base[i].gbuffer : nullptr;
+      d.path_base = cur.paths;
+      cur.jobs.push_back(d);
+      cur.paths += npix;
+    }
+    if (!cur.jobs.empty()) steps.push_back(std::move(cur));
+  }
+  for (const StepPlan& s : steps) multi_job = multi_job || s.jobs.size() > 1;
+  RunOpts o;
+  o.cfg = b->cfg;
+  o.max_bounces = b->opts.max_bounces;
+  o.flags = flags;
+  o.packed = packed;
+  o.nodes = one_scene ? jobs[0].scene->nodes.p : nullptr;
+  o.tris = one_scene ? jobs[0].scene->tris.p : nullptr;
+  o.multi_job = multi_job;
+  o.max_depth = max_depth;
+  o.graph = flags & CV_RENDER_GRAPH;
+  cv_status cs = b->pipe.reserve(cap, 1);
+  if (cs != CV_OK) return cs;
+  cs = pipeline_run(b->pipe, steps, o, &b->stats);
+  b->last_steps = (uint32_t)steps.size();
+  if (cs == CV_OK && !dev) {
+    for (size_t i = 0; i < n; i++) {
+      const size_t npix = (size_t)jobs[i].width * jobs[i].height;
+      HIPTRY(hipMemcpyAsync(jobs[i].hdr, d_hdr[i].p, npix * 12, hipMemcpyDeviceToHost, st),
+             "download hdr");
+      if (jobs[i].gbuffer)
+        HIPTRY(hipMemcpyAsync(jobs[i].gbuffer, d_gb[i].p, npix * 4, hipMemcpyDeviceToHost, st),
+               "download gbuffer");
+    }
+    HIPTRY(hipStreamSynchronize(st), "download");
+  }
+  return cs;
+}
+
+cv_status cv_batch_get_stats(const cv_batch* b, cv_render_stats* stats) {
+  if (!b || !stats) return fail(CV_ERR_INVALID_ARGUMENT, "null argument");
+  *stats = b->stats;
+  return CV_OK;
+}
+
+}  // extern "C"

@@ -300,6 +300,48 @@ def config1(subdivisions: int = 8, n_free: int = 20000, seed: int = 2,
                    name=f"config1-sub{subdivisions}-free{n_free}")
 
 
+def procedural_scene(n_tris: int, seed: int, emission=(10.0, 10.0, 10.0)) -> SceneData:
+    """A scene of the small configuration's kind with about n_tris
+    triangles: the box with its light (26 triangles), the largest icosphere
+    (20 * 4^k faces, radius 0.4, radially displaced by 0.05*u) that fits,
+    and free triangles for the rest."""
+    budget = max(int(n_tris) - 26, 20)
+    k = 0
+    while 20 * 4 ** (k + 1) <= budget and k < 9:
+        k += 1
+    st = Stream(seed)
+    box = box_with_light(Stream(seed ^ 0x5EED0B0C))
+    v, f = icosphere(k)
+    u = st.uniform(v.shape[0], -1.0, 1.0)
+    v = v * (0.4 + 0.05 * u)[:, None]
+    sph = (_sphere_tris(v, f), np.full(f.shape[0], MAT_GLOSSY_WHITE, dtype=np.uint32))
+    parts = [box, sph]
+    n_free = budget - f.shape[0]
+    if n_free > 0:
+        parts.append(free_triangles(Stream(seed * 7919 + 17), n_free))
+    return _finish(parts, emission_override=emission, name=f"procedural-{n_tris}-s{seed}")
+
+
+def batch_jobs(n_jobs: int = 64, seed0: int = 1000, tri_range=(1e3, 2e6),
+               width_range=(64, 1920), height_range=(64, 1080), max_spp: int = 64):
+    """The many-jobs configuration: job j uses seed seed0 + j to draw its
+    triangle count (log-uniform in tri_range), its width and height
+    (log-uniform, rounded down to multiples of 16) and its samples per pixel
+    (uniform in {1, 2, 4, ..., max_spp}).  Returns (n_tris, seed, width,
+    height, spp) tuples; procedural_scene(n_tris, seed) makes the scene."""
+    out = []
+    n_spp = int(math.log2(max_spp)) + 1
+    for j in range(n_jobs):
+        st = Stream(seed0 + j)
+        a, b, c, d = st.uniform(4, 0.0, 1.0)
+        n = int(math.exp(math.log(tri_range[0]) + a * (math.log(tri_range[1]) - math.log(tri_range[0]))))
+        w = int(math.exp(math.log(width_range[0]) + b * (math.log(width_range[1]) - math.log(width_range[0]))))
+        h = int(math.exp(math.log(height_range[0]) + c * (math.log(height_range[1]) - math.log(height_range[0]))))
+        spp = 1 << min(int(d * n_spp), n_spp - 1)
+        out.append((n, seed0 + j, max(16, w // 16 * 16), max(16, h // 16 * 16), spp))
+    return out
+
+
 def mesh_from_obj_arrays(vertices: np.ndarray, faces: np.ndarray, materials: np.ndarray,
                          **kw) -> SceneData:
     tris = np.concatenate([vertices[faces[:, 0]], vertices[faces[:, 1]],

@@ -210,6 +210,8 @@ typedef struct cv_render_stats {
 
 #define CV_RENDER_COUNT 2u /* count box/triangle tests (slower) */
 #define CV_RENDER_TIMING 4u /* record HIP events around every launch */
+#define CV_RENDER_GRAPH 8u  /* capture a step's launches once as a HIP graph and replay it
+                               (ignored with CV_RENDER_TIMING) */
 
 /* Renderer::new (renderer.rs:121-130).  The renderer keeps a reference to
  * the scene (which must outlive it).  width must be a multiple of 16 and
@@ -248,6 +250,43 @@ cv_status cv_renderer_buffer_f32_into_bitmap(cv_renderer* r, const float* hdr, u
 /* Counters and device times of the last render call. */
 cv_status cv_renderer_get_stats(const cv_renderer* r, cv_render_stats* stats);
 void cv_renderer_destroy(cv_renderer* r);
+
+/* ---------------------------------------------------------------- batches */
+
+/* Many render jobs in shared launches: the patch loop of main.rs:174-214
+ * generalised to jobs of different scenes, sizes and sample counts (no
+ * reference counterpart beyond that loop).  Step s renders frame
+ * first_frame + s of every job with more than s frames, all of them in the
+ * same generate / extend / shade launches, so a job's result is identical
+ * to cv_renderer_accumulate_frames_f32 of the same scene, size and
+ * frames, alone or in any mix. */
+typedef struct cv_batch cv_batch;
+
+typedef struct cv_job {
+    cv_scene* scene;       /* resident scene; its camera at the call is used */
+    uint32_t width;        /* multiple of 16 */
+    uint32_t height;       /* multiple of 4 */
+    uint32_t first_frame;  /* frames first_frame .. first_frame + num_frames - 1 */
+    uint32_t num_frames;   /* samples per pixel, >= 1 */
+    float* hdr;            /* accumulated into: 16x4-block layout, width*height*3 f32 */
+    uint32_t* gbuffer;     /* NULL, or the last frame's gbuffer (width*height u32) */
+} cv_job;
+
+/* max_paths_per_step bounds the path queues (0: 2^24); a step whose jobs
+ * hold more pixels is split into several steps by jobs (a job larger than
+ * the bound gets a step of its own).  opts may be NULL. */
+cv_status cv_batch_create(const cv_render_opts* opts, uint64_t max_paths_per_step,
+                          cv_batch** out);
+/* Launch shape of the batch's traversal (see cv_tuning). */
+cv_status cv_batch_set_tuning(cv_batch* b, const cv_tuning* tuning);
+cv_status cv_batch_get_tuning(const cv_batch* b, cv_tuning* tuning);
+/* Renders all jobs and returns when their outputs are written.  flags:
+ * CV_PTR_DEVICE (hdr and gbuffer are device pointers), CV_RENDER_COUNT,
+ * CV_RENDER_TIMING, CV_RENDER_GRAPH. */
+cv_status cv_batch_render(cv_batch* b, const cv_job* jobs, size_t n, uint32_t flags);
+/* Counters and device times of the last cv_batch_render. */
+cv_status cv_batch_get_stats(const cv_batch* b, cv_render_stats* stats);
+void cv_batch_destroy(cv_batch* b);
 
 /* ----------------------------------------------------------------- device */
 
