@@ -114,6 +114,137 @@ def cpu_baseline(sd, gpu_scene, args):
     }
 
 
+def bench_batch(args):
+    """The many-jobs workload: 64 seeded procedural scenes of the small
+    configuration's kind (triangle counts log-uniform in [1e3, 2e6],
+    resolutions log-uniform from 64x64 to 1920x1080, spp in {1, 2, ..., 64}),
+    rendered by one cv_batch (every step's launches shared by all jobs,
+    steps replayed from a HIP graph) against the same jobs rendered one
+    after the other by per-job renderers."""
+    import convector_amd as cv
+    from convector_amd import scenes
+
+    specs = scenes.batch_jobs(args.jobs, 1000)
+    t0 = time.perf_counter()
+    sds = [scenes.procedural_scene(n, seed) for (n, seed, _, _, _) in specs]
+    log("batch: %d scenes, %d triangles generated in %.1f s"
+        % (len(sds), sum(sd.ntris for sd in sds), time.perf_counter() - t0))
+    t0 = time.perf_counter()
+    gs = [cv.Scene.from_scene_data(sd) for sd in sds]
+    build_ms = sum(g.bvh_info()["build_ms"] for g in gs)
+    log("batch: GPU BVH builds %.1f ms device (%.1f s with uploads)"
+        % (build_ms, time.perf_counter() - t0))
+    npix = [w * h for (_, _, w, h, _) in specs]
+    offs = np.concatenate([[0], np.cumsum(npix)]).astype(np.int64)
+    hdr = DeviceBuffer(int(offs[-1]) * 12)
+    hdr.zero()
+    jobs = [cv.Job(g, w, h, 1, spp, hdr=hdr.ptr.value + int(offs[i]) * 12)
+            for i, (g, (_, _, w, h, spp)) in enumerate(zip(gs, specs))]
+    paths = sum(w * h * spp for (_, _, w, h, spp) in specs)
+    b = cv.Batch()
+    tuning, tune_ms = b.autotune(jobs, max_sample_paths=1 << 22)
+    log("batch: autotune chose %s (sample step %.2f ms)" % (tuning, tune_ms))
+    flags = cv.PTR_DEVICE | cv.RENDER_GRAPH
+    b.render(jobs, flags)  # warm-up (captures the graphs)
+    cv.device_synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        b.render(jobs, flags)
+        log("batch step %d: %.1f ms device" % (k, b.stats()["total_ms"]))
+    cv.device_synchronize()
+    wall = (time.perf_counter() - t0) / args.steps
+    st = b.stats()
+    # the same jobs one after the other, one renderer per job
+    rs = [cv.Renderer(g, w, h) for g, (_, _, w, h, _) in zip(gs, specs)]
+    for r, j in zip(rs, jobs):  # warm-up
+        r.accumulate_frames_device(j.hdr, 1, 1)
+    cv.device_synchronize()
+    t0 = time.perf_counter()
+    for r, j in zip(rs, jobs):
+        r.accumulate_frames_device(j.hdr, 1, j.num_frames)
+    cv.device_synchronize()
+    seq = time.perf_counter() - t0
+    del rs
+    out = {
+        "metric": "paths/s, 64 mixed render jobs in shared launches",
+        "value": round(paths / wall, 1),
+        "unit": "paths/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": 1,
+        "ms_per_step": round(wall * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "%d procedural jobs (seeds 1000..%d): %d triangles in total, %d paths, "
+                        "resolutions 64x64..1920x1080, 1..64 spp, 5 bounces, PIXEL rng"
+                        % (len(specs), 999 + len(specs), sum(sd.ntris for sd in sds), paths),
+            "paths_per_step": paths,
+        },
+        "mrays_per_s": round(st["extend_rays"] / wall / 1e6, 2),
+        "batch_steps_device_ms": round(st["total_ms"], 3),
+        "sequential": {"paths_per_s": round(paths / seq, 1), "ms": round(seq * 1e3, 3),
+                       "note": "64 renderers, cv_renderer_accumulate_frames_f32 one job after "
+                               "the other"},
+        "speedup_vs_sequential": round(seq / wall, 3),
+        "autotune": {"chosen": tuning, "sample_step_ms": round(tune_ms, 3)},
+        "bvh_build_ms_total": round(build_ms, 1),
+    }
+    if not args.no_cpu:
+        try:
+            out["cpu_baseline"] = cpu_baseline_batch(sds, specs, args)
+            out["gpu_over_cpu"] = round(out["value"] / out["cpu_baseline"]["value"], 1)
+        except Exception as e:
+            log("cpu baseline failed: %r" % (e,))
+            out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline_batch(sds, specs, args):
+    """The oracle (reference semantics) on every 8th job: the first frame's
+    32x32 patches in a strided order until the time budget is spent."""
+    from oracle import oracle as O
+
+    cores = min(args.cpu_threads, os.cpu_count() or 1)
+    picks = list(range(0, len(specs), 8))
+    q = paths = 0
+    t_used = 0.0
+    budget = args.cpu_seconds / len(picks)
+    for i in picks:
+        sd = sds[i]
+        (_, _, w, h, _) = specs[i]
+        osc = O.Scene.from_scene_data(sd, approx=O.APPROX_HOST_X86)
+        npatch = ((w + 31) // 32) * ((h + 31) // 32)
+        order = [(k * 37) % npatch for k in range(npatch)] if npatch % 37 else list(range(npatch))
+        t_job, done = 0.0, 0
+        while t_job < budget and done < npatch:
+            ids = order[done:done + 4 * cores]
+            t1 = time.perf_counter()
+            qq, _, _ = osc.sample_patches(w, h, 1, ids, rng_mode=O.RNG_REF_PATCH, nthreads=cores)
+            t_job += time.perf_counter() - t1
+            q += qq
+            # patches at the right and top edges hold fewer pixels
+            for pid in ids:
+                px, py = (pid // ((h + 31) // 32)) * 32, (pid % ((h + 31) // 32)) * 32
+                paths += min(32, w - px) * min(32, h - py)
+            done += len(ids)
+        t_used += t_job
+    return {
+        "value": paths / t_used,
+        "unit": "paths/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": "jobs %s (every 8th), frame 1, 32x32 patches in stride-37 order for up to "
+                  "%.1f s each, reference Rng per patch, host rcpps/rsqrtps, %d threads: "
+                  "%d paths, %d extend queries in %.1f s" % (picks, budget, cores, paths, q,
+                                                            t_used),
+        "mrays_per_s": q / t_used / 1e6,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,6 +254,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--subdivisions", type=int, default=8)
+    ap.add_argument("--workload", choices=["frame", "batch"], default="frame",
+                    help="frame: the large 16-spp 1080p render (default); batch: 64 mixed "
+                         "jobs in shared launches against sequential launches")
+    ap.add_argument("--jobs", type=int, default=64)
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="device-to-device copy of this many MiB before the steps, so that "
                          "rocprofv3 --pmc counters can be scaled by a copy of known size")
@@ -132,6 +267,10 @@ def main():
               "path is built for one GPU (see the sharding notes); --gpus %d is not run."
               % args.gpus, file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "batch":
+        bench_batch(args)
+        return
 
     import convector_amd as cv
     from convector_amd import scenes

@@ -332,7 +332,8 @@ class Batch:
         check(lib().cv_batch_get_tuning(self._h, ctypes.byref(t)))
         return {k: getattr(t, k) for k, _ in t._fields_}
 
-    def render(self, jobs: Sequence[Job], flags: int = 0):
+    @staticmethod
+    def _jobs(jobs: Sequence[Job]):
         arr = (_native.Job * max(1, len(jobs)))()
         for i, j in enumerate(jobs):
             arr[i].scene = j.scene._h
@@ -340,7 +341,19 @@ class Batch:
             arr[i].first_frame, arr[i].num_frames = j.first_frame, j.num_frames
             arr[i].hdr = j._ptr(j.hdr)
             arr[i].gbuffer = j._ptr(j.gbuffer)
-        check(lib().cv_batch_render(self._h, arr, len(jobs), flags))
+        return arr
+
+    def render(self, jobs: Sequence[Job], flags: int = 0):
+        check(lib().cv_batch_render(self._h, self._jobs(jobs), len(jobs), flags))
+
+    def autotune(self, jobs: Sequence[Job], max_sample_paths: int = 0):
+        """Times launch shapes on a sample step of these jobs and keeps the
+        fastest; returns (tuning dict, step ms)."""
+        t = _native.Tuning()
+        ms = ctypes.c_double()
+        check(lib().cv_batch_autotune(self._h, self._jobs(jobs), len(jobs), max_sample_paths,
+                                      ctypes.byref(t), ctypes.byref(ms)))
+        return {k: getattr(t, k) for k, _ in t._fields_}, ms.value
 
     def stats(self) -> dict:
         s = _native.RenderStats()

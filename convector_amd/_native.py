@@ -39,7 +39,7 @@ EXPORTS = [
     "cv_renderer_accumulate_patch_f32", "cv_renderer_render_frame_u8",
     "cv_renderer_accumulate_frames_f32", "cv_renderer_buffer_f32_into_bitmap",
     "cv_renderer_get_stats", "cv_renderer_destroy", "cv_batch_create", "cv_batch_set_tuning",
-    "cv_batch_get_tuning", "cv_batch_render", "cv_batch_get_stats", "cv_batch_destroy",
+    "cv_batch_get_tuning", "cv_batch_render", "cv_batch_autotune", "cv_batch_get_stats", "cv_batch_destroy",
     "cv_device_synchronize", "cv_last_error", "cv_status_string", "cv_version",
 ]
 
@@ -142,6 +142,7 @@ def _declare(L):
         "cv_batch_set_tuning": (I, [P, P]),
         "cv_batch_get_tuning": (I, [P, P]),
         "cv_batch_render": (I, [P, P, SZ, U32]),
+        "cv_batch_autotune": (I, [P, P, SZ, ctypes.c_uint64, P, P]),
         "cv_batch_get_stats": (I, [P, P]),
         "cv_batch_destroy": (None, [P]),
         "cv_device_synchronize": (I, []),

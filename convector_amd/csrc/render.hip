@@ -756,6 +756,98 @@ cv_status cv_batch_render(cv_batch* b, const cv_job* jobs, size_t n, uint32_t fl
   return cs;
 }
 
+cv_status cv_batch_autotune(cv_batch* b, const cv_job* jobs, size_t n, uint64_t max_sample_paths,
+                            cv_tuning* chosen, double* chosen_ms) {
+  if (!b || !jobs || n == 0) return fail(CV_ERR_INVALID_ARGUMENT, "null argument");
+  if (n > 65536) return fail(CV_ERR_INVALID_ARGUMENT, "too many jobs");
+  std::lock_guard<std::mutex> g(b->mu);
+  if (max_sample_paths == 0) max_sample_paths = 1ull << 22;
+  // sample: the first frame of the jobs in order, up to max_sample_paths
+  // paths (at least one job), written to scratch outputs
+  StepPlan step;
+  uint32_t max_depth = 0;
+  bool packed = true, one_scene = true;
+  for (size_t i = 0; i < n; i++) {
+    const cv_job& J = jobs[i];
+    if (!J.scene) return fail(CV_ERR_INVALID_ARGUMENT, "job without scene");
+    cv_status cs = check_frame(J.scene, J.width, J.height);
+    if (cs != CV_OK) return cs;
+    const uint32_t npix = J.width * J.height;
+    if (!step.jobs.empty() && step.paths + (uint64_t)npix > max_sample_paths) continue;
+    cvk::JobDesc d = cvh::job_for_scene(J.scene);
+    d.width = J.width;
+    d.height = J.height;
+    d.rect_w = J.width;
+    d.rect_h = J.height;
+    d.frame = J.first_frame;
+    d.path_base = step.paths;
+    step.jobs.push_back(d);
+    step.paths += npix;
+    max_depth = std::max(max_depth, J.scene->info.depth);
+    packed = packed && J.scene->packed;
+    one_scene = one_scene && J.scene == jobs[0].scene;
+  }
+  DevBuf<float> scratch;
+  HIPTRY(scratch.alloc((size_t)step.paths * 3), "alloc scratch");
+  for (cvk::JobDesc& d : step.jobs) d.hdr = scratch.p + (size_t)d.path_base * 3;
+  std::vector<StepPlan> steps(1, step);
+  RunOpts o;
+  o.max_bounces = b->opts.max_bounces;
+  o.flags = 0;
+  o.packed = packed;
+  o.nodes = one_scene ? jobs[0].scene->nodes.p : nullptr;
+  o.tris = one_scene ? jobs[0].scene->tris.p : nullptr;
+  o.multi_job = step.jobs.size() > 1;
+  o.max_depth = max_depth;
+  auto time_cfg = [&](const cvh::LaunchCfg& c, double* ms) -> cv_status {
+    o.cfg = c;
+    double best = 1e30;
+    for (int rep = 0; rep < 2; rep++) {
+      cv_render_stats st{};
+      cv_status cs = pipeline_run(b->pipe, steps, o, &st);
+      if (cs != CV_OK) return cs;
+      best = std::min(best, st.total_ms);
+    }
+    *ms = best;
+    return CV_OK;
+  };
+  // coordinate descent over the launch knobs, two rounds
+  static const uint32_t stacks[] = {8, 16, 24, 32};
+  static const uint32_t waves[] = {16, 24, 32, 40, 48};
+  static const uint32_t refills[] = {16, 24, 32, 48, 64, 65};
+  cvh::LaunchCfg best = b->cfg;
+  double best_ms = 0;
+  cv_status cs = time_cfg(best, &best_ms);
+  if (cs != CV_OK) return cs;
+  for (int round = 0; round < 2; round++) {
+    for (uint32_t v : stacks) {
+      cvh::LaunchCfg c = best;
+      c.stack = (int)v;
+      double ms;
+      if ((cs = time_cfg(c, &ms)) != CV_OK) return cs;
+      if (ms < best_ms) best = c, best_ms = ms;
+    }
+    for (uint32_t v : waves) {
+      cvh::LaunchCfg c = best;
+      c.blocks = cvh::num_cus() * v;
+      double ms;
+      if ((cs = time_cfg(c, &ms)) != CV_OK) return cs;
+      if (ms < best_ms) best = c, best_ms = ms;
+    }
+    for (uint32_t v : refills) {
+      cvh::LaunchCfg c = best;
+      c.refill = v;
+      double ms;
+      if ((cs = time_cfg(c, &ms)) != CV_OK) return cs;
+      if (ms < best_ms) best = c, best_ms = ms;
+    }
+  }
+  b->cfg = best;
+  if (chosen) *chosen = cvh::tuning_of(best);
+  if (chosen_ms) *chosen_ms = best_ms;
+  return CV_OK;
+}
+
 cv_status cv_batch_get_stats(const cv_batch* b, cv_render_stats* stats) {
   if (!b || !stats) return fail(CV_ERR_INVALID_ARGUMENT, "null argument");
   *stats = b->stats;

@@ -42,6 +42,12 @@ def mat_diffuse(r: float, g: float, b: float) -> int:
     return (bi << 16) | (gi << 8) | ri
 
 
+def mat_diffuse_array(rgb: np.ndarray) -> np.ndarray:
+    """mat_diffuse of every row of an (n, 3) array."""
+    q = (rgb.astype(np.float32) * np.float32(255.0)).astype(np.float32).astype(np.uint32)
+    return ((q[:, 2] << 16) | (q[:, 1] << 8) | q[:, 0]).astype(np.uint32)
+
+
 def mat_with_glossiness(mat: int, gloss: int) -> int:
     """SMaterial::with_glossiness (material.rs:111-118)."""
     assert 0 <= gloss <= 6
@@ -261,8 +267,7 @@ def free_triangles(stream: Stream, n: int):
         p = np.repeat(local[k][None, :], n, 0) * (edge / s3)[:, None]
         tris[:, 3 * k:3 * k + 3] = c + _rotate(q, p)
     rgb = stream.uniform(3 * n, 0.2, 0.8).reshape(n, 3)
-    mats = np.array([mat_diffuse(*x) for x in rgb], dtype=np.uint32)
-    return tris, mats
+    return tris, mat_diffuse_array(rgb)
 
 
 def _finish(parts, **kw) -> SceneData:

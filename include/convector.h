@@ -284,6 +284,14 @@ cv_status cv_batch_get_tuning(const cv_batch* b, cv_tuning* tuning);
  * CV_PTR_DEVICE (hdr and gbuffer are device pointers), CV_RENDER_COUNT,
  * CV_RENDER_TIMING, CV_RENDER_GRAPH. */
 cv_status cv_batch_render(cv_batch* b, const cv_job* jobs, size_t n, uint32_t flags);
+/* The autotuner: times launch shapes (stack entries 8/16/24/32, 16-48
+ * persistent waves per CU, refill thresholds 16-64 and whole batches) on a
+ * sample step -- the first frame of the jobs in order, up to
+ * max_sample_paths paths (0: 2^22), rendered into scratch buffers -- by
+ * two rounds of coordinate descent, and keeps the fastest on the batch.
+ * chosen and chosen_ms (may be NULL) receive it and its step time. */
+cv_status cv_batch_autotune(cv_batch* b, const cv_job* jobs, size_t n, uint64_t max_sample_paths,
+                            cv_tuning* chosen, double* chosen_ms);
 /* Counters and device times of the last cv_batch_render. */
 cv_status cv_batch_get_stats(const cv_batch* b, cv_render_stats* stats);
 void cv_batch_destroy(cv_batch* b);

@@ -142,3 +142,17 @@ def test_batch_errors(cv, world):
     with pytest.raises(cv.CvError) as e:
         b.set_tuning(stack_entries=12)
     assert e.value.name == "CV_ERR_INVALID_ARGUMENT"
+
+
+def test_batch_autotune(cv, world):
+    """The autotuner picks a valid launch shape, keeps it on the batch, and
+    the batch's results do not change with it."""
+    sds, gs, expect = world
+    b = cv.Batch()
+    jobs = _jobs(cv, gs)
+    t, ms = b.autotune(jobs, max_sample_paths=20000)
+    print("autotune", t, ms, flush=True)
+    assert t["stack_entries"] in (8, 16, 24, 32) and ms > 0
+    assert b.tuning() == t
+    b.render(jobs)
+    _check(jobs, expect)
