@@ -269,6 +269,15 @@ class Renderer:
         check(lib().cv_renderer_accumulate_frames_f32(self._h, _p(hdr), _p(gbuffer), first_frame,
                                                       n_frames, flags))
 
+    def new_features(self) -> np.ndarray:
+        """First-hit features: 8 f32 per pixel in rows (albedo rgb, normal
+        xyz, distance, count), summed over frames."""
+        return np.zeros(self.width * self.height * 8, np.float32)
+
+    def accumulate_aov_f32(self, hdr, features, first_frame, n_frames, flags=0):
+        check(lib().cv_renderer_accumulate_aov_f32(self._h, _p(hdr), _p(features), first_frame,
+                                                   n_frames, flags))
+
     def accumulate_frames_device(self, hdr_ptr: int, first_frame, n_frames, flags=0):
         check(lib().cv_renderer_accumulate_frames_f32(self._h, ctypes.c_void_p(hdr_ptr), None,
                                                       first_frame, n_frames,
@@ -298,12 +307,13 @@ class Job:
     16x4-block layout; a host array, or a device pointer with PTR_DEVICE)."""
 
     def __init__(self, scene: Scene, width: int, height: int, first_frame: int = 1,
-                 num_frames: int = 1, hdr=None, gbuffer=None):
+                 num_frames: int = 1, hdr=None, gbuffer=None, features=None):
         self.scene = scene
         self.width, self.height = int(width), int(height)
         self.first_frame, self.num_frames = int(first_frame), int(num_frames)
         self.hdr = np.zeros(self.width * self.height * 3, np.float32) if hdr is None else hdr
         self.gbuffer = gbuffer
+        self.features = features
 
     def _ptr(self, a):
         if a is None:
@@ -341,6 +351,7 @@ class Batch:
             arr[i].first_frame, arr[i].num_frames = j.first_frame, j.num_frames
             arr[i].hdr = j._ptr(j.hdr)
             arr[i].gbuffer = j._ptr(j.gbuffer)
+            arr[i].features = j._ptr(j.features)
         return arr
 
     def render(self, jobs: Sequence[Job], flags: int = 0):
@@ -364,6 +375,87 @@ class Batch:
         h = getattr(self, "_h", None)
         if h:
             lib().cv_batch_destroy(h)
+            self._h = None
+
+
+class Denoiser:
+    """The learned denoiser (cv_denoiser): a 3-level U-Net trained and run
+    on the GPU's matrix cores.  Arrays are host float32 unless a device
+    pointer (int) is passed with device=True."""
+
+    NUM_INPUT = 10
+
+    def __init__(self, seed: int = 1):
+        h = ctypes.c_void_p()
+        check(lib().cv_denoiser_create(seed, ctypes.byref(h)))
+        self._h = h
+        self.num_params = int(lib().cv_denoiser_num_params())
+
+    def params(self) -> np.ndarray:
+        p = np.zeros(self.num_params, np.float32)
+        check(lib().cv_denoiser_get_params(self._h, _p(p), None))
+        return p
+
+    def grads(self) -> np.ndarray:
+        g = np.zeros(self.num_params, np.float32)
+        check(lib().cv_denoiser_get_params(self._h, None, _p(g)))
+        return g
+
+    def set_params(self, p: np.ndarray):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.num_params
+        check(lib().cv_denoiser_set_params(self._h, _p(p)))
+
+    def forward(self, inp: np.ndarray) -> np.ndarray:
+        """inp: (nb, h, w, 10) float32 -> (nb, h, w, 3) log1p radiance."""
+        inp = np.ascontiguousarray(inp, np.float32)
+        nb, h, w, c = inp.shape
+        assert c == self.NUM_INPUT
+        out = np.zeros((nb, h, w, 3), np.float32)
+        check(lib().cv_denoiser_forward(self._h, _p(inp), _p(out), nb, h, w, 0))
+        return out
+
+    def train_step(self, inp: np.ndarray, target: np.ndarray, lr: float) -> float:
+        """One Adam step on L1(log1p) against linear target (nb, h, w, 3);
+        lr = 0 computes the gradients only.  Returns the loss."""
+        inp = np.ascontiguousarray(inp, np.float32)
+        target = np.ascontiguousarray(target, np.float32)
+        nb, h, w, _ = inp.shape
+        loss = ctypes.c_float()
+        check(lib().cv_denoiser_train_step(self._h, _p(inp), _p(target), nb, h, w, float(lr), 0,
+                                           ctypes.byref(loss)))
+        return loss.value
+
+    def train_step_device(self, inp_ptr: int, target_ptr: int, nb, h, w, lr) -> float:
+        loss = ctypes.c_float()
+        check(lib().cv_denoiser_train_step(self._h, ctypes.c_void_p(inp_ptr),
+                                           ctypes.c_void_p(target_ptr), nb, h, w, float(lr),
+                                           PTR_DEVICE, ctypes.byref(loss)))
+        return loss.value
+
+    def forward_device(self, inp_ptr: int, out_ptr: int, nb, h, w):
+        check(lib().cv_denoiser_forward(self._h, ctypes.c_void_p(inp_ptr), ctypes.c_void_p(out_ptr),
+                                        nb, h, w, PTR_DEVICE))
+
+    @staticmethod
+    def prepare_input(hdr, features, width, height, spp) -> np.ndarray:
+        out = np.zeros((height, width, 10), np.float32)
+        check(lib().cv_denoiser_prepare_input(_p(np.ascontiguousarray(hdr, np.float32)),
+                                              _p(np.ascontiguousarray(features, np.float32)),
+                                              width, height, spp, _p(out), 0))
+        return out
+
+    @staticmethod
+    def output_to_hdr(output, width, height, spp) -> np.ndarray:
+        hdr = np.zeros(width * height * 3, np.float32)
+        check(lib().cv_denoiser_output_to_hdr(_p(np.ascontiguousarray(output, np.float32)), width,
+                                              height, spp, _p(hdr), 0))
+        return hdr
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().cv_denoiser_destroy(h)
             self._h = None
 
 

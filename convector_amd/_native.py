@@ -40,6 +40,10 @@ EXPORTS = [
     "cv_renderer_accumulate_frames_f32", "cv_renderer_buffer_f32_into_bitmap",
     "cv_renderer_get_stats", "cv_renderer_destroy", "cv_batch_create", "cv_batch_set_tuning",
     "cv_batch_get_tuning", "cv_batch_render", "cv_batch_autotune", "cv_batch_get_stats", "cv_batch_destroy",
+    "cv_renderer_accumulate_aov_f32", "cv_denoiser_num_params", "cv_denoiser_create",
+    "cv_denoiser_get_params", "cv_denoiser_set_params", "cv_denoiser_forward",
+    "cv_denoiser_train_step", "cv_denoiser_prepare_input", "cv_denoiser_output_to_hdr",
+    "cv_denoiser_destroy",
     "cv_device_synchronize", "cv_last_error", "cv_status_string", "cv_version",
 ]
 
@@ -81,7 +85,7 @@ class Job(ctypes.Structure):
     _fields_ = [("scene", ctypes.c_void_p), ("width", ctypes.c_uint32),
                 ("height", ctypes.c_uint32), ("first_frame", ctypes.c_uint32),
                 ("num_frames", ctypes.c_uint32), ("hdr", ctypes.c_void_p),
-                ("gbuffer", ctypes.c_void_p)]
+                ("gbuffer", ctypes.c_void_p), ("features", ctypes.c_void_p)]
 
 
 class RenderStats(ctypes.Structure):
@@ -145,6 +149,16 @@ def _declare(L):
         "cv_batch_autotune": (I, [P, P, SZ, ctypes.c_uint64, P, P]),
         "cv_batch_get_stats": (I, [P, P]),
         "cv_batch_destroy": (None, [P]),
+        "cv_renderer_accumulate_aov_f32": (I, [P, P, P, U32, U32, U32]),
+        "cv_denoiser_num_params": (SZ, []),
+        "cv_denoiser_create": (I, [ctypes.c_uint64, PP]),
+        "cv_denoiser_get_params": (I, [P, P, P]),
+        "cv_denoiser_set_params": (I, [P, P]),
+        "cv_denoiser_forward": (I, [P, P, P, U32, U32, U32, U32]),
+        "cv_denoiser_train_step": (I, [P, P, P, U32, U32, U32, F, U32, P]),
+        "cv_denoiser_prepare_input": (I, [P, P, U32, U32, U32, P, U32]),
+        "cv_denoiser_output_to_hdr": (I, [P, U32, U32, U32, P, U32]),
+        "cv_denoiser_destroy": (None, [P]),
         "cv_device_synchronize": (I, []),
         "cv_last_error": (ctypes.c_char_p, []),
         "cv_status_string": (ctypes.c_char_p, [I]),

@@ -1,0 +1,930 @@
+// denoise.hip -- the learned denoiser: a 3-level U-Net (32/64/128 channels,
+// 3x3 convolutions, ReLU, 2x2 average pooling, nearest upsampling, skip
+// concatenation) that maps a 1-sample render plus its first-hit features to
+// the converged image.  It stands where the reference runs its median
+// filter (src/gpu/median.glsl:51-75), with no reference counterpart.
+//
+// Every convolution -- forward, backward to the input and backward to the
+// weights -- is an implicit GEMM on the gfx950 matrix cores
+// (v_mfma_f32_16x16x32_bf16): BF16 operands, FP32 accumulation.  Activations
+// are NHWC BF16, gradients FP32, master weights and Adam moments FP32.
+//
+// MFMA 16x16x32 fragment layout (lane l, register r): A[l%16][8*(l/16)+r],
+// B[8*(l/16)+r][l%16] (8 bf16 each), D[4*(l/16)+r][l%16] (4 f32).
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/convector.h"
+#include "device_math.hpp"
+#include "host.hpp"
+
+using cvh::DevBuf;
+using cvh::fail;
+using cvh::hip_fail;
+
+namespace cvd {
+
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint16_t to_bf16(float f) {
+  // round to nearest even (NaN kept quiet)
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+__device__ __forceinline__ bf8 load8(const uint16_t* p) {
+  uint4 v = *reinterpret_cast<const uint4*>(p);
+  return __builtin_bit_cast(bf8, v);
+}
+__device__ __forceinline__ bf8 zero8() { return __builtin_bit_cast(bf8, make_uint4(0, 0, 0, 0)); }
+
+// ---------------------------------------------------------------------------
+// Implicit-GEMM 3x3 convolution (stride 1, zero padding 1), NHWC.
+//
+// out[p][j] = sum_k A[p][k] * B[k][j], k = tap * kc + c over 9 taps and kc
+// input channels.  Forward: A = input at p + offset(tap), B[k][j] = W[j][k]
+// (weights stored [cout][kpad]).  Backward to the input (flip): A = dZ at
+// p - offset(tap), B[k][j] = Wt[j][k] (stored [cin][kpad], k = tap * cout +
+// co).  One wave computes 16 consecutive pixels x NT*16 outputs; k beyond
+// 9 * kc reads zeros.  Requirements (checked by the host): pixel count a
+// multiple of 64, kc a multiple of 8, channel strides multiples of 8,
+// kpad = roundup(9 * kc, 32).
+// ---------------------------------------------------------------------------
+struct ConvArgs {
+  const uint16_t* x;  // A source (bf16), pixel stride xcs
+  uint32_t xcs, kc;
+  const uint16_t* w;  // B source, [NT*16 rows][kpad]
+  uint32_t kpad;
+  const float* bias;  // or null
+  uint32_t nout;      // valid outputs (<= NT*16)
+  uint16_t* y;        // bf16 output (pixel stride ycs) or null
+  uint32_t ycs;
+  float* yf;          // f32 output (pixel stride yfcs) or null
+  uint32_t yfcs;
+  uint32_t accumulate;  // yf += instead of =
+  uint32_t relu;
+  uint32_t flip;        // backward to the input: offsets mirrored
+  uint32_t npix, h, w_;
+};
+
+template <int NT>
+__global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t p0 = (blockIdx.x * 4 + wave) * 16;
+  if (p0 >= a.npix) return;
+  const uint32_t prow = p0 + (lane & 15);
+  const uint32_t hw = a.h * a.w_;
+  const uint32_t b = prow / hw, rem = prow % hw;
+  const int yy = (int)(rem / a.w_), xx = (int)(rem % a.w_);
+  f4 acc[NT];
+  for (int t = 0; t < NT; t++) acc[t] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+  const uint32_t kvalid = 9 * a.kc;
+  const uint32_t nchunks = a.kpad / 32;
+  for (uint32_t kc32 = 0; kc32 < nchunks; kc32++) {
+    const uint32_t k0 = kc32 * 32 + 8 * (lane >> 4);
+    bf8 av = zero8();
+    if (k0 < kvalid) {
+      const uint32_t tap = k0 / a.kc, c = k0 % a.kc;
+      int dy = (int)(tap / 3) - 1, dx = (int)(tap % 3) - 1;
+      if (a.flip) {
+        dy = -dy;
+        dx = -dx;
+      }
+      const int sy = yy + dy, sx = xx + dx;
+      if (sy >= 0 && sy < (int)a.h && sx >= 0 && sx < (int)a.w_)
+        av = load8(a.x + ((size_t)b * hw + (uint32_t)sy * a.w_ + (uint32_t)sx) * a.xcs + c);
+    }
+    for (int t = 0; t < NT; t++) {
+      const bf8 bv = load8(a.w + (size_t)(t * 16 + (lane & 15)) * a.kpad + k0);
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[t], 0, 0, 0);
+    }
+  }
+  for (int t = 0; t < NT; t++) {
+    const uint32_t j = t * 16 + (lane & 15);
+    if (j >= a.nout) continue;
+    const float bj = a.bias ? a.bias[j] : 0.0f;
+    for (int r = 0; r < 4; r++) {
+      const uint32_t p = p0 + 4 * (lane >> 4) + r;
+      float v = acc[t][r] + bj;
+      if (a.relu) v = fmaxf(v, 0.0f);
+      if (a.y) a.y[(size_t)p * a.ycs + j] = to_bf16(v);
+      if (a.yf) {
+        float* o = a.yf + (size_t)p * a.yfcs + j;
+        *o = a.accumulate ? *o + v : v;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Backward to the weights: dW[co][tap * cin + c] += sum_p dZ[p][co] *
+// X[p + offset(tap)][c], and db[co] += sum_p dZ[p][co].  One wave computes a
+// 16 (co) x 64 (k) tile over a range of pixels (split-K over blocks, FP32
+// atomics into dW).  K of the MFMA runs over 32 pixels.
+// ---------------------------------------------------------------------------
+struct WGradArgs {
+  const uint16_t* dz;  // [p][zcs] bf16
+  uint32_t zcs, cout;
+  const uint16_t* x;   // [p][xcs] bf16
+  uint32_t xcs, cin;
+  float* dw;           // [cout][9 * cin]
+  float* db;           // [cout] or null
+  uint32_t npix, h, w_;
+  uint32_t pix_per_block;
+};
+
+__global__ __launch_bounds__(256) void k_conv_wgrad(WGradArgs a) {
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t ncols = 9 * a.cin;
+  const uint32_t co = blockIdx.y * 16 + (lane & 15);            // A row
+  const uint32_t col0 = (blockIdx.x * 4 + wave) * 64;           // first of 4 column tiles
+  if (col0 >= ncols) return;
+  const uint32_t pbeg = blockIdx.z * a.pix_per_block;
+  const uint32_t pend = min(a.npix, pbeg + a.pix_per_block);
+  const uint32_t hw = a.h * a.w_;
+  f4 acc[4];
+  for (int t = 0; t < 4; t++) acc[t] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+  float bsum = 0.0f;
+  // column (tap, c) of this lane in each tile
+  int cdy[4], cdx[4];
+  uint32_t cc[4];
+  bool cvalid[4];
+  for (int t = 0; t < 4; t++) {
+    const uint32_t j = col0 + t * 16 + (lane & 15);
+    cvalid[t] = j < ncols;
+    const uint32_t tap = cvalid[t] ? j / a.cin : 0, c = cvalid[t] ? j % a.cin : 0;
+    cdy[t] = (int)(tap / 3) - 1;
+    cdx[t] = (int)(tap % 3) - 1;
+    cc[t] = c;
+  }
+  const bool co_ok = co < a.cout;
+  for (uint32_t pk = pbeg; pk < pend; pk += 32) {
+    uint16_t av[8];
+    const uint32_t pb = pk + 8 * (lane >> 4);
+    for (int r = 0; r < 8; r++) {
+      const uint32_t p = pb + r;
+      av[r] = (co_ok && p < pend) ? a.dz[(size_t)p * a.zcs + co] : (uint16_t)0;
+    }
+    if (a.db && blockIdx.x == 0 && wave == 0)
+      for (int r = 0; r < 8; r++) bsum += from_bf16(av[r]);
+    bf8 A;
+    for (int r = 0; r < 8; r++) A[r] = __builtin_bit_cast(__bf16, av[r]);
+    for (int t = 0; t < 4; t++) {
+      bf8 B;
+      for (int r = 0; r < 8; r++) {
+        const uint32_t p = pb + r;
+        uint16_t v = 0;
+        if (cvalid[t] && p < pend) {
+          const uint32_t bb = p / hw, rem = p % hw;
+          const int sy = (int)(rem / a.w_) + cdy[t], sx = (int)(rem % a.w_) + cdx[t];
+          if (sy >= 0 && sy < (int)a.h && sx >= 0 && sx < (int)a.w_)
+            v = a.x[((size_t)bb * hw + (uint32_t)sy * a.w_ + (uint32_t)sx) * a.xcs + cc[t]];
+        }
+        B[r] = __builtin_bit_cast(__bf16, v);
+      }
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc[t], 0, 0, 0);
+    }
+  }
+  // D rows are co = blockIdx.y * 16 + 4 * (lane / 16) + r, columns lane % 16
+  for (int t = 0; t < 4; t++) {
+    const uint32_t j = col0 + t * 16 + (lane & 15);
+    if (j >= ncols) continue;
+    for (int r = 0; r < 4; r++) {
+      const uint32_t o = blockIdx.y * 16 + 4 * (lane >> 4) + r;
+      if (o < a.cout) atomicAdd(&a.dw[(size_t)o * ncols + j], acc[t][r]);
+    }
+  }
+  if (a.db && blockIdx.x == 0 && wave == 0) {
+    bsum += __shfl_xor(bsum, 16);
+    bsum += __shfl_xor(bsum, 32);
+    if (lane < 16 && co_ok) atomicAdd(&a.db[co], bsum);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Elementwise and resampling kernels
+// ---------------------------------------------------------------------------
+
+// dZ = dY * (Y > 0) for ReLU layers (or dY for linear ones), as BF16 with
+// the padded channels [c, zcs) zeroed.
+__global__ void k_relu_grad(const float* dy, uint32_t dycs, const uint16_t* y, uint32_t ycs,
+                            uint32_t c, uint16_t* dz, uint32_t zcs, uint32_t npix, uint32_t relu) {
+  const size_t n = (size_t)npix * zcs;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const size_t p = i / zcs;
+    const uint32_t ch = (uint32_t)(i % zcs);
+    float v = 0.0f;
+    if (ch < c) {
+      v = dy[p * dycs + ch];
+      if (relu && !(from_bf16(y[p * ycs + ch]) > 0.0f)) v = 0.0f;
+    }
+    dz[i] = to_bf16(v);
+  }
+}
+
+// 2x2 average pooling, BF16 in and out.
+__global__ void k_pool(const uint16_t* x, uint32_t xcs, uint16_t* y, uint32_t ycs, uint32_t c,
+                       uint32_t nb, uint32_t h2, uint32_t w2) {
+  const size_t n = (size_t)nb * h2 * w2 * c;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const uint32_t ch = (uint32_t)(i % c);
+    const size_t q = i / c;
+    const uint32_t X = (uint32_t)(q % w2), Y = (uint32_t)((q / w2) % h2), B = (uint32_t)(q / ((size_t)w2 * h2));
+    const size_t base = ((size_t)B * (2 * h2) + 2 * Y) * (2 * w2) + 2 * X;
+    const float s = from_bf16(x[base * xcs + ch]) + from_bf16(x[(base + 1) * xcs + ch]) +
+                    from_bf16(x[(base + 2 * w2) * xcs + ch]) +
+                    from_bf16(x[(base + 2 * w2 + 1) * xcs + ch]);
+    y[q * ycs + ch] = to_bf16(0.25f * s);
+  }
+}
+
+// Backward of the pooling: dx += dy / 4 on each of the 4 inputs.
+__global__ void k_pool_grad(const float* dy, uint32_t dycs, float* dx, uint32_t dxcs, uint32_t c,
+                            uint32_t nb, uint32_t h2, uint32_t w2) {
+  const size_t n = (size_t)nb * h2 * w2 * c;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const uint32_t ch = (uint32_t)(i % c);
+    const size_t q = i / c;
+    const uint32_t X = (uint32_t)(q % w2), Y = (uint32_t)((q / w2) % h2), B = (uint32_t)(q / ((size_t)w2 * h2));
+    const size_t base = ((size_t)B * (2 * h2) + 2 * Y) * (2 * w2) + 2 * X;
+    const float g = 0.25f * dy[q * dycs + ch];
+    dx[base * dxcs + ch] += g;
+    dx[(base + 1) * dxcs + ch] += g;
+    dx[(base + 2 * w2) * dxcs + ch] += g;
+    dx[(base + 2 * w2 + 1) * dxcs + ch] += g;
+  }
+}
+
+// Nearest 2x upsampling, BF16.
+__global__ void k_upsample(const uint16_t* x, uint32_t xcs, uint16_t* y, uint32_t ycs, uint32_t c,
+                           uint32_t nb, uint32_t h, uint32_t w) {
+  const size_t n = (size_t)nb * h * w * c;  // output elements (h, w: output size)
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const uint32_t ch = (uint32_t)(i % c);
+    const size_t q = i / c;
+    const uint32_t X = (uint32_t)(q % w), Y = (uint32_t)((q / w) % h), B = (uint32_t)(q / ((size_t)w * h));
+    const size_t s = ((size_t)B * (h / 2) + Y / 2) * (w / 2) + X / 2;
+    y[q * ycs + ch] = x[s * xcs + ch];
+  }
+}
+
+// Backward of the upsampling: dx = sum of the 4 output gradients.
+__global__ void k_upsample_grad(const float* dy, uint32_t dycs, float* dx, uint32_t dxcs,
+                                uint32_t c, uint32_t nb, uint32_t h2, uint32_t w2) {
+  const size_t n = (size_t)nb * h2 * w2 * c;  // input elements (h2, w2: input size)
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const uint32_t ch = (uint32_t)(i % c);
+    const size_t q = i / c;
+    const uint32_t X = (uint32_t)(q % w2), Y = (uint32_t)((q / w2) % h2), B = (uint32_t)(q / ((size_t)w2 * h2));
+    const size_t base = ((size_t)B * (2 * h2) + 2 * Y) * (2 * w2) + 2 * X;
+    dx[q * dxcs + ch] = dy[base * dycs + ch] + dy[(base + 1) * dycs + ch] +
+                        dy[(base + 2 * w2) * dycs + ch] + dy[(base + 2 * w2 + 1) * dycs + ch];
+  }
+}
+
+// Input features (row-major f32, 10 channels) -> BF16 [p][16], zero padded.
+__global__ void k_to_input(const float* in, uint16_t* x, uint32_t npix) {
+  const size_t n = (size_t)npix * 16;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const size_t p = i / 16;
+    const uint32_t ch = (uint32_t)(i % 16);
+    x[i] = ch < 10 ? to_bf16(in[p * 10 + ch]) : (uint16_t)0;
+  }
+}
+
+// L1 loss in log space: out (predicted log1p radiance, [p][3]) against
+// log1p(target); writes dL/dout / npix*3 into dout ([p][3]) and adds the
+// loss sum to *loss.
+__global__ void k_l1_loss(const float* out, const float* target, float* dout, uint32_t npix,
+                          float* loss) {
+  __shared__ float part[256];
+  float s = 0.0f;
+  const float scale = 1.0f / (3.0f * (float)npix);
+  const size_t n = (size_t)npix * 3;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const float d = out[i] - log1pf(fmaxf(target[i], 0.0f));
+    s += fabsf(d);
+    dout[i] = d > 0.0f ? scale : (d < 0.0f ? -scale : 0.0f);
+  }
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (uint32_t o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) part[threadIdx.x] += part[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(loss, part[0] * scale);
+}
+
+// Adam (Kingma & Ba) on the flat FP32 parameters.
+__global__ void k_adam(float* w, const float* g, float* m, float* v, uint32_t n, float lr,
+                       float b1, float b2, float eps, float c1, float c2) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  }
+}
+
+// FP32 master weights [cout][9][cin] -> BF16 W [cout_pad][kpad] (k = tap *
+// cin + c) and Wt [cin][kpad_t] (k = tap * cout_pad + co), zero padded.
+__global__ void k_pack(const float* w, uint32_t cout, uint32_t cin, uint32_t cout_pad,
+                       uint32_t kpad, uint32_t kpad_t, uint16_t* W, uint16_t* Wt) {
+  const uint32_t n1 = cout_pad * kpad, n2 = cin * kpad_t;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n1 + n2;
+       i += gridDim.x * blockDim.x) {
+    if (i < n1) {
+      const uint32_t co = i / kpad, k = i % kpad;
+      float v = 0.0f;
+      if (co < cout && k < 9 * cin) v = w[((size_t)co * 9 + k / cin) * cin + k % cin];
+      W[i] = to_bf16(v);
+    } else {
+      const uint32_t e = i - n1;
+      const uint32_t c = e / kpad_t, k = e % kpad_t;
+      float v = 0.0f;
+      if (k < 9 * cout_pad) {
+        const uint32_t tap = k / cout_pad, co = k % cout_pad;
+        if (co < cout) v = w[((size_t)co * 9 + tap) * cin + c];
+      }
+      Wt[e] = to_bf16(v);
+    }
+  }
+}
+
+__global__ void k_fill(float* p, size_t n, float v) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// Renderer outputs -> network input: log1p of the mean radiance (16x4-block
+// HDR layout / spp), albedo, octahedral normal, log1p depth, spp -- the
+// features summed over spp frames, 8 f32 per pixel (albedo rgb, normal xyz,
+// depth, 0) -- as 10 f32 per pixel, rows y from 0.
+__global__ void k_features(const float* hdr, const float* feat, uint32_t width, uint32_t height,
+                           uint32_t spp, float* out) {
+  const uint32_t npix = width * height;
+  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < npix; p += gridDim.x * blockDim.x) {
+    const uint32_t px = p % width, py = p / width;
+    const float* h = hdr + cvk::hdr_offset(px, py, width);
+    const float inv = 1.0f / (float)spp;
+    const float* f = feat + (size_t)p * 8;
+    float* o = out + (size_t)p * 10;
+    o[0] = log1pf(fmaxf(h[0] * inv, 0.0f));
+    o[1] = log1pf(fmaxf(h[8] * inv, 0.0f));
+    o[2] = log1pf(fmaxf(h[16] * inv, 0.0f));
+    o[3] = f[0] * inv;
+    o[4] = f[1] * inv;
+    o[5] = f[2] * inv;
+    float nx = f[3] * inv, ny = f[4] * inv, nz = f[5] * inv;
+    const float l1 = fabsf(nx) + fabsf(ny) + fabsf(nz);
+    float ox = 0.0f, oy = 0.0f;
+    if (l1 > 0.0f) {
+      ox = nx / l1;
+      oy = ny / l1;
+      if (nz < 0.0f) {
+        const float tx = (1.0f - fabsf(oy)) * (ox >= 0.0f ? 1.0f : -1.0f);
+        const float ty = (1.0f - fabsf(ox)) * (oy >= 0.0f ? 1.0f : -1.0f);
+        ox = tx;
+        oy = ty;
+      }
+    }
+    o[6] = ox;
+    o[7] = oy;
+    o[8] = log1pf(fmaxf(f[6] * inv, 0.0f));
+    o[9] = (float)spp;
+  }
+}
+
+// Network output (log1p radiance, [p][3]) -> 16x4-block HDR holding
+// spp * radiance, so that it converts like an accumulation of spp frames.
+__global__ void k_output(const float* out, uint32_t width, uint32_t height, uint32_t spp,
+                         float* hdr) {
+  const uint32_t npix = width * height;
+  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < npix; p += gridDim.x * blockDim.x) {
+    const uint32_t px = p % width, py = p / width;
+    float* h = hdr + cvk::hdr_offset(px, py, width);
+    for (int c = 0; c < 3; c++) h[8 * c] = expm1f(fmaxf(out[(size_t)p * 3 + c], 0.0f)) * (float)spp;
+  }
+}
+
+}  // namespace cvd
+
+// ---------------------------------------------------------------------------
+// Host side: the network, its buffers and the training step.
+// ---------------------------------------------------------------------------
+
+namespace {
+
+inline uint32_t round_up(uint32_t x, uint32_t m) { return (x + m - 1) / m * m; }
+inline uint32_t grid1(size_t n) { return cvh::grid_for(n, 256, 65536); }
+
+struct Conv {
+  uint32_t cin, cout, cout_pad, kpad, kpad_t;
+  bool relu;
+  size_t w_off, b_off;  // offsets into the flat parameter vector
+  DevBuf<uint16_t> W, Wt;
+};
+
+// A BF16 NHWC tensor view and its FP32 gradient view.
+struct Act {
+  uint16_t* p = nullptr;
+  uint32_t cs = 0;  // pixel stride in elements
+  float* g = nullptr;
+  uint32_t gcs = 0;
+};
+
+// The 11 convolutions: enc0a enc0b enc1a enc1b bot2a bot2b dec1a dec1b
+// dec0a dec0b out.
+const uint32_t kLayers[11][3] = {
+    {16, 32, 1},  {32, 32, 1},   {32, 64, 1},  {64, 64, 1},  {64, 128, 1}, {128, 128, 1},
+    {192, 64, 1}, {64, 64, 1},   {96, 32, 1},  {32, 32, 1},  {32, 3, 0}};
+
+}  // namespace
+
+struct cv_denoiser {
+  hipStream_t stream = nullptr;
+  Conv conv[11];
+  size_t nparams = 0;
+  DevBuf<float> params, grads, m, v;
+  uint64_t step = 0;
+  // activations for the current size
+  uint32_t nb = 0, h = 0, w = 0;
+  DevBuf<uint16_t> x0, e0a, cat0, p0, e1a, cat1, p1, b2a, b2b, d1a, d1b, d0a, d0b, dz;
+  DevBuf<float> out, dout, g_e0a, g_cat0, g_p0, g_e1a, g_cat1, g_p1, g_b2a, g_b2b, g_d1a, g_d1b,
+      g_d0a, g_d0b, g_up1, g_up0, loss;
+  DevBuf<float> in_f32, tgt_f32;
+  std::mutex mu;
+};
+
+namespace {
+
+cv_status launch_conv(cv_denoiser* d, const cvd::ConvArgs& a, uint32_t nt) {
+  const uint32_t blocks = (a.npix + 63) / 64;  // npix is a multiple of 16: whole waves
+  hipStream_t s = d->stream;
+  switch (nt) {
+    case 1: hipLaunchKernelGGL(cvd::k_conv<1>, dim3(blocks), dim3(256), 0, s, a); break;
+    case 2: hipLaunchKernelGGL(cvd::k_conv<2>, dim3(blocks), dim3(256), 0, s, a); break;
+    case 4: hipLaunchKernelGGL(cvd::k_conv<4>, dim3(blocks), dim3(256), 0, s, a); break;
+    case 6: hipLaunchKernelGGL(cvd::k_conv<6>, dim3(blocks), dim3(256), 0, s, a); break;
+    case 8: hipLaunchKernelGGL(cvd::k_conv<8>, dim3(blocks), dim3(256), 0, s, a); break;
+    case 12: hipLaunchKernelGGL(cvd::k_conv<12>, dim3(blocks), dim3(256), 0, s, a); break;
+    default: return fail(CV_ERR_INVALID_ARGUMENT, "unsupported conv width");
+  }
+  HIPTRY(hipGetLastError(), "conv");
+  return CV_OK;
+}
+
+// y = relu(conv(x) + b) at level size (h, w).
+cv_status conv_fwd(cv_denoiser* d, const Conv& L, const uint16_t* x, uint32_t xcs, uint16_t* y,
+                   uint32_t ycs, float* yf, uint32_t yfcs, uint32_t h, uint32_t w) {
+  cvd::ConvArgs a{};
+  a.x = x;
+  a.xcs = xcs;
+  a.kc = L.cin;
+  a.w = L.W.p;
+  a.kpad = L.kpad;
+  a.bias = d->params.p + L.b_off;
+  a.nout = L.cout;
+  a.y = y;
+  a.ycs = ycs;
+  a.yf = yf;
+  a.yfcs = yfcs;
+  a.relu = L.relu ? 1u : 0u;
+  a.npix = d->nb * h * w;
+  a.h = h;
+  a.w_ = w;
+  return launch_conv(d, a, L.cout_pad / 16);
+}
+
+// Backward of one convolution given dY (f32 at y's shape): dZ (ReLU mask),
+// dW and db, and dX (+= into gx when gx != null).
+cv_status conv_bwd(cv_denoiser* d, const Conv& L, const uint16_t* x, uint32_t xcs,
+                   const uint16_t* y, uint32_t ycs, const float* gy, uint32_t gycs, float* gx,
+                   uint32_t gxcs, bool accumulate, uint32_t h, uint32_t w) {
+  const uint32_t npix = d->nb * h * w;
+  hipStream_t s = d->stream;
+  hipLaunchKernelGGL(cvd::k_relu_grad, dim3(grid1((size_t)npix * L.cout_pad)), dim3(256), 0, s,
+                     gy, gycs, y, ycs, L.cout, d->dz.p, L.cout_pad, npix, L.relu ? 1u : 0u);
+  HIPTRY(hipGetLastError(), "relu grad");
+  cvd::WGradArgs g{};
+  g.dz = d->dz.p;
+  g.zcs = L.cout_pad;
+  g.cout = L.cout;
+  g.x = x;
+  g.xcs = xcs;
+  g.cin = L.cin;
+  g.dw = d->grads.p + L.w_off;
+  g.db = d->grads.p + L.b_off;
+  g.npix = npix;
+  g.h = h;
+  g.w_ = w;
+  g.pix_per_block = 4096;
+  dim3 grid((9 * L.cin + 255) / 256, L.cout_pad / 16, (npix + 4095) / 4096);
+  hipLaunchKernelGGL(cvd::k_conv_wgrad, grid, dim3(256), 0, s, g);
+  HIPTRY(hipGetLastError(), "wgrad");
+  if (gx) {
+    cvd::ConvArgs a{};
+    a.x = d->dz.p;
+    a.xcs = L.cout_pad;
+    a.kc = L.cout_pad;
+    a.w = L.Wt.p;
+    a.kpad = L.kpad_t;
+    a.bias = nullptr;
+    a.nout = L.cin;
+    a.yf = gx;
+    a.yfcs = gxcs;
+    a.accumulate = accumulate ? 1u : 0u;
+    a.flip = 1;
+    a.npix = npix;
+    a.h = h;
+    a.w_ = w;
+    cv_status cs = launch_conv(d, a, round_up(L.cin, 16) / 16);
+    if (cs != CV_OK) return cs;
+  }
+  return CV_OK;
+}
+
+cv_status reserve_acts(cv_denoiser* d, uint32_t nb, uint32_t h, uint32_t w) {
+  if (d->nb == nb && d->h == h && d->w == w) return CV_OK;
+  const size_t P0 = (size_t)nb * h * w, P1 = P0 / 4, P2 = P0 / 16;
+  HIPTRY(d->x0.alloc(P0 * 16), "alloc");
+  HIPTRY(d->e0a.alloc(P0 * 32), "alloc");
+  HIPTRY(d->cat0.alloc(P0 * 96), "alloc");
+  HIPTRY(d->p0.alloc(P1 * 32), "alloc");
+  HIPTRY(d->e1a.alloc(P1 * 64), "alloc");
+  HIPTRY(d->cat1.alloc(P1 * 192), "alloc");
+  HIPTRY(d->p1.alloc(P2 * 64), "alloc");
+  HIPTRY(d->b2a.alloc(P2 * 128), "alloc");
+  HIPTRY(d->b2b.alloc(P2 * 128), "alloc");
+  HIPTRY(d->d1a.alloc(P1 * 64), "alloc");
+  HIPTRY(d->d1b.alloc(P1 * 64), "alloc");
+  HIPTRY(d->d0a.alloc(P0 * 32), "alloc");
+  HIPTRY(d->d0b.alloc(P0 * 32), "alloc");
+  HIPTRY(d->dz.alloc(P0 * 128), "alloc");
+  HIPTRY(d->out.alloc(P0 * 3), "alloc");
+  HIPTRY(d->dout.alloc(P0 * 3), "alloc");
+  HIPTRY(d->g_e0a.alloc(P0 * 32), "alloc");
+  HIPTRY(d->g_cat0.alloc(P0 * 96), "alloc");
+  HIPTRY(d->g_p0.alloc(P1 * 32), "alloc");
+  HIPTRY(d->g_e1a.alloc(P1 * 64), "alloc");
+  HIPTRY(d->g_cat1.alloc(P1 * 192), "alloc");
+  HIPTRY(d->g_p1.alloc(P2 * 64), "alloc");
+  HIPTRY(d->g_b2a.alloc(P2 * 128), "alloc");
+  HIPTRY(d->g_b2b.alloc(P2 * 128), "alloc");
+  HIPTRY(d->g_d1a.alloc(P1 * 64), "alloc");
+  HIPTRY(d->g_d1b.alloc(P1 * 64), "alloc");
+  HIPTRY(d->g_d0a.alloc(P0 * 32), "alloc");
+  HIPTRY(d->g_d0b.alloc(P0 * 32), "alloc");
+  HIPTRY(d->loss.alloc(1), "alloc");
+  d->nb = nb;
+  d->h = h;
+  d->w = w;
+  return CV_OK;
+}
+
+cv_status pack_weights(cv_denoiser* d) {
+  for (Conv& L : d->conv) {
+    const uint32_t n = L.cout_pad * L.kpad + L.cin * L.kpad_t;
+    hipLaunchKernelGGL(cvd::k_pack, dim3(grid1(n)), dim3(256), 0, d->stream, d->params.p + L.w_off,
+                       L.cout, L.cin, L.cout_pad, L.kpad, L.kpad_t, L.W.p, L.Wt.p);
+    HIPTRY(hipGetLastError(), "pack");
+  }
+  return CV_OK;
+}
+
+// Forward pass from d->x0 to d->out (f32 log1p radiance, [p][3]).
+cv_status forward(cv_denoiser* d) {
+  const uint32_t H = d->h, W = d->w, H1 = H / 2, W1 = W / 2, H2 = H / 4, W2 = W / 4;
+  auto& L = d->conv;
+  hipStream_t s = d->stream;
+  cv_status cs;
+#define CVD_TRY(x) \
+  if ((cs = (x)) != CV_OK) return cs
+  CVD_TRY(conv_fwd(d, L[0], d->x0.p, 16, d->e0a.p, 32, nullptr, 0, H, W));
+  CVD_TRY(conv_fwd(d, L[1], d->e0a.p, 32, d->cat0.p + 64, 96, nullptr, 0, H, W));
+  hipLaunchKernelGGL(cvd::k_pool, dim3(grid1((size_t)d->nb * H1 * W1 * 32)), dim3(256), 0, s,
+                     d->cat0.p + 64, 96u, d->p0.p, 32u, 32u, d->nb, H1, W1);
+  CVD_TRY(conv_fwd(d, L[2], d->p0.p, 32, d->e1a.p, 64, nullptr, 0, H1, W1));
+  CVD_TRY(conv_fwd(d, L[3], d->e1a.p, 64, d->cat1.p + 128, 192, nullptr, 0, H1, W1));
+  hipLaunchKernelGGL(cvd::k_pool, dim3(grid1((size_t)d->nb * H2 * W2 * 64)), dim3(256), 0, s,
+                     d->cat1.p + 128, 192u, d->p1.p, 64u, 64u, d->nb, H2, W2);
+  CVD_TRY(conv_fwd(d, L[4], d->p1.p, 64, d->b2a.p, 128, nullptr, 0, H2, W2));
+  CVD_TRY(conv_fwd(d, L[5], d->b2a.p, 128, d->b2b.p, 128, nullptr, 0, H2, W2));
+  hipLaunchKernelGGL(cvd::k_upsample, dim3(grid1((size_t)d->nb * H1 * W1 * 128)), dim3(256), 0, s,
+                     d->b2b.p, 128u, d->cat1.p, 192u, 128u, d->nb, H1, W1);
+  CVD_TRY(conv_fwd(d, L[6], d->cat1.p, 192, d->d1a.p, 64, nullptr, 0, H1, W1));
+  CVD_TRY(conv_fwd(d, L[7], d->d1a.p, 64, d->d1b.p, 64, nullptr, 0, H1, W1));
+  hipLaunchKernelGGL(cvd::k_upsample, dim3(grid1((size_t)d->nb * H * W * 64)), dim3(256), 0, s,
+                     d->d1b.p, 64u, d->cat0.p, 96u, 64u, d->nb, H, W);
+  CVD_TRY(conv_fwd(d, L[8], d->cat0.p, 96, d->d0a.p, 32, nullptr, 0, H, W));
+  CVD_TRY(conv_fwd(d, L[9], d->d0a.p, 32, d->d0b.p, 32, nullptr, 0, H, W));
+  CVD_TRY(conv_fwd(d, L[10], d->d0b.p, 32, nullptr, 0, d->out.p, 3, H, W));
+  HIPTRY(hipGetLastError(), "forward");
+  return CV_OK;
+}
+
+// Backward pass from d->dout to d->grads (zeroed here first).
+cv_status backward(cv_denoiser* d) {
+  const uint32_t H = d->h, W = d->w, H1 = H / 2, W1 = W / 2, H2 = H / 4, W2 = W / 4;
+  auto& L = d->conv;
+  hipStream_t s = d->stream;
+  const uint32_t nb = d->nb;
+  cv_status cs;
+  hipLaunchKernelGGL(cvd::k_fill, dim3(grid1(d->nparams)), dim3(256), 0, s, d->grads.p,
+                     d->nparams, 0.0f);
+  // out <- d0b
+  CVD_TRY(conv_bwd(d, L[10], d->d0b.p, 32, nullptr, 0, d->dout.p, 3, d->g_d0b.p, 32, false, H, W));
+  CVD_TRY(conv_bwd(d, L[9], d->d0a.p, 32, d->d0b.p, 32, d->g_d0b.p, 32, d->g_d0a.p, 32, false, H,
+                   W));
+  CVD_TRY(conv_bwd(d, L[8], d->cat0.p, 96, d->d0a.p, 32, d->g_d0a.p, 32, d->g_cat0.p, 96, false,
+                   H, W));
+  // cat0[0:64] = up(d1b)
+  hipLaunchKernelGGL(cvd::k_upsample_grad, dim3(grid1((size_t)nb * H1 * W1 * 64)), dim3(256), 0, s,
+                     d->g_cat0.p, 96u, d->g_d1b.p, 64u, 64u, nb, H1, W1);
+  CVD_TRY(conv_bwd(d, L[7], d->d1a.p, 64, d->d1b.p, 64, d->g_d1b.p, 64, d->g_d1a.p, 64, false,
+                   H1, W1));
+  CVD_TRY(conv_bwd(d, L[6], d->cat1.p, 192, d->d1a.p, 64, d->g_d1a.p, 64, d->g_cat1.p, 192, false,
+                   H1, W1));
+  // cat1[0:128] = up(b2b)
+  hipLaunchKernelGGL(cvd::k_upsample_grad, dim3(grid1((size_t)nb * H2 * W2 * 128)), dim3(256), 0,
+                     s, d->g_cat1.p, 192u, d->g_b2b.p, 128u, 128u, nb, H2, W2);
+  CVD_TRY(conv_bwd(d, L[5], d->b2a.p, 128, d->b2b.p, 128, d->g_b2b.p, 128, d->g_b2a.p, 128, false,
+                   H2, W2));
+  CVD_TRY(conv_bwd(d, L[4], d->p1.p, 64, d->b2a.p, 128, d->g_b2a.p, 128, d->g_p1.p, 64, false, H2,
+                   W2));
+  // p1 = pool(cat1[128:192]): its gradient joins the skip gradient in g_cat1
+  hipLaunchKernelGGL(cvd::k_pool_grad, dim3(grid1((size_t)nb * H2 * W2 * 64)), dim3(256), 0, s,
+                     d->g_p1.p, 64u, d->g_cat1.p + 128, 192u, 64u, nb, H2, W2);
+  CVD_TRY(conv_bwd(d, L[3], d->e1a.p, 64, d->cat1.p + 128, 192, d->g_cat1.p + 128, 192,
+                   d->g_e1a.p, 64, false, H1, W1));
+  CVD_TRY(conv_bwd(d, L[2], d->p0.p, 32, d->e1a.p, 64, d->g_e1a.p, 64, d->g_p0.p, 32, false, H1,
+                   W1));
+  hipLaunchKernelGGL(cvd::k_pool_grad, dim3(grid1((size_t)nb * H1 * W1 * 32)), dim3(256), 0, s,
+                     d->g_p0.p, 32u, d->g_cat0.p + 64, 96u, 32u, nb, H1, W1);
+  CVD_TRY(conv_bwd(d, L[1], d->e0a.p, 32, d->cat0.p + 64, 96, d->g_cat0.p + 64, 96, d->g_e0a.p,
+                   32, false, H, W));
+  CVD_TRY(conv_bwd(d, L[0], d->x0.p, 16, d->e0a.p, 32, d->g_e0a.p, 32, nullptr, 0, false, H, W));
+  HIPTRY(hipGetLastError(), "backward");
+  return CV_OK;
+#undef CVD_TRY
+}
+
+cv_status check_shape(uint32_t nb, uint32_t h, uint32_t w) {
+  if (nb == 0 || h == 0 || w == 0 || (h & 3) || (w & 63))
+    return fail(CV_ERR_INVALID_ARGUMENT, "denoiser sizes: height a multiple of 4, width of 64");
+  if ((uint64_t)nb * h * w >= (1ull << 28)) return fail(CV_ERR_INVALID_ARGUMENT, "batch too large");
+  return CV_OK;
+}
+
+// Uploads a host or device f32 array into buf.
+cv_status stage_in(cv_denoiser* d, DevBuf<float>& buf, const float* src, size_t n, bool dev,
+                   const float** out) {
+  if (dev) {
+    *out = src;
+    return CV_OK;
+  }
+  if (buf.n < n) HIPTRY(buf.alloc(n), "alloc");
+  HIPTRY(hipMemcpyAsync(buf.p, src, n * 4, hipMemcpyHostToDevice, d->stream), "upload");
+  *out = buf.p;
+  return CV_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t cv_denoiser_num_params(void) {
+  size_t n = 0;
+  for (auto& l : kLayers) n += (size_t)l[1] * 9 * l[0] + l[1];
+  return n;
+}
+
+cv_status cv_denoiser_create(uint64_t seed, cv_denoiser** out) {
+  if (!out) return fail(CV_ERR_INVALID_ARGUMENT, "null argument");
+  *out = nullptr;
+  cv_status cs = cvh::ensure_device();
+  if (cs != CV_OK) return cs;
+  cv_denoiser* d = new cv_denoiser();
+  auto bail = [&](cv_status st) {
+    cv_denoiser_destroy(d);
+    return st;
+  };
+  hipError_t e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) return bail(hip_fail(e, "hipStreamCreate"));
+  size_t off = 0;
+  for (int i = 0; i < 11; i++) {
+    Conv& L = d->conv[i];
+    L.cin = kLayers[i][0];
+    L.cout = kLayers[i][1];
+    L.relu = kLayers[i][2] != 0;
+    L.cout_pad = round_up(L.cout, 16);
+    L.kpad = round_up(9 * L.cin, 32);
+    L.kpad_t = round_up(9 * L.cout_pad, 32);
+    L.w_off = off;
+    off += (size_t)L.cout * 9 * L.cin;
+    L.b_off = off;
+    off += L.cout;
+    if ((e = L.W.alloc((size_t)L.cout_pad * L.kpad)) || (e = L.Wt.alloc((size_t)L.cin * L.kpad_t)))
+      return bail(hip_fail(e, "alloc weights"));
+  }
+  d->nparams = off;
+  if ((e = d->params.alloc(off)) || (e = d->grads.alloc(off)) || (e = d->m.alloc(off)) ||
+      (e = d->v.alloc(off)))
+    return bail(hip_fail(e, "alloc parameters"));
+  // He-normal initialisation from a splitmix64 stream (Box-Muller), zero bias
+  std::vector<float> h(off, 0.0f);
+  uint64_t z = seed;
+  auto next = [&]() {
+    z += 0x9E3779B97F4A7C15ull;
+    uint64_t x = z;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+  };
+  for (const Conv& L : d->conv) {
+    const float sd = sqrtf(2.0f / (9.0f * (float)(L.cin == 16 ? 10 : L.cin)));
+    for (size_t k = 0; k < (size_t)L.cout * 9 * L.cin; k++) {
+      double u1 = ((next() >> 11) + 1) * (1.0 / 9007199254740993.0);
+      double u2 = (next() >> 11) * (1.0 / 9007199254740992.0);
+      h[L.w_off + k] = (float)(sd * sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2));
+    }
+  }
+  if ((e = hipMemcpyAsync(d->params.p, h.data(), off * 4, hipMemcpyHostToDevice, d->stream)))
+    return bail(hip_fail(e, "upload"));
+  hipLaunchKernelGGL(cvd::k_fill, dim3(grid1(off)), dim3(256), 0, d->stream, d->m.p, off, 0.0f);
+  hipLaunchKernelGGL(cvd::k_fill, dim3(grid1(off)), dim3(256), 0, d->stream, d->v.p, off, 0.0f);
+  if ((cs = pack_weights(d)) != CV_OK) return bail(cs);
+  if ((e = hipStreamSynchronize(d->stream))) return bail(hip_fail(e, "init"));
+  *out = d;
+  return CV_OK;
+}
+
+void cv_denoiser_destroy(cv_denoiser* d) {
+  if (!d) return;
+  if (d->stream) (void)hipStreamSynchronize(d->stream);
+  if (d->stream) (void)hipStreamDestroy(d->stream);
+  delete d;
+}
+
+cv_status cv_denoiser_get_params(cv_denoiser* d, float* params, float* grads) {
+  if (!d) return fail(CV_ERR_INVALID_ARGUMENT, "null argument");
+  std::lock_guard<std::mutex> g(d->mu);
+  if (params)
+    HIPTRY(hipMemcpyAsync(params, d->params.p, d->nparams * 4, hipMemcpyDeviceToHost, d->stream),
+           "download");
+  if (grads)
+    HIPTRY(hipMemcpyAsync(grads, d->grads.p, d->nparams * 4, hipMemcpyDeviceToHost, d->stream),
+           "download");
+  HIPTRY(hipStreamSynchronize(d->stream), "sync");
+  return CV_OK;
+}
+
+cv_status cv_denoiser_set_params(cv_denoiser* d, const float* params) {
+  if (!d || !params) return fail(CV_ERR_INVALID_ARGUMENT, "null argument");
+  std::lock_guard<std::mutex> g(d->mu);
+  HIPTRY(hipMemcpyAsync(d->params.p, params, d->nparams * 4, hipMemcpyHostToDevice, d->stream),
+         "upload");
+  hipLaunchKernelGGL(cvd::k_fill, dim3(grid1(d->nparams)), dim3(256), 0, d->stream, d->m.p,
+                     d->nparams, 0.0f);
+  hipLaunchKernelGGL(cvd::k_fill, dim3(grid1(d->nparams)), dim3(256), 0, d->stream, d->v.p,
+                     d->nparams, 0.0f);
+  d->step = 0;
+  cv_status cs = pack_weights(d);
+  if (cs != CV_OK) return cs;
+  HIPTRY(hipStreamSynchronize(d->stream), "sync");
+  return CV_OK;
+}
+
+cv_status cv_denoiser_forward(cv_denoiser* d, const float* input, float* output, uint32_t nb,
+                              uint32_t h, uint32_t w, uint32_t flags) {
+  if (!d || !input || !output) return fail(CV_ERR_INVALID_ARGUMENT, "null argument");
+  cv_status cs = check_shape(nb, h, w);
+  if (cs != CV_OK) return cs;
+  std::lock_guard<std::mutex> g(d->mu);
+  const bool dev = flags & CV_PTR_DEVICE;
+  if ((cs = reserve_acts(d, nb, h, w)) != CV_OK) return cs;
+  const size_t P = (size_t)nb * h * w;
+  const float* in = nullptr;
+  if ((cs = stage_in(d, d->in_f32, input, P * 10, dev, &in)) != CV_OK) return cs;
+  hipLaunchKernelGGL(cvd::k_to_input, dim3(grid1(P * 16)), dim3(256), 0, d->stream, in, d->x0.p,
+                     (uint32_t)P);
+  if ((cs = forward(d)) != CV_OK) return cs;
+  HIPTRY(hipMemcpyAsync(output, d->out.p, P * 12, dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                        d->stream),
+         "download");
+  HIPTRY(hipStreamSynchronize(d->stream), "sync");
+  return CV_OK;
+}
+
+cv_status cv_denoiser_train_step(cv_denoiser* d, const float* input, const float* target,
+                                 uint32_t nb, uint32_t h, uint32_t w, float lr, uint32_t flags,
+                                 float* loss) {
+  if (!d || !input || !target) return fail(CV_ERR_INVALID_ARGUMENT, "null argument");
+  cv_status cs = check_shape(nb, h, w);
+  if (cs != CV_OK) return cs;
+  std::lock_guard<std::mutex> g(d->mu);
+  const bool dev = flags & CV_PTR_DEVICE;
+  if ((cs = reserve_acts(d, nb, h, w)) != CV_OK) return cs;
+  const size_t P = (size_t)nb * h * w;
+  const float *in = nullptr, *tg = nullptr;
+  if ((cs = stage_in(d, d->in_f32, input, P * 10, dev, &in)) != CV_OK) return cs;
+  if ((cs = stage_in(d, d->tgt_f32, target, P * 3, dev, &tg)) != CV_OK) return cs;
+  hipStream_t s = d->stream;
+  hipLaunchKernelGGL(cvd::k_to_input, dim3(grid1(P * 16)), dim3(256), 0, s, in, d->x0.p,
+                     (uint32_t)P);
+  if ((cs = forward(d)) != CV_OK) return cs;
+  hipLaunchKernelGGL(cvd::k_fill, dim3(1), dim3(64), 0, s, d->loss.p, (size_t)1, 0.0f);
+  hipLaunchKernelGGL(cvd::k_l1_loss, dim3(std::min<size_t>(grid1(P * 3), 1024)), dim3(256), 0, s,
+                     d->out.p, tg, d->dout.p, (uint32_t)P, d->loss.p);
+  if ((cs = backward(d)) != CV_OK) return cs;
+  if (lr > 0.0f) {
+    d->step++;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.0f - powf(b1, (float)d->step), c2 = 1.0f - powf(b2, (float)d->step);
+    hipLaunchKernelGGL(cvd::k_adam, dim3(grid1(d->nparams)), dim3(256), 0, s, d->params.p,
+                       d->grads.p, d->m.p, d->v.p, (uint32_t)d->nparams, lr, b1, b2, 1e-8f, c1, c2);
+    HIPTRY(hipGetLastError(), "adam");
+    if ((cs = pack_weights(d)) != CV_OK) return cs;
+  }
+  float l = 0.0f;
+  HIPTRY(hipMemcpyAsync(&l, d->loss.p, 4, hipMemcpyDeviceToHost, s), "download");
+  HIPTRY(hipStreamSynchronize(s), "sync");
+  if (loss) *loss = l;
+  return CV_OK;
+}
+
+cv_status cv_denoiser_prepare_input(const float* hdr, const float* features, uint32_t width,
+                                    uint32_t height, uint32_t spp, float* input, uint32_t flags) {
+  if (!hdr || !features || !input || spp == 0) return fail(CV_ERR_INVALID_ARGUMENT, "null argument");
+  if ((width & 15) || (height & 3)) return fail(CV_ERR_NOT_MULTIPLE_OF_16, "block layout sizes");
+  cv_status cs = cvh::ensure_device();
+  if (cs != CV_OK) return cs;
+  const bool dev = flags & CV_PTR_DEVICE;
+  const size_t P = (size_t)width * height;
+  DevBuf<float> h, f, o;
+  const float *hp = hdr, *fp = features;
+  float* op = input;
+  if (!dev) {
+    HIPTRY(h.alloc(P * 3), "alloc");
+    HIPTRY(f.alloc(P * 8), "alloc");
+    HIPTRY(o.alloc(P * 10), "alloc");
+    HIPTRY(hipMemcpy(h.p, hdr, P * 12, hipMemcpyHostToDevice), "upload");
+    HIPTRY(hipMemcpy(f.p, features, P * 32, hipMemcpyHostToDevice), "upload");
+    hp = h.p;
+    fp = f.p;
+    op = o.p;
+  }
+  hipLaunchKernelGGL(cvd::k_features, dim3(grid1(P)), dim3(256), 0, 0, hp, fp, width, height, spp,
+                     op);
+  HIPTRY(hipGetLastError(), "features");
+  if (!dev) HIPTRY(hipMemcpy(input, op, P * 40, hipMemcpyDeviceToHost), "download");
+  HIPTRY(hipDeviceSynchronize(), "sync");
+  return CV_OK;
+}
+
+cv_status cv_denoiser_output_to_hdr(const float* output, uint32_t width, uint32_t height,
+                                    uint32_t spp, float* hdr, uint32_t flags) {
+  if (!output || !hdr || spp == 0) return fail(CV_ERR_INVALID_ARGUMENT, "null argument");
+  if ((width & 15) || (height & 3)) return fail(CV_ERR_NOT_MULTIPLE_OF_16, "block layout sizes");
+  cv_status cs = cvh::ensure_device();
+  if (cs != CV_OK) return cs;
+  const bool dev = flags & CV_PTR_DEVICE;
+  const size_t P = (size_t)width * height;
+  DevBuf<float> o, h;
+  const float* op = output;
+  float* hp = hdr;
+  if (!dev) {
+    HIPTRY(o.alloc(P * 3), "alloc");
+    HIPTRY(h.alloc(P * 3), "alloc");
+    HIPTRY(hipMemcpy(o.p, output, P * 12, hipMemcpyHostToDevice), "upload");
+    op = o.p;
+    hp = h.p;
+  }
+  hipLaunchKernelGGL(cvd::k_output, dim3(grid1(P)), dim3(256), 0, 0, op, width, height, spp, hp);
+  HIPTRY(hipGetLastError(), "output");
+  if (!dev) HIPTRY(hipMemcpy(hdr, hp, P * 12, hipMemcpyDeviceToHost), "download");
+  HIPTRY(hipDeviceSynchronize(), "sync");
+  return CV_OK;
+}
+
+}  // extern "C"

@@ -103,6 +103,15 @@ struct DevLight {
   float area;       // 0.5 * rcp(rnorm)
 };
 
+// Offset of a pixel in the 16x4-block HDR layout (renderer.rs:154-206,356-358).
+__device__ __forceinline__ size_t hdr_offset(uint32_t px, uint32_t py, uint32_t width) {
+  uint32_t dx = px & 15u, dy = py & 3u;
+  uint32_t g = (dx >> 2) * 2 + (dy >> 1);
+  uint32_t l = (dx & 3u) + 4 * (dy & 1u);
+  size_t blk = (size_t)(py >> 2) * (width >> 4) + (px >> 4);
+  return blk * 192 + g * 24 + l;
+}
+
 struct Hit {
   uint32_t tri;
   float t, u, v;

@@ -55,6 +55,7 @@ struct JobDesc {
   float* hdr;         // 16x4-block HDR accumulation, or null
   uint32_t* bitmap;   // RGBA8 rows, or null
   uint32_t* gbuffer;  // gbuffer rows, or null
+  float* features;    // first-hit features summed over frames (8 f32 per pixel), or null
   CameraParams cam;
   float emission[3];
   uint32_t nlights;

@@ -452,14 +452,6 @@ __device__ __forceinline__ uint32_t pack_gbuffer(uint32_t tex_index, float tu, f
   return (r | g) | (b | a);
 }
 
-// Offset of a pixel in the 16x4-block HDR layout (renderer.rs:154-206,356-358).
-__device__ __forceinline__ size_t hdr_offset(uint32_t px, uint32_t py, uint32_t width) {
-  uint32_t dx = px & 15u, dy = py & 3u;
-  uint32_t g = (dx >> 2) * 2 + (dy >> 1);
-  uint32_t l = (dx & 3u) + 4 * (dy & 1u);
-  size_t blk = (size_t)(py >> 2) * (width >> 4) + (px >> 4);
-  return blk * 192 + g * 24 + l;
-}
 
 // ---------------------------------------------------------------------------
 // generate: camera rays (renderer.rs:171-206,413-416, scene.rs:81-99)
@@ -824,6 +816,22 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(ShadeArgs a) {
         tu = fmaf(uv[0], w, fmaf(uv[2], bv, uv[4] * bu));
         tv = fmaf(uv[1], w, fmaf(uv[3], bv, uv[5] * bu));
       }
+    }
+    if (a.bounce == 0 && J.features) {
+      // first-hit features of the denoiser: albedo, normal, distance, count
+      float* f = J.features + (size_t)pixel * 8;
+      const float k255 = 1.0f / 255.0f;
+      V3 alb = (mat >> 31) ? mk3(1.0f, 1.0f, 1.0f)
+                           : mk3((float)(mat & 0xffu) * k255, (float)((mat >> 8) & 0xffu) * k255,
+                                 (float)((mat >> 16) & 0xffu) * k255);
+      f[0] += alb.x;
+      f[1] += alb.y;
+      f[2] += alb.z;
+      f[3] += normal.x;
+      f[4] += normal.y;
+      f[5] += normal.z;
+      f[6] += hit.x == kNoHit ? 1.0e5f : t;
+      f[7] += 1.0f;
     }
     if (mat >> 31) {
       // The path found an emitter: colour x emission (renderer.rs:452-460).

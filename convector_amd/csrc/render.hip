@@ -363,7 +363,7 @@ struct cv_renderer {
   uint32_t width = 0, height = 0;
   cv_render_opts opts{};
   Pipeline pipe;
-  DevBuf<float> hdr;
+  DevBuf<float> hdr, features;
   DevBuf<uint32_t> bitmap, gbuffer;
   cv_render_stats stats{};
   std::mutex mu;
@@ -402,7 +402,8 @@ struct Rect {
 };
 
 cv_status run_frames(cv_renderer* r, const Rect& rc, uint32_t first, uint32_t nframes, float* hdr,
-                     uint32_t* bitmap, uint32_t* gbuffer, uint32_t flags, bool hdr_mode) {
+                     uint32_t* bitmap, uint32_t* gbuffer, uint32_t flags, bool hdr_mode,
+                     float* features = nullptr) {
   hipStream_t st = r->pipe.stream;
   const bool dev = flags & CV_PTR_DEVICE;
   const uint32_t W = r->width;
@@ -423,6 +424,18 @@ cv_status run_frames(cv_renderer* r, const Rect& rc, uint32_t first, uint32_t nf
   if (bitmap) bm_dev = dev ? bitmap : r->bitmap.p;
   uint32_t* gb_dev = nullptr;
   if (gbuffer) gb_dev = dev ? gbuffer : r->gbuffer.p;
+  float* ft_dev = nullptr;
+  const size_t npix_all = (size_t)W * r->height;
+  if (features) {
+    if (dev) {
+      ft_dev = features;
+    } else {
+      if (r->features.n < npix_all * 8) HIPTRY(r->features.alloc(npix_all * 8), "alloc features");
+      HIPTRY(hipMemcpyAsync(r->features.p, features, npix_all * 32, hipMemcpyHostToDevice, st),
+             "upload features");
+      ft_dev = r->features.p;
+    }
+  }
   cv_scene* s = r->scene;
   cvk::JobDesc base = cvh::job_for_scene(s);
   base.width = r->width;
@@ -433,6 +446,7 @@ cv_status run_frames(cv_renderer* r, const Rect& rc, uint32_t first, uint32_t nf
   base.rect_h = rc.h;
   base.hdr = hdr_dev;
   base.bitmap = bm_dev;
+  base.features = ft_dev;
   base.path_base = 0;
   std::vector<StepPlan> steps(nframes);
   for (uint32_t f = 0; f < nframes; f++) {
@@ -473,6 +487,11 @@ cv_status run_frames(cv_renderer* r, const Rect& rc, uint32_t first, uint32_t nf
                                 (size_t)rc.w * 4, rc.h, hipMemcpyDeviceToHost, st) == hipSuccess
                    ? CV_OK
                    : fail(CV_ERR_DEVICE, "download gbuffer");
+    if (status == CV_OK && features)
+      status = hipMemcpyAsync(features, r->features.p, npix_all * 32, hipMemcpyDeviceToHost, st) ==
+                       hipSuccess
+                   ? CV_OK
+                   : fail(CV_ERR_DEVICE, "download features");
     hipError_t se = hipStreamSynchronize(st);
     if (status == CV_OK && se != hipSuccess) status = hip_fail(se, "download");
   }
@@ -535,6 +554,7 @@ void cv_renderer_destroy(cv_renderer* r) {
   if (!r) return;
   r->pipe.release();
   r->hdr.reset();
+  r->features.reset();
   r->bitmap.reset();
   r->gbuffer.reset();
   delete r;
@@ -577,6 +597,15 @@ cv_status cv_renderer_accumulate_frames_f32(cv_renderer* r, float* hdr, uint32_t
   std::lock_guard<std::mutex> g(r->mu);
   return run_frames(r, Rect{0, 0, r->width, r->height}, first_frame, n_frames, hdr, nullptr,
                     gbuffer, flags, true);
+}
+
+cv_status cv_renderer_accumulate_aov_f32(cv_renderer* r, float* hdr, float* features,
+                                         uint32_t first_frame, uint32_t n_frames,
+                                         uint32_t flags) {
+  if (!r || !hdr || !features) return fail(CV_ERR_INVALID_ARGUMENT, "null argument");
+  std::lock_guard<std::mutex> g(r->mu);
+  return run_frames(r, Rect{0, 0, r->width, r->height}, first_frame, n_frames, hdr, nullptr,
+                    nullptr, flags, true, features);
 }
 
 cv_status cv_renderer_buffer_f32_into_bitmap(cv_renderer* r, const float* hdr, uint32_t* bitmap,
@@ -680,6 +709,7 @@ cv_status cv_batch_render(cv_batch* b, const cv_job* jobs, size_t n, uint32_t fl
   // host outputs: device staging per job
   std::vector<DevBuf<float>> d_hdr(dev ? 0 : n);
   std::vector<DevBuf<uint32_t>> d_gb(dev ? 0 : n);
+  std::vector<DevBuf<float>> d_ft(dev ? 0 : n);
   hipStream_t st = b->pipe.stream;
   std::vector<cvk::JobDesc> base(n);
   for (size_t i = 0; i < n; i++) {
@@ -693,6 +723,7 @@ cv_status cv_batch_render(cv_batch* b, const cv_job* jobs, size_t n, uint32_t fl
     d.rect_h = J.height;
     d.hdr = J.hdr;
     d.gbuffer = J.gbuffer;
+    d.features = J.features;
     d.bitmap = nullptr;
     if (!dev) {
       HIPTRY(d_hdr[i].alloc(npix * 3), "alloc hdr");
@@ -701,6 +732,12 @@ cv_status cv_batch_render(cv_batch* b, const cv_job* jobs, size_t n, uint32_t fl
       if (J.gbuffer) {
         HIPTRY(d_gb[i].alloc(npix), "alloc gbuffer");
         d.gbuffer = d_gb[i].p;
+      }
+      if (J.features) {
+        HIPTRY(d_ft[i].alloc(npix * 8), "alloc features");
+        HIPTRY(hipMemcpyAsync(d_ft[i].p, J.features, npix * 32, hipMemcpyHostToDevice, st),
+               "upload features");
+        d.features = d_ft[i].p;
       }
     }
     base[i] = d;
@@ -750,6 +787,9 @@ cv_status cv_batch_render(cv_batch* b, const cv_job* jobs, size_t n, uint32_t fl
       if (jobs[i].gbuffer)
         HIPTRY(hipMemcpyAsync(jobs[i].gbuffer, d_gb[i].p, npix * 4, hipMemcpyDeviceToHost, st),
                "download gbuffer");
+      if (jobs[i].features)
+        HIPTRY(hipMemcpyAsync(jobs[i].features, d_ft[i].p, npix * 32, hipMemcpyDeviceToHost, st),
+               "download features");
     }
     HIPTRY(hipStreamSynchronize(st), "download");
   }

@@ -244,6 +244,11 @@ cv_status cv_renderer_render_frame_u8(cv_renderer* r, uint32_t* bitmap, uint32_t
 cv_status cv_renderer_accumulate_frames_f32(cv_renderer* r, float* hdr, uint32_t* gbuffer,
                                             uint32_t first_frame, uint32_t n_frames,
                                             uint32_t flags);
+/* accumulate_frames_f32 plus the denoiser's first-hit features (the cv_job
+ * features layout, accumulated into; no reference counterpart). */
+cv_status cv_renderer_accumulate_aov_f32(cv_renderer* r, float* hdr, float* features,
+                                         uint32_t first_frame, uint32_t n_frames,
+                                         uint32_t flags);
 /* Renderer::buffer_f32_into_render_buffer (renderer.rs:378-410). */
 cv_status cv_renderer_buffer_f32_into_bitmap(cv_renderer* r, const float* hdr, uint32_t* bitmap,
                                              uint32_t num_samples, uint32_t flags);
@@ -270,6 +275,9 @@ typedef struct cv_job {
     uint32_t num_frames;   /* samples per pixel, >= 1 */
     float* hdr;            /* accumulated into: 16x4-block layout, width*height*3 f32 */
     uint32_t* gbuffer;     /* NULL, or the last frame's gbuffer (width*height u32) */
+    float* features;       /* NULL, or accumulated into: first-hit denoiser features summed
+                              over the frames, 8 f32 per pixel in rows (albedo rgb, shading
+                              normal xyz, hit distance (1e5 for sky), count) */
 } cv_job;
 
 /* max_paths_per_step bounds the path queues (0: 2^24); a step whose jobs
@@ -295,6 +303,46 @@ cv_status cv_batch_autotune(cv_batch* b, const cv_job* jobs, size_t n, uint64_t 
 /* Counters and device times of the last cv_batch_render. */
 cv_status cv_batch_get_stats(const cv_batch* b, cv_render_stats* stats);
 void cv_batch_destroy(cv_batch* b);
+
+/* --------------------------------------------------------------- denoiser */
+
+/* The learned denoiser that stands where the reference runs its median
+ * filter (src/gpu/median.glsl:51-75; no reference counterpart): a 3-level
+ * U-Net (32/64/128 channels, 3x3 convolutions, ReLU, 2x2 average pooling,
+ * nearest upsampling, skip concatenation), BF16 weights and activations on
+ * the matrix cores with FP32 accumulation, FP32 master weights and Adam.
+ * Input: 10 f32 per pixel in rows (log1p radiance rgb, albedo rgb,
+ * octahedral normal xy, log1p distance, spp; see
+ * cv_denoiser_prepare_input); output: 3 f32 per pixel, log1p radiance.
+ * Batches are nb images of height x width, height a multiple of 4 and width
+ * of 64.  Pointers are host memory unless flags has CV_PTR_DEVICE. */
+typedef struct cv_denoiser cv_denoiser;
+
+/* Number of f32 parameters (weights [cout][3][3][cin] then bias, per layer). */
+size_t cv_denoiser_num_params(void);
+/* He-normal weights from a splitmix64 stream seeded with seed, zero bias. */
+cv_status cv_denoiser_create(uint64_t seed, cv_denoiser** out);
+cv_status cv_denoiser_get_params(cv_denoiser* d, float* params, float* grads);
+/* Replaces the parameters and resets the Adam state. */
+cv_status cv_denoiser_set_params(cv_denoiser* d, const float* params);
+cv_status cv_denoiser_forward(cv_denoiser* d, const float* input, float* output, uint32_t nb,
+                              uint32_t height, uint32_t width, uint32_t flags);
+/* Forward, L1 loss against log1p(target) (3 f32 per pixel, linear
+ * radiance), backward into the gradients, then one Adam step with learning
+ * rate lr (lr = 0: gradients only).  loss (may be NULL) receives the mean
+ * absolute error before the step. */
+cv_status cv_denoiser_train_step(cv_denoiser* d, const float* input, const float* target,
+                                 uint32_t nb, uint32_t height, uint32_t width, float lr,
+                                 uint32_t flags, float* loss);
+/* Network input from a renderer's HDR accumulation (16x4-block layout, spp
+ * frames) and its summed features (cv_renderer_accumulate_aov_f32). */
+cv_status cv_denoiser_prepare_input(const float* hdr, const float* features, uint32_t width,
+                                    uint32_t height, uint32_t spp, float* input, uint32_t flags);
+/* Network output -> 16x4-block HDR holding spp x radiance, so that
+ * cv_renderer_buffer_f32_into_bitmap(..., spp) displays it. */
+cv_status cv_denoiser_output_to_hdr(const float* output, uint32_t width, uint32_t height,
+                                    uint32_t spp, float* hdr, uint32_t flags);
+void cv_denoiser_destroy(cv_denoiser* d);
 
 /* ----------------------------------------------------------------- device */
 
