@@ -245,6 +245,180 @@ def cpu_baseline_batch(sds, specs, args):
     }
 
 
+def _random_camera(scene, rng):
+    """A camera inside the box (|x|, |z| < 0.6, y in [0.2, 0.9]) looking
+    towards the centre, random yaw and pitch around the default view."""
+    import math as m
+    pos = (float(rng.uniform(-0.6, 0.6)), float(rng.uniform(0.2, 0.9)), float(rng.uniform(-0.6, 0.6)))
+    yaw, pitch = float(rng.uniform(-0.5, 0.5)), float(rng.uniform(-0.3, 0.3))
+    # default orientation: -90 degrees about x (looking along -y); then yaw
+    # about z and pitch about x
+    def qmul(a, b):
+        return (a[0] * b[0] - a[1] * b[1] - a[2] * b[2] - a[3] * b[3],
+                a[0] * b[1] + a[1] * b[0] + a[2] * b[3] - a[3] * b[2],
+                a[0] * b[2] - a[1] * b[3] + a[2] * b[0] + a[3] * b[1],
+                a[0] * b[3] + a[1] * b[2] - a[2] * b[1] + a[3] * b[0])
+    q0 = (0.70710677, -0.70710677, 0.0, 0.0)
+    qy = (m.cos(yaw / 2), 0.0, 0.0, m.sin(yaw / 2))
+    qp = (m.cos(pitch / 2), m.sin(pitch / 2), 0.0, 0.0)
+    q = qmul(qy, qmul(q0, qp))
+    scene.camera.set_position(pos)
+    scene.camera.set_orientation(q)
+
+
+def bench_denoise(args):
+    """The learned denoiser: trained on the GPU on 1-sample / high-sample
+    pairs rendered by the path tracer from seeded procedural scenes
+    (10k-500k triangles, random cameras), then judged on held-out seeds at
+    1280x736: the plain accumulation's RMSE against the reference falls with
+    spp; the spp at which it equals the denoised 1-sample RMSE, times the
+    per-sample render time, over the denoised path's time (1 sample with
+    features + the network), is the equal-RMSE speedup."""
+    import convector_amd as cv
+    from convector_amd import scenes
+
+    rng = np.random.default_rng(1234)
+    TW = TH = 256
+    t0 = time.perf_counter()
+    gs = []
+    for k in range(args.train_scenes):
+        n = int(np.exp(rng.uniform(np.log(1e4), np.log(5e5))))
+        g = cv.Scene.from_scene_data(scenes.procedural_scene(n, 20000 + k))
+        _random_camera(g, rng)
+        gs.append(g)
+    log("denoise: %d training scenes built in %.1f s" % (len(gs), time.perf_counter() - t0))
+    # 1-sample inputs with features and the references, each set in one batch
+    b = cv.Batch()
+    jin = [cv.Job(g, TW, TH, 1 + k, 1, features=np.zeros(TW * TH * 8, np.float32))
+           for k, g in enumerate(gs)]
+    jref = [cv.Job(g, TW, TH, 100000, args.ref_spp) for g in gs]
+    t1 = time.perf_counter()
+    b.render(jin)
+    b.render(jref, cv.RENDER_GRAPH)
+    render_s = time.perf_counter() - t1
+    xs = np.stack([cv.Denoiser.prepare_input(j.hdr, j.features, TW, TH, 1) for j in jin])
+    ys = np.stack([np.expm1(cv.Denoiser.prepare_input(jr.hdr, j.features, TW, TH,
+                                                      args.ref_spp)[..., :3])
+                   for j, jr in zip(jin, jref)])
+    log("denoise: %d training pairs (%dx%d, 1 / %d spp) rendered in %.1f s"
+        % (len(xs), TW, TH, args.ref_spp, render_s))
+    d = cv.Denoiser(7)
+    B = min(16, len(xs))
+    losses = []
+    cv.device_synchronize()
+    t0 = time.perf_counter()
+    for it in range(args.train_steps):
+        idx = rng.choice(len(xs), B, replace=False)
+        losses.append(d.train_step(xs[idx], ys[idx], 1e-3))
+        if it % 100 == 0:
+            log("denoise: step %d loss %.4f" % (it, losses[-1]))
+    train_s = time.perf_counter() - t0
+    log("denoise: %d Adam steps (batch %d x %dx%d) in %.1f s, loss %.4f -> %.4f"
+        % (args.train_steps, B, TW, TH, train_s, losses[0], float(np.mean(losses[-20:]))))
+    # held-out evaluation at 1280x736
+    W, H = 1280, 736
+    ratios, evals = [], []
+    hdr_dev = DeviceBuffer(W * H * 12)
+    for e in range(args.eval_scenes):
+        n = int(np.exp(rng.uniform(np.log(1e4), np.log(5e5))))
+        g = cv.Scene.from_scene_data(scenes.procedural_scene(n, 90000 + e))
+        _random_camera(g, rng)
+        r = cv.Renderer(g, W, H)
+        ref = r.new_buffer_f32()
+        r.accumulate_frames_f32(ref, 200000, args.eval_ref_spp)
+        feat0 = r.new_features()
+        ref_img = np.expm1(cv.Denoiser.prepare_input(ref, feat0, W, H, args.eval_ref_spp)[..., :3])
+        # the denoised path on device buffers: 1 sample with features, the
+        # network input, the network
+        def denoised_path():
+            hdr_d.zero()
+            feat_d.zero()
+            r.accumulate_aov_f32(hdr_d.ptr.value, feat_d.ptr.value, 1, 1, flags=cv.PTR_DEVICE)
+            t_render = r.stats()["total_ms"]
+            cv.Denoiser.prepare_input_device(hdr_d.ptr.value, feat_d.ptr.value, W, H, 1,
+                                             inp_d.ptr.value)
+            d.forward_device(inp_d.ptr.value, out_d.ptr.value, 1, H, W)
+            return t_render
+        hdr_d, feat_d = DeviceBuffer(W * H * 12), DeviceBuffer(W * H * 32)
+        inp_d, out_d = DeviceBuffer(W * H * 40), DeviceBuffer(W * H * 12)
+        denoised_path()
+        t_path = []
+        for _ in range(3):
+            cv.device_synchronize()
+            t1 = time.perf_counter()
+            denoised_path()
+            cv.device_synchronize()
+            t_path.append((time.perf_counter() - t1) * 1e3)
+        out = np.zeros((H, W, 3), np.float32)
+        hip = hdr_d.hip
+        hip.hipMemcpy(out.ctypes.data_as(ctypes.c_void_p), out_d.ptr, ctypes.c_size_t(out.nbytes), 2)
+        out = np.expm1(np.maximum(out, 0.0))
+        rmse_dn = float(np.sqrt(((out - ref_img) ** 2).mean()))
+        # the network alone
+        t_net = []
+        for _ in range(3):
+            cv.device_synchronize()
+            t1 = time.perf_counter()
+            d.forward_device(inp_d.ptr.value, out_d.ptr.value, 1, H, W)
+            t_net.append((time.perf_counter() - t1) * 1e3)
+        # plain accumulation: RMSE at 1, 2, 4, ... spp and the time per sample
+        acc = r.new_buffer_f32()
+        curve = []
+        spp = 0
+        while spp < args.eval_max_spp:
+            k = max(spp, 1)
+            r.accumulate_frames_f32(acc, 1 + spp, k)
+            spp += k
+            img = np.expm1(cv.Denoiser.prepare_input(acc, feat0, W, H, spp)[..., :3])
+            curve.append((spp, float(np.sqrt(((img - ref_img) ** 2).mean()))))
+        hdr_dev.zero()
+        r.accumulate_frames_device(hdr_dev.ptr.value, 1, 16, cv.RENDER_TIMING)
+        t_spp = r.stats()["total_ms"] / 16
+        # equal-RMSE spp: log-log interpolation on the curve (extrapolated
+        # with RMSE ~ 1/sqrt(spp) beyond its end)
+        spp_eq = None
+        for (s0, e0), (s1, e1) in zip(curve, curve[1:]):
+            if e0 >= rmse_dn >= e1:
+                a = (np.log(rmse_dn) - np.log(e0)) / (np.log(e1) - np.log(e0))
+                spp_eq = float(np.exp(np.log(s0) + a * (np.log(s1) - np.log(s0))))
+                break
+        if spp_eq is None:
+            s_end, e_end = curve[-1]
+            spp_eq = s_end * (e_end / rmse_dn) ** 2 if rmse_dn < e_end else 1.0
+        t_dn = min(t_path)
+        ratio = spp_eq * t_spp / t_dn
+        ratios.append(ratio)
+        evals.append({"seed": 90000 + e, "triangles": n, "rmse_denoised": round(rmse_dn, 5),
+                      "rmse_plain": [(s_, round(e_, 5)) for s_, e_ in curve],
+                      "equal_rmse_spp": round(spp_eq, 2), "ms_per_spp": round(t_spp, 3),
+                      "ms_denoised_path": round(t_dn, 3), "ms_network": round(min(t_net), 3),
+                      "speedup": round(ratio, 2)})
+        log("denoise eval %d: %s" % (e, evals[-1]))
+    value = float(np.exp(np.mean(np.log(ratios))))
+    out = {
+        "metric": "equal-RMSE time, denoised 1 spp vs plain accumulation",
+        "value": round(value, 3),
+        "unit": "x (plain time / denoised time at equal RMSE)",
+        "n_gpus": 1,
+        "steps": args.train_steps,
+        "warmup": 0,
+        "ms_per_step": round(train_s * 1e3 / args.train_steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16",
+        "data": "synthetic",
+        "config": {"workload": "U-Net 32/64/128, 10 input channels, Adam lr 1e-3, batch %d of "
+                               "%dx%d, %d training pairs (1 / %d spp) from %d procedural scenes; "
+                               "evaluated on %d held-out seeds at %dx%d against %d spp"
+                               % (B, TW, TH, len(xs), args.ref_spp, args.train_scenes,
+                                  args.eval_scenes, W, H, args.eval_ref_spp)},
+        "train_loss_first_last": [round(losses[0], 5), round(float(np.mean(losses[-20:])), 5)],
+        "evals": evals,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -254,10 +428,17 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--subdivisions", type=int, default=8)
-    ap.add_argument("--workload", choices=["frame", "batch"], default="frame",
+    ap.add_argument("--workload", choices=["frame", "batch", "denoise"], default="frame",
                     help="frame: the large 16-spp 1080p render (default); batch: 64 mixed "
-                         "jobs in shared launches against sequential launches")
+                         "jobs in shared launches against sequential launches; denoise: train "
+                         "the denoiser and measure its equal-RMSE speedup")
     ap.add_argument("--jobs", type=int, default=64)
+    ap.add_argument("--train-scenes", type=int, default=96)
+    ap.add_argument("--train-steps", type=int, default=1000)
+    ap.add_argument("--ref-spp", type=int, default=1024)
+    ap.add_argument("--eval-scenes", type=int, default=4)
+    ap.add_argument("--eval-ref-spp", type=int, default=2048)
+    ap.add_argument("--eval-max-spp", type=int, default=512)
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="device-to-device copy of this many MiB before the steps, so that "
                          "rocprofv3 --pmc counters can be scaled by a copy of known size")
@@ -270,6 +451,9 @@ def main():
 
     if args.workload == "batch":
         bench_batch(args)
+        return
+    if args.workload == "denoise":
+        bench_denoise(args)
         return
 
     import convector_amd as cv

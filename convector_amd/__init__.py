@@ -40,8 +40,13 @@ ISECT_DTYPE = np.dtype([("position", "<f4", 3), ("normal", "<f4", 3), ("distance
                         ("reserved", "<u4")])
 
 
-def _p(a: Optional[np.ndarray]):
-    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+def _p(a):
+    """A numpy array's data pointer, a device pointer given as int, or None."""
+    if a is None:
+        return None
+    if isinstance(a, int):
+        return ctypes.c_void_p(a)
+    return a.ctypes.data_as(ctypes.c_void_p)
 
 
 class Mesh:
@@ -444,6 +449,12 @@ class Denoiser:
                                               _p(np.ascontiguousarray(features, np.float32)),
                                               width, height, spp, _p(out), 0))
         return out
+
+    @staticmethod
+    def prepare_input_device(hdr_ptr: int, features_ptr: int, width, height, spp, out_ptr: int):
+        check(lib().cv_denoiser_prepare_input(ctypes.c_void_p(hdr_ptr),
+                                              ctypes.c_void_p(features_ptr), width, height, spp,
+                                              ctypes.c_void_p(out_ptr), PTR_DEVICE))
 
     @staticmethod
     def output_to_hdr(output, width, height, spp) -> np.ndarray:
