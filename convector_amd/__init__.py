@@ -470,6 +470,86 @@ class Denoiser:
             self._h = None
 
 
+class Server:
+    """Request loop with continuous batching over resident scenes
+    (cv_server).  Results are 16x4-block HDR buffers holding spp x radiance."""
+
+    def __init__(self, budget_bytes: int = 0, max_paths_per_step: int = 0, max_bounces: int = 5):
+        o = _native.RenderOpts(RNG_PIXEL, _native.CV_APPROX_PORTABLE, max_bounces)
+        h = ctypes.c_void_p()
+        check(lib().cv_server_create(ctypes.byref(o), budget_bytes, max_paths_per_step,
+                                     ctypes.byref(h)))
+        self._h = h
+        self._scenes = []
+        self._denoiser = None
+        self._live = {}
+
+    def add_scene(self, scene: Scene) -> int:
+        i = ctypes.c_uint32()
+        check(lib().cv_server_add_scene(self._h, scene._h, ctypes.byref(i)))
+        self._scenes.append(scene)
+        return i.value
+
+    def set_denoiser(self, d: "Denoiser"):
+        check(lib().cv_server_set_denoiser(self._h, d._h if d else None))
+        self._denoiser = d
+
+    def set_tuning(self, stack_entries=0, extend_waves_per_cu=0, refill_threshold=0,
+                   fetch_batches=0):
+        t = _native.Tuning(stack_entries, extend_waves_per_cu, refill_threshold, fetch_batches)
+        check(lib().cv_server_set_tuning(self._h, ctypes.byref(t)))
+
+    def submit(self, scene_id: int, width: int, height: int, spp: int, camera_pos, camera_q,
+               denoise: bool = False, screen_distance: float = 0.0, hdr=None):
+        """Queues a request; returns (ticket, hdr) -- hdr is filled when done."""
+        if hdr is None:
+            hdr = np.zeros(width * height * 3, np.float32)
+        r = _native.Request()
+        r.scene, r.width, r.height, r.spp = scene_id, width, height, spp
+        r.denoise = 1 if denoise else 0
+        r.flags = PTR_DEVICE if isinstance(hdr, int) else 0
+        r.camera_pos[:] = [float(x) for x in camera_pos]
+        r.camera_q[:] = [float(x) for x in camera_q]
+        r.screen_distance = float(screen_distance)
+        r.hdr = hdr if isinstance(hdr, int) else hdr.ctypes.data
+        t = ctypes.c_uint64()
+        check(lib().cv_server_submit(self._h, ctypes.byref(r), ctypes.byref(t)))
+        self._live[t.value] = hdr
+        return t.value, hdr
+
+    def poll(self, ticket: int) -> bool:
+        d = ctypes.c_int()
+        check(lib().cv_server_poll(self._h, ticket, ctypes.byref(d)))
+        return bool(d.value)
+
+    def wait(self, ticket: int) -> float:
+        """Blocks until the request is delivered; returns its latency (ms)."""
+        ms = ctypes.c_double()
+        try:
+            check(lib().cv_server_wait(self._h, ticket, ctypes.byref(ms)))
+        finally:
+            self._live.pop(ticket, None)
+        return ms.value
+
+    def stats(self) -> dict:
+        s = _native.ServerStats()
+        check(lib().cv_server_get_stats(self._h, ctypes.byref(s)))
+        return {k: getattr(s, k) for k, _ in s._fields_}
+
+    def reset_stats(self):
+        check(lib().cv_server_reset_stats(self._h))
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().cv_server_destroy(h)
+            self._h = None
+            self._live.clear()
+
+    def __del__(self):
+        self.close()
+
+
 def device_synchronize():
     check(lib().cv_device_synchronize())
 

@@ -43,7 +43,9 @@ EXPORTS = [
     "cv_renderer_accumulate_aov_f32", "cv_denoiser_num_params", "cv_denoiser_create",
     "cv_denoiser_get_params", "cv_denoiser_set_params", "cv_denoiser_forward",
     "cv_denoiser_train_step", "cv_denoiser_prepare_input", "cv_denoiser_output_to_hdr",
-    "cv_denoiser_destroy",
+    "cv_denoiser_destroy", "cv_server_create", "cv_server_add_scene", "cv_server_set_denoiser",
+    "cv_server_set_tuning", "cv_server_submit", "cv_server_poll", "cv_server_wait",
+    "cv_server_get_stats", "cv_server_reset_stats", "cv_server_destroy",
     "cv_device_synchronize", "cv_last_error", "cv_status_string", "cv_version",
 ]
 
@@ -86,6 +88,22 @@ class Job(ctypes.Structure):
                 ("height", ctypes.c_uint32), ("first_frame", ctypes.c_uint32),
                 ("num_frames", ctypes.c_uint32), ("hdr", ctypes.c_void_p),
                 ("gbuffer", ctypes.c_void_p), ("features", ctypes.c_void_p)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("scene", ctypes.c_uint32), ("width", ctypes.c_uint32),
+                ("height", ctypes.c_uint32), ("spp", ctypes.c_uint32),
+                ("denoise", ctypes.c_uint32), ("flags", ctypes.c_uint32),
+                ("camera_pos", ctypes.c_float * 3), ("camera_q", ctypes.c_float * 4),
+                ("screen_distance", ctypes.c_float), ("hdr", ctypes.c_void_p)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64),
+                ("steps", ctypes.c_uint64), ("p50_ms", ctypes.c_double),
+                ("p99_ms", ctypes.c_double), ("mean_ms", ctypes.c_double),
+                ("busy_ms", ctypes.c_double), ("peak_bytes", ctypes.c_uint64),
+                ("resident_bytes", ctypes.c_uint64)]
 
 
 class RenderStats(ctypes.Structure):
@@ -159,6 +177,16 @@ def _declare(L):
         "cv_denoiser_prepare_input": (I, [P, P, U32, U32, U32, P, U32]),
         "cv_denoiser_output_to_hdr": (I, [P, U32, U32, U32, P, U32]),
         "cv_denoiser_destroy": (None, [P]),
+        "cv_server_create": (I, [P, ctypes.c_uint64, ctypes.c_uint64, PP]),
+        "cv_server_add_scene": (I, [P, P, P]),
+        "cv_server_set_denoiser": (I, [P, P]),
+        "cv_server_set_tuning": (I, [P, P]),
+        "cv_server_submit": (I, [P, P, P]),
+        "cv_server_poll": (I, [P, ctypes.c_uint64, P]),
+        "cv_server_wait": (I, [P, ctypes.c_uint64, P]),
+        "cv_server_get_stats": (I, [P, P]),
+        "cv_server_reset_stats": (I, [P]),
+        "cv_server_destroy": (None, [P]),
         "cv_device_synchronize": (I, []),
         "cv_last_error": (ctypes.c_char_p, []),
         "cv_status_string": (ctypes.c_char_p, [I]),

@@ -469,6 +469,7 @@ struct cv_denoiser {
   uint64_t step = 0;
   // activations for the current size
   uint32_t nb = 0, h = 0, w = 0;
+  size_t cap_pixels = 0;
   DevBuf<uint16_t> x0, e0a, cat0, p0, e1a, cat1, p1, b2a, b2b, d1a, d1b, d0a, d0b, dz;
   DevBuf<float> out, dout, g_e0a, g_cat0, g_p0, g_e1a, g_cat1, g_p1, g_b2a, g_b2b, g_d1a, g_d1b,
       g_d0a, g_d0b, g_up1, g_up0, loss;
@@ -564,9 +565,17 @@ cv_status conv_bwd(cv_denoiser* d, const Conv& L, const uint16_t* x, uint32_t xc
   return CV_OK;
 }
 
+// Activation buffers are indexed by pixel only (NHWC), so one allocation
+// serves every shape with at most as many pixels; it grows, never shrinks.
 cv_status reserve_acts(cv_denoiser* d, uint32_t nb, uint32_t h, uint32_t w) {
-  if (d->nb == nb && d->h == h && d->w == w) return CV_OK;
-  const size_t P0 = (size_t)nb * h * w, P1 = P0 / 4, P2 = P0 / 16;
+  const size_t need = (size_t)nb * h * w;
+  if (need <= d->cap_pixels) {
+    d->nb = nb;
+    d->h = h;
+    d->w = w;
+    return CV_OK;
+  }
+  const size_t P0 = need, P1 = P0 / 4, P2 = P0 / 16;
   HIPTRY(d->x0.alloc(P0 * 16), "alloc");
   HIPTRY(d->e0a.alloc(P0 * 32), "alloc");
   HIPTRY(d->cat0.alloc(P0 * 96), "alloc");
@@ -596,6 +605,7 @@ cv_status reserve_acts(cv_denoiser* d, uint32_t nb, uint32_t h, uint32_t w) {
   HIPTRY(d->g_d0a.alloc(P0 * 32), "alloc");
   HIPTRY(d->g_d0b.alloc(P0 * 32), "alloc");
   HIPTRY(d->loss.alloc(1), "alloc");
+  d->cap_pixels = need;
   d->nb = nb;
   d->h = h;
   d->w = w;

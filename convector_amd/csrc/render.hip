@@ -12,8 +12,14 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/convector.h"
@@ -891,6 +897,375 @@ cv_status cv_batch_autotune(cv_batch* b, const cv_job* jobs, size_t n, uint64_t 
 cv_status cv_batch_get_stats(const cv_batch* b, cv_render_stats* stats) {
   if (!b || !stats) return fail(CV_ERR_INVALID_ARGUMENT, "null argument");
   *stats = b->stats;
+  return CV_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ server
+
+namespace {
+
+double now_ms() {
+  using namespace std::chrono;
+  return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+struct ServeReq {
+  uint64_t ticket = 0;
+  cv_request req{};
+  uint32_t frames_done = 0;
+  DevBuf<float> hdr, feat;
+  size_t bytes = 0;
+  double t_submit = 0, t_done = 0;
+  bool done = false;
+  cv_status status = CV_OK;
+  std::string error;
+};
+
+size_t scene_bytes(const cv_scene* s) {
+  return (size_t)s->info.num_nodes * sizeof(cvk::DevNode) +
+         (size_t)s->ntris * (sizeof(cvk::DevTri) + 4 + 24 + 4 + 4) +
+         (size_t)s->ndirect * sizeof(cvk::DevLight);
+}
+
+size_t request_bytes(const cv_request& r) {
+  const size_t npix = (size_t)r.width * r.height;
+  return npix * 12 + (r.denoise ? npix * (32 + 40 + 12 + 12) : 0);
+}
+
+}  // namespace
+
+struct cv_server {
+  uint64_t budget = 0, max_paths = 0;
+  cv_render_opts opts{};
+  cvh::LaunchCfg cfg;
+  cv_denoiser* denoiser = nullptr;
+  std::vector<cv_scene*> scenes;
+  size_t fixed_bytes = 0;  // scenes + queues
+  size_t used_bytes = 0;   // admitted requests
+  size_t peak_bytes = 0;
+  Pipeline pipe;
+  std::mutex mu;
+  std::condition_variable work_cv, done_cv;
+  std::deque<std::shared_ptr<ServeReq>> pending;
+  std::vector<std::shared_ptr<ServeReq>> active;
+  std::unordered_map<uint64_t, std::shared_ptr<ServeReq>> tickets;
+  uint64_t next_ticket = 1;
+  bool stop = false;
+  std::thread worker;
+  // statistics
+  uint64_t submitted = 0, completed = 0, steps = 0;
+  double busy_ms = 0;
+  std::vector<double> latencies;
+};
+
+namespace {
+
+// Denoise (optional) and deliver one finished request; called by the worker
+// without the server lock.
+void finish_request(cv_server* sv, ServeReq& r) {
+  const cv_request& q = r.req;
+  const size_t npix = (size_t)q.width * q.height;
+  const bool dev = q.flags & CV_PTR_DEVICE;
+  hipMemcpyKind kind = dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+  cv_status st = CV_OK;
+  if (q.denoise && sv->denoiser) {
+    DevBuf<float> inp, out, res;
+    if (inp.alloc(npix * 10) || out.alloc(npix * 3) || res.alloc(npix * 3)) {
+      st = fail(CV_ERR_OUT_OF_MEMORY, "denoiser buffers");
+    } else {
+      st = cv_denoiser_prepare_input(r.hdr.p, r.feat.p, q.width, q.height, q.spp, inp.p,
+                                     CV_PTR_DEVICE);
+      if (st == CV_OK)
+        st = cv_denoiser_forward(sv->denoiser, inp.p, out.p, 1, q.height, q.width, CV_PTR_DEVICE);
+      if (st == CV_OK)
+        st = cv_denoiser_output_to_hdr(out.p, q.width, q.height, q.spp, res.p, CV_PTR_DEVICE);
+      if (st == CV_OK && hipMemcpy(q.hdr, res.p, npix * 12, kind) != hipSuccess)
+        st = fail(CV_ERR_DEVICE, "deliver result");
+    }
+  } else if (hipMemcpy(q.hdr, r.hdr.p, npix * 12, kind) != hipSuccess) {
+    st = fail(CV_ERR_DEVICE, "deliver result");
+  }
+  r.status = st;
+  if (st != CV_OK) r.error = cv_last_error();
+  r.hdr.reset();
+  r.feat.reset();
+}
+
+void serve_loop(cv_server* sv) {
+  for (;;) {
+    std::vector<std::shared_ptr<ServeReq>> act;
+    {
+      std::unique_lock<std::mutex> lk(sv->mu);
+      sv->work_cv.wait(lk, [&] { return sv->stop || !sv->pending.empty() || !sv->active.empty(); });
+      if (sv->stop && sv->pending.empty() && sv->active.empty()) return;
+      // admission in arrival order: memory budget and queue capacity
+      size_t pixels = 0;
+      for (auto& r : sv->active) pixels += (size_t)r->req.width * r->req.height;
+      while (!sv->pending.empty()) {
+        auto r = sv->pending.front();
+        const size_t npix = (size_t)r->req.width * r->req.height;
+        const bool fits = sv->fixed_bytes + sv->used_bytes + r->bytes <= sv->budget &&
+                          pixels + npix <= sv->max_paths;
+        if (!fits && !sv->active.empty()) break;
+        sv->pending.pop_front();
+        if (!fits || r->hdr.alloc(npix * 3) != hipSuccess ||
+            hipMemset(r->hdr.p, 0, npix * 12) != hipSuccess ||
+            (r->req.denoise && (r->feat.alloc(npix * 8) != hipSuccess ||
+                                hipMemset(r->feat.p, 0, npix * 32) != hipSuccess))) {
+          r->status = !fits ? CV_ERR_OUT_OF_MEMORY : CV_ERR_DEVICE;
+          r->error = !fits ? "request exceeds the memory budget or the step capacity"
+                           : "request buffers";
+          r->hdr.reset();
+          r->feat.reset();
+          r->done = true;
+          r->t_done = now_ms();
+          sv->done_cv.notify_all();
+          continue;
+        }
+        sv->used_bytes += r->bytes;
+        sv->peak_bytes = std::max(sv->peak_bytes, sv->fixed_bytes + sv->used_bytes);
+        pixels += npix;
+        sv->active.push_back(r);
+      }
+      act = sv->active;
+    }
+    if (act.empty()) continue;
+    // one step: the next frame of every active request
+    StepPlan step;
+    uint32_t max_depth = 0;
+    bool packed = true;
+    for (auto& r : act) {
+      const cv_request& q = r->req;
+      cv_scene* sc = sv->scenes[q.scene];
+      cvk::JobDesc d = cvh::job_for_scene(sc);
+      memcpy(d.cam.pos, q.camera_pos, 12);
+      memset(d.cam.dpos, 0, 12);
+      memcpy(d.cam.q, q.camera_q, 16);
+      memset(d.cam.dq, 0, 16);
+      if (q.screen_distance > 0.0f) d.cam.screen_distance = q.screen_distance;
+      d.width = q.width;
+      d.height = q.height;
+      d.rect_w = q.width;
+      d.rect_h = q.height;
+      d.frame = 1 + r->frames_done;
+      d.hdr = r->hdr.p;
+      d.features = q.denoise ? r->feat.p : nullptr;
+      d.path_base = step.paths;
+      step.jobs.push_back(d);
+      step.paths += q.width * q.height;
+      max_depth = std::max(max_depth, sc->info.depth);
+      packed = packed && sc->packed;
+    }
+    RunOpts o;
+    o.cfg = sv->cfg;
+    o.max_bounces = sv->opts.max_bounces;
+    o.packed = packed;
+    o.multi_job = true;
+    o.max_depth = max_depth;
+    o.graph = true;
+    std::vector<StepPlan> steps(1, std::move(step));
+    cv_render_stats stt{};
+    cv_status st = pipeline_run(sv->pipe, steps, o, &stt);
+    std::string err = st != CV_OK ? cv_last_error() : "";
+    std::vector<std::shared_ptr<ServeReq>> finished;
+    {
+      std::lock_guard<std::mutex> lk(sv->mu);
+      sv->steps++;
+      sv->busy_ms += stt.total_ms;
+      for (auto& r : act) {
+        r->frames_done++;
+        if (st != CV_OK || r->frames_done >= r->req.spp) finished.push_back(r);
+      }
+      for (auto& r : finished)
+        sv->active.erase(std::find(sv->active.begin(), sv->active.end(), r));
+    }
+    for (auto& r : finished) {
+      if (st != CV_OK) {
+        r->status = st;
+        r->error = err;
+        r->hdr.reset();
+        r->feat.reset();
+      } else {
+        finish_request(sv, *r);
+      }
+    }
+    {
+      std::lock_guard<std::mutex> lk(sv->mu);
+      for (auto& r : finished) {
+        r->done = true;
+        r->t_done = now_ms();
+        sv->used_bytes -= r->bytes;
+        if (r->status == CV_OK) {
+          sv->completed++;
+          sv->latencies.push_back(r->t_done - r->t_submit);
+        }
+      }
+    }
+    if (!finished.empty()) sv->done_cv.notify_all();
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+cv_status cv_server_create(const cv_render_opts* opts, uint64_t hbm_budget_bytes,
+                           uint64_t max_paths_per_step, cv_server** out) {
+  if (!out) return fail(CV_ERR_INVALID_ARGUMENT, "null argument");
+  *out = nullptr;
+  cv_status cs = cvh::ensure_device();
+  if (cs != CV_OK) return cs;
+  cv_render_opts o;
+  if ((cs = check_opts(opts, &o)) != CV_OK) return cs;
+  if (max_paths_per_step == 0) max_paths_per_step = 1ull << 23;
+  if (max_paths_per_step >= (1ull << 31))
+    return fail(CV_ERR_INVALID_ARGUMENT, "max_paths_per_step must be below 2^31");
+  cv_server* sv = new cv_server();
+  sv->opts = o;
+  sv->budget = hbm_budget_bytes ? hbm_budget_bytes : (200ull << 30);
+  sv->max_paths = max_paths_per_step;
+  cv_tuning t{8, 32, 32, 1};
+  cvh::apply_tuning(sv->cfg, t);
+  if ((cs = sv->pipe.init()) != CV_OK || (cs = sv->pipe.reserve(sv->max_paths, 256)) != CV_OK) {
+    delete sv;
+    return cs;
+  }
+  sv->fixed_bytes = sv->max_paths * (3 * 32 + 2 * 8 + 16);
+  if (sv->fixed_bytes > sv->budget) {
+    delete sv;
+    return fail(CV_ERR_OUT_OF_MEMORY, "the path queues alone exceed the memory budget");
+  }
+  sv->worker = std::thread(serve_loop, sv);
+  *out = sv;
+  return CV_OK;
+}
+
+void cv_server_destroy(cv_server* sv) {
+  if (!sv) return;
+  {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    sv->stop = true;
+  }
+  sv->work_cv.notify_all();
+  if (sv->worker.joinable()) sv->worker.join();
+  delete sv;
+}
+
+cv_status cv_server_add_scene(cv_server* sv, cv_scene* scene, uint32_t* id) {
+  if (!sv || !scene || !id) return fail(CV_ERR_INVALID_ARGUMENT, "null argument");
+  if (scene->ndirect < 8)
+    return fail(CV_ERR_LIGHT_COUNT, "the scene has fewer than 8 direct-sample triangles");
+  std::lock_guard<std::mutex> lk(sv->mu);
+  const size_t b = scene_bytes(scene);
+  if (sv->fixed_bytes + sv->used_bytes + b > sv->budget)
+    return fail(CV_ERR_OUT_OF_MEMORY, "the scene does not fit the memory budget");
+  sv->fixed_bytes += b;
+  sv->scenes.push_back(scene);
+  *id = (uint32_t)(sv->scenes.size() - 1);
+  return CV_OK;
+}
+
+cv_status cv_server_set_denoiser(cv_server* sv, cv_denoiser* d) {
+  if (!sv) return fail(CV_ERR_INVALID_ARGUMENT, "null argument");
+  std::lock_guard<std::mutex> lk(sv->mu);
+  sv->denoiser = d;
+  return CV_OK;
+}
+
+cv_status cv_server_set_tuning(cv_server* sv, const cv_tuning* t) {
+  if (!sv || !t) return fail(CV_ERR_INVALID_ARGUMENT, "null argument");
+  cv_status v = cvh::validate_tuning(*t);
+  if (v != CV_OK) return v;
+  std::lock_guard<std::mutex> lk(sv->mu);
+  cvh::apply_tuning(sv->cfg, *t);
+  return CV_OK;
+}
+
+cv_status cv_server_submit(cv_server* sv, const cv_request* req, uint64_t* ticket) {
+  if (!sv || !req || !ticket || !req->hdr) return fail(CV_ERR_INVALID_ARGUMENT, "null argument");
+  std::unique_lock<std::mutex> lk(sv->mu);
+  if (req->scene >= sv->scenes.size()) return fail(CV_ERR_OUT_OF_RANGE, "unknown scene");
+  cv_status cs = check_frame(sv->scenes[req->scene], req->width, req->height);
+  if (cs != CV_OK) return cs;
+  if (req->spp == 0) return fail(CV_ERR_INVALID_ARGUMENT, "spp must be at least 1");
+  if (req->denoise) {
+    if (!sv->denoiser) return fail(CV_ERR_INVALID_ARGUMENT, "no denoiser set");
+    if (req->width % 64) return fail(CV_ERR_INVALID_ARGUMENT, "denoised widths are multiples of 64");
+  }
+  if ((size_t)req->width * req->height > sv->max_paths)
+    return fail(CV_ERR_OUT_OF_RANGE, "request larger than the step capacity");
+  auto r = std::make_shared<ServeReq>();
+  r->ticket = sv->next_ticket++;
+  r->req = *req;
+  r->bytes = request_bytes(*req);
+  r->t_submit = now_ms();
+  sv->pending.push_back(r);
+  sv->tickets[r->ticket] = r;
+  sv->submitted++;
+  *ticket = r->ticket;
+  lk.unlock();
+  sv->work_cv.notify_all();
+  return CV_OK;
+}
+
+cv_status cv_server_poll(cv_server* sv, uint64_t ticket, int* done) {
+  if (!sv || !done) return fail(CV_ERR_INVALID_ARGUMENT, "null argument");
+  std::lock_guard<std::mutex> lk(sv->mu);
+  auto it = sv->tickets.find(ticket);
+  if (it == sv->tickets.end()) return fail(CV_ERR_OUT_OF_RANGE, "unknown ticket");
+  *done = it->second->done ? 1 : 0;
+  return CV_OK;
+}
+
+cv_status cv_server_wait(cv_server* sv, uint64_t ticket, double* latency_ms) {
+  if (!sv) return fail(CV_ERR_INVALID_ARGUMENT, "null argument");
+  std::unique_lock<std::mutex> lk(sv->mu);
+  auto it = sv->tickets.find(ticket);
+  if (it == sv->tickets.end()) return fail(CV_ERR_OUT_OF_RANGE, "unknown ticket");
+  auto r = it->second;
+  sv->done_cv.wait(lk, [&] { return r->done; });
+  sv->tickets.erase(ticket);
+  if (latency_ms) *latency_ms = r->t_done - r->t_submit;
+  if (r->status != CV_OK) return fail(r->status, r->error);
+  return CV_OK;
+}
+
+cv_status cv_server_get_stats(cv_server* sv, cv_server_stats* out) {
+  if (!sv || !out) return fail(CV_ERR_INVALID_ARGUMENT, "null argument");
+  std::lock_guard<std::mutex> lk(sv->mu);
+  cv_server_stats s{};
+  s.submitted = sv->submitted;
+  s.completed = sv->completed;
+  s.steps = sv->steps;
+  s.busy_ms = sv->busy_ms;
+  s.peak_bytes = sv->peak_bytes;
+  s.resident_bytes = sv->fixed_bytes;
+  std::vector<double> l = sv->latencies;
+  if (!l.empty()) {
+    std::sort(l.begin(), l.end());
+    auto pct = [&](double q) {
+      size_t k = (size_t)std::min<double>((double)l.size() - 1, q * (double)(l.size() - 1) + 0.5);
+      return l[k];
+    };
+    s.p50_ms = pct(0.50);
+    s.p99_ms = pct(0.99);
+    double sum = 0;
+    for (double v : l) sum += v;
+    s.mean_ms = sum / (double)l.size();
+  }
+  *out = s;
+  return CV_OK;
+}
+
+cv_status cv_server_reset_stats(cv_server* sv) {
+  if (!sv) return fail(CV_ERR_INVALID_ARGUMENT, "null argument");
+  std::lock_guard<std::mutex> lk(sv->mu);
+  sv->submitted = sv->completed = sv->steps = 0;
+  sv->busy_ms = 0;
+  sv->latencies.clear();
+  sv->peak_bytes = sv->fixed_bytes + sv->used_bytes;
   return CV_OK;
 }
 

@@ -304,6 +304,61 @@ cv_status cv_batch_autotune(cv_batch* b, const cv_job* jobs, size_t n, uint64_t 
 cv_status cv_batch_get_stats(const cv_batch* b, cv_render_stats* stats);
 void cv_batch_destroy(cv_batch* b);
 
+/* ----------------------------------------------------------------- server */
+
+/* A request loop over resident scenes with continuous batching (no
+ * reference counterpart; the reference renders one interactive view,
+ * main.rs:124-217).  A worker thread runs steps: each step admits waiting
+ * requests in arrival order while the memory budget (resident scenes, path
+ * queues and the admitted requests' buffers) and the step's path capacity
+ * allow, then renders the next frame of every admitted request in shared
+ * launches (a captured HIP graph).  A request is finished after spp steps
+ * (then denoised if asked) and its result delivered; its image is the same
+ * as when rendered alone (PIXEL streams are keyed by pixel and frame). */
+typedef struct cv_server cv_server;
+typedef struct cv_denoiser cv_denoiser; /* see the denoiser section */
+
+typedef struct cv_request {
+    uint32_t scene;            /* index returned by cv_server_add_scene */
+    uint32_t width;            /* multiple of 16 (64 with denoise) */
+    uint32_t height;           /* multiple of 4 */
+    uint32_t spp;              /* frames 1..spp */
+    uint32_t denoise;          /* 1: run the server's denoiser on the result */
+    uint32_t flags;            /* CV_PTR_DEVICE: hdr is a device pointer */
+    float camera_pos[3];       /* camera position */
+    float camera_q[4];         /* camera orientation quaternion (scene.rs:47-63) */
+    float screen_distance;     /* 0: the scene's */
+    float* hdr;                /* result: 16x4-block HDR holding spp x radiance */
+} cv_request;
+
+typedef struct cv_server_stats {
+    uint64_t submitted, completed, steps;
+    double p50_ms, p99_ms, mean_ms; /* submit -> delivered latency of completed requests */
+    double busy_ms;                 /* summed device time of the steps */
+    uint64_t peak_bytes;            /* peak of resident + admitted bytes */
+    uint64_t resident_bytes;        /* scenes + path queues */
+} cv_server_stats;
+
+/* hbm_budget_bytes: 0 selects 200 GiB; max_paths_per_step: 0 selects 2^23. */
+cv_status cv_server_create(const cv_render_opts* opts, uint64_t hbm_budget_bytes,
+                           uint64_t max_paths_per_step, cv_server** out);
+/* Registers a resident scene (it must outlive the server). */
+cv_status cv_server_add_scene(cv_server* sv, cv_scene* scene, uint32_t* id);
+/* The denoiser used by requests with denoise = 1 (must outlive the server). */
+cv_status cv_server_set_denoiser(cv_server* sv, cv_denoiser* d);
+cv_status cv_server_set_tuning(cv_server* sv, const cv_tuning* tuning);
+/* Queues a request and returns at once; thread safe. */
+cv_status cv_server_submit(cv_server* sv, const cv_request* req, uint64_t* ticket);
+/* done = 1 when the request's result is delivered (or it failed). */
+cv_status cv_server_poll(cv_server* sv, uint64_t ticket, int* done);
+/* Blocks until the request is delivered; returns its status.  The ticket is
+ * released. */
+cv_status cv_server_wait(cv_server* sv, uint64_t ticket, double* latency_ms);
+cv_status cv_server_get_stats(cv_server* sv, cv_server_stats* stats);
+cv_status cv_server_reset_stats(cv_server* sv);
+/* Finishes every queued request, then stops the worker. */
+void cv_server_destroy(cv_server* sv);
+
 /* --------------------------------------------------------------- denoiser */
 
 /* The learned denoiser that stands where the reference runs its median

@@ -1,0 +1,108 @@
+"""cv_server: continuous batching over resident scenes.  A request's result
+is bit-identical whether it runs alone or mixed with others (and equals the
+single-job renderer with the same camera); admission respects the memory
+budget; denoised requests equal the denoiser applied by hand."""
+import numpy as np
+import pytest
+
+from convector_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+Q0 = (0.70710677, -0.70710677, 0.0, 0.0)
+# (scene, width, height, spp, camera position)
+REQS = [(0, 64, 32, 3, (0.0, 0.9, 0.0)), (1, 128, 64, 1, (0.1, 0.8, -0.2)),
+        (2, 32, 16, 5, (-0.3, 0.7, 0.1)), (0, 96, 48, 2, (0.2, 0.6, 0.3)),
+        (1, 64, 64, 4, (0.0, 0.5, 0.0)), (2, 160, 96, 2, (0.3, 0.85, -0.4)),
+        (0, 32, 32, 1, (-0.5, 0.4, 0.2)), (1, 48, 16, 6, (0.4, 0.3, 0.4))]
+
+
+@pytest.fixture(scope="module")
+def world(cv):
+    sds = [scenes.config0(subdivisions=3), scenes.procedural_scene(5000, 31),
+           scenes.procedural_scene(2000, 32)]
+    return sds, [cv.Scene.from_scene_data(sd) for sd in sds]
+
+
+def test_alone_equals_mix_equals_renderer(cv, world):
+    sds, gs = world
+    sv = cv.Server()
+    ids = [sv.add_scene(g) for g in gs]
+    alone = []
+    for (si, w, h, spp, pos) in REQS:
+        t, hdr = sv.submit(ids[si], w, h, spp, pos, Q0)
+        sv.wait(t)
+        alone.append(hdr)
+    mix = [sv.submit(ids[si], w, h, spp, pos, Q0) for (si, w, h, spp, pos) in REQS]
+    lat = [sv.wait(t) for t, _ in mix]
+    for a, (_, m) in zip(alone, mix):
+        assert a.tobytes() == m.tobytes()
+    assert all(x > 0 for x in lat)
+    # the renderer with the same camera
+    for k in (0, 5):
+        si, w, h, spp, pos = REQS[k]
+        gs[si].camera.set_position(pos)
+        gs[si].camera.set_orientation(Q0)
+        r = cv.Renderer(gs[si], w, h)
+        ref = r.new_buffer_f32()
+        r.accumulate_frames_f32(ref, 1, spp)
+        assert ref.tobytes() == alone[k].tobytes()
+    st = sv.stats()
+    assert st["completed"] == 2 * len(REQS) and st["p99_ms"] >= st["p50_ms"] > 0
+    sv.close()
+    for g, sd in zip(gs, sds):
+        g.camera.set_position(sd.camera_pos)
+        g.camera.set_orientation(sd.camera_q)
+
+
+def test_budget_admission(cv, world):
+    """A budget that admits one request at a time still finishes every
+    request with the same result; one larger than the budget fails."""
+    sds, gs = world
+    big = cv.Server()
+    ids = [big.add_scene(g) for g in gs]
+    ref = [big.submit(ids[si], w, h, spp, pos, Q0) for (si, w, h, spp, pos) in REQS[:4]]
+    for t, _ in ref:
+        big.wait(t)
+    resident = big.stats()["resident_bytes"]
+    big.close()
+    # room for the scenes, the queues and about one 128x64 request
+    small = cv.Server(budget_bytes=resident + 128 * 64 * 12 + 1024)
+    ids = [small.add_scene(g) for g in gs]
+    got = [small.submit(ids[si], w, h, spp, pos, Q0) for (si, w, h, spp, pos) in REQS[:4]]
+    for (t, hdr), (_, r) in zip(got, ref):
+        small.wait(t)
+        assert hdr.tobytes() == r.tobytes()
+    t, _ = small.submit(ids[0], 256, 128, 1, REQS[0][4], Q0)
+    with pytest.raises(cv.CvError) as e:
+        small.wait(t)
+    assert e.value.name == "CV_ERR_OUT_OF_MEMORY"
+    small.close()
+
+
+def test_denoised_requests(cv, world):
+    sds, gs = world
+    d = cv.Denoiser(3)
+    sv = cv.Server()
+    sid = sv.add_scene(gs[1])
+    sv.set_denoiser(d)
+    W, H, spp, pos = 64, 32, 2, (0.1, 0.6, 0.0)
+    t, out = sv.submit(sid, W, H, spp, pos, Q0, denoise=True)
+    t2, plain = sv.submit(sid, W, H, spp, pos, Q0)
+    sv.wait(t)
+    sv.wait(t2)
+    # the same by hand: accumulation with features, network, HDR
+    gs[1].camera.set_position(pos)
+    gs[1].camera.set_orientation(Q0)
+    r = cv.Renderer(gs[1], W, H)
+    hdr, feat = r.new_buffer_f32(), r.new_features()
+    r.accumulate_aov_f32(hdr, feat, 1, spp)
+    assert hdr.tobytes() == plain.tobytes()
+    inp = cv.Denoiser.prepare_input(hdr, feat, W, H, spp)
+    expect = cv.Denoiser.output_to_hdr(d.forward(inp[None])[0], W, H, spp)
+    assert out.tobytes() == expect.tobytes()
+    with pytest.raises(cv.CvError):
+        sv.submit(sid, 48, 32, 1, pos, Q0, denoise=True)  # width not a multiple of 64
+    sv.close()
+    gs[1].camera.set_position(sds[1].camera_pos)
+    gs[1].camera.set_orientation(sds[1].camera_q)
