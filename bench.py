@@ -419,6 +419,109 @@ def bench_denoise(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_serve(args):
+    """The request loop: 4 resident scenes (about 20k, 200k, 1.3M and 2M
+    triangles), Poisson arrivals at each rate, every request a random camera
+    in the box, a resolution from {256x256, 1280x720, 1920x1080}, spp from
+    {1, 4, 16} and the denoiser on half of them.  Continuous batching (the
+    server admits requests into shared steps as memory and queue capacity
+    free up) against the same requests one at a time through the same
+    server."""
+    import convector_amd as cv
+    from convector_amd import scenes
+
+    t0 = time.perf_counter()
+    sds = [scenes.config0(), scenes.procedural_scene(200000, 41), scenes.config1(),
+           scenes.procedural_scene(2000000, 43)]
+    gs = [cv.Scene.from_scene_data(sd) for sd in sds]
+    log("serve: scenes %s built in %.1f s" % ([sd.ntris for sd in sds], time.perf_counter() - t0))
+    d = cv.Denoiser(5)  # untrained weights: the timing does not depend on them
+    sizes = [(256, 256), (1280, 720), (1920, 1080)]
+    rng = np.random.default_rng(77)
+
+    def make_requests(n):
+        out = []
+        for _ in range(n):
+            pos = (float(rng.uniform(-0.6, 0.6)), float(rng.uniform(0.2, 0.9)),
+                   float(rng.uniform(-0.6, 0.6)))
+            w, h = sizes[int(rng.integers(0, 3))]
+            out.append((int(rng.integers(0, 4)), w, h, int((1, 4, 16)[int(rng.integers(0, 3))]),
+                        pos, bool(rng.integers(0, 2))))
+        return out
+
+    Q0 = (0.70710677, -0.70710677, 0.0, 0.0)
+    sv = cv.Server(budget_bytes=200 << 30, max_paths_per_step=1 << 23)
+    ids = [sv.add_scene(g) for g in gs]
+    sv.set_denoiser(d)
+    # warm-up: every size once
+    for w, h in sizes:
+        t, _ = sv.submit(ids[0], w, h, 1, (0.0, 0.9, 0.0), Q0, denoise=True)
+        sv.wait(t)
+    rates = []
+    for rate in args.rates:
+        reqs = make_requests(args.requests)
+        gaps = rng.exponential(1.0 / rate, len(reqs))
+        sv.reset_stats()
+        tickets = []
+        t_start = time.perf_counter()
+        t_next = t_start
+        for (si, w, h, spp, pos, dn), gap in zip(reqs, gaps):
+            t_next += gap
+            delay = t_next - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            tickets.append(sv.submit(ids[si], w, h, spp, pos, Q0, denoise=dn))
+        lat = [sv.wait(t) for t, _ in tickets]
+        wall = time.perf_counter() - t_start
+        st = sv.stats()
+        rates.append({"rate_req_per_s": rate, "requests": len(reqs),
+                      "p50_ms": round(float(np.percentile(lat, 50)), 2),
+                      "p99_ms": round(float(np.percentile(lat, 99)), 2),
+                      "frames_per_s": round(len(reqs) / wall, 2),
+                      "paths_per_s": round(sum(w * h * spp for (_, w, h, spp, _, _) in reqs) / wall, 1),
+                      "steps": st["steps"], "device_busy_ms": round(st["busy_ms"], 1),
+                      "wall_ms": round(wall * 1e3, 1)})
+        log("serve rate %s: %s" % (rate, rates[-1]))
+    # the same kind of requests one at a time
+    reqs = make_requests(args.requests)
+    t_start = time.perf_counter()
+    lat1 = []
+    for (si, w, h, spp, pos, dn) in reqs:
+        t, _ = sv.submit(ids[si], w, h, spp, pos, Q0, denoise=dn)
+        lat1.append(sv.wait(t))
+    wall1 = time.perf_counter() - t_start
+    serial = {"frames_per_s": round(len(reqs) / wall1, 2),
+              "p50_ms_service": round(float(np.percentile(lat1, 50)), 2),
+              "p99_ms_service": round(float(np.percentile(lat1, 99)), 2)}
+    log("serve one at a time: %s" % serial)
+    peak = sv.stats()["peak_bytes"]
+    sv.close()
+    best = max(rates, key=lambda r: r["frames_per_s"])
+    out = {
+        "metric": "p99 frame latency and frames/s, continuous batching",
+        "value": best["frames_per_s"],
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.requests,
+        "warmup": 3,
+        "ms_per_step": round(1e3 / best["frames_per_s"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "4 resident scenes (%s triangles), Poisson arrivals at %s req/s, "
+                               "%d requests per rate, 256^2/720p/1080p, 1/4/16 spp, denoiser on "
+                               "half, 200 GiB budget" % ([sd.ntris for sd in sds], args.rates,
+                                                         args.requests)},
+        "rates": rates,
+        "one_at_a_time": serial,
+        "speedup_vs_one_at_a_time": round(best["frames_per_s"] / serial["frames_per_s"], 3),
+        "peak_bytes": peak,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -428,10 +531,13 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--subdivisions", type=int, default=8)
-    ap.add_argument("--workload", choices=["frame", "batch", "denoise"], default="frame",
+    ap.add_argument("--rates", type=float, nargs="+", default=[50, 100, 200, 400])
+    ap.add_argument("--requests", type=int, default=100)
+    ap.add_argument("--workload", choices=["frame", "batch", "denoise", "serve"], default="frame",
                     help="frame: the large 16-spp 1080p render (default); batch: 64 mixed "
                          "jobs in shared launches against sequential launches; denoise: train "
-                         "the denoiser and measure its equal-RMSE speedup")
+                         "the denoiser and measure its equal-RMSE speedup; serve: the request "
+                         "loop's latency and throughput")
     ap.add_argument("--jobs", type=int, default=64)
     ap.add_argument("--train-scenes", type=int, default=96)
     ap.add_argument("--train-steps", type=int, default=1000)
@@ -454,6 +560,9 @@ def main():
         return
     if args.workload == "denoise":
         bench_denoise(args)
+        return
+    if args.workload == "serve":
+        bench_serve(args)
         return
 
     import convector_amd as cv
