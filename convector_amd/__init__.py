@@ -550,6 +550,20 @@ class Server:
         self.close()
 
 
+def trace_enable(limit: int = 4096):
+    """Start recording device kernel spans (trace.rs TraceLog)."""
+    check(lib().cv_trace_enable(limit))
+
+
+def trace_disable():
+    check(lib().cv_trace_disable())
+
+
+def trace_export(path: str):
+    """Write the recorded spans as a chrome://tracing JSON file."""
+    check(lib().cv_trace_export(path.encode()))
+
+
 def device_synchronize():
     check(lib().cv_device_synchronize())
 

@@ -45,7 +45,8 @@ EXPORTS = [
     "cv_denoiser_train_step", "cv_denoiser_prepare_input", "cv_denoiser_output_to_hdr",
     "cv_denoiser_destroy", "cv_server_create", "cv_server_add_scene", "cv_server_set_denoiser",
     "cv_server_set_tuning", "cv_server_submit", "cv_server_poll", "cv_server_wait",
-    "cv_server_get_stats", "cv_server_reset_stats", "cv_server_destroy",
+    "cv_server_get_stats", "cv_server_reset_stats", "cv_server_destroy", "cv_trace_enable",
+    "cv_trace_disable", "cv_trace_export",
     "cv_device_synchronize", "cv_last_error", "cv_status_string", "cv_version",
 ]
 
@@ -187,6 +188,9 @@ def _declare(L):
         "cv_server_get_stats": (I, [P, P]),
         "cv_server_reset_stats": (I, [P]),
         "cv_server_destroy": (None, [P]),
+        "cv_trace_enable": (I, [SZ]),
+        "cv_trace_disable": (I, []),
+        "cv_trace_export": (I, [ctypes.c_char_p]),
         "cv_device_synchronize": (I, []),
         "cv_last_error": (ctypes.c_char_p, []),
         "cv_status_string": (ctypes.c_char_p, [I]),

@@ -181,15 +181,57 @@ struct Pipeline {
 
 namespace {
 
-typedef std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> EventList;
+// A recorded launch: kind (0 generate, 1 extend, 2 shade), bounce, the
+// frame of the step's first job, and its start / end events.
+struct LaunchEvent {
+  int kind;
+  uint32_t bounce, frame;
+  hipEvent_t start, end;
+};
+typedef std::vector<LaunchEvent> EventList;
+
+// The trace log (trace.rs:22-152 restated for device spans): a bounded
+// ring of kernel spans, exported in the Chrome trace format.
+struct TraceSpan {
+  int kind;
+  double ts_us, dur_us;
+  uint32_t frame, id;
+};
+std::mutex g_trace_mu;
+bool g_trace_on = false;
+size_t g_trace_limit = 0;
+std::deque<TraceSpan> g_trace;
+hipEvent_t g_trace_epoch = nullptr;
+
+bool trace_enabled() {
+  std::lock_guard<std::mutex> lk(g_trace_mu);
+  return g_trace_on;
+}
+
+void trace_record(const EventList& evs) {
+  std::lock_guard<std::mutex> lk(g_trace_mu);
+  if (!g_trace_on) return;
+  for (const LaunchEvent& e : evs) {
+    float t0 = 0.0f, d = 0.0f;
+    if (hipEventElapsedTime(&t0, g_trace_epoch, e.start) != hipSuccess) continue;
+    if (hipEventElapsedTime(&d, e.start, e.end) != hipSuccess) continue;
+    if (g_trace.size() == g_trace_limit) g_trace.pop_front();
+    g_trace.push_back(TraceSpan{e.kind, t0 * 1e3, d * 1e3, e.frame, e.bounce});
+  }
+}
 
 // The launches of one step (generate, then extend + shade per bounce) for
 // job-table slot `slot`.  With graph == true every grid is sized for the
 // pipeline's capacity, so that the recorded launches serve any step.
 cv_status launch_step(Pipeline& p, const RunOpts& o, uint32_t slot, uint32_t paths, bool graph,
-                      EventList* evs) {
+                      EventList* evs, uint32_t frame = 0) {
   hipStream_t st = p.stream;
-  auto ev_pair = [&](int kind) -> std::pair<hipEvent_t, hipEvent_t>* {
+  uint32_t bounce = 0;
+  struct Pair {
+    hipEvent_t first, second;
+  };
+  Pair pair{};
+  auto ev_pair = [&](int kind) -> Pair* {
     if (!evs) return nullptr;
     hipEvent_t a, b;
     if (hipEventCreate(&a) != hipSuccess) return nullptr;
@@ -197,8 +239,9 @@ cv_status launch_step(Pipeline& p, const RunOpts& o, uint32_t slot, uint32_t pat
       (void)hipEventDestroy(a);
       return nullptr;
     }
-    evs->push_back({kind, {a, b}});
-    return &evs->back().second;
+    evs->push_back(LaunchEvent{kind, bounce, frame, a, b});
+    pair = Pair{a, b};
+    return &pair;
   };
   const size_t grid_paths = graph ? p.capacity : paths;
   const cvk::JobDesc* jobs = p.jobs[slot].p;
@@ -214,6 +257,7 @@ cv_status launch_step(Pipeline& p, const RunOpts& o, uint32_t slot, uint32_t pat
   const bool multi_scene = o.nodes == nullptr;
   for (uint32_t b = 0; b < o.max_bounces; b++) {
     const int cur = b & 1, nxt = cur ^ 1;
+    bounce = b;
     cvk::ExtendArgs x{};
     x.ctrl = p.ctrl.p;
     x.bounce = b;
@@ -260,7 +304,7 @@ cv_status pipeline_run(Pipeline& p, const std::vector<StepPlan>& steps, const Ru
   cs = p.reserve_spill(o.cfg, o.max_depth);
   if (cs != CV_OK) return cs;
   hipStream_t st = p.stream;
-  const bool timing = o.flags & CV_RENDER_TIMING;
+  const bool timing = (o.flags & CV_RENDER_TIMING) || trace_enabled();
   const bool use_graph = o.graph && !timing;
   HIPTRY(cvk::launch_zero_u32(reinterpret_cast<uint32_t*>(&p.ctrl.p->sum_aabb), 6, st), "zero");
   GraphKey key;
@@ -317,7 +361,8 @@ cv_status pipeline_run(Pipeline& p, const std::vector<StepPlan>& steps, const Ru
       }
       HIPTRY(hipGraphLaunch(p.graphs[slot], st), "graph launch");
     } else {
-      status = launch_step(p, o, slot, s.paths, false, timing ? &evs : nullptr);
+      status = launch_step(p, o, slot, s.paths, false, timing ? &evs : nullptr,
+                           s.jobs.empty() ? 0u : s.jobs[0].frame);
       if (status != CV_OK) break;
     }
   }
@@ -333,17 +378,18 @@ cv_status pipeline_run(Pipeline& p, const std::vector<StepPlan>& steps, const Ru
   float ms = 0.0f;
   (void)hipEventElapsedTime(&ms, t0, t1);
   stt.total_ms = ms;
+  trace_record(evs);
   for (auto& e : evs) {
     float m = 0.0f;
-    (void)hipEventElapsedTime(&m, e.second.first, e.second.second);
-    if (e.first == 0) stt.generate_ms += m;
-    if (e.first == 1) {
+    (void)hipEventElapsedTime(&m, e.start, e.end);
+    if (e.kind == 0) stt.generate_ms += m;
+    if (e.kind == 1) {
       stt.extend_ms += m;
       stt.extend_launches++;
     }
-    if (e.first == 2) stt.shade_ms += m;
-    (void)hipEventDestroy(e.second.first);
-    (void)hipEventDestroy(e.second.second);
+    if (e.kind == 2) stt.shade_ms += m;
+    (void)hipEventDestroy(e.start);
+    (void)hipEventDestroy(e.end);
   }
   (void)hipEventDestroy(t0);
   (void)hipEventDestroy(t1);
@@ -1267,6 +1313,52 @@ cv_status cv_server_reset_stats(cv_server* sv) {
   sv->latencies.clear();
   sv->peak_bytes = sv->fixed_bytes + sv->used_bytes;
   return CV_OK;
+}
+
+}  // extern "C"
+
+// -------------------------------------------------------------------- trace
+
+extern "C" {
+
+cv_status cv_trace_enable(size_t limit) {
+  cv_status cs = cvh::ensure_device();
+  if (cs != CV_OK) return cs;
+  if (limit == 0) return fail(CV_ERR_INVALID_ARGUMENT, "limit must be positive");
+  std::lock_guard<std::mutex> lk(g_trace_mu);
+  if (!g_trace_epoch) HIPTRY(hipEventCreate(&g_trace_epoch), "event");
+  HIPTRY(hipEventRecord(g_trace_epoch, nullptr), "event");
+  HIPTRY(hipEventSynchronize(g_trace_epoch), "event");
+  g_trace.clear();
+  g_trace_limit = limit;
+  g_trace_on = true;
+  return CV_OK;
+}
+
+cv_status cv_trace_disable(void) {
+  std::lock_guard<std::mutex> lk(g_trace_mu);
+  g_trace_on = false;
+  return CV_OK;
+}
+
+cv_status cv_trace_export(const char* path) {
+  if (!path) return fail(CV_ERR_INVALID_ARGUMENT, "null path");
+  std::lock_guard<std::mutex> lk(g_trace_mu);
+  FILE* f = fopen(path, "w");
+  if (!f) return fail(CV_ERR_IO, std::string("cannot open ") + path);
+  static const char* names[3] = {"generate", "extend", "shade"};
+  fputs("{\"traceEvents\":[", f);
+  bool first = true;
+  for (const TraceSpan& e : g_trace) {
+    fprintf(f,
+            "%s{\"name\":\"%s\",\"cat\":\"\",\"ph\":\"X\",\"ts\":%.3f,\"dur\":%.3f,\"pid\":0,"
+            "\"tid\":%d,\"args\":{\"frame\":%u,\"id\":%u}}",
+            first ? "" : ",", names[e.kind], e.ts_us, e.dur_us, e.kind, e.frame, e.id);
+    first = false;
+  }
+  fputs("],\"displayTimeUnit\":\"ms\"}", f);
+  const bool ok = fclose(f) == 0;
+  return ok ? CV_OK : fail(CV_ERR_IO, std::string("cannot write ") + path);
 }
 
 }  // extern "C"

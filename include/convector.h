@@ -399,6 +399,19 @@ cv_status cv_denoiser_output_to_hdr(const float* output, uint32_t width, uint32_
                                     uint32_t spp, float* hdr, uint32_t flags);
 void cv_denoiser_destroy(cv_denoiser* d);
 
+/* ------------------------------------------------------------------ trace */
+
+/* TraceLog (trace.rs:22-152) for the device: while enabled, every
+ * generate / extend / shade launch of renderers, batches and servers is
+ * timed with HIP events (steps are not replayed from graphs) and kept in a
+ * ring of the last `limit` spans; cv_trace_export writes them in the Chrome
+ * trace format of trace.rs:122-146 (name, ph "X", ts and dur in
+ * microseconds from cv_trace_enable, tid = kernel kind, args frame and id =
+ * bounce). */
+cv_status cv_trace_enable(size_t limit);
+cv_status cv_trace_disable(void);
+cv_status cv_trace_export(const char* path);
+
 /* ----------------------------------------------------------------- device */
 
 /* Waits for all work of this library on the current device. */

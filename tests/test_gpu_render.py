@@ -163,3 +163,35 @@ def test_tuning_variants_exact(cv, oracle):
         r.accumulate_frames_f32(hdr, 1, n, gbuffer=gb)
         assert hdr.tobytes() == ohdr.tobytes()
         assert (gb == ogb).all()
+
+
+def test_trace_export(cv, tmp_path):
+    """The device trace log: every launch of the frames as a Chrome trace
+    span (trace.rs:122-146 format), bounded by the ring limit."""
+    import json
+    sd = scenes.config0(subdivisions=3)
+    g = cv.Scene.from_scene_data(sd)
+    r = cv.Renderer(g, 64, 32)
+    hdr = r.new_buffer_f32()
+    plain = r.new_buffer_f32()
+    r.accumulate_frames_f32(plain, 1, 3)
+    cv.trace_enable(1000)
+    try:
+        r.accumulate_frames_f32(hdr, 1, 3)
+        path = str(tmp_path / "trace.json")
+        cv.trace_export(path)
+        ev = json.load(open(path))["traceEvents"]
+        assert len(ev) == 3 * (1 + 2 * 5)
+        assert [e["name"] for e in ev[:3]] == ["generate", "extend", "shade"]
+        assert all(e["ph"] == "X" and e["dur"] > 0 and e["ts"] >= 0 for e in ev)
+        assert [e["args"]["frame"] for e in ev[::11]] == [1, 2, 3]
+        assert sorted(e["ts"] for e in ev) == [e["ts"] for e in ev]
+        cv.trace_enable(5)
+        r.accumulate_frames_f32(r.new_buffer_f32(), 1, 1)
+        cv.trace_export(path)
+        ev = json.load(open(path))["traceEvents"]
+        assert len(ev) == 5 and ev[-1]["name"] == "shade" and ev[-1]["args"]["id"] == 4
+    finally:
+        cv.trace_disable()
+    # tracing changes nothing in the image
+    assert hdr.tobytes() == plain.tobytes()
