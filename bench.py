@@ -14,7 +14,9 @@ HBM.  The metric is extend queries (nearest-hit traversals) per second.
 JSON line fields beyond the contract:
   roofline     the traversal (extend) kernel: algorithmic bytes per launch
                (32 B per box test + 36 B per triangle test, the oracle's
-               counters, identical on the GPU) over its HIP-event time
+               counters, identical on the GPU) over its HIP-event time;
+               traffic = its measured HBM bytes per launch from the
+               committed rocprofv3 --pmc summary under profiles/
   valu         the same kernel against the FP32 VALU issue peak
                (23 ops per box test + 48 per triangle test)
   cpu_baseline the CPU oracle (reference semantics: per-patch Rng,
@@ -62,6 +64,21 @@ class DeviceBuffer:
     def __del__(self):
         if getattr(self, "ptr", None):
             self.hip.hipFree(self.ptr)
+
+
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "pmc_extend_config1_1080p_16spp.md")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE and WRITE_SIZE scaled by a 1 GiB copy), or None."""
+    try:
+        for line in open(TRAFFIC_PROFILE):
+            if line.startswith("traffic_bytes_per_launch " + kernel + " "):
+                return float(line.rsplit(" ", 1)[1])
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(sd, gpu_scene, args):
@@ -623,6 +640,10 @@ def main():
     avg_launch_ms = ext_ms / ext_launch
     achieved_gbs = (bytes_per_step / launches_per_step) / (avg_launch_ms * 1e-3) / 1e9
     achieved_tops = (ops_per_step / launches_per_step) / (avg_launch_ms * 1e-3) / 1e12
+    tun = scene.tuning()
+    kernel = "k_extend_refill<false, %d, %s, false>" % (tun["stack_entries"],
+                                                       "true" if info["max_leaf"] <= 31 else "false")
+    traffic = measured_traffic(kernel) if tun["refill_threshold"] <= 64 else None
     out = {
         "metric": "Mrays/s vs host AVX",
         "value": round(value, 2),
@@ -655,7 +676,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved_gbs / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": round(traffic) if traffic else None,
+            "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT) if traffic else None,
+            "traffic_kernel": kernel,
             "bytes_per_launch": round(bytes_per_step / launches_per_step),
             "avg_launch_ms": round(avg_launch_ms, 4),
         },
