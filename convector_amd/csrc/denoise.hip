@@ -126,6 +126,116 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
   }
 }
 
+// The same convolution with the operands staged in LDS: a block of 4 waves
+// computes 128 pixels x NT*16 outputs, each wave 32 pixels (two 16-row A
+// fragments, so every B fragment feeds two MFMAs).  Per 32-wide K chunk the
+// block gathers the input tile (128 pixels x 32 k, tap offsets and zero
+// padding applied) and the weight tile (NT*16 rows x 32 k) into LDS with
+// 16-byte loads, double buffered: the next chunk's global loads are in
+// flight while the current chunk's MFMAs run.  Rows are padded to 40 bf16
+// (80 B) so that the 16 rows of a fragment read fall in distinct banks.
+constexpr uint32_t kConvPix = 128;
+constexpr uint32_t kRow = 40;
+
+template <int NT>
+__global__ __launch_bounds__(256) void k_conv_lds(ConvArgs a) {
+  constexpr uint32_t NB = (NT * 64 + 255) / 256;  // weight pieces per thread
+  __shared__ __attribute__((aligned(16))) uint16_t As[2][kConvPix * kRow];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[2][NT * 16 * kRow];
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const uint32_t pblock = blockIdx.x * kConvPix;
+  const uint32_t hw = a.h * a.w_;
+  const uint32_t kvalid = 9 * a.kc, nchunks = a.kpad / 32;
+  // the two input pieces of this thread: piece q = pixel q / 4, k offset (q % 4) * 8
+  uint32_t abase[2];
+  int ay[2], ax[2];
+  bool aok[2];
+  for (int i = 0; i < 2; i++) {
+    const uint32_t q = tid + 256 * i;
+    const uint32_t p = pblock + q / 4;
+    aok[i] = p < a.npix;
+    const uint32_t pp = aok[i] ? p : 0u;
+    const uint32_t b = pp / hw, rem = pp % hw;
+    ay[i] = (int)(rem / a.w_);
+    ax[i] = (int)(rem % a.w_);
+    abase[i] = b * hw;
+  }
+  uint4 ra[2], rb[NB];
+  auto load_chunk = [&](uint32_t c) {
+    for (int i = 0; i < 2; i++) {
+      const uint32_t q = tid + 256 * i;
+      const uint32_t k = c * 32 + (q % 4) * 8;
+      ra[i] = make_uint4(0, 0, 0, 0);
+      if (aok[i] && k < kvalid) {
+        const uint32_t tap = k / a.kc, ch = k % a.kc;
+        int dy = (int)(tap / 3) - 1, dx = (int)(tap % 3) - 1;
+        if (a.flip) {
+          dy = -dy;
+          dx = -dx;
+        }
+        const int sy = ay[i] + dy, sx = ax[i] + dx;
+        if (sy >= 0 && sy < (int)a.h && sx >= 0 && sx < (int)a.w_)
+          ra[i] = *reinterpret_cast<const uint4*>(
+              a.x + ((size_t)abase[i] + (uint32_t)sy * a.w_ + (uint32_t)sx) * a.xcs + ch);
+      }
+    }
+    for (uint32_t j = 0; j < NB; j++) {
+      const uint32_t q = tid + 256 * j;
+      if (q < NT * 64)
+        rb[j] = *reinterpret_cast<const uint4*>(a.w + (size_t)(q / 4) * a.kpad + c * 32 +
+                                               (q % 4) * 8);
+    }
+  };
+  auto store_chunk = [&](uint32_t buf) {
+    for (int i = 0; i < 2; i++) {
+      const uint32_t q = tid + 256 * i;
+      *reinterpret_cast<uint4*>(&As[buf][(q / 4) * kRow + (q % 4) * 8]) = ra[i];
+    }
+    for (uint32_t j = 0; j < NB; j++) {
+      const uint32_t q = tid + 256 * j;
+      if (q < NT * 64) *reinterpret_cast<uint4*>(&Bs[buf][(q / 4) * kRow + (q % 4) * 8]) = rb[j];
+    }
+  };
+  f4 acc[2][NT];
+  for (int f = 0; f < 2; f++)
+    for (int t = 0; t < NT; t++) acc[f][t] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+  load_chunk(0);
+  store_chunk(0);
+  __syncthreads();
+  const uint32_t arow = wave * 32 + (lane & 15), koff = 8 * (lane >> 4);
+  for (uint32_t c = 0; c < nchunks; c++) {
+    const uint32_t buf = c & 1;
+    if (c + 1 < nchunks) load_chunk(c + 1);
+    const bf8 A0 = *reinterpret_cast<const bf8*>(&As[buf][arow * kRow + koff]);
+    const bf8 A1 = *reinterpret_cast<const bf8*>(&As[buf][(arow + 16) * kRow + koff]);
+    for (int t = 0; t < NT; t++) {
+      const bf8 B = *reinterpret_cast<const bf8*>(&Bs[buf][(t * 16 + (lane & 15)) * kRow + koff]);
+      acc[0][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A0, B, acc[0][t], 0, 0, 0);
+      acc[1][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A1, B, acc[1][t], 0, 0, 0);
+    }
+    if (c + 1 < nchunks) store_chunk(buf ^ 1u);
+    __syncthreads();
+  }
+  for (int f = 0; f < 2; f++) {
+    for (int t = 0; t < NT; t++) {
+      const uint32_t j = t * 16 + (lane & 15);
+      if (j >= a.nout) continue;
+      const float bj = a.bias ? a.bias[j] : 0.0f;
+      for (int r = 0; r < 4; r++) {
+        const uint32_t p = pblock + wave * 32 + f * 16 + 4 * (lane >> 4) + r;
+        if (p >= a.npix) continue;
+        float v = acc[f][t][r] + bj;
+        if (a.relu) v = fmaxf(v, 0.0f);
+        if (a.y) a.y[(size_t)p * a.ycs + j] = to_bf16(v);
+        if (a.yf) {
+          float* o = a.yf + (size_t)p * a.yfcs + j;
+          *o = a.accumulate ? *o + v : v;
+        }
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Backward to the weights: dW[co][tap * cin + c] += sum_p dZ[p][co] *
 // X[p + offset(tap)][c], and db[co] += sum_p dZ[p][co].  One wave computes a
@@ -480,15 +590,15 @@ struct cv_denoiser {
 namespace {
 
 cv_status launch_conv(cv_denoiser* d, const cvd::ConvArgs& a, uint32_t nt) {
-  const uint32_t blocks = (a.npix + 63) / 64;  // npix is a multiple of 16: whole waves
+  const uint32_t blocks = (a.npix + cvd::kConvPix - 1) / cvd::kConvPix;
   hipStream_t s = d->stream;
   switch (nt) {
-    case 1: hipLaunchKernelGGL(cvd::k_conv<1>, dim3(blocks), dim3(256), 0, s, a); break;
-    case 2: hipLaunchKernelGGL(cvd::k_conv<2>, dim3(blocks), dim3(256), 0, s, a); break;
-    case 4: hipLaunchKernelGGL(cvd::k_conv<4>, dim3(blocks), dim3(256), 0, s, a); break;
-    case 6: hipLaunchKernelGGL(cvd::k_conv<6>, dim3(blocks), dim3(256), 0, s, a); break;
-    case 8: hipLaunchKernelGGL(cvd::k_conv<8>, dim3(blocks), dim3(256), 0, s, a); break;
-    case 12: hipLaunchKernelGGL(cvd::k_conv<12>, dim3(blocks), dim3(256), 0, s, a); break;
+    case 1: hipLaunchKernelGGL(cvd::k_conv_lds<1>, dim3(blocks), dim3(256), 0, s, a); break;
+    case 2: hipLaunchKernelGGL(cvd::k_conv_lds<2>, dim3(blocks), dim3(256), 0, s, a); break;
+    case 4: hipLaunchKernelGGL(cvd::k_conv_lds<4>, dim3(blocks), dim3(256), 0, s, a); break;
+    case 6: hipLaunchKernelGGL(cvd::k_conv_lds<6>, dim3(blocks), dim3(256), 0, s, a); break;
+    case 8: hipLaunchKernelGGL(cvd::k_conv_lds<8>, dim3(blocks), dim3(256), 0, s, a); break;
+    case 12: hipLaunchKernelGGL(cvd::k_conv_lds<12>, dim3(blocks), dim3(256), 0, s, a); break;
     default: return fail(CV_ERR_INVALID_ARGUMENT, "unsupported conv width");
   }
   HIPTRY(hipGetLastError(), "conv");
