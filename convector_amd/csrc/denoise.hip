@@ -16,6 +16,7 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <algorithm>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -160,44 +161,54 @@ __global__ __launch_bounds__(256) void k_conv_lds(ConvArgs a) {
     ax[i] = (int)(rem % a.w_);
     abase[i] = b * hw;
   }
-  uint4 ra[2], rb[NB];
-  auto load_chunk = [&](uint32_t c) {
-    for (int i = 0; i < 2; i++) {
-      const uint32_t q = tid + 256 * i;
-      const uint32_t k = c * 32 + (q % 4) * 8;
-      ra[i] = make_uint4(0, 0, 0, 0);
-      if (aok[i] && k < kvalid) {
-        const uint32_t tap = k / a.kc, ch = k % a.kc;
-        int dy = (int)(tap / 3) - 1, dx = (int)(tap % 3) - 1;
-        if (a.flip) {
-          dy = -dy;
-          dx = -dx;
-        }
-        const int sy = ay[i] + dy, sx = ax[i] + dx;
-        if (sy >= 0 && sy < (int)a.h && sx >= 0 && sx < (int)a.w_)
-          ra[i] = *reinterpret_cast<const uint4*>(
-              a.x + ((size_t)abase[i] + (uint32_t)sy * a.w_ + (uint32_t)sx) * a.xcs + ch);
+  static_assert(NB <= 3, "at most 3 weight pieces per thread");
+  uint4 ra0, ra1, rb0, rb1, rb2;
+  // pieces of chunk c into registers (weight pieces clamped, so that every
+  // thread loads; the unused ones are not stored)
+  auto a_piece = [&](int i, uint32_t c) -> uint4 {
+    const uint32_t q = tid + 256 * i;
+    const uint32_t k = c * 32 + (q % 4) * 8;
+    uint4 v = make_uint4(0, 0, 0, 0);
+    if (aok[i] && k < kvalid) {
+      const uint32_t tap = k / a.kc, ch = k % a.kc;
+      int dy = (int)(tap / 3) - 1, dx = (int)(tap % 3) - 1;
+      if (a.flip) {
+        dy = -dy;
+        dx = -dx;
       }
+      const int sy = ay[i] + dy, sx = ax[i] + dx;
+      if (sy >= 0 && sy < (int)a.h && sx >= 0 && sx < (int)a.w_)
+        v = *reinterpret_cast<const uint4*>(
+            a.x + ((size_t)abase[i] + (uint32_t)sy * a.w_ + (uint32_t)sx) * a.xcs + ch);
     }
-    for (uint32_t j = 0; j < NB; j++) {
-      const uint32_t q = tid + 256 * j;
-      if (q < NT * 64)
-        rb[j] = *reinterpret_cast<const uint4*>(a.w + (size_t)(q / 4) * a.kpad + c * 32 +
-                                               (q % 4) * 8);
-    }
+    return v;
+  };
+  auto b_piece = [&](uint32_t j, uint32_t c) -> uint4 {
+    const uint32_t q = min(tid + 256 * j, (uint32_t)(NT * 64 - 1));
+    return *reinterpret_cast<const uint4*>(a.w + (size_t)(q / 4) * a.kpad + c * 32 + (q % 4) * 8);
+  };
+  auto load_chunk = [&](uint32_t c) {
+    ra0 = a_piece(0, c);
+    ra1 = a_piece(1, c);
+    rb0 = b_piece(0, c);
+    if (NB > 1) rb1 = b_piece(1, c);
+    if (NB > 2) rb2 = b_piece(2, c);
+  };
+  auto put_b = [&](uint32_t buf, uint32_t j, uint4 v) {
+    const uint32_t q = tid + 256 * j;
+    if (q < NT * 64) *reinterpret_cast<uint4*>(&Bs[buf][(q / 4) * kRow + (q % 4) * 8]) = v;
   };
   auto store_chunk = [&](uint32_t buf) {
-    for (int i = 0; i < 2; i++) {
-      const uint32_t q = tid + 256 * i;
-      *reinterpret_cast<uint4*>(&As[buf][(q / 4) * kRow + (q % 4) * 8]) = ra[i];
-    }
-    for (uint32_t j = 0; j < NB; j++) {
-      const uint32_t q = tid + 256 * j;
-      if (q < NT * 64) *reinterpret_cast<uint4*>(&Bs[buf][(q / 4) * kRow + (q % 4) * 8]) = rb[j];
-    }
+    *reinterpret_cast<uint4*>(&As[buf][(tid / 4) * kRow + (tid % 4) * 8]) = ra0;
+    *reinterpret_cast<uint4*>(&As[buf][((tid + 256) / 4) * kRow + (tid % 4) * 8]) = ra1;
+    put_b(buf, 0, rb0);
+    if (NB > 1) put_b(buf, 1, rb1);
+    if (NB > 2) put_b(buf, 2, rb2);
   };
   f4 acc[2][NT];
+#pragma unroll
   for (int f = 0; f < 2; f++)
+#pragma unroll
     for (int t = 0; t < NT; t++) acc[f][t] = f4{0.0f, 0.0f, 0.0f, 0.0f};
   load_chunk(0);
   store_chunk(0);
@@ -208,6 +219,7 @@ __global__ __launch_bounds__(256) void k_conv_lds(ConvArgs a) {
     if (c + 1 < nchunks) load_chunk(c + 1);
     const bf8 A0 = *reinterpret_cast<const bf8*>(&As[buf][arow * kRow + koff]);
     const bf8 A1 = *reinterpret_cast<const bf8*>(&As[buf][(arow + 16) * kRow + koff]);
+#pragma unroll
     for (int t = 0; t < NT; t++) {
       const bf8 B = *reinterpret_cast<const bf8*>(&Bs[buf][(t * 16 + (lane & 15)) * kRow + koff]);
       acc[0][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A0, B, acc[0][t], 0, 0, 0);
@@ -216,14 +228,17 @@ __global__ __launch_bounds__(256) void k_conv_lds(ConvArgs a) {
     if (c + 1 < nchunks) store_chunk(buf ^ 1u);
     __syncthreads();
   }
+#pragma unroll
   for (int f = 0; f < 2; f++) {
+#pragma unroll
     for (int t = 0; t < NT; t++) {
       const uint32_t j = t * 16 + (lane & 15);
-      if (j >= a.nout) continue;
-      const float bj = a.bias ? a.bias[j] : 0.0f;
+      const bool jok = j < a.nout;
+      const float bj = (a.bias && jok) ? a.bias[j] : 0.0f;
+#pragma unroll
       for (int r = 0; r < 4; r++) {
         const uint32_t p = pblock + wave * 32 + f * 16 + 4 * (lane >> 4) + r;
-        if (p >= a.npix) continue;
+        if (!jok || p >= a.npix) continue;
         float v = acc[f][t][r] + bj;
         if (a.relu) v = fmaxf(v, 0.0f);
         if (a.y) a.y[(size_t)p * a.ycs + j] = to_bf16(v);
@@ -318,6 +333,123 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(WGradArgs a) {
     bsum += __shfl_xor(bsum, 16);
     bsum += __shfl_xor(bsum, 32);
     if (lane < 16 && co_ok) atomicAdd(&a.db[co], bsum);
+  }
+}
+
+// Weight gradient with LDS-staged operands.  GEMM M = cout (NM 16-row
+// tiles), N = 64 columns (tap, c) per block, K = pixels: per 32-pixel chunk
+// the block loads dZ (32 pixels x cout) and the shifted input patch (32
+// pixels x 64 columns) with 16-byte loads and stores both transposed in
+// LDS (pixels contiguous), so each MFMA fragment is one 16-byte LDS read.
+// Wave w owns column tile w; blocks split the pixels (FP32 atomics).  The
+// blocks of column tile 0 also reduce db.
+template <int NM>
+__global__ __launch_bounds__(256) void k_wgrad_lds(WGradArgs a) {
+  constexpr uint32_t CP = NM * 16;        // cout padded
+  constexpr uint32_t AP = 4 * CP;         // A pieces per chunk (32 px x CP / 8)
+  constexpr uint32_t NA = (AP + 255) / 256;
+  __shared__ __attribute__((aligned(16))) uint16_t At[2][CP * kRow];
+  __shared__ __attribute__((aligned(16))) uint16_t Bt[2][64 * kRow];
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const uint32_t ncols = 9 * a.cin;
+  const uint32_t j0 = blockIdx.x * 64;
+  const uint32_t pbeg = blockIdx.z * a.pix_per_block;
+  const uint32_t pend = min(a.npix, pbeg + a.pix_per_block);
+  const uint32_t hw = a.h * a.w_;
+  // B piece of this thread: pixel tid / 8 of the chunk, columns j0 + (tid % 8) * 8 ..+7
+  const uint32_t bj = j0 + (tid % 8) * 8;
+  const bool bok = bj < ncols;
+  const uint32_t btap = bok ? bj / a.cin : 0, bc = bok ? bj % a.cin : 0;
+  const int bdy = (int)(btap / 3) - 1, bdx = (int)(btap % 3) - 1;
+  uint4 ra[NA], rb;
+  float bs[NA][8];
+#pragma unroll
+  for (uint32_t i = 0; i < NA; i++)
+#pragma unroll
+    for (int e = 0; e < 8; e++) bs[i][e] = 0.0f;
+  const bool do_bias = a.db && blockIdx.x == 0;
+  auto load_chunk = [&](uint32_t pk) {
+#pragma unroll
+    for (uint32_t i = 0; i < NA; i++) {
+      const uint32_t q = tid + 256 * i;
+      ra[i] = make_uint4(0, 0, 0, 0);
+      if (q < AP) {
+        const uint32_t p = pk + q / NM / 2, g = q % (2 * NM);
+        if (p < pend) ra[i] = *reinterpret_cast<const uint4*>(a.dz + (size_t)p * a.zcs + g * 8);
+      }
+    }
+    rb = make_uint4(0, 0, 0, 0);
+    const uint32_t p = pk + tid / 8;
+    if (bok && p < pend) {
+      const uint32_t b = p / hw, rem = p % hw;
+      const int sy = (int)(rem / a.w_) + bdy, sx = (int)(rem % a.w_) + bdx;
+      if (sy >= 0 && sy < (int)a.h && sx >= 0 && sx < (int)a.w_)
+        rb = *reinterpret_cast<const uint4*>(
+            a.x + ((size_t)b * hw + (uint32_t)sy * a.w_ + (uint32_t)sx) * a.xcs + bc);
+    }
+  };
+  auto store_chunk = [&](uint32_t buf) {
+#pragma unroll
+    for (uint32_t i = 0; i < NA; i++) {
+      const uint32_t q = tid + 256 * i;
+      if (q < AP) {
+        const uint32_t px = q / NM / 2, g = q % (2 * NM);
+        const uint16_t* v = reinterpret_cast<const uint16_t*>(&ra[i]);
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+          At[buf][(g * 8 + e) * kRow + px] = v[e];
+          if (do_bias) bs[i][e] += from_bf16(v[e]);
+        }
+      }
+    }
+    const uint16_t* v = reinterpret_cast<const uint16_t*>(&rb);
+#pragma unroll
+    for (int e = 0; e < 8; e++) Bt[buf][((tid % 8) * 8 + e) * kRow + tid / 8] = v[e];
+  };
+  f4 acc[NM];
+#pragma unroll
+  for (int m = 0; m < NM; m++) acc[m] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+  const uint32_t koff = 8 * (lane >> 4);
+  if (pbeg < pend) {
+    load_chunk(pbeg);
+    store_chunk(0);
+  }
+  __syncthreads();
+  uint32_t buf = 0;
+  for (uint32_t pk = pbeg; pk < pend; pk += 32) {
+    const bool more = pk + 32 < pend;
+    if (more) load_chunk(pk + 32);
+    const bf8 B = *reinterpret_cast<const bf8*>(&Bt[buf][(wave * 16 + (lane & 15)) * kRow + koff]);
+#pragma unroll
+    for (int m = 0; m < NM; m++) {
+      const bf8 A = *reinterpret_cast<const bf8*>(&At[buf][(m * 16 + (lane & 15)) * kRow + koff]);
+      acc[m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc[m], 0, 0, 0);
+    }
+    if (more) store_chunk(buf ^ 1u);
+    __syncthreads();
+    buf ^= 1u;
+  }
+  const uint32_t j = j0 + wave * 16 + (lane & 15);
+  if (j < ncols) {
+#pragma unroll
+    for (int m = 0; m < NM; m++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const uint32_t o = m * 16 + 4 * (lane >> 4) + r;
+        if (o < a.cout) atomicAdd(&a.dw[(size_t)o * ncols + j], acc[m][r]);
+      }
+  }
+  if (do_bias) {
+#pragma unroll
+    for (uint32_t i = 0; i < NA; i++) {
+      const uint32_t q = tid + 256 * i;
+      if (q < AP) {
+        const uint32_t g = q % (2 * NM);
+#pragma unroll
+        for (int e = 0; e < 8; e++)
+          if (g * 8 + e < a.cout) atomicAdd(&a.db[g * 8 + e], bs[i][e]);
+      }
+    }
   }
 }
 
@@ -649,9 +781,19 @@ cv_status conv_bwd(cv_denoiser* d, const Conv& L, const uint16_t* x, uint32_t xc
   g.npix = npix;
   g.h = h;
   g.w_ = w;
-  g.pix_per_block = 4096;
-  dim3 grid((9 * L.cin + 255) / 256, L.cout_pad / 16, (npix + 4095) / 4096);
-  hipLaunchKernelGGL(cvd::k_conv_wgrad, grid, dim3(256), 0, s, g);
+  // about 2048 blocks: pixel splits of a multiple of 32 pixels
+  const uint32_t col_tiles = (9 * L.cin + 63) / 64;
+  uint32_t splits = std::max(1u, std::min(npix / 2048u, 2048u / col_tiles));
+  g.pix_per_block = (npix / splits + 31) / 32 * 32;
+  splits = (npix + g.pix_per_block - 1) / g.pix_per_block;
+  dim3 grid(col_tiles, 1, splits);
+  switch (L.cout_pad / 16) {
+    case 1: hipLaunchKernelGGL(cvd::k_wgrad_lds<1>, grid, dim3(256), 0, s, g); break;
+    case 2: hipLaunchKernelGGL(cvd::k_wgrad_lds<2>, grid, dim3(256), 0, s, g); break;
+    case 4: hipLaunchKernelGGL(cvd::k_wgrad_lds<4>, grid, dim3(256), 0, s, g); break;
+    case 8: hipLaunchKernelGGL(cvd::k_wgrad_lds<8>, grid, dim3(256), 0, s, g); break;
+    default: return fail(CV_ERR_INVALID_ARGUMENT, "unsupported conv width");
+  }
   HIPTRY(hipGetLastError(), "wgrad");
   if (gx) {
     cvd::ConvArgs a{};
