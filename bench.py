@@ -116,6 +116,7 @@ def cpu_baseline(sd, gpu_scene, args):
         if done % npatch == 0:
             frame += 1
     paths = done * 32 * 32
+    last_frame = frame - 1 if done % npatch == 0 else frame
     return {
         "value": q / t_used / 1e6,
         "unit": "Mrays/s",
@@ -123,8 +124,8 @@ def cpu_baseline(sd, gpu_scene, args):
         "kind": "port",
         "sample": "%d patch renders (32x32, %d per frame, stride-%d order) of frames 1..%d at "
                   "1920x1088, reference Rng per patch, host rcpps/rsqrtps, %d threads; %d paths, "
-                  "%d extend queries in %.1f s" % (done, npatch, stride, frame, cores, paths, q,
-                                                   t_used),
+                  "%d extend queries in %.1f s" % (done, npatch, stride, last_frame, cores, paths,
+                                                   q, t_used),
         "paths_per_s": paths / t_used,
         "bvh_build_s": build_s,
         "bvh_identical_to_gpu": same,

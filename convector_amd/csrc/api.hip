@@ -64,7 +64,8 @@ cv_status validate_tuning(const cv_tuning& t) {
   if (t.stack_entries && t.stack_entries != 8 && t.stack_entries != 16 && t.stack_entries != 24 &&
       t.stack_entries != 32)
     return fail(CV_ERR_INVALID_ARGUMENT, "stack_entries must be 8, 16, 24 or 32");
-  if (t.extend_waves_per_cu > 64 || t.refill_threshold > 65 || t.fetch_batches > 64)
+  if (t.extend_waves_per_cu > 64 || t.refill_threshold > 65 || t.fetch_batches > 64 ||
+      (t.block_waves && t.block_waves != 1 && t.block_waves != 2 && t.block_waves != 4))
     return fail(CV_ERR_INVALID_ARGUMENT, "tuning value out of range");
   return CV_OK;
 }
@@ -74,6 +75,7 @@ void apply_tuning(LaunchCfg& c, const cv_tuning& t) {
   if (t.extend_waves_per_cu) c.blocks = num_cus() * t.extend_waves_per_cu;
   if (t.refill_threshold) c.refill = t.refill_threshold;
   if (t.fetch_batches) c.fetch_batches = t.fetch_batches;
+  if (t.block_waves) c.block_waves = t.block_waves;
 }
 
 cv_tuning tuning_of(const LaunchCfg& c) {
@@ -82,6 +84,7 @@ cv_tuning tuning_of(const LaunchCfg& c) {
   t.extend_waves_per_cu = c.blocks / num_cus();
   t.refill_threshold = c.refill;
   t.fetch_batches = c.fetch_batches;
+  t.block_waves = c.block_waves;
   return t;
 }
 
@@ -511,7 +514,7 @@ static cv_status scene_finish(cv_scene* s, const std::vector<float>& v9,
   // leaves of at most 31 triangles
   s->packed = s->info.max_leaf <= 31 && N < (1u << 27) && s->info.num_nodes < (1u << 27);
   HIPTRY(s->cursor.alloc(4), "alloc");
-  cv_tuning t{8, 32, 32, 1};
+  cv_tuning t{8, 32, 32, 1, 1};
   return scene_apply_tuning(s, t);
 }
 

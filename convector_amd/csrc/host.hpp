@@ -81,6 +81,7 @@ struct LaunchCfg {
   uint32_t blocks = 0;        // persistent 64-lane workgroups
   uint32_t refill = 32;       // 1..64 lane refill threshold, 65 whole 64-ray batches
   uint32_t fetch_batches = 1; // 64-ray batches per fetch in batch mode
+  uint32_t block_waves = 1;   // waves per workgroup (lane refill)
 };
 
 }  // namespace cvh

@@ -114,8 +114,10 @@ hipError_t launch_intersect(const IntersectArgs& a, bool count, bool packed, int
                             uint32_t blocks, hipStream_t s);
 // multi: rays of the step belong to more than one scene (per-ray node and
 // triangle pointers from the job table).
+// blocks = persistent waves; block_waves (1, 2, 4) waves per workgroup in
+// lane-refill mode.
 hipError_t launch_extend(const ExtendArgs& a, bool count, bool packed, bool multi, int stack,
-                         uint32_t blocks, hipStream_t s);
+                         uint32_t blocks, uint32_t block_waves, hipStream_t s);
 hipError_t launch_generate(const GenerateArgs& a, uint32_t blocks, hipStream_t s);
 // multi: the step has more than one job.
 hipError_t launch_shade(const ShadeArgs& a, bool multi, uint32_t max_paths, hipStream_t s);

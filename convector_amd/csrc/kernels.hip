@@ -571,14 +571,17 @@ __global__ __launch_bounds__(64) void k_extend(ExtendArgs a) {
 // the queue is empty.  A wave refills when at least `refill` lanes are idle
 // (one atomic for all of them).  Every ray is traversed exactly as by
 // traverse(), one trav_step per loop iteration.
+// Workgroups of 1, 2 or 4 independent waves (the launch's choice); each
+// wave owns S * 64 entries of the dynamic LDS.
 template <bool COUNT, int S, bool PACKED, bool MULTI>
-__global__ __launch_bounds__(64) void k_extend_refill(ExtendArgs a) {
-  __shared__ uint2 lds[S * 64];
+__global__ __launch_bounds__(256) void k_extend_refill(ExtendArgs a) {
+  extern __shared__ uint2 lds_waves[];
   const uint32_t lane = lane_id();
-  const uint32_t gtid = blockIdx.x * 64 + lane;
-  const uint32_t stride = gridDim.x * 64;
+  uint2* lds = lds_waves + (threadIdx.x >> 6) * (S * 64);
+  const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t stride = gridDim.x * blockDim.x;
   const uint32_t n = a.ctrl->count[a.bounce];
-  if (blockIdx.x == 0 && lane == 0) atomicAdd(&a.ctrl->total_rays, (unsigned long long)n);
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&a.ctrl->total_rays, (unsigned long long)n);
   uint32_t* cursor = &a.ctrl->ext_cursor[a.bounce];
   const uint64_t lt = (1ull << lane) - 1ull;
   Stack<S, PACKED> st(lds + lane, a.spill + gtid, stride, 0);
@@ -967,18 +970,20 @@ hipError_t launch_intersect(const IntersectArgs& a, bool count, bool packed, int
 
 template <bool COUNT, int S>
 static hipError_t launch_extend_s(const ExtendArgs& a, bool packed, bool multi, uint32_t blocks,
-                                  hipStream_t s) {
+                                  uint32_t block_waves, hipStream_t s) {
+  const uint32_t bw = block_waves == 2 || block_waves == 4 ? block_waves : 1u;
+  const dim3 grid((blocks + bw - 1) / bw), block(64 * bw);
+  const size_t shmem = (size_t)S * 64 * sizeof(uint2) * bw;
   if (multi) {
     if (packed)
-      hipLaunchKernelGGL((k_extend_refill<COUNT, S, true, true>), dim3(blocks), dim3(64), 0, s, a);
+      hipLaunchKernelGGL((k_extend_refill<COUNT, S, true, true>), grid, block, shmem, s, a);
     else
-      hipLaunchKernelGGL((k_extend_refill<COUNT, S, false, true>), dim3(blocks), dim3(64), 0, s, a);
+      hipLaunchKernelGGL((k_extend_refill<COUNT, S, false, true>), grid, block, shmem, s, a);
   } else if (a.refill) {
     if (packed)
-      hipLaunchKernelGGL((k_extend_refill<COUNT, S, true, false>), dim3(blocks), dim3(64), 0, s, a);
+      hipLaunchKernelGGL((k_extend_refill<COUNT, S, true, false>), grid, block, shmem, s, a);
     else
-      hipLaunchKernelGGL((k_extend_refill<COUNT, S, false, false>), dim3(blocks), dim3(64), 0, s,
-                         a);
+      hipLaunchKernelGGL((k_extend_refill<COUNT, S, false, false>), grid, block, shmem, s, a);
   } else if (packed) {
     hipLaunchKernelGGL((k_extend<COUNT, S, true>), dim3(blocks), dim3(64), 0, s, a);
   } else {
@@ -988,12 +993,12 @@ static hipError_t launch_extend_s(const ExtendArgs& a, bool packed, bool multi, 
 }
 
 hipError_t launch_extend(const ExtendArgs& a, bool count, bool packed, bool multi, int stack,
-                         uint32_t blocks, hipStream_t s) {
+                         uint32_t blocks, uint32_t block_waves, hipStream_t s) {
   if (multi && !a.refill) return hipErrorInvalidValue;
-#define CASE(S)                                                                     \
-  if (stack == S)                                                                   \
-    return count ? launch_extend_s<true, S>(a, packed, multi, blocks, s)            \
-                 : launch_extend_s<false, S>(a, packed, multi, blocks, s);
+#define CASE(S)                                                                        \
+  if (stack == S)                                                                      \
+    return count ? launch_extend_s<true, S>(a, packed, multi, blocks, block_waves, s)  \
+                 : launch_extend_s<false, S>(a, packed, multi, blocks, block_waves, s);
   CVK_STACK_VARIANTS(CASE)
 #undef CASE
   return hipErrorInvalidValue;

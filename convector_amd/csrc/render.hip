@@ -54,13 +54,14 @@ struct RunOpts {
 
 struct GraphKey {
   int stack = 0;
-  uint32_t blocks = 0, refill = 0, fetch_batches = 0, max_bounces = 0;
+  uint32_t blocks = 0, refill = 0, fetch_batches = 0, max_bounces = 0, block_waves = 0;
   bool packed = false, count = false, multi_job = false;
   const void* nodes = nullptr;
   const void* tris = nullptr;
   size_t capacity = 0;
   bool operator==(const GraphKey& o) const {
     return stack == o.stack && blocks == o.blocks && refill == o.refill &&
+           block_waves == o.block_waves &&
            fetch_batches == o.fetch_batches && max_bounces == o.max_bounces &&
            packed == o.packed && count == o.count && multi_job == o.multi_job &&
            nodes == o.nodes && tris == o.tris && capacity == o.capacity;
@@ -169,7 +170,8 @@ struct Pipeline {
   cv_status reserve_spill(const cvh::LaunchCfg& cfg, uint32_t max_depth) {
     uint32_t need = max_depth + 4;
     size_t per_lane = need > (uint32_t)cfg.stack ? (size_t)(need - cfg.stack) : 0;
-    size_t entries = per_lane * cfg.blocks * 64 + 1;
+    // lanes of the launch: the waves rounded up to whole 4-wave workgroups
+    size_t entries = per_lane * (((size_t)cfg.blocks + 3) / 4 * 4) * 64 + 1;
     if (spill.n < entries) {
       HIPTRY(hipStreamSynchronize(stream), "sync");
       drop_graphs();
@@ -272,7 +274,8 @@ cv_status launch_step(Pipeline& p, const RunOpts& o, uint32_t slot, uint32_t pat
     x.refill = o.cfg.refill <= 64 ? o.cfg.refill : (multi_scene ? 64u : 0u);
     ep = ev_pair(1);
     if (ep) (void)hipEventRecord(ep->first, st);
-    HIPTRY(cvk::launch_extend(x, count, o.packed, multi_scene, o.cfg.stack, o.cfg.blocks, st),
+    HIPTRY(cvk::launch_extend(x, count, o.packed, multi_scene, o.cfg.stack, o.cfg.blocks,
+                              o.cfg.block_waves, st),
            "extend");
     if (ep) (void)hipEventRecord(ep->second, st);
     cvk::ShadeArgs sh{};
@@ -311,6 +314,7 @@ cv_status pipeline_run(Pipeline& p, const std::vector<StepPlan>& steps, const Ru
   key.stack = o.cfg.stack;
   key.blocks = o.cfg.blocks;
   key.refill = o.cfg.refill;
+  key.block_waves = o.cfg.block_waves;
   key.fetch_batches = o.cfg.fetch_batches;
   key.max_bounces = o.max_bounces;
   key.packed = o.packed;
@@ -701,7 +705,7 @@ cv_status cv_batch_create(const cv_render_opts* opts, uint64_t max_paths_per_ste
   cv_batch* b = new cv_batch();
   b->opts = o;
   b->max_paths_per_step = max_paths_per_step ? max_paths_per_step : (1ull << 24);
-  cv_tuning t{8, 32, 32, 1};
+  cv_tuning t{8, 32, 32, 1, 1};
   cvh::apply_tuning(b->cfg, t);
   if ((cs = b->pipe.init()) != CV_OK) {
     cv_batch_destroy(b);
@@ -907,6 +911,7 @@ cv_status cv_batch_autotune(cv_batch* b, const cv_job* jobs, size_t n, uint64_t 
   static const uint32_t stacks[] = {8, 16, 24, 32};
   static const uint32_t waves[] = {16, 24, 32, 40, 48};
   static const uint32_t refills[] = {16, 24, 32, 48, 64, 65};
+  static const uint32_t block_waves[] = {1, 2, 4};
   cvh::LaunchCfg best = b->cfg;
   double best_ms = 0;
   cv_status cs = time_cfg(best, &best_ms);
@@ -929,6 +934,13 @@ cv_status cv_batch_autotune(cv_batch* b, const cv_job* jobs, size_t n, uint64_t 
     for (uint32_t v : refills) {
       cvh::LaunchCfg c = best;
       c.refill = v;
+      double ms;
+      if ((cs = time_cfg(c, &ms)) != CV_OK) return cs;
+      if (ms < best_ms) best = c, best_ms = ms;
+    }
+    for (uint32_t v : block_waves) {
+      cvh::LaunchCfg c = best;
+      c.block_waves = v;
       double ms;
       if ((cs = time_cfg(c, &ms)) != CV_OK) return cs;
       if (ms < best_ms) best = c, best_ms = ms;
@@ -1172,7 +1184,7 @@ cv_status cv_server_create(const cv_render_opts* opts, uint64_t hbm_budget_bytes
   sv->opts = o;
   sv->budget = hbm_budget_bytes ? hbm_budget_bytes : (200ull << 30);
   sv->max_paths = max_paths_per_step;
-  cv_tuning t{8, 32, 32, 1};
+  cv_tuning t{8, 32, 32, 1, 1};
   cvh::apply_tuning(sv->cfg, t);
   if ((cs = sv->pipe.init()) != CV_OK || (cs = sv->pipe.reserve(sv->max_paths, 256)) != CV_OK) {
     delete sv;

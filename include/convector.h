@@ -134,6 +134,7 @@ typedef struct cv_tuning {
     uint32_t refill_threshold;    /* 1..64: idle lanes of a wave take new rays when at least
                                      this many are idle; 65: whole 64-ray batches */
     uint32_t fetch_batches;       /* batch mode: 64-ray batches per fetch */
+    uint32_t block_waves;         /* lane refill: 1, 2 or 4 waves (64-256 threads) per block */
 } cv_tuning;
 cv_status cv_scene_set_tuning(cv_scene* scene, const cv_tuning* tuning);
 cv_status cv_scene_get_tuning(const cv_scene* scene, cv_tuning* tuning);
@@ -293,7 +294,8 @@ cv_status cv_batch_get_tuning(const cv_batch* b, cv_tuning* tuning);
  * CV_RENDER_TIMING, CV_RENDER_GRAPH. */
 cv_status cv_batch_render(cv_batch* b, const cv_job* jobs, size_t n, uint32_t flags);
 /* The autotuner: times launch shapes (stack entries 8/16/24/32, 16-48
- * persistent waves per CU, refill thresholds 16-64 and whole batches) on a
+ * persistent waves per CU, refill thresholds 16-64 and whole batches, 1/2/4
+ * waves per workgroup) on a
  * sample step -- the first frame of the jobs in order, up to
  * max_sample_paths paths (0: 2^22), rendered into scratch buffers -- by
  * two rounds of coordinate descent, and keeps the fastest on the batch.

@@ -122,7 +122,8 @@ def test_batch_tuning_variants(cv, world):
     sds, gs, expect = world
     b = cv.Batch()
     for t in (dict(stack_entries=16, refill_threshold=8), dict(stack_entries=32, refill_threshold=65),
-              dict(stack_entries=8, extend_waves_per_cu=4, refill_threshold=64)):
+              dict(stack_entries=8, extend_waves_per_cu=4, refill_threshold=64),
+              dict(stack_entries=24, extend_waves_per_cu=8, refill_threshold=32, block_waves=4)):
         print("batch tuning", t, flush=True)
         b.set_tuning(**t)
         jobs = _jobs(cv, gs)

@@ -154,7 +154,9 @@ def test_tuning_variants_exact(cv, oracle):
     for t in (dict(stack_entries=8, refill_threshold=1), dict(stack_entries=16, refill_threshold=32),
               dict(stack_entries=24, refill_threshold=65, fetch_batches=1),
               dict(stack_entries=32, refill_threshold=65, fetch_batches=4),
-              dict(stack_entries=8, extend_waves_per_cu=1, refill_threshold=64)):
+              dict(stack_entries=8, extend_waves_per_cu=1, refill_threshold=64),
+              dict(stack_entries=16, extend_waves_per_cu=8, refill_threshold=32, block_waves=4),
+              dict(stack_entries=8, extend_waves_per_cu=6, refill_threshold=16, block_waves=2)):
         print("tuning case", t, flush=True)
         g.set_tuning(**t)
         r = cv.Renderer(g, W, H)
