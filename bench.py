@@ -284,6 +284,28 @@ def _random_camera(scene, rng):
     scene.camera.set_orientation(q)
 
 
+def _lit_camera(cv, scene, rng, tries=16):
+    """A random camera (as above) outside the central sphere whose 64x64,
+    4-sample preview has light in at least a quarter of its pixels, or the
+    best lit of `tries` draws: a camera inside the sphere or facing an unlit
+    corner sees an image that no sample count improves."""
+    r = cv.Renderer(scene, 64, 64)
+    best = (-1.0, None)
+    for _ in range(tries):
+        _random_camera(scene, rng)
+        while float(np.linalg.norm(scene.camera.position)) < 0.5:
+            _random_camera(scene, rng)
+        img = r.new_buffer_f32()
+        r.accumulate_frames_f32(img, 1, 4)
+        lit = float((img.reshape(-1, 3).max(axis=1) > 0).mean())
+        if lit > best[0]:
+            best = (lit, (scene.camera.position, scene.camera.orientation))
+        if lit >= 0.25:
+            return
+    scene.camera.set_position(best[1][0])
+    scene.camera.set_orientation(best[1][1])
+
+
 def bench_denoise(args):
     """The learned denoiser: trained on the GPU on 1-sample / high-sample
     pairs rendered by the path tracer from seeded procedural scenes
@@ -302,7 +324,7 @@ def bench_denoise(args):
     for k in range(args.train_scenes):
         n = int(np.exp(rng.uniform(np.log(1e4), np.log(5e5))))
         g = cv.Scene.from_scene_data(scenes.procedural_scene(n, 20000 + k))
-        _random_camera(g, rng)
+        _lit_camera(cv, g, rng)
         gs.append(g)
     log("denoise: %d training scenes built in %.1f s" % (len(gs), time.perf_counter() - t0))
     # 1-sample inputs with features and the references, each set in one batch
@@ -340,7 +362,7 @@ def bench_denoise(args):
     for e in range(args.eval_scenes):
         n = int(np.exp(rng.uniform(np.log(1e4), np.log(5e5))))
         g = cv.Scene.from_scene_data(scenes.procedural_scene(n, 90000 + e))
-        _random_camera(g, rng)
+        _lit_camera(cv, g, rng)
         r = cv.Renderer(g, W, H)
         ref = r.new_buffer_f32()
         r.accumulate_frames_f32(ref, 200000, args.eval_ref_spp)

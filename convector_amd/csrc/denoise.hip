@@ -973,6 +973,37 @@ cv_status stage_in(cv_denoiser* d, DevBuf<float>& buf, const float* src, size_t 
 
 }  // namespace
 
+namespace cvh {
+
+cv_status denoise_image(cv_denoiser* d, const float* hdr, const float* features, uint32_t width,
+                        uint32_t height, uint32_t spp, float* dst, bool dst_device) {
+  if (!d || !hdr || !features || !dst || spp == 0)
+    return fail(CV_ERR_INVALID_ARGUMENT, "null argument");
+  cv_status cs = check_shape(1, height, width);
+  if (cs != CV_OK) return cs;
+  std::lock_guard<std::mutex> g(d->mu);
+  if ((cs = reserve_acts(d, 1, height, width)) != CV_OK) return cs;
+  const size_t P = (size_t)width * height;
+  HIPTRY(d->in_f32.reserve(P * 10), "alloc");
+  HIPTRY(d->tgt_f32.reserve(P * 3), "alloc");
+  hipStream_t s = d->stream;
+  hipLaunchKernelGGL(cvd::k_features, dim3(grid1(P)), dim3(256), 0, s, hdr, features, width,
+                     height, spp, d->in_f32.p);
+  hipLaunchKernelGGL(cvd::k_to_input, dim3(grid1(P * 16)), dim3(256), 0, s, d->in_f32.p, d->x0.p,
+                     (uint32_t)P);
+  if ((cs = forward(d)) != CV_OK) return cs;
+  hipLaunchKernelGGL(cvd::k_output, dim3(grid1(P)), dim3(256), 0, s, d->out.p, width, height, spp,
+                     d->tgt_f32.p);
+  HIPTRY(hipGetLastError(), "denoise");
+  HIPTRY(hipMemcpyAsync(dst, d->tgt_f32.p, P * 12,
+                        dst_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s),
+         "download");
+  HIPTRY(hipStreamSynchronize(s), "sync");
+  return CV_OK;
+}
+
+}  // namespace cvh
+
 extern "C" {
 
 size_t cv_denoiser_num_params(void) {

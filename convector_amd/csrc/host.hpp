@@ -134,4 +134,11 @@ cv_tuning tuning_of(const LaunchCfg& c);
 // The job of a step for scene s (geometry, lights, camera, emission).
 cvk::JobDesc job_for_scene(cv_scene* s);
 
+// Denoises one accumulated image (hdr and features on the device, in the
+// renderer's layouts) into dst (device or host, hdr layout) on the
+// denoiser's own stream; allocates only when the image is larger than any
+// before.
+cv_status denoise_image(cv_denoiser* d, const float* hdr, const float* features, uint32_t width,
+                        uint32_t height, uint32_t spp, float* dst, bool dst_device);
+
 }  // namespace cvh

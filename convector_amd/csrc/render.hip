@@ -50,6 +50,9 @@ struct RunOpts {
   bool multi_job = false;
   uint32_t max_depth = 0;
   bool graph = false;
+  // held around graph capture: a HIP call from another thread of the
+  // caller (a copy, an allocation) during the capture would invalidate it
+  std::mutex* capture_mu = nullptr;
 };
 
 struct GraphKey {
@@ -353,12 +356,22 @@ cv_status pipeline_run(Pipeline& p, const std::vector<StepPlan>& steps, const Ru
         if (p.graph_defs[slot]) (void)hipGraphDestroy(p.graph_defs[slot]);
         p.graphs[slot] = nullptr;
         p.graph_defs[slot] = nullptr;
+        std::unique_lock<std::mutex> cl;
+        if (o.capture_mu) cl = std::unique_lock<std::mutex>(*o.capture_mu);
         HIPTRY(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed), "graph capture");
         cv_status ls = launch_step(p, o, slot, s.paths, true, nullptr);
         hipGraph_t gdef = nullptr;
         hipError_t ce = hipStreamEndCapture(st, &gdef);
-        if (ls != CV_OK) return ls;
-        HIPTRY(ce, "graph capture");
+        cl = std::unique_lock<std::mutex>();
+        if (ls != CV_OK || ce != hipSuccess) {
+          // nothing of a failed capture ran: clear the error and launch the
+          // step directly (a real launch error fails there again)
+          if (gdef) (void)hipGraphDestroy(gdef);
+          (void)hipGetLastError();
+          status = launch_step(p, o, slot, s.paths, false, nullptr);
+          if (status != CV_OK) break;
+          continue;
+        }
         p.graph_defs[slot] = gdef;
         HIPTRY(hipGraphInstantiate(&p.graphs[slot], gdef, nullptr, nullptr, 0), "graph instantiate");
         p.keys[slot] = key;
@@ -1012,6 +1025,19 @@ struct cv_server {
   uint64_t next_ticket = 1;
   bool stop = false;
   std::thread worker;
+  // requests whose frames are done, denoised and delivered by the finisher
+  // thread while the worker runs the next steps
+  std::deque<std::shared_ptr<ServeReq>> finishing;
+  std::condition_variable fin_cv;
+  bool fin_stop = false;
+  std::thread finisher;
+  hipStream_t fin_stream = nullptr;
+  // held by the finisher around its HIP calls and by the worker around
+  // graph capture, which another thread's HIP call would invalidate
+  std::mutex hip_mu;
+  // device buffers of released requests, freed by the worker between steps
+  // (hipFree waits for the device, which the finisher should not)
+  std::vector<void*> to_free;
   // statistics
   uint64_t submitted = 0, completed = 0, steps = 0;
   double busy_ms = 0;
@@ -1020,60 +1046,72 @@ struct cv_server {
 
 namespace {
 
-// Denoise (optional) and deliver one finished request; called by the worker
-// without the server lock.
-void finish_request(cv_server* sv, ServeReq& r) {
+// Denoise (optional) and deliver one finished request; called by the
+// finisher thread without the server lock, holding hip_mu.
+cv_status finish_request(cv_server* sv, cv_denoiser* dn, ServeReq& r) {
   const cv_request& q = r.req;
   const size_t npix = (size_t)q.width * q.height;
   const bool dev = q.flags & CV_PTR_DEVICE;
-  hipMemcpyKind kind = dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
-  cv_status st = CV_OK;
-  if (q.denoise && sv->denoiser) {
-    DevBuf<float> inp, out, res;
-    if (inp.alloc(npix * 10) || out.alloc(npix * 3) || res.alloc(npix * 3)) {
-      st = fail(CV_ERR_OUT_OF_MEMORY, "denoiser buffers");
-    } else {
-      st = cv_denoiser_prepare_input(r.hdr.p, r.feat.p, q.width, q.height, q.spp, inp.p,
-                                     CV_PTR_DEVICE);
-      if (st == CV_OK)
-        st = cv_denoiser_forward(sv->denoiser, inp.p, out.p, 1, q.height, q.width, CV_PTR_DEVICE);
-      if (st == CV_OK)
-        st = cv_denoiser_output_to_hdr(out.p, q.width, q.height, q.spp, res.p, CV_PTR_DEVICE);
-      if (st == CV_OK && hipMemcpy(q.hdr, res.p, npix * 12, kind) != hipSuccess)
-        st = fail(CV_ERR_DEVICE, "deliver result");
-    }
-  } else if (hipMemcpy(q.hdr, r.hdr.p, npix * 12, kind) != hipSuccess) {
-    st = fail(CV_ERR_DEVICE, "deliver result");
-  }
-  r.status = st;
-  if (st != CV_OK) r.error = cv_last_error();
-  r.hdr.reset();
-  r.feat.reset();
+  if (q.denoise && dn)
+    return cvh::denoise_image(dn, r.hdr.p, r.feat.p, q.width, q.height, q.spp, q.hdr, dev);
+  HIPTRY(hipMemcpyAsync(q.hdr, r.hdr.p, npix * 12, dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                        sv->fin_stream),
+         "deliver result");
+  HIPTRY(hipStreamSynchronize(sv->fin_stream), "deliver result");
+  return CV_OK;
 }
 
 void serve_loop(cv_server* sv) {
   for (;;) {
     std::vector<std::shared_ptr<ServeReq>> act;
+    std::vector<void*> frees;
     {
       std::unique_lock<std::mutex> lk(sv->mu);
-      sv->work_cv.wait(lk, [&] { return sv->stop || !sv->pending.empty() || !sv->active.empty(); });
-      if (sv->stop && sv->pending.empty() && sv->active.empty()) return;
+      // a request that could never fit fails; one that does not fit now
+      // waits for active requests to finish and finished ones to be released
+      auto never_fits = [&](const ServeReq& r) {
+        return sv->fixed_bytes + r.bytes > sv->budget ||
+               (size_t)r.req.width * r.req.height > sv->max_paths;
+      };
+      auto fits_now = [&](const ServeReq& r, size_t pixels) {
+        return sv->fixed_bytes + sv->used_bytes + r.bytes <= sv->budget &&
+               pixels + (size_t)r.req.width * r.req.height <= sv->max_paths;
+      };
+      sv->work_cv.wait(lk, [&] {
+        if (!sv->active.empty() || !sv->to_free.empty()) return true;
+        if (sv->pending.empty()) return sv->stop;
+        const ServeReq& f = *sv->pending.front();
+        return never_fits(f) || fits_now(f, 0);
+      });
+      frees.swap(sv->to_free);
+      if (frees.empty() && sv->stop && sv->pending.empty() && sv->active.empty()) return;
+    }
+    for (void* p : frees) (void)hipFree(p);
+    {
+      std::unique_lock<std::mutex> lk(sv->mu);
+      auto never_fits = [&](const ServeReq& r) {
+        return sv->fixed_bytes + r.bytes > sv->budget ||
+               (size_t)r.req.width * r.req.height > sv->max_paths;
+      };
+      auto fits_now = [&](const ServeReq& r, size_t pixels) {
+        return sv->fixed_bytes + sv->used_bytes + r.bytes <= sv->budget &&
+               pixels + (size_t)r.req.width * r.req.height <= sv->max_paths;
+      };
       // admission in arrival order: memory budget and queue capacity
       size_t pixels = 0;
       for (auto& r : sv->active) pixels += (size_t)r->req.width * r->req.height;
       while (!sv->pending.empty()) {
         auto r = sv->pending.front();
         const size_t npix = (size_t)r->req.width * r->req.height;
-        const bool fits = sv->fixed_bytes + sv->used_bytes + r->bytes <= sv->budget &&
-                          pixels + npix <= sv->max_paths;
-        if (!fits && !sv->active.empty()) break;
+        const bool never = never_fits(*r);
+        if (!never && !fits_now(*r, pixels)) break;
         sv->pending.pop_front();
-        if (!fits || r->hdr.alloc(npix * 3) != hipSuccess ||
+        if (never || r->hdr.alloc(npix * 3) != hipSuccess ||
             hipMemset(r->hdr.p, 0, npix * 12) != hipSuccess ||
             (r->req.denoise && (r->feat.alloc(npix * 8) != hipSuccess ||
                                 hipMemset(r->feat.p, 0, npix * 32) != hipSuccess))) {
-          r->status = !fits ? CV_ERR_OUT_OF_MEMORY : CV_ERR_DEVICE;
-          r->error = !fits ? "request exceeds the memory budget or the step capacity"
+          r->status = never ? CV_ERR_OUT_OF_MEMORY : CV_ERR_DEVICE;
+          r->error = never ? "request exceeds the memory budget or the step capacity"
                            : "request buffers";
           r->hdr.reset();
           r->feat.reset();
@@ -1123,6 +1161,7 @@ void serve_loop(cv_server* sv) {
     o.multi_job = true;
     o.max_depth = max_depth;
     o.graph = true;
+    o.capture_mu = &sv->hip_mu;
     std::vector<StepPlan> steps(1, std::move(step));
     cv_render_stats stt{};
     cv_status st = pipeline_run(sv->pipe, steps, o, &stt);
@@ -1139,29 +1178,61 @@ void serve_loop(cv_server* sv) {
       for (auto& r : finished)
         sv->active.erase(std::find(sv->active.begin(), sv->active.end(), r));
     }
-    for (auto& r : finished) {
+    if (finished.empty()) continue;
+    {
+      std::lock_guard<std::mutex> lk(sv->mu);
+      for (auto& r : finished) {
+        if (st != CV_OK) {
+          r->status = st;
+          r->error = err;
+        }
+        sv->finishing.push_back(r);
+      }
+    }
+    sv->fin_cv.notify_all();
+  }
+}
+
+// Finisher thread: denoises and delivers finished requests (on the
+// denoiser's stream and synchronous copies) while the worker's next steps
+// run, then releases their memory and wakes their waiters.
+void finish_loop(cv_server* sv) {
+  for (;;) {
+    std::shared_ptr<ServeReq> r;
+    cv_denoiser* dn = nullptr;
+    {
+      std::unique_lock<std::mutex> lk(sv->mu);
+      sv->fin_cv.wait(lk, [&] { return sv->fin_stop || !sv->finishing.empty(); });
+      if (sv->finishing.empty()) return;
+      r = sv->finishing.front();
+      sv->finishing.pop_front();
+      dn = sv->denoiser;
+    }
+    if (r->status == CV_OK) {
+      std::lock_guard<std::mutex> hl(sv->hip_mu);
+      const cv_status st = finish_request(sv, dn, *r);
       if (st != CV_OK) {
         r->status = st;
-        r->error = err;
-        r->hdr.reset();
-        r->feat.reset();
-      } else {
-        finish_request(sv, *r);
+        r->error = cv_last_error();
       }
     }
     {
       std::lock_guard<std::mutex> lk(sv->mu);
-      for (auto& r : finished) {
-        r->done = true;
-        r->t_done = now_ms();
-        sv->used_bytes -= r->bytes;
-        if (r->status == CV_OK) {
-          sv->completed++;
-          sv->latencies.push_back(r->t_done - r->t_submit);
-        }
+      for (DevBuf<float>* b : {&r->hdr, &r->feat}) {
+        if (b->p) sv->to_free.push_back(b->p);
+        b->p = nullptr;
+        b->n = 0;
+      }
+      r->done = true;
+      r->t_done = now_ms();
+      sv->used_bytes -= r->bytes;
+      if (r->status == CV_OK) {
+        sv->completed++;
+        sv->latencies.push_back(r->t_done - r->t_submit);
       }
     }
-    if (!finished.empty()) sv->done_cv.notify_all();
+    sv->done_cv.notify_all();
+    sv->work_cv.notify_all();  // freed memory may admit waiting requests
   }
 }
 
@@ -1195,7 +1266,13 @@ cv_status cv_server_create(const cv_render_opts* opts, uint64_t hbm_budget_bytes
     delete sv;
     return fail(CV_ERR_OUT_OF_MEMORY, "the path queues alone exceed the memory budget");
   }
+  hipError_t e = hipStreamCreateWithFlags(&sv->fin_stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete sv;
+    return hip_fail(e, "hipStreamCreate");
+  }
   sv->worker = std::thread(serve_loop, sv);
+  sv->finisher = std::thread(finish_loop, sv);
   *out = sv;
   return CV_OK;
 }
@@ -1208,6 +1285,14 @@ void cv_server_destroy(cv_server* sv) {
   }
   sv->work_cv.notify_all();
   if (sv->worker.joinable()) sv->worker.join();
+  {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    sv->fin_stop = true;
+  }
+  sv->fin_cv.notify_all();
+  if (sv->finisher.joinable()) sv->finisher.join();
+  for (void* p : sv->to_free) (void)hipFree(p);
+  if (sv->fin_stream) (void)hipStreamDestroy(sv->fin_stream);
   delete sv;
 }
 

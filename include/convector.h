@@ -346,7 +346,9 @@ cv_status cv_server_create(const cv_render_opts* opts, uint64_t hbm_budget_bytes
                            uint64_t max_paths_per_step, cv_server** out);
 /* Registers a resident scene (it must outlive the server). */
 cv_status cv_server_add_scene(cv_server* sv, cv_scene* scene, uint32_t* id);
-/* The denoiser used by requests with denoise = 1 (must outlive the server). */
+/* The denoiser used by requests with denoise = 1 (must outlive the server).
+   The server's finisher thread runs it; the caller must not use the same
+   denoiser from another thread while denoised requests are in flight. */
 cv_status cv_server_set_denoiser(cv_server* sv, cv_denoiser* d);
 cv_status cv_server_set_tuning(cv_server* sv, const cv_tuning* tuning);
 /* Queues a request and returns at once; thread safe. */
