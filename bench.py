@@ -81,9 +81,24 @@ def measured_traffic(kernel):
     return None
 
 
+def _cpu_sampler(O, osc, cores):
+    """The host path that is timed: the AVX2 8-wide packet restatement of the
+    reference (packet traversal, rcpps/rsqrtps, reference Rng per patch)
+    where the host has AVX2 + FMA, else the scalar oracle.  Returns
+    (description, f(width, height, frame, patch_ids) -> extend queries)."""
+    if O.avx_available():
+        return ("AVX2 8-wide packet restatement of the reference (packet BVH traversal, "
+                "host rcpps/rsqrtps, reference Rng per patch)",
+                lambda w, h, f, ids: osc.avx_sample_patches(w, h, f, ids, nthreads=cores))
+    return ("scalar restatement (single-ray traversal, host rcpps/rsqrtps, reference Rng "
+            "per patch)",
+            lambda w, h, f, ids: osc.sample_patches(w, h, f, ids, rng_mode=O.RNG_REF_PATCH,
+                                                    nthreads=cores)[0])
+
+
 def cpu_baseline(sd, gpu_scene, args):
-    """Times the oracle (reference semantics) on a strided sample of the
-    frame's 32x32 patches with the host cores."""
+    """Times the reference's algorithm on the host cores over a strided
+    sample of the frame's 32x32 patches."""
     from oracle import oracle as O
 
     cores = min(args.cpu_threads, os.cpu_count() or 1)
@@ -95,6 +110,7 @@ def cpu_baseline(sd, gpu_scene, args):
     gn, go = gpu_scene.bvh()
     on, oo = osc.bvh()
     same = len(gn) == len(on) and gn.tobytes() == on.tobytes() and bool((go == oo).all())
+    what, sample = _cpu_sampler(O, osc, cores)
     hp = 1088  # the reference's renderer needs heights divisible by 16
     npy = (hp + 31) // 32
     npatch = (W // 32) * npy
@@ -109,9 +125,8 @@ def cpu_baseline(sd, gpu_scene, args):
         k = done % npatch
         ids = order[k:k + chunk]
         t1 = time.perf_counter()
-        qq, _, _ = osc.sample_patches(W, hp, frame, ids, rng_mode=O.RNG_REF_PATCH, nthreads=cores)
+        q += sample(W, hp, frame, ids)
         t_used += time.perf_counter() - t1
-        q += qq
         done += len(ids)
         if done % npatch == 0:
             frame += 1
@@ -122,10 +137,9 @@ def cpu_baseline(sd, gpu_scene, args):
         "unit": "Mrays/s",
         "cores": cores,
         "kind": "port",
-        "sample": "%d patch renders (32x32, %d per frame, stride-%d order) of frames 1..%d at "
-                  "1920x1088, reference Rng per patch, host rcpps/rsqrtps, %d threads; %d paths, "
-                  "%d extend queries in %.1f s" % (done, npatch, stride, last_frame, cores, paths,
-                                                   q, t_used),
+        "sample": "%s: %d patch renders (32x32, %d per frame, stride-%d order) of frames 1..%d "
+                  "at 1920x1088, %d threads; %d paths, %d extend queries in %.1f s"
+                  % (what, done, npatch, stride, last_frame, cores, paths, q, t_used),
         "paths_per_s": paths / t_used,
         "bvh_build_s": build_s,
         "bvh_identical_to_gpu": same,
@@ -222,8 +236,9 @@ def bench_batch(args):
 
 
 def cpu_baseline_batch(sds, specs, args):
-    """The oracle (reference semantics) on every 8th job: the first frame's
-    32x32 patches in a strided order until the time budget is spent."""
+    """The reference's algorithm on the host (as cpu_baseline) on every 8th
+    job: the first frame's 32x32 patches in a strided order until the time
+    budget is spent."""
     from oracle import oracle as O
 
     cores = min(args.cpu_threads, os.cpu_count() or 1)
@@ -231,19 +246,20 @@ def cpu_baseline_batch(sds, specs, args):
     q = paths = 0
     t_used = 0.0
     budget = args.cpu_seconds / len(picks)
+    what = None
     for i in picks:
         sd = sds[i]
         (_, _, w, h, _) = specs[i]
         osc = O.Scene.from_scene_data(sd, approx=O.APPROX_HOST_X86)
+        what, sample = _cpu_sampler(O, osc, cores)
         npatch = ((w + 31) // 32) * ((h + 31) // 32)
         order = [(k * 37) % npatch for k in range(npatch)] if npatch % 37 else list(range(npatch))
         t_job, done = 0.0, 0
         while t_job < budget and done < npatch:
             ids = order[done:done + 4 * cores]
             t1 = time.perf_counter()
-            qq, _, _ = osc.sample_patches(w, h, 1, ids, rng_mode=O.RNG_REF_PATCH, nthreads=cores)
+            q += sample(w, h, 1, ids)
             t_job += time.perf_counter() - t1
-            q += qq
             # patches at the right and top edges hold fewer pixels
             for pid in ids:
                 px, py = (pid // ((h + 31) // 32)) * 32, (pid % ((h + 31) // 32)) * 32
@@ -255,10 +271,9 @@ def cpu_baseline_batch(sds, specs, args):
         "unit": "paths/s",
         "cores": cores,
         "kind": "port",
-        "sample": "jobs %s (every 8th), frame 1, 32x32 patches in stride-37 order for up to "
-                  "%.1f s each, reference Rng per patch, host rcpps/rsqrtps, %d threads: "
-                  "%d paths, %d extend queries in %.1f s" % (picks, budget, cores, paths, q,
-                                                            t_used),
+        "sample": "%s; jobs %s (every 8th), frame 1, 32x32 patches in stride-37 order for up "
+                  "to %.1f s each, %d threads: %d paths, %d extend queries in %.1f s"
+                  % (what, picks, budget, cores, paths, q, t_used),
         "mrays_per_s": q / t_used / 1e6,
     }
 

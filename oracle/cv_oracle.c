@@ -1016,6 +1016,30 @@ void or_scene_set_emission_override(or_scene* s, int enable, const float rgb[3])
     if (enable) s->emission = ld3(rgb);
 }
 void or_scene_set_approx(or_scene* s, int approx) { s->approx = approx; }
+void or_scene_get_view(const or_scene* s, or_scene_view* out) {
+    out->v = s->v;
+    out->uv = s->uv;
+    out->mat = s->mat;
+    out->nodes = s->nodes;
+    out->direct = s->direct;
+    out->ntris = s->ntris;
+    out->nnodes = s->nnodes;
+    out->ndirect = s->ndirect;
+    const float pos[3] = {s->cam_pos.x, s->cam_pos.y, s->cam_pos.z};
+    const float dpos[3] = {s->cam_dpos.x, s->cam_dpos.y, s->cam_dpos.z};
+    const float q[4] = {s->cam_q.a, s->cam_q.b, s->cam_q.c, s->cam_q.d};
+    const float dq[4] = {s->cam_dq.a, s->cam_dq.b, s->cam_dq.c, s->cam_dq.d};
+    memcpy(out->cam_pos, pos, 12);
+    memcpy(out->cam_dpos, dpos, 12);
+    memcpy(out->cam_q, q, 16);
+    memcpy(out->cam_dq, dq, 16);
+    out->screen_distance = s->screen_distance;
+    out->emission_override = s->emission_override;
+    out->emission[0] = s->emission.x;
+    out->emission[1] = s->emission.y;
+    out->emission[2] = s->emission.z;
+    out->approx = s->approx;
+}
 uint32_t or_scene_num_nodes(const or_scene* s) { return s->nnodes; }
 uint32_t or_scene_num_direct(const or_scene* s) { return s->ndirect; }
 void or_scene_copy_bvh(const or_scene* s, or_node* nodes, uint32_t* order) {

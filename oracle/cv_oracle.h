@@ -169,14 +169,38 @@ int or_sample_patches(const or_scene* s, uint32_t width, uint32_t height, uint32
 void or_hdr_blocks_to_rows(uint32_t width, uint32_t height, const float* hdr, float* rows);
 
 /* ---- AVX2 packet baseline (cv_oracle_avx.c) ---- */
+/* Read-only view of a scene's arrays (BVH order) and camera. */
+typedef struct {
+    const float* v;          /* 9 per triangle: v0, v1, v2 */
+    const float* uv;         /* 6 per triangle */
+    const uint32_t* mat;
+    const or_node* nodes;
+    const uint32_t* direct;  /* light triangles (BVH indices) */
+    uint32_t ntris, nnodes, ndirect;
+    float cam_pos[3], cam_dpos[3], cam_q[4], cam_dq[4], screen_distance;
+    int emission_override;
+    float emission[3];
+    int approx;
+} or_scene_view;
+void or_scene_get_view(const or_scene* s, or_scene_view* out);
+
+/* 1 when the library was built for AVX2 + FMA and the host has both. */
 int or_avx_available(void);
-/* The reference AVX path (REF_PATCH rng, host rcpps/rsqrtps, packet traversal)
- * over the patches [patch_begin, patch_end) of the frame in patch order
- * (i over width/32 columns outer, j over height/32 rows inner).  Accumulates
- * into hdr; returns the number of patches rendered. */
+#define OR_AVX_PACKET 0   /* the reference's packet traversal (bvh.rs:589-657) */
+#define OR_AVX_PER_LANE 1 /* each lane traversed alone: the scalar oracle's */
+/* The reference's 8-wide packet path (REF_PATCH rng, the scene's approx
+ * mode) over whole frames first_frame .. first_frame + n_frames - 1 in the
+ * patch order of main.rs:174-214, accumulated into hdr (16x4 block layout).
+ * n_queries (optional) receives the number of extend queries (active rays
+ * traced).  Returns 0 or a negative status. */
 int or_avx_accumulate_frames(const or_scene* s, uint32_t width, uint32_t height, float* hdr,
-                             uint32_t first_frame, uint32_t n_frames, uint32_t patch_begin,
-                             uint32_t patch_end, int nthreads);
+                             uint32_t first_frame, uint32_t n_frames, int mode, int nthreads,
+                             uint64_t* n_queries);
+/* The packet path over the listed 32x32 patches of one frame (a bounded
+ * timing sample); the image is discarded. */
+int or_avx_sample_patches(const or_scene* s, uint32_t width, uint32_t height, uint32_t frame,
+                          int nthreads, const uint32_t* patch_ids, uint32_t n_ids,
+                          uint64_t* n_queries);
 
 #ifdef __cplusplus
 }

@@ -97,7 +97,8 @@ def _declare(L):
         "or_hdr_blocks_to_rows": (None, [U32, U32, P, P]),
         "or_sample_patches": (I32, [P, U32, U32, U32, I32, I32, P, U32, P]),
         "or_avx_available": (I32, []),
-        "or_avx_accumulate_frames": (I32, [P, U32, U32, P, U32, U32, U32, U32, I32]),
+        "or_avx_accumulate_frames": (I32, [P, U32, U32, P, U32, U32, I32, I32, P]),
+        "or_avx_sample_patches": (I32, [P, U32, U32, U32, I32, P, U32, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -275,18 +276,35 @@ class Scene:
             raise RuntimeError("oracle sample_patches status %d" % st)
         return int(out[0]), int(out[1]), int(out[2])
 
-    def avx_accumulate_frames(self, width, height, first_frame, n_frames, patch_begin=0,
-                              patch_end=None, nthreads=8, hdr=None):
+    def avx_accumulate_frames(self, width, height, first_frame, n_frames, per_lane=False,
+                              nthreads=8, hdr=None):
+        """The 8-wide AVX2 path (reference Rng per patch, the scene's approx
+        mode) with the reference's packet traversal, or with each lane
+        traversed alone (per_lane): returns (hdr in the 16x4 block layout,
+        extend queries)."""
         if hdr is None:
             hdr = np.zeros(width * height * 3, dtype=np.float32)
-        npatch = ((width + 31) // 32) * ((height + 31) // 32)
-        if patch_end is None:
-            patch_end = npatch
-        n = lib().or_avx_accumulate_frames(self._h, width, height, ptr(hdr), first_frame,
-                                           n_frames, patch_begin, patch_end, nthreads)
-        if n < 0:
-            raise RuntimeError("oracle avx_accumulate_frames status %d" % n)
-        return hdr, n
+        q = np.zeros(1, dtype=np.uint64)
+        st = lib().or_avx_accumulate_frames(self._h, width, height, ptr(hdr), first_frame,
+                                            n_frames, 1 if per_lane else 0, nthreads, ptr(q))
+        if st:
+            raise RuntimeError("oracle avx_accumulate_frames status %d" % st)
+        return hdr, int(q[0])
+
+    def avx_sample_patches(self, width, height, frame, patch_ids, nthreads=8):
+        """The AVX2 packet path over the listed 32x32 patches of one frame;
+        returns the number of extend queries."""
+        ids = u32(patch_ids)
+        q = np.zeros(1, dtype=np.uint64)
+        st = lib().or_avx_sample_patches(self._h, width, height, frame, nthreads, ptr(ids),
+                                         len(ids), ptr(q))
+        if st:
+            raise RuntimeError("oracle avx_sample_patches status %d" % st)
+        return int(q[0])
+
+
+def avx_available():
+    return bool(lib().or_avx_available())
 
 
 def hdr_blocks_to_rows(width, height, hdr):
