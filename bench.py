@@ -237,8 +237,10 @@ def bench_batch(args):
 
 def cpu_baseline_batch(sds, specs, args):
     """The reference's algorithm on the host (as cpu_baseline) on every 8th
-    job: the first frame's 32x32 patches in a strided order until the time
-    budget is spent."""
+    job: frame 1's 32x32 patches in a strided order until the job's time
+    budget is spent gives the job's host rate; the rates are combined with
+    the batch's own weights (each job's width x height x spp paths), i.e. the
+    host's paths/s on these jobs as the batch defines them."""
     from oracle import oracle as O
 
     cores = min(args.cpu_threads, os.cpu_count() or 1)
@@ -247,14 +249,17 @@ def cpu_baseline_batch(sds, specs, args):
     t_used = 0.0
     budget = args.cpu_seconds / len(picks)
     what = None
+    work = est_s = 0.0
+    rates = []
     for i in picks:
         sd = sds[i]
-        (_, _, w, h, _) = specs[i]
+        (_, _, w, h, spp) = specs[i]
         osc = O.Scene.from_scene_data(sd, approx=O.APPROX_HOST_X86)
         what, sample = _cpu_sampler(O, osc, cores)
-        npatch = ((w + 31) // 32) * ((h + 31) // 32)
+        npy = (h + 31) // 32
+        npatch = ((w + 31) // 32) * npy
         order = [(k * 37) % npatch for k in range(npatch)] if npatch % 37 else list(range(npatch))
-        t_job, done = 0.0, 0
+        t_job, done, p_job = 0.0, 0, 0
         while t_job < budget and done < npatch:
             ids = order[done:done + 4 * cores]
             t1 = time.perf_counter()
@@ -262,18 +267,26 @@ def cpu_baseline_batch(sds, specs, args):
             t_job += time.perf_counter() - t1
             # patches at the right and top edges hold fewer pixels
             for pid in ids:
-                px, py = (pid // ((h + 31) // 32)) * 32, (pid % ((h + 31) // 32)) * 32
-                paths += min(32, w - px) * min(32, h - py)
+                px, py = (pid // npy) * 32, (pid % npy) * 32
+                p_job += min(32, w - px) * min(32, h - py)
             done += len(ids)
         t_used += t_job
+        paths += p_job
+        rate = p_job / t_job
+        rates.append(round(rate))
+        work += w * h * spp
+        est_s += w * h * spp / rate
     return {
-        "value": paths / t_used,
+        "value": work / est_s,
         "unit": "paths/s",
         "cores": cores,
         "kind": "port",
         "sample": "%s; jobs %s (every 8th), frame 1, 32x32 patches in stride-37 order for up "
-                  "to %.1f s each, %d threads: %d paths, %d extend queries in %.1f s"
+                  "to %.1f s each, %d threads: %d paths, %d extend queries in %.1f s; per-job "
+                  "rates weighted by each job's width x height x spp"
                   % (what, picks, budget, cores, paths, q, t_used),
+        "job_rates_paths_per_s": rates,
+        "unweighted_paths_per_s": paths / t_used,
         "mrays_per_s": q / t_used / 1e6,
     }
 
