@@ -87,8 +87,13 @@ struct Stack {
       : lds((lds_u64*)l), spill((glb_u64*)g), stride(str), sp(p) {}
   // Pushes e when p is set.  Below S the slot at sp is written either way
   // (it is above the top of the stack, so an unused write is harmless).
+  // The LDS-only case is tested for the whole wave, so the common path has
+  // no divergent branch.
   __device__ __forceinline__ void push_if(bool p, entry_t e) {
-    if (sp < S) {
+    if (__builtin_expect(!__any(sp >= S), 1)) {
+      lds[sp * 64] = e;
+      sp += (int)p;
+    } else if (sp < S) {
       lds[sp * 64] = e;
       sp += (int)p;
     } else if (p) {
@@ -98,6 +103,7 @@ struct Stack {
   }
   __device__ __forceinline__ entry_t pop() {
     sp--;
+    if (__builtin_expect(!__any(sp >= S), 1)) return lds[sp * 64];
     if (sp < S) return lds[sp * 64];
     return spill[(size_t)(sp - S) * stride];
   }
@@ -108,32 +114,34 @@ struct Stack {
 // iteration pops the last push.  Here the last push stays in registers as
 // `cur` (have = true) and only the other child goes to the stack, which
 // visits the nodes in exactly the reference's order.
-template <bool EXACT_NAN, int S, bool PACKED>
+template <int S, bool PACKED>
 __device__ __forceinline__ void visit_pair(const DevNode* __restrict__ nodes, uint32_t base, V3 o,
-                                           V3 inv, Stack<S, PACKED>& st, entry_t& cur,
-                                           bool& have) {
+                                           V3 inv, bool exact, Stack<S, PACKED>& st,
+                                           entry_t& cur, bool& have) {
   const float4* p = reinterpret_cast<const float4*>(nodes + base);
   float4 a0 = p[0], b0 = p[1], a1 = p[2], b1 = p[3];
   float tmin0, tmax0, tmin1, tmax1;
-  uint32_t s01;
-  if (EXACT_NAN) {
-    slab<true>(a0, b0, o, inv, tmin0, tmax0);
-    slab<true>(a1, b1, o, inv, tmin1, tmax1);
-    s01 = sgn_x86_sub(tmin0, tmin1);
-  } else {
-    slab_hw(a0, b0, o, inv, tmin0, tmax0);
-    slab_hw(a1, b1, o, inv, tmin1, tmax1);
-    // Zero results are where the signs of the hardware min / max may differ
-    // from minps / maxps; the sign of tmax decides the hit and the signs of
-    // two zero tmins decide the order.  Redo those with the x86 chains.
-    const bool zero = (tmax0 == 0.0f) | (tmax1 == 0.0f) | ((tmin0 == 0.0f) & (tmin1 == 0.0f));
-    if (__builtin_expect(zero, 0)) {
+  slab_hw(a0, b0, o, inv, tmin0, tmax0);
+  slab_hw(a1, b1, o, inv, tmin1, tmax1);
+  // Zero results are where the signs of the hardware min / max may differ
+  // from minps / maxps; the sign of tmax decides the hit and the signs of two
+  // zero tmins decide the order (tmax0 == 0 | tmax1 == 0 | (tmin0 == 0 &
+  // tmin1 == 0), as one min3).  Those, and waves with a lane whose slab
+  // distances can be NaN (wave-uniform `exact`), redo the x86 chains.
+  const float z = fminf(fminf(fabsf(tmax0), fabsf(tmax1)), fmaxf(fabsf(tmin0), fabsf(tmin1)));
+  if (__builtin_expect(exact || z == 0.0f, 0)) {
+    if (exact) {
+      slab<true>(a0, b0, o, inv, tmin0, tmax0);
+      slab<true>(a1, b1, o, inv, tmin1, tmax1);
+    } else {
       slab<false>(a0, b0, o, inv, tmin0, tmax0);
       slab<false>(a1, b1, o, inv, tmin1, tmax1);
     }
-    // sign bit of tmin0 - tmin1; inf - inf is the negative default NaN
-    s01 = neg_or_nan(tmin0 - tmin1);
   }
+  // sign bit of tmin0 - tmin1 as x86 computes it: inf - inf (and NaN inputs)
+  // give the negative default NaN
+  const bool swap = exact ? sgn_x86_sub(tmin0, tmin1) != 0
+                          : __builtin_amdgcn_classf(tmin0 - tmin1, 0x3f);
   // any_masked: mask (tmax NLT tmin) and tmax's sign bit 0 (aabb.rs:155-158)
   const bool h0 = !(tmax0 < tmin0) && sgn(tmax0) == 0;
   const bool h1 = !(tmax1 < tmin1) && sgn(tmax1) == 0;
@@ -145,15 +153,15 @@ __device__ __forceinline__ void visit_pair(const DevNode* __restrict__ nodes, ui
     x0 = base;
     x1 = base + 1;
   }
-  // should_try_before (bvh.rs:616-622): with s01 == 0 child 0 is pushed
-  // first and child 1 is popped first.
-  const entry_t e0 = pack_entry(x0, tmin0), e1 = pack_entry(x1, tmin1);
-  const bool swap = s01 != 0;
-  const entry_t ef = swap ? e1 : e0, es = swap ? e0 : e1;
-  const bool hf = swap ? h1 : h0, hs = swap ? h0 : h1;
-  st.push_if(hf && hs, ef);
-  cur = hs ? es : ef;
-  have = hf || hs;
+  // should_try_before (bvh.rs:616-622): without swap child 0 is pushed first
+  // and child 1 is popped first.  The first push goes to the stack when both
+  // children are hit; the entry popped next (the second push if it was hit,
+  // else the first) stays in registers: child 0 when h0 and (swap or !h1).
+  // (when neither child is hit `cur` is not used, so c0 may be false then)
+  const bool c0 = h0 & (swap | !h1);
+  cur = pack_entry(c0 ? x0 : x1, c0 ? tmin0 : tmin1);
+  have = h0 | h1;
+  st.push_if(h0 && h1, swap ? pack_entry(x1, tmin1) : pack_entry(x0, tmin0));
 }
 
 // Triangle::intersect, keeping only what decides the pick (triangle.rs:59-127).
@@ -188,11 +196,11 @@ __device__ __forceinline__ void test_tri(const DevTri* __restrict__ tris, uint32
 // triangles.  The ray is finished when have == false and the stack is empty.
 // The root is the entry (index 0, len 0, tmin -inf), so that its pair is
 // visited first and counted like every other pair.
-template <bool EXACT_NAN, bool COUNT, int S, bool PACKED>
+template <bool COUNT, int S, bool PACKED>
 __device__ __forceinline__ void trav_step(const DevNode* __restrict__ nodes,
                                           const DevTri* __restrict__ tris, V3 o, V3 d, V3 inv,
-                                          Stack<S, PACKED>& st, TravState& ts, entry_t& cur,
-                                          bool& have) {
+                                          bool exact, Stack<S, PACKED>& st, TravState& ts,
+                                          entry_t& cur, bool& have) {
   entry_t e = cur;
   if (!have) e = st.pop();
   have = false;
@@ -212,7 +220,7 @@ __device__ __forceinline__ void trav_step(const DevNode* __restrict__ nodes,
   }
   if (len == 0) {
     if (COUNT) ts.na += 2;
-    visit_pair<EXACT_NAN>(nodes, idx, o, inv, st, cur, have);
+    visit_pair(nodes, idx, o, inv, exact, st, cur, have);
   } else {
     for (uint32_t k = idx; k < idx + len; k++) {
       test_tri(tris, k, o, d, ts);
@@ -252,11 +260,8 @@ __device__ __forceinline__ TravState traverse(const DevNode* __restrict__ nodes,
   if (!valid) return ts;
   entry_t cur = root_entry<PACKED>();
   bool have = true;
-  if (__builtin_expect(__any(exact), 0)) {
-    while (have || st.sp > 0) trav_step<true, COUNT>(nodes, tris, o, d, inv, st, ts, cur, have);
-  } else {
-    while (have || st.sp > 0) trav_step<false, COUNT>(nodes, tris, o, d, inv, st, ts, cur, have);
-  }
+  const bool wave_exact = __any(exact);
+  while (have || st.sp > 0) trav_step<COUNT>(nodes, tris, o, d, inv, wave_exact, st, ts, cur, have);
   return ts;
 }
 
@@ -632,10 +637,7 @@ __global__ __launch_bounds__(256) void k_extend_refill(ExtendArgs a) {
     }
     const bool wave_exact = __any(active && exact);
     if (active) {
-      if (wave_exact)
-        trav_step<true, COUNT>(nodes, tris, o, d, inv, st, ts, cur, have);
-      else
-        trav_step<false, COUNT>(nodes, tris, o, d, inv, st, ts, cur, have);
+      trav_step<COUNT>(nodes, tris, o, d, inv, wave_exact, st, ts, cur, have);
       if (!have && st.sp == 0) {
         a.hits[ray] = make_uint4(ts.best, f2u(ts.best_t), f2u(ts.bu), f2u(ts.bv));
         if (COUNT) {
