@@ -30,6 +30,7 @@ RNG_REF_PATCH = _native.CV_RNG_REF_PATCH
 PTR_DEVICE = _native.CV_PTR_DEVICE
 RENDER_COUNT = _native.CV_RENDER_COUNT
 RENDER_TIMING = _native.CV_RENDER_TIMING
+FRAMES_AUTO = 255  # cv_tuning.frames_per_step: about 2^24 paths per step
 RENDER_GRAPH = _native.CV_RENDER_GRAPH
 
 NODE_DTYPE = np.dtype([("min", "<f4", 3), ("max", "<f4", 3), ("index", "<u4"), ("len", "<u4")])
@@ -191,10 +192,10 @@ class Scene:
         return s
 
     def set_tuning(self, stack_entries=0, extend_waves_per_cu=0, refill_threshold=0,
-                   fetch_batches=0, block_waves=0):
+                   fetch_batches=0, block_waves=0, frames_per_step=0):
         """Launch shape of the traversal (the autotuner's knobs)."""
         t = _native.Tuning(stack_entries, extend_waves_per_cu, refill_threshold, fetch_batches,
-                           block_waves)
+                           block_waves, frames_per_step)
         check(lib().cv_scene_set_tuning(self._h, ctypes.byref(t)))
 
     def tuning(self) -> dict:
@@ -339,9 +340,9 @@ class Batch:
         self._h = h
 
     def set_tuning(self, stack_entries=0, extend_waves_per_cu=0, refill_threshold=0,
-                   fetch_batches=0, block_waves=0):
+                   fetch_batches=0, block_waves=0, frames_per_step=0):
         t = _native.Tuning(stack_entries, extend_waves_per_cu, refill_threshold, fetch_batches,
-                           block_waves)
+                           block_waves, frames_per_step)
         check(lib().cv_batch_set_tuning(self._h, ctypes.byref(t)))
 
     def tuning(self) -> dict:
@@ -497,9 +498,9 @@ class Server:
         self._denoiser = d
 
     def set_tuning(self, stack_entries=0, extend_waves_per_cu=0, refill_threshold=0,
-                   fetch_batches=0, block_waves=0):
+                   fetch_batches=0, block_waves=0, frames_per_step=0):
         t = _native.Tuning(stack_entries, extend_waves_per_cu, refill_threshold, fetch_batches,
-                           block_waves)
+                           block_waves, frames_per_step)
         check(lib().cv_server_set_tuning(self._h, ctypes.byref(t)))
 
     def submit(self, scene_id: int, width: int, height: int, spp: int, camera_pos, camera_q,

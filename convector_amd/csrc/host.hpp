@@ -82,6 +82,7 @@ struct LaunchCfg {
   uint32_t refill = 32;       // 1..64 lane refill threshold, 65 whole 64-ray batches
   uint32_t fetch_batches = 1; // 64-ray batches per fetch in batch mode
   uint32_t block_waves = 1;   // waves per workgroup (lane refill)
+  uint32_t frames_per_step = CV_FRAMES_AUTO;  // accumulation frames sharing launches
 };
 
 }  // namespace cvh

@@ -128,6 +128,11 @@ hipError_t launch_prepare_tris(const float* v9, const uint32_t* order, uint32_t 
 hipError_t launch_prepare_lights(const DevTri* tris, const uint32_t* direct, uint32_t n,
                                  DevLight* out, hipStream_t s);
 hipError_t launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t s);
+// hdr += slots[0] + ... + slots[count-1] (in that order) over the rectangle,
+// clearing the slots; slot k starts at slots + k * slot_stride.
+hipError_t launch_fold_hdr(float* hdr, float* slots, size_t slot_stride, uint32_t count,
+                           uint32_t width, uint32_t x, uint32_t y, uint32_t w, uint32_t h,
+                           hipStream_t s);
 
 // ---------------------------------------------------------------- builder
 

@@ -934,6 +934,32 @@ __global__ void k_prepare_lights(const DevTri* tris, const uint32_t* direct, uin
   out[k] = L;
 }
 
+// Frames rendered in the same launches accumulate into separate buffers;
+// this adds buffers 1..count to the HDR image in frame order and clears
+// them, so that every pixel receives the additions of a sequential
+// accumulation, in the same order.  One thread per pixel of the rectangle.
+__global__ void k_fold_hdr(float* __restrict__ hdr, float* __restrict__ slots, size_t slot_stride,
+                           uint32_t count, uint32_t width, uint32_t x0, uint32_t y0, uint32_t w,
+                           uint32_t npix) {
+  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < npix; p += gridDim.x * blockDim.x) {
+    const uint32_t px = x0 + p % w, py = y0 + p / w;
+    const size_t off = hdr_offset(px, py, width);
+    float r = hdr[off], g = hdr[off + 8], b = hdr[off + 16];
+    for (uint32_t k = 0; k < count; k++) {
+      float* s = slots + k * slot_stride + off;
+      r = r + s[0];
+      g = g + s[8];
+      b = b + s[16];
+      s[0] = 0.0f;
+      s[8] = 0.0f;
+      s[16] = 0.0f;
+    }
+    hdr[off] = r;
+    hdr[off + 8] = g;
+    hdr[off + 16] = b;
+  }
+}
+
 __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0u;
@@ -1045,6 +1071,16 @@ hipError_t launch_prepare_lights(const DevTri* tris, const uint32_t* direct, uin
   uint32_t blocks = (n + 63) / 64;
   if (blocks == 0) return hipSuccess;
   hipLaunchKernelGGL(k_prepare_lights, dim3(blocks), dim3(64), 0, s, tris, direct, n, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_fold_hdr(float* hdr, float* slots, size_t slot_stride, uint32_t count,
+                           uint32_t width, uint32_t x, uint32_t y, uint32_t w, uint32_t h,
+                           hipStream_t s) {
+  const uint32_t npix = w * h;
+  if (count == 0 || npix == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fold_hdr, dim3(std::min<uint32_t>((npix + 255) / 256, 8192)), dim3(256), 0,
+                     s, hdr, slots, slot_stride, count, width, x, y, w, npix);
   return hipGetLastError();
 }
 

@@ -38,6 +38,12 @@ constexpr uint32_t kRingSlots = 8;
 struct StepPlan {
   std::vector<cvk::JobDesc> jobs;  // path_base filled, sorted
   uint32_t paths = 0;
+  // after the step: hdr += fold_slots[0 .. fold_count-1] over the rectangle
+  // (frames rendered in the same launches, added in frame order)
+  float* fold_hdr = nullptr;
+  float* fold_slots = nullptr;
+  size_t fold_stride = 0;
+  uint32_t fold_count = 0, fold_width = 0, fold_rect[4] = {0, 0, 0, 0};
 };
 
 struct RunOpts {
@@ -370,17 +376,24 @@ cv_status pipeline_run(Pipeline& p, const std::vector<StepPlan>& steps, const Ru
           (void)hipGetLastError();
           status = launch_step(p, o, slot, s.paths, false, nullptr);
           if (status != CV_OK) break;
-          continue;
+        } else {
+          p.graph_defs[slot] = gdef;
+          HIPTRY(hipGraphInstantiate(&p.graphs[slot], gdef, nullptr, nullptr, 0),
+                 "graph instantiate");
+          p.keys[slot] = key;
         }
-        p.graph_defs[slot] = gdef;
-        HIPTRY(hipGraphInstantiate(&p.graphs[slot], gdef, nullptr, nullptr, 0), "graph instantiate");
-        p.keys[slot] = key;
       }
-      HIPTRY(hipGraphLaunch(p.graphs[slot], st), "graph launch");
+      if (p.graphs[slot]) HIPTRY(hipGraphLaunch(p.graphs[slot], st), "graph launch");
     } else {
       status = launch_step(p, o, slot, s.paths, false, timing ? &evs : nullptr,
                            s.jobs.empty() ? 0u : s.jobs[0].frame);
       if (status != CV_OK) break;
+    }
+    if (s.fold_count) {
+      HIPTRY(cvk::launch_fold_hdr(s.fold_hdr, s.fold_slots, s.fold_stride, s.fold_count,
+                                  s.fold_width, s.fold_rect[0], s.fold_rect[1], s.fold_rect[2],
+                                  s.fold_rect[3], st),
+             "fold");
     }
   }
   (void)hipEventRecord(t1, st);
@@ -433,6 +446,9 @@ struct cv_renderer {
   cv_render_opts opts{};
   Pipeline pipe;
   DevBuf<float> hdr, features;
+  // accumulation buffers of the 2nd..Kth frame of a step (zero between calls)
+  DevBuf<float> slots;
+  bool slots_clean = false;
   DevBuf<uint32_t> bitmap, gbuffer;
   cv_render_stats stats{};
   std::mutex mu;
@@ -517,14 +533,53 @@ cv_status run_frames(cv_renderer* r, const Rect& rc, uint32_t first, uint32_t nf
   base.bitmap = bm_dev;
   base.features = ft_dev;
   base.path_base = 0;
-  std::vector<StepPlan> steps(nframes);
-  for (uint32_t f = 0; f < nframes; f++) {
-    cvk::JobDesc j = base;
-    j.frame = first + f;
-    // every frame rewrites the gbuffer; the last one is what remains
-    j.gbuffer = f + 1 == nframes ? gb_dev : nullptr;
-    steps[f].jobs.push_back(j);
-    steps[f].paths = rc.w * rc.h;
+  // Frames per step: an accumulation renders K frames in the same launches
+  // (a launch's last rays keep most of the GPU idle, so larger launches are
+  // faster per ray); frame k > 0 of a step accumulates into its own zeroed
+  // buffer and a fold adds those to the image in frame order afterwards.
+  // The features of the denoiser are summed in place, so they keep K = 1.
+  const uint32_t npix = rc.w * rc.h;
+  uint32_t K = s->cfg.frames_per_step;
+  if (K == CV_FRAMES_AUTO) K = std::max<uint32_t>(1u, (1u << 24) / std::max(npix, 1u));
+  K = std::min({K, nframes, 64u});
+  if (!hdr_mode || features || bitmap) K = 1;
+  const size_t slot_floats = (size_t)W * r->height * 3;
+  if (K > 1) {
+    const size_t need = (size_t)(K - 1) * slot_floats;
+    if (r->slots.n < need) {
+      HIPTRY(r->slots.alloc(need), "alloc accumulation buffers");
+      r->slots_clean = false;
+    }
+    if (!r->slots_clean)
+      HIPTRY(hipMemsetAsync(r->slots.p, 0, r->slots.n * sizeof(float), st), "clear");
+    r->slots_clean = false;
+  }
+  std::vector<StepPlan> steps;
+  for (uint32_t f0 = 0; f0 < nframes; f0 += K) {
+    const uint32_t kk = std::min(K, nframes - f0);
+    StepPlan sp;
+    for (uint32_t k = 0; k < kk; k++) {
+      cvk::JobDesc j = base;
+      j.frame = first + f0 + k;
+      j.hdr = k == 0 ? hdr_dev : r->slots.p + (size_t)(k - 1) * slot_floats;
+      // every frame rewrites the gbuffer; the last one is what remains
+      j.gbuffer = f0 + k + 1 == nframes ? gb_dev : nullptr;
+      j.path_base = k * npix;
+      sp.jobs.push_back(j);
+    }
+    sp.paths = kk * npix;
+    if (kk > 1) {
+      sp.fold_hdr = hdr_dev;
+      sp.fold_slots = r->slots.p;
+      sp.fold_stride = slot_floats;
+      sp.fold_count = kk - 1;
+      sp.fold_width = W;
+      sp.fold_rect[0] = rc.x;
+      sp.fold_rect[1] = rc.y;
+      sp.fold_rect[2] = rc.w;
+      sp.fold_rect[3] = rc.h;
+    }
+    steps.push_back(std::move(sp));
   }
   RunOpts o;
   o.cfg = s->cfg;
@@ -533,10 +588,11 @@ cv_status run_frames(cv_renderer* r, const Rect& rc, uint32_t first, uint32_t nf
   o.packed = s->packed;
   o.nodes = s->nodes.p;
   o.tris = s->tris.p;
-  o.multi_job = false;
+  o.multi_job = K > 1;
   o.max_depth = s->info.depth;
   o.graph = flags & CV_RENDER_GRAPH;
   cv_status status = pipeline_run(r->pipe, steps, o, &r->stats);
+  if (status == CV_OK && K > 1) r->slots_clean = true;
   if (status == CV_OK && !dev) {
     const size_t row = (size_t)W * 4;
     const size_t off = ((size_t)rc.y * W + rc.x) * 4;
