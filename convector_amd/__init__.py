@@ -30,7 +30,7 @@ RNG_REF_PATCH = _native.CV_RNG_REF_PATCH
 PTR_DEVICE = _native.CV_PTR_DEVICE
 RENDER_COUNT = _native.CV_RENDER_COUNT
 RENDER_TIMING = _native.CV_RENDER_TIMING
-FRAMES_AUTO = 255  # cv_tuning.frames_per_step: about 2^24 paths per step
+FRAMES_AUTO = 255  # cv_tuning.frames_per_step: about 2^25 paths per step
 RENDER_GRAPH = _native.CV_RENDER_GRAPH
 
 NODE_DTYPE = np.dtype([("min", "<f4", 3), ("max", "<f4", 3), ("index", "<u4"), ("len", "<u4")])

@@ -35,15 +35,20 @@ namespace {
 
 constexpr uint32_t kRingSlots = 8;
 
+// After a step: hdr += slots[0] + ... + slots[count-1] over the rectangle
+// (further frames of one image rendered in the same launches, each into its
+// own buffer, added in frame order).
+struct FoldDesc {
+  float* hdr = nullptr;
+  float* slots = nullptr;
+  size_t stride = 0;
+  uint32_t count = 0, width = 0, rect[4] = {0, 0, 0, 0};
+};
+
 struct StepPlan {
   std::vector<cvk::JobDesc> jobs;  // path_base filled, sorted
   uint32_t paths = 0;
-  // after the step: hdr += fold_slots[0 .. fold_count-1] over the rectangle
-  // (frames rendered in the same launches, added in frame order)
-  float* fold_hdr = nullptr;
-  float* fold_slots = nullptr;
-  size_t fold_stride = 0;
-  uint32_t fold_count = 0, fold_width = 0, fold_rect[4] = {0, 0, 0, 0};
+  std::vector<FoldDesc> folds;
 };
 
 struct RunOpts {
@@ -389,12 +394,10 @@ cv_status pipeline_run(Pipeline& p, const std::vector<StepPlan>& steps, const Ru
                            s.jobs.empty() ? 0u : s.jobs[0].frame);
       if (status != CV_OK) break;
     }
-    if (s.fold_count) {
-      HIPTRY(cvk::launch_fold_hdr(s.fold_hdr, s.fold_slots, s.fold_stride, s.fold_count,
-                                  s.fold_width, s.fold_rect[0], s.fold_rect[1], s.fold_rect[2],
-                                  s.fold_rect[3], st),
+    for (const FoldDesc& fd : s.folds)
+      HIPTRY(cvk::launch_fold_hdr(fd.hdr, fd.slots, fd.stride, fd.count, fd.width, fd.rect[0],
+                                  fd.rect[1], fd.rect[2], fd.rect[3], st),
              "fold");
-    }
   }
   (void)hipEventRecord(t1, st);
   hipError_t se = hipStreamSynchronize(st);
@@ -540,7 +543,7 @@ cv_status run_frames(cv_renderer* r, const Rect& rc, uint32_t first, uint32_t nf
   // The features of the denoiser are summed in place, so they keep K = 1.
   const uint32_t npix = rc.w * rc.h;
   uint32_t K = s->cfg.frames_per_step;
-  if (K == CV_FRAMES_AUTO) K = std::max<uint32_t>(1u, (1u << 24) / std::max(npix, 1u));
+  if (K == CV_FRAMES_AUTO) K = std::max<uint32_t>(1u, (1u << 25) / std::max(npix, 1u));
   K = std::min({K, nframes, 64u});
   if (!hdr_mode || features || bitmap) K = 1;
   const size_t slot_floats = (size_t)W * r->height * 3;
@@ -569,15 +572,17 @@ cv_status run_frames(cv_renderer* r, const Rect& rc, uint32_t first, uint32_t nf
     }
     sp.paths = kk * npix;
     if (kk > 1) {
-      sp.fold_hdr = hdr_dev;
-      sp.fold_slots = r->slots.p;
-      sp.fold_stride = slot_floats;
-      sp.fold_count = kk - 1;
-      sp.fold_width = W;
-      sp.fold_rect[0] = rc.x;
-      sp.fold_rect[1] = rc.y;
-      sp.fold_rect[2] = rc.w;
-      sp.fold_rect[3] = rc.h;
+      FoldDesc fd;
+      fd.hdr = hdr_dev;
+      fd.slots = r->slots.p;
+      fd.stride = slot_floats;
+      fd.count = kk - 1;
+      fd.width = W;
+      fd.rect[0] = rc.x;
+      fd.rect[1] = rc.y;
+      fd.rect[2] = rc.w;
+      fd.rect[3] = rc.h;
+      sp.folds.push_back(fd);
     }
     steps.push_back(std::move(sp));
   }
@@ -823,11 +828,11 @@ cv_status cv_batch_render(cv_batch* b, const cv_job* jobs, size_t n, uint32_t fl
     max_frames = std::max(max_frames, J.num_frames);
     max_depth = std::max(max_depth, J.scene->info.depth);
     largest = std::max(largest, (size_t)J.width * J.height);
-    total += (size_t)J.width * J.height;
+    total += (size_t)J.width * J.height * J.num_frames;
     packed = packed && J.scene->packed;
     one_scene = one_scene && J.scene == jobs[0].scene;
   }
-  // queue capacity: the first step's pixels, within the bound (a job larger
+  // queue capacity: all the batch's paths, within the bound (a job larger
   // than the bound still fits)
   const size_t cap = std::max<size_t>(largest, std::min<size_t>(b->max_paths_per_step, total));
   if (cap >= (1ull << 31)) return fail(CV_ERR_INVALID_ARGUMENT, "step too large");
@@ -867,29 +872,84 @@ cv_status cv_batch_render(cv_batch* b, const cv_job* jobs, size_t n, uint32_t fl
     }
     base[i] = d;
   }
-  // step s renders frame first_frame + s of every job with more than s
-  // frames; a step that would exceed the queue capacity is split by jobs
+  // Steps take (job, frame) pairs in frame-major order (frame 0 of every
+  // job, then frame 1 of every job with more than one frame, ...) up to the
+  // queue capacity, so a step can hold several frames of one job once few
+  // jobs are left: the first of them accumulates into the job's image, the
+  // k-th further one into the job's k-th zeroed buffer, and the step's fold
+  // adds those in frame order.  A job with features appears once per step
+  // (its features are summed in place).
   std::vector<StepPlan> steps;
-  bool multi_job = false;
+  std::vector<std::vector<uint32_t>> extra_of;   // per step entry: 0 (image) or k (k-th buffer)
+  std::vector<std::vector<size_t>> job_of_entry; // per step entry: the job
+  std::vector<std::vector<size_t>> fold_jobs;    // per step: the job of each fold
+  std::vector<uint32_t> occ(n, 0), max_extra(n, 0);
+  StepPlan cur;
+  std::vector<uint32_t> cur_extra;
+  std::vector<size_t> cur_job;
+  auto flush = [&]() {
+    if (cur.jobs.empty()) return;
+    std::vector<size_t> fj;
+    for (size_t i = 0; i < n; i++) {
+      if (occ[i] > 1) {
+        FoldDesc fd;
+        fd.hdr = base[i].hdr;
+        fd.stride = (size_t)jobs[i].width * jobs[i].height * 3;
+        fd.count = occ[i] - 1;
+        fd.width = jobs[i].width;
+        fd.rect[2] = jobs[i].width;
+        fd.rect[3] = jobs[i].height;
+        cur.folds.push_back(fd);
+        fj.push_back(i);
+      }
+      occ[i] = 0;
+    }
+    steps.push_back(std::move(cur));
+    extra_of.push_back(std::move(cur_extra));
+    job_of_entry.push_back(std::move(cur_job));
+    fold_jobs.push_back(std::move(fj));
+    cur = StepPlan();
+    cur_extra.clear();
+    cur_job.clear();
+  };
   for (uint32_t f = 0; f < max_frames; f++) {
-    StepPlan cur;
     for (size_t i = 0; i < n; i++) {
       if (jobs[i].num_frames <= f) continue;
       const uint32_t npix = jobs[i].width * jobs[i].height;
-      if (cur.paths + (size_t)npix > cap) {
-        steps.push_back(std::move(cur));
-        cur = StepPlan();
-      }
+      if (cur.paths + (size_t)npix > cap || (occ[i] > 0 && jobs[i].features)) flush();
       cvk::JobDesc d = base[i];
       d.frame = jobs[i].first_frame + f;
       d.gbuffer = f + 1 == jobs[i].num_frames ? base[i].gbuffer : nullptr;
       d.path_base = cur.paths;
+      cur_extra.push_back(occ[i]);
+      cur_job.push_back(i);
+      max_extra[i] = std::max(max_extra[i], occ[i]);
+      occ[i]++;
       cur.jobs.push_back(d);
       cur.paths += npix;
     }
-    if (!cur.jobs.empty()) steps.push_back(std::move(cur));
   }
-  for (const StepPlan& s : steps) multi_job = multi_job || s.jobs.size() > 1;
+  flush();
+  // the further frames' buffers, zeroed (each fold clears what it read)
+  std::vector<DevBuf<float>> slot_bufs(n);
+  for (size_t i = 0; i < n; i++) {
+    if (!max_extra[i]) continue;
+    const size_t floats = (size_t)max_extra[i] * jobs[i].width * jobs[i].height * 3;
+    HIPTRY(slot_bufs[i].alloc(floats), "alloc accumulation buffers");
+    HIPTRY(hipMemsetAsync(slot_bufs[i].p, 0, floats * sizeof(float), st), "clear");
+  }
+  bool multi_job = false;
+  for (size_t k = 0; k < steps.size(); k++) {
+    StepPlan& sp = steps[k];
+    multi_job = multi_job || sp.jobs.size() > 1;
+    for (size_t e = 0; e < sp.jobs.size(); e++) {
+      const uint32_t x = extra_of[k][e];
+      if (!x) continue;
+      const size_t i = job_of_entry[k][e];
+      sp.jobs[e].hdr = slot_bufs[i].p + (size_t)(x - 1) * jobs[i].width * jobs[i].height * 3;
+    }
+    for (size_t q = 0; q < sp.folds.size(); q++) sp.folds[q].slots = slot_bufs[fold_jobs[k][q]].p;
+  }
   RunOpts o;
   o.cfg = b->cfg;
   o.max_bounces = b->opts.max_bounces;

@@ -137,7 +137,7 @@ typedef struct cv_tuning {
     uint32_t block_waves;         /* lane refill: 1, 2 or 4 waves (64-256 threads) per block */
     uint32_t frames_per_step;     /* frame accumulation: 1..64 frames rendered in the same
                                      launches (each into its own buffer, added in frame order),
-                                     or CV_FRAMES_AUTO (about 2^24 paths per step) */
+                                     or CV_FRAMES_AUTO (about 2^25 paths per step) */
 } cv_tuning;
 #define CV_FRAMES_AUTO 255u
 cv_status cv_scene_set_tuning(cv_scene* scene, const cv_tuning* tuning);

@@ -75,6 +75,36 @@ def test_batch_split_steps(cv, world):
     _check(jobs, expect)
 
 
+def test_batch_packed_frames_and_features(cv, world):
+    """Late steps hold several frames of the jobs that still have frames
+    (each further frame in its own buffer, folded in frame order); jobs with
+    features take one frame per step.  Both give the renderer's bits, also
+    over an image that already holds samples."""
+    sds, gs, expect = world
+    specs = [(0, 64, 48, 1, 9), (1, 32, 16, 2, 1), (2, 48, 32, 3, 6), (1, 64, 32, 1, 4)]
+    ref = []
+    for k, (si, w, h, first, n) in enumerate(specs):
+        r = cv.Renderer(gs[si], w, h)
+        hdr = np.full(w * h * 3, 0.25, np.float32)
+        feat = r.new_features() if k == 3 else None
+        if feat is not None:
+            r.accumulate_aov_f32(hdr, feat, first, n)
+        else:
+            r.accumulate_frames_f32(hdr, first, n)
+        ref.append((hdr, feat))
+    for bound in (0, 5000):
+        print("queue bound", bound, flush=True)
+        b = cv.Batch(max_paths_per_step=bound)
+        jobs = [cv.Job(gs[si], w, h, first, n, hdr=np.full(w * h * 3, 0.25, np.float32),
+                       features=np.zeros(w * h * 8, np.float32) if k == 3 else None)
+                for k, (si, w, h, first, n) in enumerate(specs)]
+        b.render(jobs)
+        for j, (hdr, feat) in zip(jobs, ref):
+            assert j.hdr.tobytes() == hdr.tobytes()
+            if feat is not None:
+                assert j.features.tobytes() == feat.tobytes()
+
+
 def test_batch_alone_and_in_mix(cv, world):
     sds, gs, expect = world
     b = cv.Batch()
