@@ -69,16 +69,22 @@ class DeviceBuffer:
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "pmc_extend_config1_1080p_16spp.md")
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
-    summary (FETCH_SIZE and WRITE_SIZE scaled by a 1 GiB copy), or None."""
+def profile_value(key, kernel):
+    """A figure of `kernel` from the committed rocprofv3 --pmc summary, or
+    None: traffic_bytes_per_launch (HBM bytes per launch, FETCH_SIZE and
+    WRITE_SIZE scaled by a 1 GiB copy) or valu_issue_frac (SQ_INSTS_VALU over
+    one VALU instruction per CU per cycle)."""
     try:
         for line in open(TRAFFIC_PROFILE):
-            if line.startswith("traffic_bytes_per_launch " + kernel + " "):
+            if line.startswith(key + " " + kernel + " "):
                 return float(line.rsplit(" ", 1)[1])
     except OSError:
         pass
     return None
+
+
+def measured_traffic(kernel):
+    return profile_value("traffic_bytes_per_launch", kernel)
 
 
 def _cpu_sampler(O, osc, cores):
@@ -732,12 +738,18 @@ def main():
             "traffic_kernel": kernel,
             "bytes_per_launch": round(bytes_per_step / launches_per_step),
             "avg_launch_ms": round(avg_launch_ms, 4),
+            "note": "achieved = algorithmic node and triangle bytes (32 B per box test, 36 B per "
+                    "triangle test) per launch / launch time; above the HBM peak because L2 and "
+                    "MALL serve the working set (measured HBM bytes per launch: traffic); the "
+                    "traversal is bound by VALU issue (valu.issue_frac)",
         },
         "valu": {
             "achieved": round(achieved_tops, 3),
             "peak": VALU_PEAK_TOPS,
             "unit": "T lane-ops/s",
             "frac": round(achieved_tops / VALU_PEAK_TOPS, 4),
+            "issue_frac": (profile_value("valu_issue_frac", kernel)
+                           if tun["refill_threshold"] <= 64 else None),
         },
     }
     if not args.no_cpu:
