@@ -1103,7 +1103,10 @@ struct ServeReq {
   cv_request req{};
   uint32_t frames_done = 0;
   DevBuf<float> hdr, feat;
-  size_t bytes = 0;
+  // buffers of the further frames a step renders for this request (zero
+  // between steps); slot_bytes of them count against the budget
+  DevBuf<float> slots;
+  size_t bytes = 0, slot_bytes = 0;
   double t_submit = 0, t_done = 0;
   bool done = false;
   cv_status status = CV_OK;
@@ -1244,12 +1247,58 @@ void serve_loop(cv_server* sv) {
       act = sv->active;
     }
     if (act.empty()) continue;
-    // one step: the next frame of every active request
+    // one step: the next frame of every active request, then further
+    // frames (frame-major, in admission order) while the step's path
+    // capacity lasts, so that few requests still fill the GPU.  A further
+    // frame accumulates into the request's own zeroed buffer and is folded
+    // into its image in frame order after the step; denoised requests sum
+    // their features in place and take one frame per step.
+    std::vector<uint32_t> take(act.size(), 1);
+    {
+      size_t paths = 0;
+      for (auto& r : act) paths += (size_t)r->req.width * r->req.height;
+      for (bool more = true; more;) {
+        more = false;
+        for (size_t k = 0; k < act.size(); k++) {
+          const ServeReq& r = *act[k];
+          const size_t npix = (size_t)r.req.width * r.req.height;
+          if (r.req.denoise || r.frames_done + take[k] >= r.req.spp || take[k] >= 64 ||
+              paths + npix > sv->max_paths)
+            continue;
+          take[k]++;
+          paths += npix;
+          more = true;
+        }
+      }
+      // the further frames' buffers, within the memory budget
+      std::lock_guard<std::mutex> lk(sv->mu);
+      for (size_t k = 0; k < act.size(); k++) {
+        ServeReq& r = *act[k];
+        const size_t frame_floats = (size_t)r.req.width * r.req.height * 3;
+        const size_t need = (size_t)(take[k] - 1) * frame_floats;
+        if (need <= r.slots.n) continue;
+        const size_t extra = (need - r.slots.n) * sizeof(float);
+        if (sv->fixed_bytes + sv->used_bytes + extra > sv->budget ||
+            r.slots.alloc(need) != hipSuccess ||
+            hipMemset(r.slots.p, 0, need * sizeof(float)) != hipSuccess) {
+          sv->used_bytes -= r.slot_bytes;
+          r.slots.reset();
+          r.slot_bytes = 0;
+          (void)hipGetLastError();
+          take[k] = 1;
+          continue;
+        }
+        sv->used_bytes += need * sizeof(float) - r.slot_bytes;
+        r.slot_bytes = need * sizeof(float);
+        sv->peak_bytes = std::max(sv->peak_bytes, sv->fixed_bytes + sv->used_bytes);
+      }
+    }
     StepPlan step;
     uint32_t max_depth = 0;
     bool packed = true;
-    for (auto& r : act) {
-      const cv_request& q = r->req;
+    for (size_t k = 0; k < act.size(); k++) {
+      const ServeReq& r = *act[k];
+      const cv_request& q = r.req;
       cv_scene* sc = sv->scenes[q.scene];
       cvk::JobDesc d = cvh::job_for_scene(sc);
       memcpy(d.cam.pos, q.camera_pos, 12);
@@ -1261,12 +1310,26 @@ void serve_loop(cv_server* sv) {
       d.height = q.height;
       d.rect_w = q.width;
       d.rect_h = q.height;
-      d.frame = 1 + r->frames_done;
-      d.hdr = r->hdr.p;
-      d.features = q.denoise ? r->feat.p : nullptr;
-      d.path_base = step.paths;
-      step.jobs.push_back(d);
-      step.paths += q.width * q.height;
+      d.features = q.denoise ? r.feat.p : nullptr;
+      const size_t frame_floats = (size_t)q.width * q.height * 3;
+      for (uint32_t f = 0; f < take[k]; f++) {
+        d.frame = 1 + r.frames_done + f;
+        d.hdr = f == 0 ? r.hdr.p : r.slots.p + (size_t)(f - 1) * frame_floats;
+        d.path_base = step.paths;
+        step.jobs.push_back(d);
+        step.paths += q.width * q.height;
+      }
+      if (take[k] > 1) {
+        FoldDesc fd;
+        fd.hdr = r.hdr.p;
+        fd.slots = r.slots.p;
+        fd.stride = frame_floats;
+        fd.count = take[k] - 1;
+        fd.width = q.width;
+        fd.rect[2] = q.width;
+        fd.rect[3] = q.height;
+        step.folds.push_back(fd);
+      }
       max_depth = std::max(max_depth, sc->info.depth);
       packed = packed && sc->packed;
     }
@@ -1287,8 +1350,9 @@ void serve_loop(cv_server* sv) {
       std::lock_guard<std::mutex> lk(sv->mu);
       sv->steps++;
       sv->busy_ms += stt.total_ms;
-      for (auto& r : act) {
-        r->frames_done++;
+      for (size_t k = 0; k < act.size(); k++) {
+        auto& r = act[k];
+        r->frames_done += take[k];
         if (st != CV_OK || r->frames_done >= r->req.spp) finished.push_back(r);
       }
       for (auto& r : finished)
@@ -1334,14 +1398,15 @@ void finish_loop(cv_server* sv) {
     }
     {
       std::lock_guard<std::mutex> lk(sv->mu);
-      for (DevBuf<float>* b : {&r->hdr, &r->feat}) {
+      for (DevBuf<float>* b : {&r->hdr, &r->feat, &r->slots}) {
         if (b->p) sv->to_free.push_back(b->p);
         b->p = nullptr;
         b->n = 0;
       }
       r->done = true;
       r->t_done = now_ms();
-      sv->used_bytes -= r->bytes;
+      sv->used_bytes -= r->bytes + r->slot_bytes;
+      r->slot_bytes = 0;
       if (r->status == CV_OK) {
         sv->completed++;
         sv->latencies.push_back(r->t_done - r->t_submit);
