@@ -243,8 +243,8 @@ def bench_batch(args):
 
 def cpu_baseline_batch(sds, specs, args):
     """The reference's algorithm on the host (as cpu_baseline) on every 8th
-    job: frame 1's 32x32 patches in a strided order until the job's time
-    budget is spent gives the job's host rate; the rates are combined with
+    job: its frames' 32x32 patches in a strided order until the job's time
+    budget is spent give the job's host rate; the rates are combined with
     the batch's own weights (each job's width x height x spp paths), i.e. the
     host's paths/s on these jobs as the batch defines them."""
     from oracle import oracle as O
@@ -266,10 +266,14 @@ def cpu_baseline_batch(sds, specs, args):
         npatch = ((w + 31) // 32) * npy
         order = [(k * 37) % npatch for k in range(npatch)] if npatch % 37 else list(range(npatch))
         t_job, done, p_job = 0.0, 0, 0
-        while t_job < budget and done < npatch:
-            ids = order[done:done + 4 * cores]
+        # the job's frames in order (frame f = its f-th sample) while the
+        # job's share of the time budget lasts
+        while t_job < budget and done < npatch * spp:
+            frame = 1 + done // npatch
+            k = done % npatch
+            ids = order[k:k + 4 * cores]
             t1 = time.perf_counter()
-            q += sample(w, h, 1, ids)
+            q += sample(w, h, frame, ids)
             t_job += time.perf_counter() - t1
             # patches at the right and top edges hold fewer pixels
             for pid in ids:
@@ -287,9 +291,9 @@ def cpu_baseline_batch(sds, specs, args):
         "unit": "paths/s",
         "cores": cores,
         "kind": "port",
-        "sample": "%s; jobs %s (every 8th), frame 1, 32x32 patches in stride-37 order for up "
-                  "to %.1f s each, %d threads: %d paths, %d extend queries in %.1f s; per-job "
-                  "rates weighted by each job's width x height x spp"
+        "sample": "%s; jobs %s (every 8th), their frames in order, 32x32 patches in stride-37 "
+                  "order, for up to %.1f s each, %d threads: %d paths, %d extend queries in "
+                  "%.1f s; per-job rates weighted by each job's width x height x spp"
                   % (what, picks, budget, cores, paths, q, t_used),
         "job_rates_paths_per_s": rates,
         "unweighted_paths_per_s": paths / t_used,
