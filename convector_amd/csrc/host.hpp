@@ -83,6 +83,7 @@ struct LaunchCfg {
   uint32_t fetch_batches = 1; // 64-ray batches per fetch in batch mode
   uint32_t block_waves = 1;   // waves per workgroup (lane refill)
   uint32_t frames_per_step = CV_FRAMES_AUTO;  // accumulation frames sharing launches
+  uint32_t refill_interval = 16;              // traversal steps between refill checks
 };
 
 }  // namespace cvh

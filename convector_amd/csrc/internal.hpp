@@ -84,6 +84,7 @@ struct ExtendArgs {
   uint2* spill;
   uint32_t batches;  // 64-ray batches per cursor fetch (batch mode)
   uint32_t refill;   // > 0: lane refill when this many lanes are idle
+  uint32_t interval; // lane refill: traversal steps between refill checks (>= 1)
 };
 
 struct ShadeArgs {

@@ -636,15 +636,21 @@ __global__ __launch_bounds__(256) void k_extend_refill(ExtendArgs a) {
       continue;
     }
     const bool wave_exact = __any(active && exact);
-    if (active) {
-      trav_step<COUNT>(nodes, tris, o, d, inv, wave_exact, st, ts, cur, have);
-      if (!have && st.sp == 0) {
-        a.hits[ray] = make_uint4(ts.best, f2u(ts.best_t), f2u(ts.bu), f2u(ts.bv));
-        if (COUNT) {
-          na += ts.na;
-          nt += ts.nt;
+    // several traversal steps per refill check: the check's votes cost
+    // about as much as a step's mask logic, and a lane that finishes early
+    // idles for at most interval - 1 steps
+#pragma unroll 1
+    for (uint32_t rep = 0; rep < a.interval; rep++) {
+      if (active) {
+        trav_step<COUNT>(nodes, tris, o, d, inv, wave_exact, st, ts, cur, have);
+        if (!have && st.sp == 0) {
+          a.hits[ray] = make_uint4(ts.best, f2u(ts.best_t), f2u(ts.bu), f2u(ts.bv));
+          if (COUNT) {
+            na += ts.na;
+            nt += ts.nt;
+          }
+          active = false;
         }
-        active = false;
       }
     }
   }

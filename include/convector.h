@@ -138,6 +138,7 @@ typedef struct cv_tuning {
     uint32_t frames_per_step;     /* frame accumulation: 1..64 frames rendered in the same
                                      launches (each into its own buffer, added in frame order),
                                      or CV_FRAMES_AUTO (about 2^25 paths per step) */
+    uint32_t refill_interval;     /* lane refill: 1..64 traversal steps between refill checks */
 } cv_tuning;
 #define CV_FRAMES_AUTO 255u
 cv_status cv_scene_set_tuning(cv_scene* scene, const cv_tuning* tuning);
@@ -299,7 +300,7 @@ cv_status cv_batch_get_tuning(const cv_batch* b, cv_tuning* tuning);
 cv_status cv_batch_render(cv_batch* b, const cv_job* jobs, size_t n, uint32_t flags);
 /* The autotuner: times launch shapes (stack entries 8/16/24/32, 16-48
  * persistent waves per CU, refill thresholds 16-64 and whole batches, 1/2/4
- * waves per workgroup) on a
+ * waves per workgroup, 1/4/16/32 steps between refill checks) on a
  * sample step -- the first frame of the jobs in order, up to
  * max_sample_paths paths (0: 2^22), rendered into scratch buffers -- by
  * two rounds of coordinate descent, and keeps the fastest on the batch.
