@@ -457,96 +457,147 @@ __global__ __launch_bounds__(256) void k_wgrad_lds(WGradArgs a) {
 // Elementwise and resampling kernels
 // ---------------------------------------------------------------------------
 
+// The layout kernels below move 8 BF16 (16 B) or 4 FP32 channels per
+// thread with 32-bit index arithmetic: every channel count and channel
+// stride they are given is a multiple of 8, every base pointer 16-B aligned.
+__device__ __forceinline__ void unpack8(uint4 v, float* f) {
+  const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    f[2 * k] = from_bf16((uint16_t)(w[k] & 0xffffu));
+    f[2 * k + 1] = from_bf16((uint16_t)(w[k] >> 16));
+  }
+}
+__device__ __forceinline__ uint4 pack8(const float* f) {
+  uint32_t w[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) w[k] = (uint32_t)to_bf16(f[2 * k]) | ((uint32_t)to_bf16(f[2 * k + 1]) << 16);
+  return make_uint4(w[0], w[1], w[2], w[3]);
+}
+
 // dZ = dY * (Y > 0) for ReLU layers (or dY for linear ones), as BF16 with
 // the padded channels [c, zcs) zeroed.
 __global__ void k_relu_grad(const float* dy, uint32_t dycs, const uint16_t* y, uint32_t ycs,
                             uint32_t c, uint16_t* dz, uint32_t zcs, uint32_t npix, uint32_t relu) {
-  const size_t n = (size_t)npix * zcs;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
-       i += (size_t)gridDim.x * blockDim.x) {
-    const size_t p = i / zcs;
-    const uint32_t ch = (uint32_t)(i % zcs);
+  const uint32_t n = npix * zcs;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t p = i / zcs;
+    const uint32_t ch = i % zcs;
     float v = 0.0f;
     if (ch < c) {
-      v = dy[p * dycs + ch];
-      if (relu && !(from_bf16(y[p * ycs + ch]) > 0.0f)) v = 0.0f;
+      v = dy[(size_t)p * dycs + ch];
+      if (relu && !(from_bf16(y[(size_t)p * ycs + ch]) > 0.0f)) v = 0.0f;
     }
     dz[i] = to_bf16(v);
+  }
+}
+
+// The same for c == zcs, all multiples of 8: 8 channels per thread.
+__global__ void k_relu_grad8(const float* dy, uint32_t dycs, const uint16_t* y, uint32_t ycs,
+                             uint32_t c, uint16_t* dz, uint32_t npix, uint32_t relu) {
+  const uint32_t g = c / 8, n = npix * g;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t p = i / g, ch = (i % g) * 8;
+    const float4 a = *reinterpret_cast<const float4*>(dy + (size_t)p * dycs + ch);
+    const float4 b = *reinterpret_cast<const float4*>(dy + (size_t)p * dycs + ch + 4);
+    float v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    if (relu) {
+      float yv[8];
+      unpack8(*reinterpret_cast<const uint4*>(y + (size_t)p * ycs + ch), yv);
+#pragma unroll
+      for (int k = 0; k < 8; k++)
+        if (!(yv[k] > 0.0f)) v[k] = 0.0f;
+    }
+    *reinterpret_cast<uint4*>(dz + (size_t)p * c + ch) = pack8(v);
   }
 }
 
 // 2x2 average pooling, BF16 in and out.
 __global__ void k_pool(const uint16_t* x, uint32_t xcs, uint16_t* y, uint32_t ycs, uint32_t c,
                        uint32_t nb, uint32_t h2, uint32_t w2) {
-  const size_t n = (size_t)nb * h2 * w2 * c;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
-       i += (size_t)gridDim.x * blockDim.x) {
-    const uint32_t ch = (uint32_t)(i % c);
-    const size_t q = i / c;
-    const uint32_t X = (uint32_t)(q % w2), Y = (uint32_t)((q / w2) % h2), B = (uint32_t)(q / ((size_t)w2 * h2));
-    const size_t base = ((size_t)B * (2 * h2) + 2 * Y) * (2 * w2) + 2 * X;
-    const float s = from_bf16(x[base * xcs + ch]) + from_bf16(x[(base + 1) * xcs + ch]) +
-                    from_bf16(x[(base + 2 * w2) * xcs + ch]) +
-                    from_bf16(x[(base + 2 * w2 + 1) * xcs + ch]);
-    y[q * ycs + ch] = to_bf16(0.25f * s);
+  const uint32_t g = c / 8, n = nb * h2 * w2 * g;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t ch = (i % g) * 8, q = i / g;
+    const uint32_t X = q % w2, t = q / w2, Y = t % h2, B = t / h2;
+    const uint32_t base = (B * (2 * h2) + 2 * Y) * (2 * w2) + 2 * X;
+    float a[8], b[8], c2[8], d[8], r[8];
+    unpack8(*reinterpret_cast<const uint4*>(x + (size_t)base * xcs + ch), a);
+    unpack8(*reinterpret_cast<const uint4*>(x + (size_t)(base + 1) * xcs + ch), b);
+    unpack8(*reinterpret_cast<const uint4*>(x + (size_t)(base + 2 * w2) * xcs + ch), c2);
+    unpack8(*reinterpret_cast<const uint4*>(x + (size_t)(base + 2 * w2 + 1) * xcs + ch), d);
+#pragma unroll
+    for (int k = 0; k < 8; k++) r[k] = 0.25f * (((a[k] + b[k]) + c2[k]) + d[k]);
+    *reinterpret_cast<uint4*>(y + (size_t)q * ycs + ch) = pack8(r);
   }
 }
 
-// Backward of the pooling: dx += dy / 4 on each of the 4 inputs.
+// Backward of the pooling: dx += dy / 4 on each of the 4 inputs (FP32).
 __global__ void k_pool_grad(const float* dy, uint32_t dycs, float* dx, uint32_t dxcs, uint32_t c,
                             uint32_t nb, uint32_t h2, uint32_t w2) {
-  const size_t n = (size_t)nb * h2 * w2 * c;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
-       i += (size_t)gridDim.x * blockDim.x) {
-    const uint32_t ch = (uint32_t)(i % c);
-    const size_t q = i / c;
-    const uint32_t X = (uint32_t)(q % w2), Y = (uint32_t)((q / w2) % h2), B = (uint32_t)(q / ((size_t)w2 * h2));
-    const size_t base = ((size_t)B * (2 * h2) + 2 * Y) * (2 * w2) + 2 * X;
-    const float g = 0.25f * dy[q * dycs + ch];
-    dx[base * dxcs + ch] += g;
-    dx[(base + 1) * dxcs + ch] += g;
-    dx[(base + 2 * w2) * dxcs + ch] += g;
-    dx[(base + 2 * w2 + 1) * dxcs + ch] += g;
+  const uint32_t g = c / 4, n = nb * h2 * w2 * g;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t ch = (i % g) * 4, q = i / g;
+    const uint32_t X = q % w2, t = q / w2, Y = t % h2, B = t / h2;
+    const uint32_t base = (B * (2 * h2) + 2 * Y) * (2 * w2) + 2 * X;
+    const float4 v = *reinterpret_cast<const float4*>(dy + (size_t)q * dycs + ch);
+    const float4 gv = make_float4(0.25f * v.x, 0.25f * v.y, 0.25f * v.z, 0.25f * v.w);
+    const uint32_t px[4] = {base, base + 1, base + 2 * w2, base + 2 * w2 + 1};
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      float4* o = reinterpret_cast<float4*>(dx + (size_t)px[k] * dxcs + ch);
+      float4 u = *o;
+      u.x += gv.x;
+      u.y += gv.y;
+      u.z += gv.z;
+      u.w += gv.w;
+      *o = u;
+    }
   }
 }
 
 // Nearest 2x upsampling, BF16.
 __global__ void k_upsample(const uint16_t* x, uint32_t xcs, uint16_t* y, uint32_t ycs, uint32_t c,
                            uint32_t nb, uint32_t h, uint32_t w) {
-  const size_t n = (size_t)nb * h * w * c;  // output elements (h, w: output size)
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
-       i += (size_t)gridDim.x * blockDim.x) {
-    const uint32_t ch = (uint32_t)(i % c);
-    const size_t q = i / c;
-    const uint32_t X = (uint32_t)(q % w), Y = (uint32_t)((q / w) % h), B = (uint32_t)(q / ((size_t)w * h));
-    const size_t s = ((size_t)B * (h / 2) + Y / 2) * (w / 2) + X / 2;
-    y[q * ycs + ch] = x[s * xcs + ch];
+  const uint32_t g = c / 8, n = nb * h * w * g;  // output vectors (h, w: output size)
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t ch = (i % g) * 8, q = i / g;
+    const uint32_t X = q % w, t = q / w, Y = t % h, B = t / h;
+    const uint32_t src = (B * (h / 2) + Y / 2) * (w / 2) + X / 2;
+    *reinterpret_cast<uint4*>(y + (size_t)q * ycs + ch) =
+        *reinterpret_cast<const uint4*>(x + (size_t)src * xcs + ch);
   }
 }
 
-// Backward of the upsampling: dx = sum of the 4 output gradients.
+// Backward of the upsampling: dx = sum of the 4 output gradients (FP32).
 __global__ void k_upsample_grad(const float* dy, uint32_t dycs, float* dx, uint32_t dxcs,
                                 uint32_t c, uint32_t nb, uint32_t h2, uint32_t w2) {
-  const size_t n = (size_t)nb * h2 * w2 * c;  // input elements (h2, w2: input size)
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
-       i += (size_t)gridDim.x * blockDim.x) {
-    const uint32_t ch = (uint32_t)(i % c);
-    const size_t q = i / c;
-    const uint32_t X = (uint32_t)(q % w2), Y = (uint32_t)((q / w2) % h2), B = (uint32_t)(q / ((size_t)w2 * h2));
-    const size_t base = ((size_t)B * (2 * h2) + 2 * Y) * (2 * w2) + 2 * X;
-    dx[q * dxcs + ch] = dy[base * dycs + ch] + dy[(base + 1) * dycs + ch] +
-                        dy[(base + 2 * w2) * dycs + ch] + dy[(base + 2 * w2 + 1) * dycs + ch];
+  const uint32_t g = c / 4, n = nb * h2 * w2 * g;  // input vectors (h2, w2: input size)
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t ch = (i % g) * 4, q = i / g;
+    const uint32_t X = q % w2, t = q / w2, Y = t % h2, B = t / h2;
+    const uint32_t base = (B * (2 * h2) + 2 * Y) * (2 * w2) + 2 * X;
+    const float4 a = *reinterpret_cast<const float4*>(dy + (size_t)base * dycs + ch);
+    const float4 b = *reinterpret_cast<const float4*>(dy + (size_t)(base + 1) * dycs + ch);
+    const float4 e = *reinterpret_cast<const float4*>(dy + (size_t)(base + 2 * w2) * dycs + ch);
+    const float4 f = *reinterpret_cast<const float4*>(dy + (size_t)(base + 2 * w2 + 1) * dycs + ch);
+    *reinterpret_cast<float4*>(dx + (size_t)q * dxcs + ch) =
+        make_float4(((a.x + b.x) + e.x) + f.x, ((a.y + b.y) + e.y) + f.y,
+                    ((a.z + b.z) + e.z) + f.z, ((a.w + b.w) + e.w) + f.w);
   }
 }
 
-// Input features (row-major f32, 10 channels) -> BF16 [p][16], zero padded.
+// Input features (row-major f32, 10 channels) -> BF16 [p][16], zero padded;
+// one pixel per thread, two 16-B stores.
 __global__ void k_to_input(const float* in, uint16_t* x, uint32_t npix) {
-  const size_t n = (size_t)npix * 16;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
-       i += (size_t)gridDim.x * blockDim.x) {
-    const size_t p = i / 16;
-    const uint32_t ch = (uint32_t)(i % 16);
-    x[i] = ch < 10 ? to_bf16(in[p * 10 + ch]) : (uint16_t)0;
+  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < npix; p += gridDim.x * blockDim.x) {
+    float f[16];
+#pragma unroll
+    for (int k = 0; k < 10; k++) f[k] = in[(size_t)p * 10 + k];
+#pragma unroll
+    for (int k = 10; k < 16; k++) f[k] = 0.0f;
+    uint4* o = reinterpret_cast<uint4*>(x + (size_t)p * 16);
+    o[0] = pack8(f);
+    o[1] = pack8(f + 8);
   }
 }
 
@@ -766,8 +817,12 @@ cv_status conv_bwd(cv_denoiser* d, const Conv& L, const uint16_t* x, uint32_t xc
                    uint32_t gxcs, bool accumulate, uint32_t h, uint32_t w) {
   const uint32_t npix = d->nb * h * w;
   hipStream_t s = d->stream;
-  hipLaunchKernelGGL(cvd::k_relu_grad, dim3(grid1((size_t)npix * L.cout_pad)), dim3(256), 0, s,
-                     gy, gycs, y, ycs, L.cout, d->dz.p, L.cout_pad, npix, L.relu ? 1u : 0u);
+  if (L.cout == L.cout_pad && L.cout % 8 == 0 && gycs % 8 == 0 && (!y || ycs % 8 == 0))
+    hipLaunchKernelGGL(cvd::k_relu_grad8, dim3(grid1((size_t)npix * L.cout / 8)), dim3(256), 0, s,
+                       gy, gycs, y, ycs, L.cout, d->dz.p, npix, L.relu ? 1u : 0u);
+  else
+    hipLaunchKernelGGL(cvd::k_relu_grad, dim3(grid1((size_t)npix * L.cout_pad)), dim3(256), 0, s,
+                       gy, gycs, y, ycs, L.cout, d->dz.p, L.cout_pad, npix, L.relu ? 1u : 0u);
   HIPTRY(hipGetLastError(), "relu grad");
   cvd::WGradArgs g{};
   g.dz = d->dz.p;
@@ -884,19 +939,19 @@ cv_status forward(cv_denoiser* d) {
   if ((cs = (x)) != CV_OK) return cs
   CVD_TRY(conv_fwd(d, L[0], d->x0.p, 16, d->e0a.p, 32, nullptr, 0, H, W));
   CVD_TRY(conv_fwd(d, L[1], d->e0a.p, 32, d->cat0.p + 64, 96, nullptr, 0, H, W));
-  hipLaunchKernelGGL(cvd::k_pool, dim3(grid1((size_t)d->nb * H1 * W1 * 32)), dim3(256), 0, s,
+  hipLaunchKernelGGL(cvd::k_pool, dim3(grid1((size_t)d->nb * H1 * W1 * 4)), dim3(256), 0, s,
                      d->cat0.p + 64, 96u, d->p0.p, 32u, 32u, d->nb, H1, W1);
   CVD_TRY(conv_fwd(d, L[2], d->p0.p, 32, d->e1a.p, 64, nullptr, 0, H1, W1));
   CVD_TRY(conv_fwd(d, L[3], d->e1a.p, 64, d->cat1.p + 128, 192, nullptr, 0, H1, W1));
-  hipLaunchKernelGGL(cvd::k_pool, dim3(grid1((size_t)d->nb * H2 * W2 * 64)), dim3(256), 0, s,
+  hipLaunchKernelGGL(cvd::k_pool, dim3(grid1((size_t)d->nb * H2 * W2 * 8)), dim3(256), 0, s,
                      d->cat1.p + 128, 192u, d->p1.p, 64u, 64u, d->nb, H2, W2);
   CVD_TRY(conv_fwd(d, L[4], d->p1.p, 64, d->b2a.p, 128, nullptr, 0, H2, W2));
   CVD_TRY(conv_fwd(d, L[5], d->b2a.p, 128, d->b2b.p, 128, nullptr, 0, H2, W2));
-  hipLaunchKernelGGL(cvd::k_upsample, dim3(grid1((size_t)d->nb * H1 * W1 * 128)), dim3(256), 0, s,
+  hipLaunchKernelGGL(cvd::k_upsample, dim3(grid1((size_t)d->nb * H1 * W1 * 16)), dim3(256), 0, s,
                      d->b2b.p, 128u, d->cat1.p, 192u, 128u, d->nb, H1, W1);
   CVD_TRY(conv_fwd(d, L[6], d->cat1.p, 192, d->d1a.p, 64, nullptr, 0, H1, W1));
   CVD_TRY(conv_fwd(d, L[7], d->d1a.p, 64, d->d1b.p, 64, nullptr, 0, H1, W1));
-  hipLaunchKernelGGL(cvd::k_upsample, dim3(grid1((size_t)d->nb * H * W * 64)), dim3(256), 0, s,
+  hipLaunchKernelGGL(cvd::k_upsample, dim3(grid1((size_t)d->nb * H * W * 8)), dim3(256), 0, s,
                      d->d1b.p, 64u, d->cat0.p, 96u, 64u, d->nb, H, W);
   CVD_TRY(conv_fwd(d, L[8], d->cat0.p, 96, d->d0a.p, 32, nullptr, 0, H, W));
   CVD_TRY(conv_fwd(d, L[9], d->d0a.p, 32, d->d0b.p, 32, nullptr, 0, H, W));
@@ -921,27 +976,27 @@ cv_status backward(cv_denoiser* d) {
   CVD_TRY(conv_bwd(d, L[8], d->cat0.p, 96, d->d0a.p, 32, d->g_d0a.p, 32, d->g_cat0.p, 96, false,
                    H, W));
   // cat0[0:64] = up(d1b)
-  hipLaunchKernelGGL(cvd::k_upsample_grad, dim3(grid1((size_t)nb * H1 * W1 * 64)), dim3(256), 0, s,
+  hipLaunchKernelGGL(cvd::k_upsample_grad, dim3(grid1((size_t)nb * H1 * W1 * 16)), dim3(256), 0, s,
                      d->g_cat0.p, 96u, d->g_d1b.p, 64u, 64u, nb, H1, W1);
   CVD_TRY(conv_bwd(d, L[7], d->d1a.p, 64, d->d1b.p, 64, d->g_d1b.p, 64, d->g_d1a.p, 64, false,
                    H1, W1));
   CVD_TRY(conv_bwd(d, L[6], d->cat1.p, 192, d->d1a.p, 64, d->g_d1a.p, 64, d->g_cat1.p, 192, false,
                    H1, W1));
   // cat1[0:128] = up(b2b)
-  hipLaunchKernelGGL(cvd::k_upsample_grad, dim3(grid1((size_t)nb * H2 * W2 * 128)), dim3(256), 0,
+  hipLaunchKernelGGL(cvd::k_upsample_grad, dim3(grid1((size_t)nb * H2 * W2 * 32)), dim3(256), 0,
                      s, d->g_cat1.p, 192u, d->g_b2b.p, 128u, 128u, nb, H2, W2);
   CVD_TRY(conv_bwd(d, L[5], d->b2a.p, 128, d->b2b.p, 128, d->g_b2b.p, 128, d->g_b2a.p, 128, false,
                    H2, W2));
   CVD_TRY(conv_bwd(d, L[4], d->p1.p, 64, d->b2a.p, 128, d->g_b2a.p, 128, d->g_p1.p, 64, false, H2,
                    W2));
   // p1 = pool(cat1[128:192]): its gradient joins the skip gradient in g_cat1
-  hipLaunchKernelGGL(cvd::k_pool_grad, dim3(grid1((size_t)nb * H2 * W2 * 64)), dim3(256), 0, s,
+  hipLaunchKernelGGL(cvd::k_pool_grad, dim3(grid1((size_t)nb * H2 * W2 * 16)), dim3(256), 0, s,
                      d->g_p1.p, 64u, d->g_cat1.p + 128, 192u, 64u, nb, H2, W2);
   CVD_TRY(conv_bwd(d, L[3], d->e1a.p, 64, d->cat1.p + 128, 192, d->g_cat1.p + 128, 192,
                    d->g_e1a.p, 64, false, H1, W1));
   CVD_TRY(conv_bwd(d, L[2], d->p0.p, 32, d->e1a.p, 64, d->g_e1a.p, 64, d->g_p0.p, 32, false, H1,
                    W1));
-  hipLaunchKernelGGL(cvd::k_pool_grad, dim3(grid1((size_t)nb * H1 * W1 * 32)), dim3(256), 0, s,
+  hipLaunchKernelGGL(cvd::k_pool_grad, dim3(grid1((size_t)nb * H1 * W1 * 8)), dim3(256), 0, s,
                      d->g_p0.p, 32u, d->g_cat0.p + 64, 96u, 32u, nb, H1, W1);
   CVD_TRY(conv_bwd(d, L[1], d->e0a.p, 32, d->cat0.p + 64, 96, d->g_cat0.p + 64, 96, d->g_e0a.p,
                    32, false, H, W));
@@ -989,7 +1044,7 @@ cv_status denoise_image(cv_denoiser* d, const float* hdr, const float* features,
   hipStream_t s = d->stream;
   hipLaunchKernelGGL(cvd::k_features, dim3(grid1(P)), dim3(256), 0, s, hdr, features, width,
                      height, spp, d->in_f32.p);
-  hipLaunchKernelGGL(cvd::k_to_input, dim3(grid1(P * 16)), dim3(256), 0, s, d->in_f32.p, d->x0.p,
+  hipLaunchKernelGGL(cvd::k_to_input, dim3(grid1(P)), dim3(256), 0, s, d->in_f32.p, d->x0.p,
                      (uint32_t)P);
   if ((cs = forward(d)) != CV_OK) return cs;
   hipLaunchKernelGGL(cvd::k_output, dim3(grid1(P)), dim3(256), 0, s, d->out.p, width, height, spp,
@@ -1119,7 +1174,7 @@ cv_status cv_denoiser_forward(cv_denoiser* d, const float* input, float* output,
   const size_t P = (size_t)nb * h * w;
   const float* in = nullptr;
   if ((cs = stage_in(d, d->in_f32, input, P * 10, dev, &in)) != CV_OK) return cs;
-  hipLaunchKernelGGL(cvd::k_to_input, dim3(grid1(P * 16)), dim3(256), 0, d->stream, in, d->x0.p,
+  hipLaunchKernelGGL(cvd::k_to_input, dim3(grid1(P)), dim3(256), 0, d->stream, in, d->x0.p,
                      (uint32_t)P);
   if ((cs = forward(d)) != CV_OK) return cs;
   HIPTRY(hipMemcpyAsync(output, d->out.p, P * 12, dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
@@ -1143,7 +1198,7 @@ cv_status cv_denoiser_train_step(cv_denoiser* d, const float* input, const float
   if ((cs = stage_in(d, d->in_f32, input, P * 10, dev, &in)) != CV_OK) return cs;
   if ((cs = stage_in(d, d->tgt_f32, target, P * 3, dev, &tg)) != CV_OK) return cs;
   hipStream_t s = d->stream;
-  hipLaunchKernelGGL(cvd::k_to_input, dim3(grid1(P * 16)), dim3(256), 0, s, in, d->x0.p,
+  hipLaunchKernelGGL(cvd::k_to_input, dim3(grid1(P)), dim3(256), 0, s, in, d->x0.p,
                      (uint32_t)P);
   if ((cs = forward(d)) != CV_OK) return cs;
   hipLaunchKernelGGL(cvd::k_fill, dim3(1), dim3(64), 0, s, d->loss.p, (size_t)1, 0.0f);
