@@ -383,11 +383,29 @@ def bench_denoise(args):
     d = cv.Denoiser(7)
     B = min(16, len(xs))
     losses = []
+    # the training pairs stay on the device; a batch is gathered with
+    # device-to-device copies
+    xs = np.ascontiguousarray(xs, np.float32)
+    ys = np.ascontiguousarray(ys, np.float32)
+    xs_d, ys_d = DeviceBuffer(xs.nbytes), DeviceBuffer(ys.nbytes)
+    bx_d, by_d = DeviceBuffer(xs[0].nbytes * B), DeviceBuffer(ys[0].nbytes * B)
+    hip = xs_d.hip
+    for dst, src in ((xs_d, xs), (ys_d, ys)):
+        hip.hipMemcpy(dst.ptr, src.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(src.nbytes), 1)
+    sx, sy = xs[0].nbytes, ys[0].nbytes
     cv.device_synchronize()
     t0 = time.perf_counter()
     for it in range(args.train_steps):
         idx = rng.choice(len(xs), B, replace=False)
-        losses.append(d.train_step(xs[idx], ys[idx], 1e-3))
+        for k, i in enumerate(idx):
+            hip.hipMemcpyAsync(ctypes.c_void_p(bx_d.ptr.value + k * sx),
+                               ctypes.c_void_p(xs_d.ptr.value + int(i) * sx), ctypes.c_size_t(sx), 3,
+                               None)
+            hip.hipMemcpyAsync(ctypes.c_void_p(by_d.ptr.value + k * sy),
+                               ctypes.c_void_p(ys_d.ptr.value + int(i) * sy), ctypes.c_size_t(sy), 3,
+                               None)
+        cv.device_synchronize()
+        losses.append(d.train_step_device(bx_d.ptr.value, by_d.ptr.value, B, TH, TW, 1e-3))
         if it % 100 == 0:
             log("denoise: step %d loss %.4f" % (it, losses[-1]))
     train_s = time.perf_counter() - t0
@@ -397,10 +415,13 @@ def bench_denoise(args):
     W, H = 1280, 736
     ratios, evals = [], []
     hdr_dev = DeviceBuffer(W * H * 12)
+    # the held-out scenes and cameras come from their own stream, so they do
+    # not depend on the training set's size
+    rng_eval = np.random.default_rng(4321)
     for e in range(args.eval_scenes):
-        n = int(np.exp(rng.uniform(np.log(1e4), np.log(5e5))))
+        n = int(np.exp(rng_eval.uniform(np.log(1e4), np.log(5e5))))
         g = cv.Scene.from_scene_data(scenes.procedural_scene(n, 90000 + e))
-        _lit_camera(cv, g, rng)
+        _lit_camera(cv, g, rng_eval)
         r = cv.Renderer(g, W, H)
         ref = r.new_buffer_f32()
         r.accumulate_frames_f32(ref, 200000, args.eval_ref_spp)
@@ -617,10 +638,10 @@ def main():
                          "the denoiser and measure its equal-RMSE speedup; serve: the request "
                          "loop's latency and throughput")
     ap.add_argument("--jobs", type=int, default=64)
-    ap.add_argument("--train-scenes", type=int, default=96)
+    ap.add_argument("--train-scenes", type=int, default=128)
     ap.add_argument("--train-steps", type=int, default=1000)
     ap.add_argument("--ref-spp", type=int, default=1024)
-    ap.add_argument("--eval-scenes", type=int, default=4)
+    ap.add_argument("--eval-scenes", type=int, default=8)
     ap.add_argument("--eval-ref-spp", type=int, default=2048)
     ap.add_argument("--eval-max-spp", type=int, default=512)
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
