@@ -519,7 +519,7 @@ static cv_status scene_finish(cv_scene* s, const std::vector<float>& v9,
   // leaves of at most 31 triangles
   s->packed = s->info.max_leaf <= 31 && N < (1u << 27) && s->info.num_nodes < (1u << 27);
   HIPTRY(s->cursor.alloc(4), "alloc");
-  cv_tuning t{8, 32, 32, 1, 1};
+  cv_tuning t{8, 32, 16, 1, 1};
   return scene_apply_tuning(s, t);
 }
 

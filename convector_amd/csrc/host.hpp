@@ -79,7 +79,7 @@ struct HostBuf {
 struct LaunchCfg {
   int stack = 8;              // per-lane LDS stack entries
   uint32_t blocks = 0;        // persistent 64-lane workgroups
-  uint32_t refill = 32;       // 1..64 lane refill threshold, 65 whole 64-ray batches
+  uint32_t refill = 16;       // 1..64 lane refill threshold, 65 whole 64-ray batches
   uint32_t fetch_batches = 1; // 64-ray batches per fetch in batch mode
   uint32_t block_waves = 1;   // waves per workgroup (lane refill)
   uint32_t frames_per_step = CV_FRAMES_AUTO;  // accumulation frames sharing launches

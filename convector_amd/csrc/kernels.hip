@@ -597,17 +597,36 @@ __global__ __launch_bounds__(256) void k_extend_refill(ExtendArgs a) {
   uint32_t ray = 0;
   entry_t cur = 0;
   bool have = false;
-  bool active = false, exact = false, exhausted = false;
+  bool active = false, exact = false;
+  // The wave's reserved rays [pool, pool_end): the cursor hands out chunks
+  // of max(idle lanes, 64 * batches) rays, so most refills take no atomic
+  // and a low refill threshold does not contend on the one cursor.
+  // `drained`: the cursor has passed the last ray.  (Wave-uniform values.)
+  uint32_t pool = 0, pool_end = 0;
+  bool drained = false;
+  const uint32_t chunk = 64u * (a.batches ? a.batches : 1u);
   uint64_t na = 0, nt = 0;
   for (;;) {
     const uint64_t idle = __ballot(!active);
     const uint32_t nidle = __popcll(idle);
-    if (!exhausted && (nidle >= a.refill || nidle == 64)) {
-      uint32_t base = fetch(cursor, nidle);
-      if (base + nidle >= n) exhausted = true;
+    if ((pool < pool_end || !drained) && (nidle >= a.refill || nidle == 64)) {
+      const uint32_t from_pool = min(nidle, pool_end - pool);
+      const uint32_t base0 = pool;
+      pool += from_pool;
+      const uint32_t need = nidle - from_pool;
+      uint32_t base1 = 0, end1 = 0;
+      if (need && !drained) {
+        const uint32_t sz = max(need, chunk);
+        base1 = fetch(cursor, sz);
+        end1 = base1 >= n ? base1 : min(base1 + sz, n);
+        if (base1 + sz >= n) drained = true;
+        pool = min(base1 + need, end1);
+        pool_end = end1;
+      }
       if (!active) {
-        uint32_t i = base + __popcll(idle & lt);
-        if (i < n) {
+        const uint32_t rank = __popcll(idle & lt);
+        const uint32_t i = rank < from_pool ? base0 + rank : base1 + (rank - from_pool);
+        if (rank < from_pool || i < end1) {
           float4 r0 = a.r0[i], r1 = a.r1[i];
           o = mk3(r0.x, r0.y, r0.z);
           d = mk3(r1.x, r1.y, r1.z);
@@ -632,7 +651,7 @@ __global__ __launch_bounds__(256) void k_extend_refill(ExtendArgs a) {
       }
     }
     if (!__any(active)) {
-      if (exhausted) break;
+      if (drained && pool >= pool_end) break;
       continue;
     }
     const bool wave_exact = __any(active && exact);

@@ -782,7 +782,7 @@ cv_status cv_batch_create(const cv_render_opts* opts, uint64_t max_paths_per_ste
   cv_batch* b = new cv_batch();
   b->opts = o;
   b->max_paths_per_step = max_paths_per_step ? max_paths_per_step : (1ull << 24);
-  cv_tuning t{8, 32, 32, 1, 1};
+  cv_tuning t{8, 32, 16, 1, 1};
   cvh::apply_tuning(b->cfg, t);
   if ((cs = b->pipe.init()) != CV_OK) {
     cv_batch_destroy(b);
@@ -1447,7 +1447,7 @@ cv_status cv_server_create(const cv_render_opts* opts, uint64_t hbm_budget_bytes
   sv->opts = o;
   sv->budget = hbm_budget_bytes ? hbm_budget_bytes : (200ull << 30);
   sv->max_paths = max_paths_per_step;
-  cv_tuning t{8, 32, 32, 1, 1};
+  cv_tuning t{8, 32, 16, 1, 1};
   cvh::apply_tuning(sv->cfg, t);
   if ((cs = sv->pipe.init()) != CV_OK || (cs = sv->pipe.reserve(sv->max_paths, 256)) != CV_OK) {
     delete sv;

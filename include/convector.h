@@ -133,7 +133,8 @@ typedef struct cv_tuning {
     uint32_t extend_waves_per_cu; /* persistent traversal waves per CU */
     uint32_t refill_threshold;    /* 1..64: idle lanes of a wave take new rays when at least
                                      this many are idle; 65: whole 64-ray batches */
-    uint32_t fetch_batches;       /* batch mode: 64-ray batches per fetch */
+    uint32_t fetch_batches;       /* 64-ray batches per cursor fetch (whole-batch mode), or
+                                     rays a wave reserves per fetch / 64 (lane refill) */
     uint32_t block_waves;         /* lane refill: 1, 2 or 4 waves (64-256 threads) per block */
     uint32_t frames_per_step;     /* frame accumulation: 1..64 frames rendered in the same
                                      launches (each into its own buffer, added in frame order),
