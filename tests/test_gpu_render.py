@@ -237,6 +237,15 @@ def test_trace_export(cv, tmp_path):
         cv.trace_export(path)
         ev = json.load(open(path))["traceEvents"]
         assert len(ev) == 5 and ev[-1]["name"] == "shade" and ev[-1]["args"]["id"] == 4
+        # the three frames in the same launches: one generate, 5 x (extend, shade)
+        g.set_tuning(frames_per_step=cv.FRAMES_AUTO)
+        cv.trace_enable(1000)
+        together = r.new_buffer_f32()
+        r.accumulate_frames_f32(together, 1, 3)
+        cv.trace_export(path)
+        ev = json.load(open(path))["traceEvents"]
+        assert len(ev) == 1 + 2 * 5
+        assert together.tobytes() == plain.tobytes()
     finally:
         cv.trace_disable()
     # tracing changes nothing in the image

@@ -55,6 +55,33 @@ def test_alone_equals_mix_equals_renderer(cv, world):
         g.camera.set_orientation(sd.camera_q)
 
 
+def test_frames_packed_per_step(cv, world):
+    """A lone request renders all its frames in one step (further frames in
+    their own buffers, folded in frame order) with the renderer's bits; a
+    denoised request takes one frame per step."""
+    sds, gs = world
+    sv = cv.Server()
+    sid = sv.add_scene(gs[2])
+    W, H, spp, pos = 64, 32, 8, (0.2, 0.7, -0.1)
+    t, hdr = sv.submit(sid, W, H, spp, pos, Q0)
+    sv.wait(t)
+    assert sv.stats()["steps"] == 1
+    gs[2].camera.set_position(pos)
+    gs[2].camera.set_orientation(Q0)
+    r = cv.Renderer(gs[2], W, H)
+    ref = r.new_buffer_f32()
+    r.accumulate_frames_f32(ref, 1, spp)
+    assert hdr.tobytes() == ref.tobytes()
+    sv.reset_stats()
+    sv.set_denoiser(cv.Denoiser(1))
+    t, _ = sv.submit(sid, W, H, 3, pos, Q0, denoise=True)
+    sv.wait(t)
+    assert sv.stats()["steps"] == 3
+    sv.close()
+    gs[2].camera.set_position(sds[2].camera_pos)
+    gs[2].camera.set_orientation(sds[2].camera_q)
+
+
 def test_budget_admission(cv, world):
     """A budget that admits one request at a time still finishes every
     request with the same result; one larger than the budget fails."""
