@@ -638,8 +638,8 @@ def main():
                          "the denoiser and measure its equal-RMSE speedup; serve: the request "
                          "loop's latency and throughput")
     ap.add_argument("--jobs", type=int, default=64)
-    ap.add_argument("--train-scenes", type=int, default=128)
-    ap.add_argument("--train-steps", type=int, default=1000)
+    ap.add_argument("--train-scenes", type=int, default=256)
+    ap.add_argument("--train-steps", type=int, default=3000)
     ap.add_argument("--ref-spp", type=int, default=1024)
     ap.add_argument("--eval-scenes", type=int, default=8)
     ap.add_argument("--eval-ref-spp", type=int, default=2048)
