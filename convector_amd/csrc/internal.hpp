@@ -121,7 +121,10 @@ hipError_t launch_extend(const ExtendArgs& a, bool count, bool packed, bool mult
                          uint32_t blocks, uint32_t block_waves, hipStream_t s);
 hipError_t launch_generate(const GenerateArgs& a, uint32_t blocks, hipStream_t s);
 // multi: the step has more than one job.
-hipError_t launch_shade(const ShadeArgs& a, bool multi, uint32_t max_paths, hipStream_t s);
+// shared_scene: every job of the step has the same scene (tris, lights,
+// emission), read once per wave from job 0.
+hipError_t launch_shade(const ShadeArgs& a, bool multi, bool shared_scene, uint32_t max_paths,
+                        hipStream_t s);
 hipError_t launch_hdr_to_bitmap(const float* hdr, uint32_t* bitmap, uint32_t width,
                                 uint32_t height, uint32_t num_samples, hipStream_t s);
 hipError_t launch_prepare_tris(const float* v9, const uint32_t* order, uint32_t ntris, DevTri* out,

@@ -798,7 +798,7 @@ __device__ __forceinline__ ShadeOut continue_path(const JobDesc& J, V3 ray_d, V3
 // One path per thread; blocks beyond the queue exit at once.  Survivors are
 // appended with one atomic per block: ballots per wave, a prefix over the
 // block's waves in LDS.  MULTI == false: every path belongs to job 0.
-template <bool MULTI>
+template <bool MULTI, bool SHARED>
 __global__ __launch_bounds__(kShadeBlock) void k_shade(ShadeArgs a) {
   __shared__ uint32_t wcount[kShadeBlock / 64];
   __shared__ uint32_t block_base;
@@ -818,7 +818,10 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(ShadeArgs a) {
     float4 col = a.in.col[i];
     st = a.in.rng[i];
     uint4 hit = a.hits[i];
+    // J: the path's job (image, outputs); S: its scene, read from the first
+    // job when every job of the step shares one (wave-uniform scalar loads)
     const JobDesc& J = a.jobs[MULTI ? f2u(r1.w) : 0u];
+    const JobDesc& S = SHARED ? a.jobs[0] : J;
     V3 o = mk3(r0.x, r0.y, r0.z), d = mk3(r1.x, r1.y, r1.z);
     V3 color = mk3(col.x, col.y, col.z);
     uint32_t pixel = f2u(r0.w);
@@ -834,13 +837,13 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(ShadeArgs a) {
       normal = d;
       mat = kMatSky;
     } else {
-      const float4* tp = reinterpret_cast<const float4*>(J.tris + hit.x);
+      const float4* tp = reinterpret_cast<const float4*>(S.tris + hit.x);
       V3 nn = mk3(tp[0].w, tp[1].w, tp[2].w);
       pos = muladd3(d, t, o);
       normal = normalized3(nn);
-      mat = J.tri_mat[hit.x];
+      mat = S.tri_mat[hit.x];
       if (gbuffer) {
-        const float* uv = J.tri_uv + 6 * (size_t)hit.x;
+        const float* uv = S.tri_uv + 6 * (size_t)hit.x;
         float bu = u2f(hit.z), bv = u2f(hit.w);
         float w = (1.0f - bu) - bv;
         tu = fmaf(uv[0], w, fmaf(uv[2], bv, uv[4] * bu));
@@ -865,8 +868,8 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(ShadeArgs a) {
     }
     if (mat >> 31) {
       // The path found an emitter: colour x emission (renderer.rs:452-460).
-      V3 em = (J.has_override && hit.x != kNoHit)
-                  ? mk3(J.emission[0], J.emission[1], J.emission[2])
+      V3 em = (S.has_override && hit.x != kNoHit)
+                  ? mk3(S.emission[0], S.emission[1], S.emission[2])
                   : sky_intensity(d);
       V3 c = mulc3(color, em);
       if (J.hdr) {
@@ -877,7 +880,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(ShadeArgs a) {
       }
       if (J.bitmap) J.bitmap[pixel] = pack_color(c);
     } else if (!last || gbuffer) {
-      so = continue_path(J, d, pos, normal, mat, st, a.bounce == 0);
+      so = continue_path(S, d, pos, normal, mat, st, a.bounce == 0);
       if (gbuffer) {
         uint32_t tex_index = ((int32_t)mat >> 24) & 3;
         gbuffer[pixel] = pack_gbuffer(tex_index, tu, tv, so.fresnel);
@@ -1062,13 +1065,16 @@ hipError_t launch_generate(const GenerateArgs& a, uint32_t blocks, hipStream_t s
   return hipGetLastError();
 }
 
-hipError_t launch_shade(const ShadeArgs& a, bool multi, uint32_t max_paths, hipStream_t s) {
+hipError_t launch_shade(const ShadeArgs& a, bool multi, bool shared_scene, uint32_t max_paths,
+                        hipStream_t s) {
   uint32_t blocks = (max_paths + kShadeBlock - 1) / kShadeBlock;
   if (blocks == 0) return hipSuccess;
-  if (multi)
-    hipLaunchKernelGGL(k_shade<true>, dim3(blocks), dim3(kShadeBlock), 0, s, a);
+  if (multi && shared_scene)
+    hipLaunchKernelGGL((k_shade<true, true>), dim3(blocks), dim3(kShadeBlock), 0, s, a);
+  else if (multi)
+    hipLaunchKernelGGL((k_shade<true, false>), dim3(blocks), dim3(kShadeBlock), 0, s, a);
   else
-    hipLaunchKernelGGL(k_shade<false>, dim3(blocks), dim3(kShadeBlock), 0, s, a);
+    hipLaunchKernelGGL((k_shade<false, false>), dim3(blocks), dim3(kShadeBlock), 0, s, a);
   return hipGetLastError();
 }
 

@@ -304,7 +304,7 @@ cv_status launch_step(Pipeline& p, const RunOpts& o, uint32_t slot, uint32_t pat
     sh.jobs = jobs;
     ep = ev_pair(2);
     if (ep) (void)hipEventRecord(ep->first, st);
-    HIPTRY(cvk::launch_shade(sh, o.multi_job, (uint32_t)grid_paths, st), "shade");
+    HIPTRY(cvk::launch_shade(sh, o.multi_job, !multi_scene, (uint32_t)grid_paths, st), "shade");
     if (ep) (void)hipEventRecord(ep->second, st);
   }
   return CV_OK;
@@ -1349,6 +1349,16 @@ void serve_loop(cv_server* sv) {
     o.max_bounces = sv->opts.max_bounces;
     o.packed = packed;
     o.multi_job = true;
+    {
+      // one scene for the whole step: the shared-scene traversal and shading
+      const uint32_t s0 = act[0]->req.scene;
+      bool one = true;
+      for (auto& r : act) one = one && r->req.scene == s0;
+      if (one) {
+        o.nodes = sv->scenes[s0]->nodes.p;
+        o.tris = sv->scenes[s0]->tris.p;
+      }
+    }
     o.max_depth = max_depth;
     o.graph = true;
     o.capture_mu = &sv->hip_mu;
