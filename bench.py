@@ -384,19 +384,27 @@ def bench_denoise(args):
     B = min(16, len(xs))
     losses = []
     # the training pairs stay on the device; a batch is gathered with
-    # device-to-device copies
+    # device-to-device copies.  With --augment A the set also holds each
+    # pair rotated by 90-degree steps and mirrored (the first A of the 8
+    # symmetries of the square crop), all resident on the device.
     xs = np.ascontiguousarray(xs, np.float32)
     ys = np.ascontiguousarray(ys, np.float32)
-    xs_d, ys_d = DeviceBuffer(xs.nbytes), DeviceBuffer(ys.nbytes)
-    bx_d, by_d = DeviceBuffer(xs[0].nbytes * B), DeviceBuffer(ys[0].nbytes * B)
-    hip = xs_d.hip
-    for dst, src in ((xs_d, xs), (ys_d, ys)):
-        hip.hipMemcpy(dst.ptr, src.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(src.nbytes), 1)
+    n_aug = max(1, min(8, args.augment))
     sx, sy = xs[0].nbytes, ys[0].nbytes
+    xs_d, ys_d = DeviceBuffer(xs.nbytes * n_aug), DeviceBuffer(ys.nbytes * n_aug)
+    bx_d, by_d = DeviceBuffer(sx * B), DeviceBuffer(sy * B)
+    hip = xs_d.hip
+    for a in range(n_aug):
+        for dst, src in ((xs_d, xs), (ys_d, ys)):
+            v = np.rot90(src, a % 4, axes=(1, 2))
+            v = np.ascontiguousarray(v[:, :, ::-1] if a >= 4 else v)
+            hip.hipMemcpy(ctypes.c_void_p(dst.ptr.value + a * v.nbytes),
+                          v.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(v.nbytes), 1)
+    n_pairs = len(xs) * n_aug
     cv.device_synchronize()
     t0 = time.perf_counter()
     for it in range(args.train_steps):
-        idx = rng.choice(len(xs), B, replace=False)
+        idx = rng.choice(n_pairs, B, replace=False)
         for k, i in enumerate(idx):
             hip.hipMemcpyAsync(ctypes.c_void_p(bx_d.ptr.value + k * sx),
                                ctypes.c_void_p(xs_d.ptr.value + int(i) * sx), ctypes.c_size_t(sx), 3,
@@ -405,7 +413,10 @@ def bench_denoise(args):
                                ctypes.c_void_p(ys_d.ptr.value + int(i) * sy), ctypes.c_size_t(sy), 3,
                                None)
         cv.device_synchronize()
-        losses.append(d.train_step_device(bx_d.ptr.value, by_d.ptr.value, B, TH, TW, 1e-3))
+        lr = 1e-3
+        if args.lr_schedule == "cosine":
+            lr *= 0.5 * (1.0 + np.cos(np.pi * it / args.train_steps))
+        losses.append(d.train_step_device(bx_d.ptr.value, by_d.ptr.value, B, TH, TW, lr))
         if it % 100 == 0:
             log("denoise: step %d loss %.4f" % (it, losses[-1]))
     train_s = time.perf_counter() - t0
@@ -507,10 +518,12 @@ def bench_denoise(args):
         "vs_baseline": None,
         "dtype": "bf16",
         "data": "synthetic",
-        "config": {"workload": "U-Net 32/64/128, 10 input channels, Adam lr 1e-3, batch %d of "
-                               "%dx%d, %d training pairs (1 / %d spp) from %d procedural scenes; "
-                               "evaluated on %d held-out seeds at %dx%d against %d spp"
-                               % (B, TW, TH, len(xs), args.ref_spp, args.train_scenes,
+        "config": {"workload": "U-Net 32/64/128, 10 input channels, Adam lr 1e-3 (%s), batch %d of "
+                               "%dx%d, %d training pairs (1 / %d spp) from %d procedural scenes "
+                               "x %d crop symmetries; evaluated on %d held-out seeds at %dx%d "
+                               "against %d spp"
+                               % (args.lr_schedule, B, TW, TH, len(xs), args.ref_spp,
+                                  args.train_scenes, n_aug,
                                   args.eval_scenes, W, H, args.eval_ref_spp)},
         "train_loss_first_last": [round(losses[0], 5), round(float(np.mean(losses[-20:])), 5)],
         "evals": evals,
@@ -639,7 +652,12 @@ def main():
                          "loop's latency and throughput")
     ap.add_argument("--jobs", type=int, default=64)
     ap.add_argument("--train-scenes", type=int, default=256)
-    ap.add_argument("--train-steps", type=int, default=3000)
+    ap.add_argument("--train-steps", type=int, default=18000)
+    ap.add_argument("--lr-schedule", choices=("cosine", "constant"), default="cosine",
+                    help="denoise: Adam learning rate 1e-3, decayed to 0 on a half cosine "
+                         "or constant")
+    ap.add_argument("--augment", type=int, default=8,
+                    help="denoise: symmetries of each training crop kept (1-8)")
     ap.add_argument("--ref-spp", type=int, default=1024)
     ap.add_argument("--eval-scenes", type=int, default=8)
     ap.add_argument("--eval-ref-spp", type=int, default=2048)

@@ -43,11 +43,6 @@ __device__ __forceinline__ uint16_t to_bf16(float f) {
 }
 __device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-__device__ __forceinline__ bf8 load8(const uint16_t* p) {
-  uint4 v = *reinterpret_cast<const uint4*>(p);
-  return __builtin_bit_cast(bf8, v);
-}
-__device__ __forceinline__ bf8 zero8() { return __builtin_bit_cast(bf8, make_uint4(0, 0, 0, 0)); }
 
 // ---------------------------------------------------------------------------
 // Implicit-GEMM 3x3 convolution (stride 1, zero padding 1), NHWC.
@@ -56,10 +51,9 @@ __device__ __forceinline__ bf8 zero8() { return __builtin_bit_cast(bf8, make_uin
 // input channels.  Forward: A = input at p + offset(tap), B[k][j] = W[j][k]
 // (weights stored [cout][kpad]).  Backward to the input (flip): A = dZ at
 // p - offset(tap), B[k][j] = Wt[j][k] (stored [cin][kpad], k = tap * cout +
-// co).  One wave computes 16 consecutive pixels x NT*16 outputs; k beyond
-// 9 * kc reads zeros.  Requirements (checked by the host): pixel count a
-// multiple of 64, kc a multiple of 8, channel strides multiples of 8,
-// kpad = roundup(9 * kc, 32).
+// co).  k beyond 9 * kc reads zeros.  Requirements (checked by the host):
+// kc a multiple of 8, channel strides multiples of 8, kpad = roundup(9 *
+// kc, 32).
 // ---------------------------------------------------------------------------
 struct ConvArgs {
   const uint16_t* x;  // A source (bf16), pixel stride xcs
@@ -78,56 +72,7 @@ struct ConvArgs {
   uint32_t npix, h, w_;
 };
 
-template <int NT>
-__global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
-  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const uint32_t p0 = (blockIdx.x * 4 + wave) * 16;
-  if (p0 >= a.npix) return;
-  const uint32_t prow = p0 + (lane & 15);
-  const uint32_t hw = a.h * a.w_;
-  const uint32_t b = prow / hw, rem = prow % hw;
-  const int yy = (int)(rem / a.w_), xx = (int)(rem % a.w_);
-  f4 acc[NT];
-  for (int t = 0; t < NT; t++) acc[t] = f4{0.0f, 0.0f, 0.0f, 0.0f};
-  const uint32_t kvalid = 9 * a.kc;
-  const uint32_t nchunks = a.kpad / 32;
-  for (uint32_t kc32 = 0; kc32 < nchunks; kc32++) {
-    const uint32_t k0 = kc32 * 32 + 8 * (lane >> 4);
-    bf8 av = zero8();
-    if (k0 < kvalid) {
-      const uint32_t tap = k0 / a.kc, c = k0 % a.kc;
-      int dy = (int)(tap / 3) - 1, dx = (int)(tap % 3) - 1;
-      if (a.flip) {
-        dy = -dy;
-        dx = -dx;
-      }
-      const int sy = yy + dy, sx = xx + dx;
-      if (sy >= 0 && sy < (int)a.h && sx >= 0 && sx < (int)a.w_)
-        av = load8(a.x + ((size_t)b * hw + (uint32_t)sy * a.w_ + (uint32_t)sx) * a.xcs + c);
-    }
-    for (int t = 0; t < NT; t++) {
-      const bf8 bv = load8(a.w + (size_t)(t * 16 + (lane & 15)) * a.kpad + k0);
-      acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[t], 0, 0, 0);
-    }
-  }
-  for (int t = 0; t < NT; t++) {
-    const uint32_t j = t * 16 + (lane & 15);
-    if (j >= a.nout) continue;
-    const float bj = a.bias ? a.bias[j] : 0.0f;
-    for (int r = 0; r < 4; r++) {
-      const uint32_t p = p0 + 4 * (lane >> 4) + r;
-      float v = acc[t][r] + bj;
-      if (a.relu) v = fmaxf(v, 0.0f);
-      if (a.y) a.y[(size_t)p * a.ycs + j] = to_bf16(v);
-      if (a.yf) {
-        float* o = a.yf + (size_t)p * a.yfcs + j;
-        *o = a.accumulate ? *o + v : v;
-      }
-    }
-  }
-}
-
-// The same convolution with the operands staged in LDS: a block of 4 waves
+// The convolution with the operands staged in LDS: a block of 4 waves
 // computes 128 pixels x NT*16 outputs, each wave 32 pixels (two 16-row A
 // fragments, so every B fragment feeds two MFMAs).  Per 32-wide K chunk the
 // block gathers the input tile (128 pixels x 32 k, tap offsets and zero
@@ -253,9 +198,8 @@ __global__ __launch_bounds__(256) void k_conv_lds(ConvArgs a) {
 
 // ---------------------------------------------------------------------------
 // Backward to the weights: dW[co][tap * cin + c] += sum_p dZ[p][co] *
-// X[p + offset(tap)][c], and db[co] += sum_p dZ[p][co].  One wave computes a
-// 16 (co) x 64 (k) tile over a range of pixels (split-K over blocks, FP32
-// atomics into dW).  K of the MFMA runs over 32 pixels.
+// X[p + offset(tap)][c], and db[co] += sum_p dZ[p][co]: a GEMM whose K runs
+// over the pixels, split over blocks (split-K).
 // ---------------------------------------------------------------------------
 struct WGradArgs {
   const uint16_t* dz;  // [p][zcs] bf16
@@ -266,190 +210,221 @@ struct WGradArgs {
   float* db;           // [cout] or null
   uint32_t npix, h, w_;
   uint32_t pix_per_block;
+  float* part;         // k_wgrad_t: per pixel split [z][cout * 9 * cin + cout]
 };
 
-__global__ __launch_bounds__(256) void k_conv_wgrad(WGradArgs a) {
-  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const uint32_t ncols = 9 * a.cin;
-  const uint32_t co = blockIdx.y * 16 + (lane & 15);            // A row
-  const uint32_t col0 = (blockIdx.x * 4 + wave) * 64;           // first of 4 column tiles
-  if (col0 >= ncols) return;
-  const uint32_t pbeg = blockIdx.z * a.pix_per_block;
-  const uint32_t pend = min(a.npix, pbeg + a.pix_per_block);
-  const uint32_t hw = a.h * a.w_;
-  f4 acc[4];
-  for (int t = 0; t < 4; t++) acc[t] = f4{0.0f, 0.0f, 0.0f, 0.0f};
-  float bsum = 0.0f;
-  // column (tap, c) of this lane in each tile
-  int cdy[4], cdx[4];
-  uint32_t cc[4];
-  bool cvalid[4];
-  for (int t = 0; t < 4; t++) {
-    const uint32_t j = col0 + t * 16 + (lane & 15);
-    cvalid[t] = j < ncols;
-    const uint32_t tap = cvalid[t] ? j / a.cin : 0, c = cvalid[t] ? j % a.cin : 0;
-    cdy[t] = (int)(tap / 3) - 1;
-    cdx[t] = (int)(tap % 3) - 1;
-    cc[t] = c;
-  }
-  const bool co_ok = co < a.cout;
-  for (uint32_t pk = pbeg; pk < pend; pk += 32) {
-    uint16_t av[8];
-    const uint32_t pb = pk + 8 * (lane >> 4);
-    for (int r = 0; r < 8; r++) {
-      const uint32_t p = pb + r;
-      av[r] = (co_ok && p < pend) ? a.dz[(size_t)p * a.zcs + co] : (uint16_t)0;
+// 8 pixels x 8 bf16 channels (in[i] = pixel i) to 8 channels x 8 pixels
+// (out[c] = channel c of pixels 0..7): halfword moves only.
+__device__ __forceinline__ void transpose8x8(const uint4 (&in)[8], uint4 (&out)[8]) {
+#pragma unroll
+  for (int c = 0; c < 8; c++) {
+    uint32_t o[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const uint32_t lo = (&in[2 * k].x)[c / 2], hi = (&in[2 * k + 1].x)[c / 2];
+      o[k] = (c & 1) ? (lo >> 16) | (hi & 0xffff0000u) : (lo & 0xffffu) | (hi << 16);
     }
-    if (a.db && blockIdx.x == 0 && wave == 0)
-      for (int r = 0; r < 8; r++) bsum += from_bf16(av[r]);
-    bf8 A;
-    for (int r = 0; r < 8; r++) A[r] = __builtin_bit_cast(__bf16, av[r]);
-    for (int t = 0; t < 4; t++) {
-      bf8 B;
-      for (int r = 0; r < 8; r++) {
-        const uint32_t p = pb + r;
-        uint16_t v = 0;
-        if (cvalid[t] && p < pend) {
-          const uint32_t bb = p / hw, rem = p % hw;
-          const int sy = (int)(rem / a.w_) + cdy[t], sx = (int)(rem % a.w_) + cdx[t];
-          if (sy >= 0 && sy < (int)a.h && sx >= 0 && sx < (int)a.w_)
-            v = a.x[((size_t)bb * hw + (uint32_t)sy * a.w_ + (uint32_t)sx) * a.xcs + cc[t]];
-        }
-        B[r] = __builtin_bit_cast(__bf16, v);
-      }
-      acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc[t], 0, 0, 0);
-    }
-  }
-  // D rows are co = blockIdx.y * 16 + 4 * (lane / 16) + r, columns lane % 16
-  for (int t = 0; t < 4; t++) {
-    const uint32_t j = col0 + t * 16 + (lane & 15);
-    if (j >= ncols) continue;
-    for (int r = 0; r < 4; r++) {
-      const uint32_t o = blockIdx.y * 16 + 4 * (lane >> 4) + r;
-      if (o < a.cout) atomicAdd(&a.dw[(size_t)o * ncols + j], acc[t][r]);
-    }
-  }
-  if (a.db && blockIdx.x == 0 && wave == 0) {
-    bsum += __shfl_xor(bsum, 16);
-    bsum += __shfl_xor(bsum, 32);
-    if (lane < 16 && co_ok) atomicAdd(&a.db[co], bsum);
+    out[c] = make_uint4(o[0], o[1], o[2], o[3]);
   }
 }
 
-// Weight gradient with LDS-staged operands.  GEMM M = cout (NM 16-row
-// tiles), N = 64 columns (tap, c) per block, K = pixels: per 32-pixel chunk
-// the block loads dZ (32 pixels x cout) and the shifted input patch (32
-// pixels x 64 columns) with 16-byte loads and stores both transposed in
-// LDS (pixels contiguous), so each MFMA fragment is one 16-byte LDS read.
-// Wave w owns column tile w; blocks split the pixels (FP32 atomics).  The
-// blocks of column tile 0 also reduce db.
+// Weight gradient, transposed in registers.  GEMM M = cout (NM 16-row
+// tiles), N = 128 columns (tap, c) per block, K = pixels in chunks of 64.
+// Waves 0-1 gather the shifted input (64 pixels x 128 columns), waves 2-3
+// dZ (64 pixels x cout): each thread loads an 8-pixel x 8-channel tile with
+// eight 16-byte loads, transposes it in registers and stores eight 16-byte
+// rows (pixels contiguous), so every MFMA fragment is one 16-byte LDS read.
+// Wave w owns column tiles 2w and 2w + 1: 4 * NM MFMAs per chunk.  The next
+// chunk's loads are in flight while the current chunk's MFMAs run (LDS
+// double buffered, one barrier per chunk).  Blocks split the pixels; each
+// block stores its dW tile (and, in column tile 0, its db sums) into its
+// split's partial slice, which k_wgrad_reduce adds up in split order: no
+// atomics, so the gradient is the same on every run.
+__device__ const uint4 kZero16 = {0u, 0u, 0u, 0u};
+constexpr uint32_t kWgPix = 64;   // pixels per chunk
+constexpr uint32_t kWgCols = 128; // columns per block
+constexpr uint32_t kWgRow = 72;   // LDS row: 64 pixels + 8 pad (distinct banks per fragment)
 template <int NM>
-__global__ __launch_bounds__(256) void k_wgrad_lds(WGradArgs a) {
-  constexpr uint32_t CP = NM * 16;        // cout padded
-  constexpr uint32_t AP = 4 * CP;         // A pieces per chunk (32 px x CP / 8)
-  constexpr uint32_t NA = (AP + 255) / 256;
-  __shared__ __attribute__((aligned(16))) uint16_t At[2][CP * kRow];
-  __shared__ __attribute__((aligned(16))) uint16_t Bt[2][64 * kRow];
+__global__ __launch_bounds__(256) void k_wgrad_t(WGradArgs a) {
+  constexpr uint32_t CP = NM * 16;
+  __shared__ __attribute__((aligned(16))) uint16_t At[2][CP * kWgRow];
+  __shared__ __attribute__((aligned(16))) uint16_t Bt[2][kWgCols * kWgRow];
   const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const uint32_t ncols = 9 * a.cin;
-  const uint32_t j0 = blockIdx.x * 64;
   const uint32_t pbeg = blockIdx.z * a.pix_per_block;
   const uint32_t pend = min(a.npix, pbeg + a.pix_per_block);
   const uint32_t hw = a.h * a.w_;
-  // B piece of this thread: pixel tid / 8 of the chunk, columns j0 + (tid % 8) * 8 ..+7
-  const uint32_t bj = j0 + (tid % 8) * 8;
-  const bool bok = bj < ncols;
-  const uint32_t btap = bok ? bj / a.cin : 0, bc = bok ? bj % a.cin : 0;
+  // tile of this thread: pixel group pg (8 pixels), channel group cg
+  const bool is_b = tid < 128;
+  const uint32_t t = is_b ? tid : tid - 128;
+  const uint32_t pg = t % 8, cg = t / 8;  // cg < 16 (B) or < 2 * NM (A)
+  const bool a_on = !is_b && cg < 2 * NM;
+  const uint32_t bj = blockIdx.x * kWgCols + cg * 8;
+  const bool b_on = is_b && bj < ncols;
+  const uint32_t btap = b_on ? bj / a.cin : 0, bc = b_on ? bj % a.cin : 0;
   const int bdy = (int)(btap / 3) - 1, bdx = (int)(btap % 3) - 1;
-  uint4 ra[NA], rb;
-  float bs[NA][8];
+  const bool do_bias = blockIdx.x == 0 && a_on;
+  float* part = a.part + (size_t)blockIdx.z * (a.cout * ncols + a.cout);
+  float bs[8];
 #pragma unroll
-  for (uint32_t i = 0; i < NA; i++)
-#pragma unroll
-    for (int e = 0; e < 8; e++) bs[i][e] = 0.0f;
-  const bool do_bias = a.db && blockIdx.x == 0;
-  auto load_chunk = [&](uint32_t pk) {
-#pragma unroll
-    for (uint32_t i = 0; i < NA; i++) {
-      const uint32_t q = tid + 256 * i;
-      ra[i] = make_uint4(0, 0, 0, 0);
-      if (q < AP) {
-        const uint32_t p = pk + q / NM / 2, g = q % (2 * NM);
-        if (p < pend) ra[i] = *reinterpret_cast<const uint4*>(a.dz + (size_t)p * a.zcs + g * 8);
-      }
-    }
-    rb = make_uint4(0, 0, 0, 0);
-    const uint32_t p = pk + tid / 8;
-    if (bok && p < pend) {
-      const uint32_t b = p / hw, rem = p % hw;
-      const int sy = (int)(rem / a.w_) + bdy, sx = (int)(rem % a.w_) + bdx;
-      if (sy >= 0 && sy < (int)a.h && sx >= 0 && sx < (int)a.w_)
-        rb = *reinterpret_cast<const uint4*>(
-            a.x + ((size_t)b * hw + (uint32_t)sy * a.w_ + (uint32_t)sx) * a.xcs + bc);
-    }
+  for (int e = 0; e < 8; e++) bs[e] = 0.0f;
+  uint4 r[8];
+  // Element offsets are 32-bit (the host checks npix * channels < 2^31).
+  // Every load is issued: a padding pixel reads 16 zero bytes (kZero16), so
+  // no branch joins the loaded registers and the loads stay in flight over
+  // the MFMAs.  When the 8 pixels of a group lie in one row (always when
+  // w % 8 == 0) the shifted row is checked once and each column once.
+  auto pick = [&](bool ok, const uint16_t* q) -> uint4 {
+    return *(ok ? reinterpret_cast<const uint4*>(q) : &kZero16);
   };
-  auto store_chunk = [&](uint32_t buf) {
+  auto load_chunk = [&](uint32_t pk) {
+    const uint32_t p0 = pk + pg * 8;
+    if (a_on) {
+      const uint16_t* src = a.dz + cg * 8;
 #pragma unroll
-    for (uint32_t i = 0; i < NA; i++) {
-      const uint32_t q = tid + 256 * i;
-      if (q < AP) {
-        const uint32_t px = q / NM / 2, g = q % (2 * NM);
-        const uint16_t* v = reinterpret_cast<const uint16_t*>(&ra[i]);
+      for (int i = 0; i < 8; i++) r[i] = pick(p0 + i < pend, src + (p0 + i) * a.zcs);
+    } else if (is_b) {
+      const uint32_t pc = min(p0, pend - 1);
+      const uint32_t b = pc / hw, rem = pc % hw;
+      const uint32_t y = rem / a.w_, x = rem % a.w_;
+      const int sy = (int)y + bdy;
+      if (p0 + 7 < pend && x + 7 < a.w_) {
+        const bool rowok = b_on && sy >= 0 && sy < (int)a.h;
+        const int sx0 = (int)x + bdx;
+        // wraps for sx0 = -1; pixel 0 is then not loaded and pixels 1..7
+        // get their true offsets
+        const uint32_t off0 = (b * hw + (uint32_t)sy * a.w_ + (uint32_t)sx0) * a.xcs + bc;
 #pragma unroll
-        for (int e = 0; e < 8; e++) {
-          At[buf][(g * 8 + e) * kRow + px] = v[e];
-          if (do_bias) bs[i][e] += from_bf16(v[e]);
+        for (int i = 0; i < 8; i++)
+          r[i] = pick(rowok && (uint32_t)(sx0 + i) < a.w_, a.x + (off0 + (uint32_t)i * a.xcs));
+      } else {
+        uint32_t bb = b, yy = y, xx = x;
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+          const int syy = (int)yy + bdy, sxx = (int)xx + bdx;
+          const bool ok = b_on && p0 + i < pend && syy >= 0 && syy < (int)a.h && sxx >= 0 &&
+                          sxx < (int)a.w_;
+          r[i] = pick(ok, a.x + ((bb * hw + (uint32_t)syy * a.w_ + (uint32_t)sxx) * a.xcs + bc));
+          if (++xx == a.w_) {
+            xx = 0;
+            if (++yy == a.h) {
+              yy = 0;
+              bb++;
+            }
+          }
         }
       }
     }
-    const uint16_t* v = reinterpret_cast<const uint16_t*>(&rb);
-#pragma unroll
-    for (int e = 0; e < 8; e++) Bt[buf][((tid % 8) * 8 + e) * kRow + tid / 8] = v[e];
   };
-  f4 acc[NM];
+  auto store_chunk = [&](uint32_t buf) {
+    if (!a_on && !is_b) return;
+    uint4 o[8];
+    transpose8x8(r, o);
+    uint16_t* dst = is_b ? &Bt[buf][(cg * 8) * kWgRow + pg * 8] : &At[buf][(cg * 8) * kWgRow + pg * 8];
 #pragma unroll
-  for (int m = 0; m < NM; m++) acc[m] = f4{0.0f, 0.0f, 0.0f, 0.0f};
-  const uint32_t koff = 8 * (lane >> 4);
+    for (int c = 0; c < 8; c++) *reinterpret_cast<uint4*>(dst + c * kWgRow) = o[c];
+    if (do_bias) {
+#pragma unroll
+      for (int i = 0; i < 8; i++) {
+        const uint32_t* w = &r[i].x;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          bs[2 * k] += from_bf16((uint16_t)(w[k] & 0xffffu));
+          bs[2 * k + 1] += from_bf16((uint16_t)(w[k] >> 16));
+        }
+      }
+    }
+  };
+  f4 acc[NM][2];
+#pragma unroll
+  for (int m = 0; m < NM; m++) acc[m][0] = acc[m][1] = f4{0.0f, 0.0f, 0.0f, 0.0f};
   if (pbeg < pend) {
     load_chunk(pbeg);
     store_chunk(0);
   }
   __syncthreads();
+  const uint32_t row = lane & 15, koff = 8 * (lane >> 4);
   uint32_t buf = 0;
-  for (uint32_t pk = pbeg; pk < pend; pk += 32) {
-    const bool more = pk + 32 < pend;
-    if (more) load_chunk(pk + 32);
-    const bf8 B = *reinterpret_cast<const bf8*>(&Bt[buf][(wave * 16 + (lane & 15)) * kRow + koff]);
+  for (uint32_t pk = pbeg; pk < pend; pk += kWgPix) {
+    const bool more = pk + kWgPix < pend;
+    if (more) load_chunk(pk + kWgPix);
 #pragma unroll
-    for (int m = 0; m < NM; m++) {
-      const bf8 A = *reinterpret_cast<const bf8*>(&At[buf][(m * 16 + (lane & 15)) * kRow + koff]);
-      acc[m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc[m], 0, 0, 0);
+    for (uint32_t ks = 0; ks < kWgPix / 32; ks++) {
+      bf8 B[2];
+#pragma unroll
+      for (int n = 0; n < 2; n++)
+        B[n] = *reinterpret_cast<const bf8*>(&Bt[buf][((wave * 2 + n) * 16 + row) * kWgRow + ks * 32 + koff]);
+#pragma unroll
+      for (int m = 0; m < NM; m++) {
+        const bf8 A = *reinterpret_cast<const bf8*>(&At[buf][(m * 16 + row) * kWgRow + ks * 32 + koff]);
+#pragma unroll
+        for (int n = 0; n < 2; n++)
+          acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[n], acc[m][n], 0, 0, 0);
+      }
     }
     if (more) store_chunk(buf ^ 1u);
     __syncthreads();
     buf ^= 1u;
   }
-  const uint32_t j = j0 + wave * 16 + (lane & 15);
-  if (j < ncols) {
+#pragma unroll
+  for (int n = 0; n < 2; n++) {
+    const uint32_t j = blockIdx.x * kWgCols + (wave * 2 + n) * 16 + row;
+    if (j >= ncols) continue;
 #pragma unroll
     for (int m = 0; m < NM; m++)
 #pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const uint32_t o = m * 16 + 4 * (lane >> 4) + r;
-        if (o < a.cout) atomicAdd(&a.dw[(size_t)o * ncols + j], acc[m][r]);
+      for (int q = 0; q < 4; q++) {
+        const uint32_t o = m * 16 + 4 * (lane >> 4) + q;
+        if (o < a.cout) part[o * ncols + j] = acc[m][n][q];
       }
   }
-  if (do_bias) {
+  // db: the 8 pixel groups of a channel group are lanes 8 cg .. 8 cg + 7
+  if (!is_b) {
 #pragma unroll
-    for (uint32_t i = 0; i < NA; i++) {
-      const uint32_t q = tid + 256 * i;
-      if (q < AP) {
-        const uint32_t g = q % (2 * NM);
-#pragma unroll
-        for (int e = 0; e < 8; e++)
-          if (g * 8 + e < a.cout) atomicAdd(&a.db[g * 8 + e], bs[i][e]);
-      }
+    for (int e = 0; e < 8; e++) {
+      bs[e] += __shfl_xor(bs[e], 1);
+      bs[e] += __shfl_xor(bs[e], 2);
+      bs[e] += __shfl_xor(bs[e], 4);
     }
+    if (do_bias && pg == 0) {
+#pragma unroll
+      for (int e = 0; e < 8; e++)
+        if (cg * 8 + e < a.cout) part[a.cout * ncols + cg * 8 + e] = bs[e];
+    }
+  }
+}
+
+// dW += sum over the pixel splits of the partial slices, and db likewise
+// from the slices' tails.  A block takes 64 consecutive entries; its 16
+// waves take every 16th split (four independent sums each, coalesced
+// 256-byte reads), then wave 0 adds the 16 wave sums: a fixed order.
+__global__ __launch_bounds__(1024) void k_wgrad_reduce(const float* part, uint32_t nsplit,
+                                                       uint32_t nw, uint32_t nb, float* dw,
+                                                       float* db) {
+  __shared__ float red[16][64];
+  const uint32_t lx = threadIdx.x & 63, zg = threadIdx.x >> 6;
+  const uint32_t n = nw + nb, e = blockIdx.x * 64 + lx;
+  float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
+  if (e < n) {
+    uint32_t z = zg;
+    for (; z + 48 < nsplit; z += 64) {
+      s0 += part[(size_t)z * n + e];
+      s1 += part[(size_t)(z + 16) * n + e];
+      s2 += part[(size_t)(z + 32) * n + e];
+      s3 += part[(size_t)(z + 48) * n + e];
+    }
+    for (; z < nsplit; z += 16) s0 += part[(size_t)z * n + e];
+  }
+  red[zg][lx] = (s0 + s1) + (s2 + s3);
+  __syncthreads();
+  if (zg == 0 && e < n) {
+    float s = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 16; k++) s += red[k][lx];
+    if (e < nw)
+      dw[e] += s;
+    else
+      db[e - nw] += s;
   }
 }
 
@@ -767,6 +742,7 @@ struct cv_denoiser {
   DevBuf<float> out, dout, g_e0a, g_cat0, g_p0, g_e1a, g_cat1, g_p1, g_b2a, g_b2b, g_d1a, g_d1b,
       g_d0a, g_d0b, g_up1, g_up0, loss;
   DevBuf<float> in_f32, tgt_f32;
+  DevBuf<float> wpart;  // weight-gradient partials of the pixel splits
   std::mutex mu;
 };
 
@@ -824,6 +800,8 @@ cv_status conv_bwd(cv_denoiser* d, const Conv& L, const uint16_t* x, uint32_t xc
     hipLaunchKernelGGL(cvd::k_relu_grad, dim3(grid1((size_t)npix * L.cout_pad)), dim3(256), 0, s,
                        gy, gycs, y, ycs, L.cout, d->dz.p, L.cout_pad, npix, L.relu ? 1u : 0u);
   HIPTRY(hipGetLastError(), "relu grad");
+  if ((uint64_t)npix * std::max(xcs, L.cout_pad) >= (1ull << 31))
+    return fail(CV_ERR_INVALID_ARGUMENT, "training batch too large for 32-bit offsets");
   cvd::WGradArgs g{};
   g.dz = d->dz.p;
   g.zcs = L.cout_pad;
@@ -836,20 +814,26 @@ cv_status conv_bwd(cv_denoiser* d, const Conv& L, const uint16_t* x, uint32_t xc
   g.npix = npix;
   g.h = h;
   g.w_ = w;
-  // about 2048 blocks: pixel splits of a multiple of 32 pixels
-  const uint32_t col_tiles = (9 * L.cin + 63) / 64;
+  // about 2048 blocks: pixel splits of a multiple of 64 pixels
+  const uint32_t col_tiles = (9 * L.cin + cvd::kWgCols - 1) / cvd::kWgCols;
   uint32_t splits = std::max(1u, std::min(npix / 2048u, 2048u / col_tiles));
-  g.pix_per_block = (npix / splits + 31) / 32 * 32;
+  g.pix_per_block = (npix / splits + cvd::kWgPix - 1) / cvd::kWgPix * cvd::kWgPix;
   splits = (npix + g.pix_per_block - 1) / g.pix_per_block;
+  const uint32_t nw = L.cout * 9 * L.cin;
+  HIPTRY(d->wpart.reserve((size_t)splits * (nw + L.cout)), "alloc");
+  g.part = d->wpart.p;
   dim3 grid(col_tiles, 1, splits);
   switch (L.cout_pad / 16) {
-    case 1: hipLaunchKernelGGL(cvd::k_wgrad_lds<1>, grid, dim3(256), 0, s, g); break;
-    case 2: hipLaunchKernelGGL(cvd::k_wgrad_lds<2>, grid, dim3(256), 0, s, g); break;
-    case 4: hipLaunchKernelGGL(cvd::k_wgrad_lds<4>, grid, dim3(256), 0, s, g); break;
-    case 8: hipLaunchKernelGGL(cvd::k_wgrad_lds<8>, grid, dim3(256), 0, s, g); break;
+    case 1: hipLaunchKernelGGL(cvd::k_wgrad_t<1>, grid, dim3(256), 0, s, g); break;
+    case 2: hipLaunchKernelGGL(cvd::k_wgrad_t<2>, grid, dim3(256), 0, s, g); break;
+    case 4: hipLaunchKernelGGL(cvd::k_wgrad_t<4>, grid, dim3(256), 0, s, g); break;
+    case 8: hipLaunchKernelGGL(cvd::k_wgrad_t<8>, grid, dim3(256), 0, s, g); break;
     default: return fail(CV_ERR_INVALID_ARGUMENT, "unsupported conv width");
   }
   HIPTRY(hipGetLastError(), "wgrad");
+  hipLaunchKernelGGL(cvd::k_wgrad_reduce, dim3((nw + L.cout + 63) / 64), dim3(1024), 0, s,
+                     d->wpart.p, splits, nw, L.cout, g.dw, g.db);
+  HIPTRY(hipGetLastError(), "wgrad reduce");
   if (gx) {
     cvd::ConvArgs a{};
     a.x = d->dz.p;

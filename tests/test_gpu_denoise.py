@@ -65,6 +65,29 @@ def test_gradients_match_torch(cv):
     assert (d.params() == p).all()
 
 
+def test_gradients_split_and_ragged(cv):
+    """3 x 36 x 64 pixels: the full-size weight gradients split the pixels
+    over 3 blocks, and at quarter size (3 x 9 x 16 = 432 pixels) the last
+    64-pixel chunk is partial.  The splits are summed in a fixed order, so a
+    second run gives the same gradient bits (the loss is a float atomic sum,
+    equal to rounding)."""
+    d = cv.Denoiser(9)
+    p = d.params()
+    inp, target = _random_batch(10, nb=3, h=36, w=64)
+    loss = d.train_step(inp, target, 0.0)
+    g = d.grads()
+    assert abs(d.train_step(inp, target, 0.0) - loss) <= 1e-6 * abs(loss)
+    assert d.grads().tobytes() == g.tobytes()
+    rloss, rg, _ = R.loss_and_grads(p, inp, target, bf16=True)
+    assert abs(loss - rloss) <= 2e-3 * abs(rloss)
+    for k, (ws, bs) in enumerate(R.layer_slices()):
+        gw, rw = g[ws], rg[ws]
+        cos = float(np.dot(gw, rw) / (np.linalg.norm(gw) * np.linalg.norm(rw) + 1e-30))
+        print("layer", k, "rel", _rel(gw, rw), "cos", cos, flush=True)
+        assert cos > 0.999 and _rel(gw, rw) < 5e-2
+        assert _rel(g[bs], rg[bs]) < 5e-2
+
+
 def test_adam_first_step(cv):
     """Adam's first step moves every weight by lr * g / (|g| + eps)."""
     d = cv.Denoiser(7)
