@@ -531,6 +531,89 @@ def bench_denoise(args):
     print(json.dumps(out), flush=True)
 
 
+# (cin, cout, level) of the 11 convolutions; level l runs at 1 / 4^l of the pixels
+_DENOISER_CONVS = [(16, 32, 0), (32, 32, 0), (32, 64, 1), (64, 64, 1), (64, 128, 2),
+                   (128, 128, 2), (192, 64, 1), (64, 64, 1), (96, 32, 0), (32, 32, 0), (32, 3, 0)]
+
+
+def _denoiser_flops(npix):
+    """Multiply-adds x 2 of the network's 3x3 convolutions over npix pixels."""
+    return sum(2 * 9 * cin * cout * npix // (4 ** lv) for cin, cout, lv in _DENOISER_CONVS)
+
+
+def bench_denoise_kernels(args):
+    """The denoiser's kernels alone, on device buffers of seeded random
+    values: --steps forwards of the network over one 1280x736 image (the
+    time a denoised frame pays), then --steps Adam steps at batch 16 of
+    256x256 (forward, backward, update).  No rendering, so a kernel trace of
+    this run holds the network's launches only."""
+    import convector_amd as cv
+
+    W, H = 1280, 736
+    B, S = 16, 256
+    rng = np.random.default_rng(5)
+    d = cv.Denoiser(3)
+
+    def upload(a):
+        a = np.ascontiguousarray(a, np.float32)
+        buf = DeviceBuffer(a.nbytes)
+        rc = buf.hip.hipMemcpy(buf.ptr, a.ctypes.data_as(ctypes.c_void_p),
+                               ctypes.c_size_t(a.nbytes), 1)
+        if rc != 0:
+            raise RuntimeError("hipMemcpy failed: %d" % rc)
+        return buf
+
+    inp = rng.normal(0.0, 1.0, (1, H, W, 10)).astype(np.float32)
+    inp[..., 9] = 1.0
+    x_d, y_d = upload(inp), DeviceBuffer(W * H * 12)
+    tin = rng.normal(0.0, 1.0, (B, S, S, 10)).astype(np.float32)
+    tin[..., 9] = 1.0
+    tin_d = upload(tin)
+    ttg_d = upload(np.exp(rng.normal(-1.0, 1.0, (B, S, S, 3))))
+
+    def timed(fn, n):
+        cv.device_synchronize()
+        t = time.perf_counter()
+        for _ in range(n):
+            fn()
+        cv.device_synchronize()
+        return (time.perf_counter() - t) * 1e3 / n
+
+    fwd = lambda: d.forward_device(x_d.ptr.value, y_d.ptr.value, 1, H, W)
+    trn = lambda: d.train_step_device(tin_d.ptr.value, ttg_d.ptr.value, B, S, S, 1e-4)
+    for _ in range(args.warmup):
+        fwd()
+        trn()
+    ms_fwd = timed(fwd, args.steps)
+    log("denoise-kernels: network over %dx%d in %.3f ms" % (W, H, ms_fwd))
+    ms_train = timed(trn, args.steps)
+    log("denoise-kernels: training step (batch %d of %dx%d) in %.3f ms" % (B, S, S, ms_train))
+    flops = _denoiser_flops(W * H)
+    tflops = flops / (ms_fwd * 1e-3) / 1e12
+    out = {
+        "metric": "denoiser network time per 1280x736 frame",
+        "value": round(ms_fwd, 4),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_fwd, 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16",
+        "data": "synthetic",
+        "config": {"workload": "U-Net 32/64/128, 10 input channels, one 1280x736 image; "
+                               "training at batch 16 of 256x256"},
+        "train_ms_per_step": round(ms_train, 4),
+        "roofline": {"bound": "mfma", "achieved": round(tflops, 2), "peak": 2500.0,
+                     "unit": "TFLOP/s", "frac": round(tflops / 2500.0, 4), "traffic": None,
+                     "note": "whole network (%.3g FLOP of 3x3 convolutions) over its wall "
+                             "time, pooling and upsampling included" % flops},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def bench_serve(args):
     """The request loop: 4 resident scenes (about 20k, 200k, 1.3M and 2M
     triangles), Poisson arrivals at each rate, every request a random camera
@@ -645,11 +728,13 @@ def main():
     ap.add_argument("--subdivisions", type=int, default=8)
     ap.add_argument("--rates", type=float, nargs="+", default=[50, 100, 200, 400])
     ap.add_argument("--requests", type=int, default=100)
-    ap.add_argument("--workload", choices=["frame", "batch", "denoise", "serve"], default="frame",
+    ap.add_argument("--workload", choices=["frame", "batch", "denoise", "denoise-kernels", "serve"],
+                    default="frame",
                     help="frame: the large 16-spp 1080p render (default); batch: 64 mixed "
                          "jobs in shared launches against sequential launches; denoise: train "
-                         "the denoiser and measure its equal-RMSE speedup; serve: the request "
-                         "loop's latency and throughput")
+                         "the denoiser and measure its equal-RMSE speedup; denoise-kernels: the "
+                         "network at 1280x736 and training steps at batch 16 of 256x256 on "
+                         "device buffers; serve: the request loop's latency and throughput")
     ap.add_argument("--jobs", type=int, default=64)
     ap.add_argument("--train-scenes", type=int, default=256)
     ap.add_argument("--train-steps", type=int, default=18000)
@@ -667,9 +752,9 @@ def main():
                          "rocprofv3 --pmc counters can be scaled by a copy of known size")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: every box of this pool has a single MI355X and the "
-              "path is built for one GPU (see the sharding notes); --gpus %d is not run."
-              % args.gpus, file=sys.stderr)
+        print("bench.py measures one GPU: the path is built for one GPU's share of the work "
+              "(see the sharding notes in DESIGN.md); --gpus %d is not run." % args.gpus,
+              file=sys.stderr)
         sys.exit(2)
 
     if args.workload == "batch":
@@ -680,6 +765,9 @@ def main():
         return
     if args.workload == "serve":
         bench_serve(args)
+        return
+    if args.workload == "denoise-kernels":
+        bench_denoise_kernels(args)
         return
 
     import convector_amd as cv
