@@ -15,9 +15,17 @@
 //     sort, one workgroup per node and axis);
 //   k_partition reorders the node's triangle references as the reference
 //     collects them (stable by bin) and creates the children.
+// A node of more than kWide triangles is split over chunks of kChunk
+// references: k_bin_chunks bins each chunk into a partial that k_decide
+// adds up (counts and ordered-integer min / max, so the bins are exact),
+// k_decide also turns the chunks' counts into each chunk's first output
+// position per bin, and k_part_chunks scatters the chunks and bounds their
+// children, so that the top levels, where one node holds most triangles,
+// use the whole GPU.
 // Nodes of at most 32 triangles are finished by k_small, one wave per node,
-// with the triangles in the lanes.  k_bottom_up counts leaves and the area
-// ratio statistic; k_layout places every node where crystallize puts it.
+// with the triangles in the lanes.  k_depth and k_bottom_up (one launch per
+// depth, deepest first) count leaves and the area ratio statistic; k_layout
+// places every node where crystallize puts it.
 //
 // The TreeSAH cost uses powf/log2f; their values for every triangle count are
 // tabulated on the host with the C library so that near-ties resolve as in the
@@ -32,7 +40,12 @@ namespace cvk {
 namespace bvh {
 
 constexpr uint32_t kLeaf = 1, kParity = 2, kIsC1 = 4, kPending = 8, kSplit = 16;
+// an id that a small subtree reserved and did not use (no node)
+constexpr uint32_t kHole = 32;
 constexpr uint32_t kBlock = 512;
+constexpr uint32_t kChunk = 2048;   // references per chunk of a wide node
+constexpr uint32_t kWide = 16384;   // nodes above this many triangles are chunked
+constexpr uint32_t kBinWords = 3 * 64 * 7;  // counts, then outer bounds, of 3 x 64 bins
 
 struct INode {
   float box[6];   // outer: min.xyz, max.xyz
@@ -43,8 +56,9 @@ struct INode {
   uint32_t flags;
   uint32_t leaves;
   float sar;
-  uint32_t arrive;
+  uint32_t depth;  // set by k_depth
   uint32_t axis, split_bin, left_count, uniform_mask;
+  uint32_t chunk_base, nchunks;  // chunks of a wide node in the level's chunk list
 };
 
 struct Ctx {
@@ -52,6 +66,8 @@ struct Ctx {
   uint32_t next_large;
   uint32_t small_count;
   uint32_t error;
+  uint32_t next_wide;     // wide nodes in the next level's list
+  uint32_t total_chunks;  // chunks of the current level
   uint32_t max_depth;
   uint32_t max_leaf;
   uint32_t root_box[6];   // okeys
@@ -75,6 +91,13 @@ struct Args {
   const float* tpm1;  // powf(0.8f, log2f(n) - 1.0f)
   uint32_t mode;
   uint32_t ntris;
+  // chunked wide nodes: per chunk the node, the bins (kBinWords), the first
+  // output position per bin of the split axis (64) and the children's
+  // bounds (2 sides x 12 okeys)
+  uint32_t* chunk_node;
+  uint32_t* part;
+  uint32_t* offs;
+  uint32_t* side;
 };
 
 __device__ __forceinline__ uint32_t okey(float f) {
@@ -190,10 +213,84 @@ __global__ void k_root(INode* nodes, Ctx* ctx, uint32_t n) {
   r.flags = 0;
   r.leaves = 0;
   r.sar = 0.0f;
-  r.arrive = 0;
+  r.depth = 0;
   r.axis = r.split_bin = r.left_count = r.uniform_mask = 0;
+  r.chunk_base = r.nchunks = 0;
   nodes[0] = r;
   ctx->node_count = 1;
+}
+
+// Centroid bins of refs[0, n) on the axes not in umask (bin_triangles,
+// bvh.rs:149-156): counts and outer bounds as ordered integers in LDS.
+__device__ __forceinline__ void bin_centroids(const Args& a, const uint32_t* refs, uint32_t n,
+                                              const float* cmn, const float* csz, uint32_t umask,
+                                              uint32_t (*cnt)[64], uint32_t (*bx)[64][6]) {
+  for (uint32_t e = threadIdx.x; e < n; e += kBlock) {
+    uint32_t r = refs[e];
+    float4 A = a.A[r], B = a.B[r];
+    float cz = a.C[r];
+    float c[3] = {A.w, B.w, cz};
+    uint32_t kmn[3] = {okey(A.x), okey(A.y), okey(A.z)};
+    uint32_t kmx[3] = {okey(B.x), okey(B.y), okey(B.z)};
+    for (uint32_t axis = 0; axis < 3; axis++) {
+      if ((umask >> axis) & 1u) continue;
+      uint32_t b = centroid_bin(c[axis], cmn[axis], csz[axis]);
+      atomicAdd(&cnt[axis][b], 1u);
+      for (int j = 0; j < 3; j++) {
+        atomicMin(&bx[axis][b][j], kmn[j]);
+        atomicMax(&bx[axis][b][3 + j], kmx[j]);
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ void clear_bins(uint32_t (*cnt)[64], uint32_t (*bx)[64][6]) {
+  for (uint32_t k = threadIdx.x; k < 3 * 64; k += kBlock) {
+    (&cnt[0][0])[k] = 0;
+    for (int j = 0; j < 6; j++) (&bx[0][0][0])[k * 6 + j] = j < 3 ? 0xffffffffu : 0u;
+  }
+}
+
+// Chunk list of the level's wide nodes (one thread; only levels that have
+// wide nodes run it, and those lists are short).
+__global__ void k_wide_plan(Args a) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  uint32_t total = 0;
+  for (uint32_t i = 0; i < a.nlist; i++) {
+    const uint32_t id = a.list[i];
+    INode* nd = &a.nodes[id];
+    const uint32_t n = nd->count;
+    const uint32_t nc = n > kWide ? (n + kChunk - 1) / kChunk : 0u;
+    nd->chunk_base = total;
+    nd->nchunks = nc;
+    for (uint32_t c = 0; c < nc; c++) a.chunk_node[total + c] = id;
+    total += nc;
+  }
+  a.ctx->total_chunks = total;
+}
+
+// Centroid bins of one chunk of a wide node into its partial.
+__global__ __launch_bounds__(kBlock) void k_bin_chunks(Args a) {
+  const uint32_t c = blockIdx.x;
+  if (c >= a.ctx->total_chunks) return;
+  const INode* nd = &a.nodes[a.chunk_node[c]];
+  const uint32_t lo = (c - nd->chunk_base) * kChunk;
+  const uint32_t n = min(nd->count, lo + kChunk) - lo;
+  const uint32_t* refs = a.buf[(nd->flags & kParity) ? 1 : 0] + nd->begin + lo;
+  __shared__ uint32_t cnt[3][64];
+  __shared__ uint32_t bx[3][64][6];
+  clear_bins(cnt, bx);
+  __syncthreads();
+  float cmn[3], csz[3];
+  for (int k = 0; k < 3; k++) {
+    cmn[k] = nd->cbox[k];
+    csz[k] = nd->cbox[3 + k] - cmn[k];
+  }
+  bin_centroids(a, refs, n, cmn, csz, 0u, cnt, bx);
+  __syncthreads();
+  uint32_t* out = a.part + (size_t)c * kBinWords;
+  for (uint32_t k = threadIdx.x; k < 3 * 64; k += kBlock) out[k] = (&cnt[0][0])[k];
+  for (uint32_t k = threadIdx.x; k < 3 * 64 * 6; k += kBlock) out[192 + k] = (&bx[0][0][0])[k];
 }
 
 // ---------------------------------------------------------------------------
@@ -216,33 +313,55 @@ __global__ __launch_bounds__(kBlock) void k_decide(Args a) {
   __shared__ uint32_t bx[3][64][6];
   __shared__ float cost[3][64];
   __shared__ uint32_t valid_axes;
+  __shared__ int s_axis;
+  __shared__ uint32_t s_bin;
   const uint32_t tid = threadIdx.x;
-  for (uint32_t k = tid; k < 3 * 64; k += kBlock) {
-    (&cnt[0][0])[k] = 0;
-    for (int j = 0; j < 6; j++) (&bx[0][0][0])[k * 6 + j] = j < 3 ? 0xffffffffu : 0u;
-  }
+  const uint32_t nch = nd->nchunks, cbase = nd->chunk_base;
+  clear_bins(cnt, bx);
   __syncthreads();
   float cmn[3], csz[3];
   for (int k = 0; k < 3; k++) {
     cmn[k] = nd->cbox[k];
     csz[k] = nd->cbox[3 + k] - cmn[k];
   }
-  for (uint32_t e = tid; e < n; e += kBlock) {
-    uint32_t r = refs[e];
-    float4 A = a.A[r], B = a.B[r];
-    float cz = a.C[r];
-    float c[3] = {A.w, B.w, cz};
-    uint32_t kmn[3] = {okey(A.x), okey(A.y), okey(A.z)};
-    uint32_t kmx[3] = {okey(B.x), okey(B.y), okey(B.z)};
-    for (uint32_t axis = 0; axis < 3; axis++) {
+  if (nch) {
+    // a wide node: its chunks' partials added up (integer sums and ordered
+    // min / max: any order gives the same bins; 16 loads are issued before
+    // they are combined, so that the latency of the partials' reads, written
+    // by other XCDs, is paid once per 16 chunks)
+    const uint32_t* part = a.part + (size_t)cbase * kBinWords;
+    for (uint32_t k = tid; k < kBinWords; k += kBlock) {
+      const uint32_t axis = k < 192 ? k / 64 : (k - 192) / 384;
       if ((umask >> axis) & 1u) continue;
-      uint32_t b = centroid_bin(c[axis], cmn[axis], csz[axis]);
-      atomicAdd(&cnt[axis][b], 1u);
-      for (int j = 0; j < 3; j++) {
-        atomicMin(&bx[axis][b][j], kmn[j]);
-        atomicMax(&bx[axis][b][3 + j], kmx[j]);
+      const uint32_t* q = part + k;
+      const int op = k < 192 ? 0 : ((k - 192) % 6) < 3 ? 1 : 2;  // sum, min, max
+      const uint32_t id0 = op == 1 ? 0xffffffffu : 0u;
+      uint32_t v0 = id0, v1 = id0, v2 = id0, v3 = id0;
+      auto acc = [&](uint32_t v, uint32_t w) {
+        return op == 0 ? v + w : op == 1 ? min(v, w) : max(v, w);
+      };
+      uint32_t c = 0;
+      for (; c + 16 <= nch; c += 16) {
+        uint32_t w[16];
+#pragma unroll
+        for (uint32_t j = 0; j < 16; j++) w[j] = q[(size_t)(c + j) * kBinWords];
+#pragma unroll
+        for (uint32_t j = 0; j < 16; j += 4) {
+          v0 = acc(v0, w[j]);
+          v1 = acc(v1, w[j + 1]);
+          v2 = acc(v2, w[j + 2]);
+          v3 = acc(v3, w[j + 3]);
+        }
       }
+      for (; c < nch; c++) v0 = acc(v0, q[(size_t)c * kBinWords]);
+      const uint32_t v = acc(acc(v0, v1), acc(v2, v3));
+      if (k < 192)
+        (&cnt[0][0])[k] = v;
+      else
+        (&bx[0][0][0])[k - 192] = v;
     }
+  } else {
+    bin_centroids(a, refs, n, cmn, csz, umask, cnt, bx);
   }
   if (umask) {
     // bin_triangles_uniform (bvh.rs:176-201): chunks of ceil(n/64) of the
@@ -351,6 +470,7 @@ __global__ __launch_bounds__(kBlock) void k_decide(Args a) {
       }
     }
     uint32_t f = flags & ~kPending;
+    s_axis = -1;
     if (best_axis < 0) {
       atomicOr(&a.ctx->error, 2u);  // assert!(!is_first), bvh.rs:303
       f |= kLeaf;
@@ -364,33 +484,65 @@ __global__ __launch_bounds__(kBlock) void k_decide(Args a) {
       nd->left_count = lc;
       if (a.mode == 0) nd->uniform_mask = 0;
       f |= kSplit;
+      s_axis = best_axis;
+      s_bin = best_i;
     }
     nd->flags = f;
+  }
+  if (!nch) return;
+  __syncthreads();
+  // a wide node split on centroid bins: where each chunk's references of
+  // each bin start in the stably sorted segment (bins in order, chunks in
+  // order within a bin)
+  const int ax = s_axis;
+  if (ax < 0 || ((umask >> ax) & 1u) || tid >= 64) return;
+  uint32_t run = 0;
+  for (uint32_t j = 0; j < tid; j++) run += cnt[ax][j];
+  const uint32_t* part = a.part + (size_t)cbase * kBinWords + (uint32_t)ax * 64 + tid;
+  uint32_t* offs = a.offs + (size_t)cbase * 64 + tid;
+  // 16 chunks' counts are loaded before they are summed, so that the scan
+  // does not wait for one load per chunk
+  for (uint32_t c0 = 0; c0 < nch; c0 += 16) {
+    uint32_t v[16];
+#pragma unroll
+    for (uint32_t j = 0; j < 16; j++) v[j] = c0 + j < nch ? part[(size_t)(c0 + j) * kBinWords] : 0u;
+#pragma unroll
+    for (uint32_t j = 0; j < 16; j++) {
+      if (c0 + j < nch) offs[(size_t)(c0 + j) * 64] = run;
+      run += v[j];
+    }
   }
 }
 
 // ---------------------------------------------------------------------------
 // Stable counting-sort pass of a segment inside one workgroup: elements are
 // taken in tiles of kBlock in order; within a tile a lane's rank among equal
-// digits comes from ballots, across waves from LDS counts.
+// digits comes from ballots, across waves from LDS counts.  Digit d starts
+// at start[d] when start is given (a chunk of a larger sort), else after
+// the segment's smaller digits.
 // ---------------------------------------------------------------------------
 
 template <int BITS, typename DigitFn, typename LoadFn>
 __device__ __forceinline__ void wg_stable_pass(uint32_t n, LoadFn load, DigitFn digit_of,
-                                               uint32_t* dst, uint32_t* hist, uint32_t* wcount) {
+                                               uint32_t* dst, uint32_t* hist, uint32_t* wcount,
+                                               const uint32_t* start = nullptr) {
   constexpr uint32_t R = 1u << BITS;
   constexpr uint32_t W = kBlock / 64;
   const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  for (uint32_t k = tid; k < R; k += kBlock) hist[k] = 0;
-  __syncthreads();
-  for (uint32_t e = tid; e < n; e += kBlock) atomicAdd(&hist[digit_of(load(e))], 1u);
-  __syncthreads();
-  if (tid == 0) {
-    uint32_t s = 0;
-    for (uint32_t k = 0; k < R; k++) {
-      uint32_t c = hist[k];
-      hist[k] = s;
-      s += c;
+  if (start) {
+    for (uint32_t k = tid; k < R; k += kBlock) hist[k] = start[k];
+  } else {
+    for (uint32_t k = tid; k < R; k += kBlock) hist[k] = 0;
+    __syncthreads();
+    for (uint32_t e = tid; e < n; e += kBlock) atomicAdd(&hist[digit_of(load(e))], 1u);
+    __syncthreads();
+    if (tid == 0) {
+      uint32_t s = 0;
+      for (uint32_t k = 0; k < R; k++) {
+        uint32_t c = hist[k];
+        hist[k] = s;
+        s += c;
+      }
     }
   }
   for (uint32_t k = tid; k < W * R; k += kBlock) wcount[k] = 0;
@@ -452,40 +604,17 @@ __global__ __launch_bounds__(kBlock) void k_sort(Args a) {
   }
 }
 
-// Reorders the segment as split() collects the children (bins in order,
-// references in push order) and creates the children.
-__global__ __launch_bounds__(kBlock) void k_partition(Args a) {
-  const uint32_t id = a.list[blockIdx.x];
-  INode* nd = &a.nodes[id];
-  const uint32_t flags = nd->flags;
-  if (!(flags & kSplit)) return;
-  const uint32_t n = nd->count, begin = nd->begin;
-  const uint32_t par = (flags & kParity) ? 1u : 0u;
-  const uint32_t* src = a.buf[par] + begin;
-  uint32_t* dst = a.buf[par ^ 1u] + begin;
-  const uint32_t axis = nd->axis, split = nd->split_bin, L = nd->left_count;
-  const bool uniform = (nd->uniform_mask >> axis) & 1u;
-  __shared__ uint32_t hist[64];
-  __shared__ uint32_t wcount[(kBlock / 64) * 64];
-  __shared__ uint32_t cb[2][12];  // per side: outer min.xyz max.xyz, barycenter min.xyz max.xyz
-  const uint32_t tid = threadIdx.x;
-  if (tid < 24) (&cb[0][0])[tid] = ((tid % 12) % 6) < 3 ? 0xffffffffu : 0u;
-  __syncthreads();
-  const float mn = nd->cbox[axis], size = nd->cbox[3 + axis] - mn;
-  auto bin_of = [&](uint32_t r) { return centroid_bin(centroid(a, r, axis), mn, size); };
-  if (uniform) {
-    const uint32_t* srt = a.sorted[axis] + begin;
-    for (uint32_t s = tid; s < n; s += kBlock) dst[s] = srt[s];
-  } else {
-    auto load = [&](uint32_t e) { return src[e]; };
-    wg_stable_pass<6>(n, load, bin_of, dst, hist, wcount);
-  }
-  // children bounds
-  const uint32_t* order = uniform ? a.sorted[axis] + begin : src;
+// Outer and barycenter bounds of the two children over order[lo, hi) (side
+// 0: position below L when uniform, else bin below split), reduced into
+// cb (LDS, initialised by the caller) with one atomic per wave and value.
+template <typename BinFn>
+__device__ __forceinline__ void side_bounds(const Args& a, const uint32_t* order, uint32_t lo,
+                                            uint32_t hi, bool uniform, uint32_t L, uint32_t split,
+                                            BinFn bin_of, uint32_t (*cb)[12]) {
   uint32_t lk[2][12];
   for (int s = 0; s < 2; s++)
     for (int k = 0; k < 12; k++) lk[s][k] = (k % 6) < 3 ? 0xffffffffu : 0u;
-  for (uint32_t e = tid; e < n; e += kBlock) {
+  for (uint32_t e = lo + threadIdx.x; e < hi; e += kBlock) {
     uint32_t r = order[e];
     int side = uniform ? (e < L ? 0 : 1) : (bin_of(r) < split ? 0 : 1);
     float4 A = a.A[r], B = a.B[r];
@@ -512,7 +641,7 @@ __global__ __launch_bounds__(kBlock) void k_partition(Args a) {
         uint32_t o = __shfl_xor(v, off);
         v = (k % 6) < 3 ? min(v, o) : max(v, o);
       }
-      if ((tid & 63) == 0) {
+      if ((threadIdx.x & 63) == 0) {
         if ((k % 6) < 3)
           atomicMin(&cb[s][k], v);
         else
@@ -520,6 +649,97 @@ __global__ __launch_bounds__(kBlock) void k_partition(Args a) {
       }
     }
   __syncthreads();
+}
+
+// One chunk of a wide node that splits: its references go to the positions
+// k_decide computed (a stable sort by bin, or the sorted order's copy for a
+// uniform axis), and its part of the children's bounds to its side partial.
+__global__ __launch_bounds__(kBlock) void k_part_chunks(Args a) {
+  const uint32_t c = blockIdx.x;
+  if (c >= a.ctx->total_chunks) return;
+  const INode* nd = &a.nodes[a.chunk_node[c]];
+  if (!(nd->flags & kSplit)) return;
+  const uint32_t n = nd->count, begin = nd->begin;
+  const uint32_t lo = (c - nd->chunk_base) * kChunk, hi = min(n, lo + kChunk);
+  const uint32_t par = (nd->flags & kParity) ? 1u : 0u;
+  const uint32_t* src = a.buf[par] + begin;
+  uint32_t* dst = a.buf[par ^ 1u] + begin;
+  const uint32_t axis = nd->axis, split = nd->split_bin, L = nd->left_count;
+  const bool uniform = (nd->uniform_mask >> axis) & 1u;
+  __shared__ uint32_t hist[64];
+  __shared__ uint32_t wcount[(kBlock / 64) * 64];
+  __shared__ uint32_t cb[2][12];
+  const uint32_t tid = threadIdx.x;
+  if (tid < 24) (&cb[0][0])[tid] = ((tid % 12) % 6) < 3 ? 0xffffffffu : 0u;
+  __syncthreads();
+  const float mn = nd->cbox[axis], size = nd->cbox[3 + axis] - mn;
+  auto bin_of = [&](uint32_t r) { return centroid_bin(centroid(a, r, axis), mn, size); };
+  if (uniform) {
+    const uint32_t* srt = a.sorted[axis] + begin;
+    for (uint32_t s = lo + tid; s < hi; s += kBlock) dst[s] = srt[s];
+  } else {
+    auto load = [&](uint32_t e) { return src[lo + e]; };
+    wg_stable_pass<6>(hi - lo, load, bin_of, dst, hist, wcount, a.offs + (size_t)c * 64);
+  }
+  side_bounds(a, uniform ? a.sorted[axis] + begin : src, lo, hi, uniform, L, split, bin_of, cb);
+  if (tid < 24) a.side[(size_t)c * 24 + tid] = (&cb[0][0])[tid];
+}
+
+// Reorders the segment as split() collects the children (bins in order,
+// references in push order) and creates the children.
+__global__ __launch_bounds__(kBlock) void k_partition(Args a) {
+  const uint32_t id = a.list[blockIdx.x];
+  INode* nd = &a.nodes[id];
+  const uint32_t flags = nd->flags;
+  if (!(flags & kSplit)) return;
+  const uint32_t n = nd->count, begin = nd->begin;
+  const uint32_t par = (flags & kParity) ? 1u : 0u;
+  const uint32_t* src = a.buf[par] + begin;
+  uint32_t* dst = a.buf[par ^ 1u] + begin;
+  const uint32_t axis = nd->axis, split = nd->split_bin, L = nd->left_count;
+  const bool uniform = (nd->uniform_mask >> axis) & 1u;
+  __shared__ uint32_t hist[64];
+  __shared__ uint32_t wcount[(kBlock / 64) * 64];
+  __shared__ uint32_t cb[2][12];  // per side: outer min.xyz max.xyz, barycenter min.xyz max.xyz
+  const uint32_t tid = threadIdx.x;
+  if (tid < 24) (&cb[0][0])[tid] = ((tid % 12) % 6) < 3 ? 0xffffffffu : 0u;
+  __syncthreads();
+  const float mn = nd->cbox[axis], size = nd->cbox[3 + axis] - mn;
+  auto bin_of = [&](uint32_t r) { return centroid_bin(centroid(a, r, axis), mn, size); };
+  if (nd->nchunks) {
+    // a wide node: k_part_chunks reordered it; its chunks' bounds are added
+    // up here
+    // (ordered min / max: any order gives the same bounds; 16 waves' lanes
+    // take every 16th chunk, then wave 0 combines them)
+    const uint32_t* sp = a.side + (size_t)nd->chunk_base * 24;
+    __shared__ uint32_t red[kBlock / 32][24];
+    const uint32_t k = tid % 24, g = tid / 24;  // 21 groups of 24 lanes
+    const bool lo = ((k % 12) % 6) < 3;
+    if (g < kBlock / 32) {
+      uint32_t v = lo ? 0xffffffffu : 0u;
+      for (uint32_t c = g; c < nd->nchunks; c += kBlock / 32) {
+        const uint32_t w = sp[(size_t)c * 24 + k];
+        v = lo ? min(v, w) : max(v, w);
+      }
+      red[g][k] = v;
+    }
+    __syncthreads();
+    if (tid < 24) {
+      uint32_t v = lo ? 0xffffffffu : 0u;
+      for (uint32_t j = 0; j < kBlock / 32; j++) v = lo ? min(v, red[j][tid]) : max(v, red[j][tid]);
+      (&cb[0][0])[tid] = v;
+    }
+    __syncthreads();
+  } else {
+    if (uniform) {
+      const uint32_t* srt = a.sorted[axis] + begin;
+      for (uint32_t s = tid; s < n; s += kBlock) dst[s] = srt[s];
+    } else {
+      auto load = [&](uint32_t e) { return src[e]; };
+      wg_stable_pass<6>(n, load, bin_of, dst, hist, wcount);
+    }
+    side_bounds(a, uniform ? a.sorted[axis] + begin : src, 0, n, uniform, L, split, bin_of, cb);
+  }
   if (tid == 0) {
     uint32_t c = atomicAdd(&a.ctx->node_count, 2u);
     INode ch[2];
@@ -535,8 +755,9 @@ __global__ __launch_bounds__(kBlock) void k_partition(Args a) {
       ch[s].flags = (par ^ 1u) ? kParity : 0u;
       ch[s].leaves = 0;
       ch[s].sar = 0.0f;
-      ch[s].arrive = 0;
+      ch[s].depth = 0;
       ch[s].axis = ch[s].split_bin = ch[s].left_count = ch[s].uniform_mask = 0;
+      ch[s].chunk_base = ch[s].nchunks = 0;
     }
     // smaller area first (bvh.rs:337-345)
     bool swap = area6(ch[0].box) > area6(ch[1].box);
@@ -549,6 +770,7 @@ __global__ __launch_bounds__(kBlock) void k_partition(Args a) {
       if (ch[s].count > 32) {
         uint32_t slot = atomicAdd(&a.ctx->next_large, 1u);
         a.next_list[slot] = cid;
+        if (ch[s].count > kWide) atomicAdd(&a.ctx->next_wide, 1u);
       } else if (ch[s].count >= 2) {
         uint32_t slot = atomicAdd(&a.ctx->small_count, 1u);
         a.small_list[slot] = cid;
@@ -563,23 +785,44 @@ __global__ __launch_bounds__(kBlock) void k_partition(Args a) {
 // k_small: one wave finishes the subtree of a node of 2..32 triangles.  With
 // at most 32 triangles every split uses sorted single-triangle bins
 // (bvh.rs:145-147), so a node is three sorts of the lanes and two scans.
+// A sort ranks each lane of the node's segment [off, off + cnt) by its key
+// (ties by lane: the stable order) with cnt readlanes, and one ds_permute
+// sends every lane to its rank; lanes outside the segment keep their place.
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ uint64_t bitonic64(uint64_t key, uint32_t lane) {
-  for (uint32_t k = 2; k <= 64; k <<= 1)
-    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-      uint64_t other = __shfl_xor(key, (int)j);
-      bool up = (lane & k) == 0;
-      bool lower = (lane & j) == 0;
-      uint64_t lo = key < other ? key : other;
-      uint64_t hi = key < other ? other : key;
-      key = (lower == up) ? lo : hi;
+__device__ __forceinline__ uint32_t stable_rank_src(uint32_t key, bool inside, uint32_t off,
+                                                    uint32_t cnt, uint32_t lane) {
+  uint32_t rank = lane;
+  if (inside) {
+    rank = off;
+    for (uint32_t j = off; j < off + cnt; j++) {
+      const uint32_t kj = __builtin_amdgcn_readlane(key, j);
+      rank += (kj < key || (kj == key && j < lane)) ? 1u : 0u;
     }
-  return key;
+  }
+  // lane `rank` receives this lane's index: the source of each sorted position
+  return (uint32_t)__builtin_amdgcn_ds_permute((int)(rank * 4), (int)lane);
+}
+
+// tree_sah with the power terms of counts up to 32 from LDS
+__device__ __forceinline__ float tree_sah_small(const float (*tp)[33], float parent_area,
+                                                const float* b, uint32_t n) {
+  const float p = 0.8f;
+  float ac_ap = area6(b) / parent_area;
+  float nf = (float)n;
+  float aabb_term = 1.0f + (ac_ap * (2.0f * p - nf * tp[0][n])) / (p - (2.0f * p) * p);
+  float tri_term = (nf * tp[1][n]) * ac_ap;
+  return aabb_term * 40.0f + tri_term * 120.0f;
 }
 
 __global__ __launch_bounds__(256) void k_small(Args a) {
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __shared__ float tp[2][33];
+  if (threadIdx.x < 66) {
+    const uint32_t k = threadIdx.x % 33;
+    tp[threadIdx.x / 33][k] = k <= a.ntris ? (threadIdx.x < 33 ? a.tpm[k] : a.tpm1[k]) : 0.0f;
+  }
+  __syncthreads();
   const uint32_t si = blockIdx.x * 4 + wave;
   if (si >= a.ctx->small_count) return;
   const uint32_t root = a.small_list[si];
@@ -597,6 +840,13 @@ __global__ __launch_bounds__(256) void k_small(Args a) {
     bmx[0] = B.x; bmx[1] = B.y; bmx[2] = B.z;
     c[0] = A.w; c[1] = B.w; c[2] = a.C[ref];
   }
+  // The subtree has at most n - 1 inner nodes, so its nodes take ids from
+  // 2 (n - 1) reserved with one atomic (one counter for every wave would
+  // serialise them); ids left over are written as holes at the end.
+  uint32_t ids = 0;
+  if (lane == 0) ids = atomicAdd(&a.ctx->node_count, 2u * (n - 1u));
+  ids = __shfl(ids, 0);
+  const uint32_t ids_end = ids + 2u * (n - 1u);
   // Pending nodes: off, cnt, id, parent, base flags, outer box.  Every node
   // record of the subtree is written once, when its node is popped.
   __shared__ uint32_t stk[4][40][11];
@@ -631,8 +881,9 @@ __global__ __launch_bounds__(256) void k_small(Args a) {
       w.flags = flags;
       w.leaves = 0;
       w.sar = 0.0f;
-      w.arrive = 0;
+      w.depth = 0;
       w.axis = w.split_bin = w.left_count = w.uniform_mask = 0;
+      w.chunk_base = w.nchunks = 0;
       a.nodes[id] = w;
     };
     if (cnt <= 1) {
@@ -646,13 +897,7 @@ __global__ __launch_bounds__(256) void k_small(Args a) {
     uint32_t best_i = 0;
     float best = 0.0f;
     for (uint32_t axis = 0; axis < 3; axis++) {
-      uint64_t key;
-      if (inside)
-        key = (1ull << 40) | ((uint64_t)sort_key(c[axis]) << 8) | lane;
-      else
-        key = lane < off ? (uint64_t)lane : ((2ull << 40) | lane);
-      key = bitonic64(key, lane);
-      uint32_t src = (uint32_t)(key & 0xffu);
+      const uint32_t src = stable_rank_src(sort_key(c[axis]), inside, off, cnt, lane);
       perm[axis] = src;
       float s[6];
       for (int k = 0; k < 3; k++) {
@@ -665,19 +910,21 @@ __global__ __launch_bounds__(256) void k_small(Args a) {
         pre[k] = inside ? s[k] : idv;
         suf[k] = pre[k];
       }
-      for (uint32_t d = 1; d < 64; d <<= 1)
+      // the segment lies in lanes 0..31: scans over 32 lanes
+      for (uint32_t d = 1; d < 32; d <<= 1)
         for (int k = 0; k < 6; k++) {
           float up = __shfl_up(pre[k], d);
           float dn = __shfl_down(suf[k], d);
           if (lane >= d) pre[k] = k < 3 ? tmin_f(pre[k], up) : tmax_f(pre[k], up);
-          if (lane + d < 64) suf[k] = k < 3 ? tmin_f(suf[k], dn) : tmax_f(suf[k], dn);
+          if (lane + d < 32) suf[k] = k < 3 ? tmin_f(suf[k], dn) : tmax_f(suf[k], dn);
         }
       float rsuf[6];
       for (int k = 0; k < 6; k++) rsuf[k] = __shfl_down(suf[k], 1);
       float cst = 0.0f;
       if (lane >= off && lane + 1 < off + cnt) {
         uint32_t lc = lane - off + 1;
-        cst = tree_sah(a, parent_area, pre, lc) + tree_sah(a, parent_area, rsuf, cnt - lc);
+        cst = tree_sah_small(tp, parent_area, pre, lc) +
+              tree_sah_small(tp, parent_area, rsuf, cnt - lc);
       }
       // first strictly smallest position; then strictly smaller than earlier axes
       int abest = -1;
@@ -720,9 +967,8 @@ __global__ __launch_bounds__(256) void k_small(Args a) {
         cbx[side][k] = v;
       }
     }
-    uint32_t cid = 0;
-    if (lane == 0) cid = atomicAdd(&a.ctx->node_count, 2u);
-    cid = __shfl(cid, 0);
+    const uint32_t cid = ids;
+    ids += 2;
     // smaller area first (bvh.rs:337-345)
     const bool swap = area6(cbx[0]) > area6(cbx[1]);
     write_node((bflags & ~kLeaf) | kSplit, (int32_t)(swap ? cid + 1 : cid),
@@ -742,41 +988,50 @@ __global__ __launch_bounds__(256) void k_small(Args a) {
     sp += 2;
   }
   if (lane < n) seg[lane] = ref;
+  for (uint32_t h = ids + lane; h < ids_end; h += 64) {
+    INode w{};
+    w.flags = kHole;
+    w.parent = -1;
+    w.child[0] = w.child[1] = -1;
+    a.nodes[h] = w;
+  }
 }
 
 // ---------------------------------------------------------------------------
 // Statistics and layout
 // ---------------------------------------------------------------------------
 
-// Leaf counts and summed_area_ratio (bvh.rs:380-386) bottom-up: the second
-// child to arrive at a parent completes it.
-__global__ __launch_bounds__(256) void k_bottom_up(Args a, uint32_t ninodes) {
-  uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+// Depth of every node (a walk to the root), the tree's depth, and the
+// leaves' statistics (one leaf, summed_area_ratio 0 + 0).
+__global__ __launch_bounds__(256) void k_depth(Args a, uint32_t ninodes) {
+  const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= ninodes) return;
   INode* nodes = a.nodes;
-  if (!(nodes[x].flags & kLeaf)) return;
-  __hip_atomic_store(&nodes[x].leaves, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __hip_atomic_store(reinterpret_cast<uint32_t*>(&nodes[x].sar), f2u(0.0f + 0.0f),
-                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  for (;;) {
-    int32_t p = nodes[x].parent;
-    if (p < 0) break;
-    __threadfence();
-    uint32_t old = atomicAdd(&nodes[p].arrive, 1u);
-    if (old == 0) break;
-    __threadfence();
-    int32_t c0 = nodes[p].child[0], c1 = nodes[p].child[1];
-    uint32_t l0 = load_coherent(&nodes[c0].leaves), l1 = load_coherent(&nodes[c1].leaves);
-    float s0 = u2f(load_coherent(reinterpret_cast<const uint32_t*>(&nodes[c0].sar)));
-    float s1 = u2f(load_coherent(reinterpret_cast<const uint32_t*>(&nodes[c1].sar)));
-    float A = area6(nodes[p].box);
-    float cc = (0.0f + s0) + s1;
-    float cs = (0.0f + area6(nodes[c0].box) / A) + area6(nodes[c1].box) / A;
-    __hip_atomic_store(&nodes[p].leaves, l0 + l1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(reinterpret_cast<uint32_t*>(&nodes[p].sar), f2u(cs + cc),
-                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    x = (uint32_t)p;
+  if (nodes[x].flags & kHole) return;
+  uint32_t d = 0;
+  for (int32_t u = nodes[x].parent; u >= 0; u = nodes[u].parent) d++;
+  nodes[x].depth = d;
+  if (nodes[x].flags & kLeaf) {
+    nodes[x].leaves = 1;
+    nodes[x].sar = 0.0f + 0.0f;
+    atomicMax(&a.ctx->max_depth, d);
   }
+}
+
+// Leaf counts and summed_area_ratio (bvh.rs:380-386) of the inner nodes at
+// one depth, from their children (one launch per depth, deepest first).
+__global__ __launch_bounds__(256) void k_bottom_up(Args a, uint32_t ninodes, uint32_t depth) {
+  const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= ninodes) return;
+  INode* nodes = a.nodes;
+  if (nodes[x].depth != depth || (nodes[x].flags & (kLeaf | kHole))) return;
+  const int32_t c0 = nodes[x].child[0], c1 = nodes[x].child[1];
+  const float s0 = nodes[c0].sar, s1 = nodes[c1].sar;
+  const float A = area6(nodes[x].box);
+  const float cc = (0.0f + s0) + s1;
+  const float cs = (0.0f + area6(nodes[c0].box) / A) + area6(nodes[c1].box) / A;
+  nodes[x].leaves = nodes[c0].leaves + nodes[c1].leaves;
+  nodes[x].sar = cs + cc;
 }
 
 // crystallize (bvh.rs:389-428): node x is written at L(parent) + [x is c1],
@@ -788,6 +1043,7 @@ __global__ __launch_bounds__(256) void k_layout(Args a, uint32_t ninodes, DevNod
   if (x >= ninodes || x == 0) return;
   const INode* nodes = a.nodes;
   const INode& nx = nodes[x];
+  if (nx.flags & kHole) return;
   uint32_t Lx = 0, Tx = 0, depth = 0;
   uint32_t c = x;
   for (;;) {
@@ -867,6 +1123,12 @@ hipError_t build_reference_bvh(const float* d_v9, uint32_t ntris, DevNode* nodes
     BVH_CHECK(alloc((void**)&a.sorted[k], N * 4));
     BVH_CHECK(alloc((void**)&a.tmp[k], N * 4));
   }
+  // Node ids: the levels take 1 + 2 S for S splits of nodes of more than 32
+  // triangles, and each small subtree of n triangles reserves 2 (n - 1).
+  // Those S splits form a full binary tree whose leaves are the small
+  // subtrees, the single triangles and the unsplit large nodes, so S is
+  // their number - 1 and all ids together are 2 (singles + unsplit large
+  // nodes + triangles of small subtrees) - 1 <= 2 N - 1.
   const size_t ninodes_cap = 2 * N + 2;
   BVH_CHECK(alloc((void**)&a.nodes, ninodes_cap * sizeof(INode)));
   Ctx* ctx;
@@ -878,6 +1140,13 @@ hipError_t build_reference_bvh(const float* d_v9, uint32_t ntris, DevNode* nodes
   BVH_CHECK(alloc((void**)&listB, list_cap * 4));
   BVH_CHECK(alloc((void**)&small, (N / 2 + 2) * 4));
   a.small_list = small;
+  // chunks of the wide nodes of one level: their triangles are disjoint, so
+  // at most N / kChunk + (number of wide nodes) chunks
+  const size_t max_chunks = N / kChunk + N / kWide + 2;
+  BVH_CHECK(alloc((void**)&a.chunk_node, max_chunks * 4));
+  BVH_CHECK(alloc((void**)&a.part, max_chunks * kBinWords * 4));
+  BVH_CHECK(alloc((void**)&a.offs, max_chunks * 64 * 4));
+  BVH_CHECK(alloc((void**)&a.side, max_chunks * 24 * 4));
   // TreeSAH power tables with the C library (same values as the CPU build)
   std::vector<float> tpm(N + 1), tpm1(N + 1);
   for (size_t n = 1; n <= N; n++) {
@@ -925,11 +1194,18 @@ hipError_t build_reference_bvh(const float* d_v9, uint32_t ntris, DevNode* nodes
   }
   uint32_t* cur = listA;
   uint32_t* nxt = listB;
+  bool wide = ntris > kWide;
   while (nlist > 0) {
     a.list = cur;
     a.nlist = nlist;
     a.next_list = nxt;
     a.mode = 0;
+    if (wide) {
+      hipLaunchKernelGGL(k_wide_plan, dim3(1), dim3(64), 0, s, a);
+      BVH_CHECK(hipGetLastError());
+      hipLaunchKernelGGL(k_bin_chunks, dim3((uint32_t)max_chunks), dim3(kBlock), 0, s, a);
+      BVH_CHECK(hipGetLastError());
+    }
     hipLaunchKernelGGL(k_decide, dim3(nlist), dim3(kBlock), 0, s, a);
     BVH_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_sort, dim3(nlist * 3), dim3(kBlock), 0, s, a);
@@ -938,13 +1214,19 @@ hipError_t build_reference_bvh(const float* d_v9, uint32_t ntris, DevNode* nodes
     hipLaunchKernelGGL(k_decide, dim3(nlist), dim3(kBlock), 0, s, a);
     BVH_CHECK(hipGetLastError());
     BVH_CHECK(hipMemcpyAsync(&ctx->next_large, &zero, 4, hipMemcpyHostToDevice, s));
+    BVH_CHECK(hipMemcpyAsync(&ctx->next_wide, &zero, 4, hipMemcpyHostToDevice, s));
+    if (wide) {
+      hipLaunchKernelGGL(k_part_chunks, dim3((uint32_t)max_chunks), dim3(kBlock), 0, s, a);
+      BVH_CHECK(hipGetLastError());
+    }
     hipLaunchKernelGGL(k_partition, dim3(nlist), dim3(kBlock), 0, s, a);
     BVH_CHECK(hipGetLastError());
-    uint32_t hctl[4];
-    BVH_CHECK(hipMemcpyAsync(hctl, ctx, 16, hipMemcpyDeviceToHost, s));
+    uint32_t hctl[5];
+    BVH_CHECK(hipMemcpyAsync(hctl, ctx, 20, hipMemcpyDeviceToHost, s));
     BVH_CHECK(hipStreamSynchronize(s));
     if (hctl[3]) break;
     nlist = hctl[1];
+    wide = hctl[4] > 0;
     uint32_t* t = cur;
     cur = nxt;
     nxt = t;
@@ -978,8 +1260,14 @@ hipError_t build_reference_bvh(const float* d_v9, uint32_t ntris, DevNode* nodes
     return hipSuccess;
   }
   uint32_t blocks = (ninodes + 255) / 256;
-  hipLaunchKernelGGL(k_bottom_up, dim3(blocks), dim3(256), 0, s, a, ninodes);
+  hipLaunchKernelGGL(k_depth, dim3(blocks), dim3(256), 0, s, a, ninodes);
   BVH_CHECK(hipGetLastError());
+  BVH_CHECK(hipMemcpyAsync(&hc, ctx, sizeof(Ctx), hipMemcpyDeviceToHost, s));
+  BVH_CHECK(hipStreamSynchronize(s));
+  for (uint32_t d = hc.max_depth; d-- > 0;) {
+    hipLaunchKernelGGL(k_bottom_up, dim3(blocks), dim3(256), 0, s, a, ninodes, d);
+    BVH_CHECK(hipGetLastError());
+  }
   hipLaunchKernelGGL(k_layout, dim3(blocks), dim3(256), 0, s, a, ninodes, nodes_out, order_out);
   BVH_CHECK(hipGetLastError());
   BVH_CHECK(hipEventRecord(ev1, s));

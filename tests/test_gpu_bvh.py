@@ -70,6 +70,16 @@ def test_sorted_fallback_large(cv, oracle):
     _compare(cv, oracle, tris.astype(np.float32), label="fallback")
 
 
+def test_sorted_fallback_wide(cv, oracle):
+    """The same cluster and outliers at 40000 triangles: the top nodes are
+    binned and partitioned in chunks (nodes of more than 16384 triangles)
+    and fall back to sorted bins there."""
+    rng = np.random.default_rng(9)
+    c = np.concatenate([rng.normal(0, 0.001, (36000, 3)), rng.uniform(-50, 50, (4000, 3))])
+    tris = np.concatenate([c + rng.normal(0, 0.0005, c.shape) for _ in range(3)], 1)
+    _compare(cv, oracle, tris.astype(np.float32), label="fallback wide")
+
+
 def test_grid_ties(cv, oracle):
     """A regular grid of equal triangles: many equal barycenter coordinates,
     so the stable order of the sorts decides the tree."""
