@@ -31,11 +31,6 @@ __device__ __forceinline__ float neg0(float x) { return 0.0f - x; }
 __device__ __forceinline__ float absb(float x) { return u2f(f2u(x) & 0x7fffffffu); }
 // A NaN produced by an invalid operation is negative on x86.
 __device__ __forceinline__ float x86nan(float x) { return x != x ? u2f(kNegNaN) : x; }
-// 1 when x has its sign bit set or is a NaN (v_cmp_class with the classes
-// signaling NaN, quiet NaN, -inf, -normal, -denormal, -0).
-__device__ __forceinline__ uint32_t neg_or_nan(float x) {
-  return __builtin_amdgcn_classf(x, 0x3f) ? 1u : 0u;
-}
 // Sign bit of the x86 difference a - b, NaN propagation included.
 __device__ __forceinline__ uint32_t sgn_x86_sub(float a, float b) {
   float r = a - b;

@@ -139,7 +139,8 @@ __device__ __forceinline__ void visit_pair(const DevNode* __restrict__ nodes, ui
     }
   }
   // sign bit of tmin0 - tmin1 as x86 computes it: inf - inf (and NaN inputs)
-  // give the negative default NaN
+  // give the negative default NaN.  Without NaN inputs the class test (mask
+  // 0x3f: NaN, -inf, -normal, -denormal, -0) is that sign bit.
   const bool swap = exact ? sgn_x86_sub(tmin0, tmin1) != 0
                           : __builtin_amdgcn_classf(tmin0 - tmin1, 0x3f);
   // any_masked: mask (tmax NLT tmin) and tmax's sign bit 0 (aabb.rs:155-158)
