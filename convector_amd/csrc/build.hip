@@ -45,6 +45,7 @@ constexpr uint32_t kHole = 32;
 constexpr uint32_t kBlock = 512;
 constexpr uint32_t kChunk = 2048;   // references per chunk of a wide node
 constexpr uint32_t kWide = 16384;   // nodes above this many triangles are chunked
+constexpr uint32_t kNarrow = 2048;  // levels with no larger node use 64-thread blocks
 constexpr uint32_t kBinWords = 3 * 64 * 7;  // counts, then outer bounds, of 3 x 64 bins
 
 struct INode {
@@ -68,6 +69,7 @@ struct Ctx {
   uint32_t error;
   uint32_t next_wide;     // wide nodes in the next level's list
   uint32_t total_chunks;  // chunks of the current level
+  uint32_t next_max;      // largest triangle count in the next level's list
   uint32_t max_depth;
   uint32_t max_leaf;
   uint32_t root_box[6];   // okeys
@@ -222,10 +224,11 @@ __global__ void k_root(INode* nodes, Ctx* ctx, uint32_t n) {
 
 // Centroid bins of refs[0, n) on the axes not in umask (bin_triangles,
 // bvh.rs:149-156): counts and outer bounds as ordered integers in LDS.
+template <uint32_t BS>
 __device__ __forceinline__ void bin_centroids(const Args& a, const uint32_t* refs, uint32_t n,
                                               const float* cmn, const float* csz, uint32_t umask,
                                               uint32_t (*cnt)[64], uint32_t (*bx)[64][6]) {
-  for (uint32_t e = threadIdx.x; e < n; e += kBlock) {
+  for (uint32_t e = threadIdx.x; e < n; e += BS) {
     uint32_t r = refs[e];
     float4 A = a.A[r], B = a.B[r];
     float cz = a.C[r];
@@ -244,8 +247,9 @@ __device__ __forceinline__ void bin_centroids(const Args& a, const uint32_t* ref
   }
 }
 
+template <uint32_t BS>
 __device__ __forceinline__ void clear_bins(uint32_t (*cnt)[64], uint32_t (*bx)[64][6]) {
-  for (uint32_t k = threadIdx.x; k < 3 * 64; k += kBlock) {
+  for (uint32_t k = threadIdx.x; k < 3 * 64; k += BS) {
     (&cnt[0][0])[k] = 0;
     for (int j = 0; j < 6; j++) (&bx[0][0][0])[k * 6 + j] = j < 3 ? 0xffffffffu : 0u;
   }
@@ -279,14 +283,14 @@ __global__ __launch_bounds__(kBlock) void k_bin_chunks(Args a) {
   const uint32_t* refs = a.buf[(nd->flags & kParity) ? 1 : 0] + nd->begin + lo;
   __shared__ uint32_t cnt[3][64];
   __shared__ uint32_t bx[3][64][6];
-  clear_bins(cnt, bx);
+  clear_bins<kBlock>(cnt, bx);
   __syncthreads();
   float cmn[3], csz[3];
   for (int k = 0; k < 3; k++) {
     cmn[k] = nd->cbox[k];
     csz[k] = nd->cbox[3 + k] - cmn[k];
   }
-  bin_centroids(a, refs, n, cmn, csz, 0u, cnt, bx);
+  bin_centroids<kBlock>(a, refs, n, cmn, csz, 0u, cnt, bx);
   __syncthreads();
   uint32_t* out = a.part + (size_t)c * kBinWords;
   for (uint32_t k = threadIdx.x; k < 3 * 64; k += kBlock) out[k] = (&cnt[0][0])[k];
@@ -301,7 +305,11 @@ __global__ __launch_bounds__(kBlock) void k_bin_chunks(Args a) {
 // mode 1: pending nodes only; sorted (uniform) bins from k_sort's output.
 // ---------------------------------------------------------------------------
 
-__global__ __launch_bounds__(kBlock) void k_decide(Args a) {
+// BS threads per node: 512, or 64 on levels whose nodes all have at most
+// kNarrow triangles (8 times as many nodes in flight per CU; the decision's
+// serial scans dominate there).
+template <uint32_t BS>
+__global__ __launch_bounds__(BS) void k_decide(Args a) {
   const uint32_t id = a.list[blockIdx.x];
   INode* nd = &a.nodes[id];
   const uint32_t flags = nd->flags;
@@ -317,7 +325,7 @@ __global__ __launch_bounds__(kBlock) void k_decide(Args a) {
   __shared__ uint32_t s_bin;
   const uint32_t tid = threadIdx.x;
   const uint32_t nch = nd->nchunks, cbase = nd->chunk_base;
-  clear_bins(cnt, bx);
+  clear_bins<BS>(cnt, bx);
   __syncthreads();
   float cmn[3], csz[3];
   for (int k = 0; k < 3; k++) {
@@ -330,7 +338,7 @@ __global__ __launch_bounds__(kBlock) void k_decide(Args a) {
     // they are combined, so that the latency of the partials' reads, written
     // by other XCDs, is paid once per 16 chunks)
     const uint32_t* part = a.part + (size_t)cbase * kBinWords;
-    for (uint32_t k = tid; k < kBinWords; k += kBlock) {
+    for (uint32_t k = tid; k < kBinWords; k += BS) {
       const uint32_t axis = k < 192 ? k / 64 : (k - 192) / 384;
       if ((umask >> axis) & 1u) continue;
       const uint32_t* q = part + k;
@@ -361,7 +369,7 @@ __global__ __launch_bounds__(kBlock) void k_decide(Args a) {
         (&bx[0][0][0])[k - 192] = v;
     }
   } else {
-    bin_centroids(a, refs, n, cmn, csz, umask, cnt, bx);
+    bin_centroids<BS>(a, refs, n, cmn, csz, umask, cnt, bx);
   }
   if (umask) {
     // bin_triangles_uniform (bvh.rs:176-201): chunks of ceil(n/64) of the
@@ -370,7 +378,7 @@ __global__ __launch_bounds__(kBlock) void k_decide(Args a) {
     for (uint32_t axis = 0; axis < 3; axis++) {
       if (!((umask >> axis) & 1u)) continue;
       const uint32_t* srt = a.sorted[axis] + begin;
-      for (uint32_t s = tid; s < n; s += kBlock) {
+      for (uint32_t s = tid; s < n; s += BS) {
         uint32_t r = srt[s];
         float4 A = a.A[r], B = a.B[r];
         uint32_t b = s / tpb;
@@ -516,25 +524,25 @@ __global__ __launch_bounds__(kBlock) void k_decide(Args a) {
 
 // ---------------------------------------------------------------------------
 // Stable counting-sort pass of a segment inside one workgroup: elements are
-// taken in tiles of kBlock in order; within a tile a lane's rank among equal
+// taken in tiles of BS in order; within a tile a lane's rank among equal
 // digits comes from ballots, across waves from LDS counts.  Digit d starts
 // at start[d] when start is given (a chunk of a larger sort), else after
 // the segment's smaller digits.
 // ---------------------------------------------------------------------------
 
-template <int BITS, typename DigitFn, typename LoadFn>
+template <int BITS, uint32_t BS, typename DigitFn, typename LoadFn>
 __device__ __forceinline__ void wg_stable_pass(uint32_t n, LoadFn load, DigitFn digit_of,
                                                uint32_t* dst, uint32_t* hist, uint32_t* wcount,
                                                const uint32_t* start = nullptr) {
   constexpr uint32_t R = 1u << BITS;
-  constexpr uint32_t W = kBlock / 64;
+  constexpr uint32_t W = BS / 64;
   const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   if (start) {
-    for (uint32_t k = tid; k < R; k += kBlock) hist[k] = start[k];
+    for (uint32_t k = tid; k < R; k += BS) hist[k] = start[k];
   } else {
-    for (uint32_t k = tid; k < R; k += kBlock) hist[k] = 0;
+    for (uint32_t k = tid; k < R; k += BS) hist[k] = 0;
     __syncthreads();
-    for (uint32_t e = tid; e < n; e += kBlock) atomicAdd(&hist[digit_of(load(e))], 1u);
+    for (uint32_t e = tid; e < n; e += BS) atomicAdd(&hist[digit_of(load(e))], 1u);
     __syncthreads();
     if (tid == 0) {
       uint32_t s = 0;
@@ -545,10 +553,10 @@ __device__ __forceinline__ void wg_stable_pass(uint32_t n, LoadFn load, DigitFn 
       }
     }
   }
-  for (uint32_t k = tid; k < W * R; k += kBlock) wcount[k] = 0;
+  for (uint32_t k = tid; k < W * R; k += BS) wcount[k] = 0;
   __syncthreads();
   const uint64_t lt = (1ull << lane) - 1ull;
-  for (uint32_t t0 = 0; t0 < n; t0 += kBlock) {
+  for (uint32_t t0 = 0; t0 < n; t0 += BS) {
     uint32_t e = t0 + tid;
     bool valid = e < n;
     uint32_t v = valid ? load(e) : 0u;
@@ -567,7 +575,7 @@ __device__ __forceinline__ void wg_stable_pass(uint32_t n, LoadFn load, DigitFn 
       dst[hist[d] + prior + rank] = v;
     }
     __syncthreads();
-    for (uint32_t k = tid; k < R; k += kBlock) {
+    for (uint32_t k = tid; k < R; k += BS) {
       uint32_t s = 0;
       for (uint32_t w = 0; w < W; w++) {
         s += wcount[w * R + k];
@@ -598,7 +606,7 @@ __global__ __launch_bounds__(kBlock) void k_sort(Args a) {
     const uint32_t shift = 8 * pass;
     auto load = [&](uint32_t e) { return pass == 0 ? src[e] : load_coherent(src + e); };
     auto digit = [&](uint32_t r) { return (sort_key(centroid(a, r, axis)) >> shift) & 255u; };
-    wg_stable_pass<8>(n, load, digit, dst, hist, wcount);
+    wg_stable_pass<8, kBlock>(n, load, digit, dst, hist, wcount);
     __threadfence();
     __syncthreads();
   }
@@ -607,14 +615,14 @@ __global__ __launch_bounds__(kBlock) void k_sort(Args a) {
 // Outer and barycenter bounds of the two children over order[lo, hi) (side
 // 0: position below L when uniform, else bin below split), reduced into
 // cb (LDS, initialised by the caller) with one atomic per wave and value.
-template <typename BinFn>
+template <uint32_t BS, typename BinFn>
 __device__ __forceinline__ void side_bounds(const Args& a, const uint32_t* order, uint32_t lo,
                                             uint32_t hi, bool uniform, uint32_t L, uint32_t split,
                                             BinFn bin_of, uint32_t (*cb)[12]) {
   uint32_t lk[2][12];
   for (int s = 0; s < 2; s++)
     for (int k = 0; k < 12; k++) lk[s][k] = (k % 6) < 3 ? 0xffffffffu : 0u;
-  for (uint32_t e = lo + threadIdx.x; e < hi; e += kBlock) {
+  for (uint32_t e = lo + threadIdx.x; e < hi; e += BS) {
     uint32_t r = order[e];
     int side = uniform ? (e < L ? 0 : 1) : (bin_of(r) < split ? 0 : 1);
     float4 A = a.A[r], B = a.B[r];
@@ -679,15 +687,17 @@ __global__ __launch_bounds__(kBlock) void k_part_chunks(Args a) {
     for (uint32_t s = lo + tid; s < hi; s += kBlock) dst[s] = srt[s];
   } else {
     auto load = [&](uint32_t e) { return src[lo + e]; };
-    wg_stable_pass<6>(hi - lo, load, bin_of, dst, hist, wcount, a.offs + (size_t)c * 64);
+    wg_stable_pass<6, kBlock>(hi - lo, load, bin_of, dst, hist, wcount, a.offs + (size_t)c * 64);
   }
-  side_bounds(a, uniform ? a.sorted[axis] + begin : src, lo, hi, uniform, L, split, bin_of, cb);
+  side_bounds<kBlock>(a, uniform ? a.sorted[axis] + begin : src, lo, hi, uniform, L, split, bin_of,
+                     cb);
   if (tid < 24) a.side[(size_t)c * 24 + tid] = (&cb[0][0])[tid];
 }
 
 // Reorders the segment as split() collects the children (bins in order,
 // references in push order) and creates the children.
-__global__ __launch_bounds__(kBlock) void k_partition(Args a) {
+template <uint32_t BS>
+__global__ __launch_bounds__(BS) void k_partition(Args a) {
   const uint32_t id = a.list[blockIdx.x];
   INode* nd = &a.nodes[id];
   const uint32_t flags = nd->flags;
@@ -699,7 +709,7 @@ __global__ __launch_bounds__(kBlock) void k_partition(Args a) {
   const uint32_t axis = nd->axis, split = nd->split_bin, L = nd->left_count;
   const bool uniform = (nd->uniform_mask >> axis) & 1u;
   __shared__ uint32_t hist[64];
-  __shared__ uint32_t wcount[(kBlock / 64) * 64];
+  __shared__ uint32_t wcount[(BS / 64) * 64];
   __shared__ uint32_t cb[2][12];  // per side: outer min.xyz max.xyz, barycenter min.xyz max.xyz
   const uint32_t tid = threadIdx.x;
   if (tid < 24) (&cb[0][0])[tid] = ((tid % 12) % 6) < 3 ? 0xffffffffu : 0u;
@@ -709,15 +719,15 @@ __global__ __launch_bounds__(kBlock) void k_partition(Args a) {
   if (nd->nchunks) {
     // a wide node: k_part_chunks reordered it; its chunks' bounds are added
     // up here
-    // (ordered min / max: any order gives the same bounds; 16 waves' lanes
-    // take every 16th chunk, then wave 0 combines them)
+    // (ordered min / max: any order gives the same bounds; BS / 32 groups of
+    // 24 lanes take every (BS / 32)-th chunk, then 24 lanes combine them)
     const uint32_t* sp = a.side + (size_t)nd->chunk_base * 24;
-    __shared__ uint32_t red[kBlock / 32][24];
+    __shared__ uint32_t red[BS / 32][24];
     const uint32_t k = tid % 24, g = tid / 24;  // 21 groups of 24 lanes
     const bool lo = ((k % 12) % 6) < 3;
-    if (g < kBlock / 32) {
+    if (g < BS / 32) {
       uint32_t v = lo ? 0xffffffffu : 0u;
-      for (uint32_t c = g; c < nd->nchunks; c += kBlock / 32) {
+      for (uint32_t c = g; c < nd->nchunks; c += BS / 32) {
         const uint32_t w = sp[(size_t)c * 24 + k];
         v = lo ? min(v, w) : max(v, w);
       }
@@ -726,19 +736,19 @@ __global__ __launch_bounds__(kBlock) void k_partition(Args a) {
     __syncthreads();
     if (tid < 24) {
       uint32_t v = lo ? 0xffffffffu : 0u;
-      for (uint32_t j = 0; j < kBlock / 32; j++) v = lo ? min(v, red[j][tid]) : max(v, red[j][tid]);
+      for (uint32_t j = 0; j < BS / 32; j++) v = lo ? min(v, red[j][tid]) : max(v, red[j][tid]);
       (&cb[0][0])[tid] = v;
     }
     __syncthreads();
   } else {
     if (uniform) {
       const uint32_t* srt = a.sorted[axis] + begin;
-      for (uint32_t s = tid; s < n; s += kBlock) dst[s] = srt[s];
+      for (uint32_t s = tid; s < n; s += BS) dst[s] = srt[s];
     } else {
       auto load = [&](uint32_t e) { return src[e]; };
-      wg_stable_pass<6>(n, load, bin_of, dst, hist, wcount);
+      wg_stable_pass<6, BS>(n, load, bin_of, dst, hist, wcount);
     }
-    side_bounds(a, uniform ? a.sorted[axis] + begin : src, 0, n, uniform, L, split, bin_of, cb);
+    side_bounds<BS>(a, uniform ? a.sorted[axis] + begin : src, 0, n, uniform, L, split, bin_of, cb);
   }
   if (tid == 0) {
     uint32_t c = atomicAdd(&a.ctx->node_count, 2u);
@@ -771,6 +781,7 @@ __global__ __launch_bounds__(kBlock) void k_partition(Args a) {
         uint32_t slot = atomicAdd(&a.ctx->next_large, 1u);
         a.next_list[slot] = cid;
         if (ch[s].count > kWide) atomicAdd(&a.ctx->next_wide, 1u);
+        atomicMax(&a.ctx->next_max, ch[s].count);
       } else if (ch[s].count >= 2) {
         uint32_t slot = atomicAdd(&a.ctx->small_count, 1u);
         a.small_list[slot] = cid;
@@ -1195,7 +1206,9 @@ hipError_t build_reference_bvh(const float* d_v9, uint32_t ntris, DevNode* nodes
   uint32_t* cur = listA;
   uint32_t* nxt = listB;
   bool wide = ntris > kWide;
+  uint32_t level_max = ntris;
   while (nlist > 0) {
+    const bool narrow = level_max <= kNarrow;
     a.list = cur;
     a.nlist = nlist;
     a.next_list = nxt;
@@ -1206,27 +1219,36 @@ hipError_t build_reference_bvh(const float* d_v9, uint32_t ntris, DevNode* nodes
       hipLaunchKernelGGL(k_bin_chunks, dim3((uint32_t)max_chunks), dim3(kBlock), 0, s, a);
       BVH_CHECK(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_decide, dim3(nlist), dim3(kBlock), 0, s, a);
+    auto decide = [&]() {
+      if (narrow)
+        hipLaunchKernelGGL(k_decide<64>, dim3(nlist), dim3(64), 0, s, a);
+      else
+        hipLaunchKernelGGL(k_decide<kBlock>, dim3(nlist), dim3(kBlock), 0, s, a);
+    };
+    decide();
     BVH_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_sort, dim3(nlist * 3), dim3(kBlock), 0, s, a);
     BVH_CHECK(hipGetLastError());
     a.mode = 1;
-    hipLaunchKernelGGL(k_decide, dim3(nlist), dim3(kBlock), 0, s, a);
+    decide();
     BVH_CHECK(hipGetLastError());
     BVH_CHECK(hipMemcpyAsync(&ctx->next_large, &zero, 4, hipMemcpyHostToDevice, s));
     BVH_CHECK(hipMemcpyAsync(&ctx->next_wide, &zero, 4, hipMemcpyHostToDevice, s));
+    BVH_CHECK(hipMemcpyAsync(&ctx->next_max, &zero, 4, hipMemcpyHostToDevice, s));
     if (wide) {
       hipLaunchKernelGGL(k_part_chunks, dim3((uint32_t)max_chunks), dim3(kBlock), 0, s, a);
       BVH_CHECK(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_partition, dim3(nlist), dim3(kBlock), 0, s, a);
+    // (64-thread blocks measured slower here: the stable pass's tiles)
+    hipLaunchKernelGGL(k_partition<kBlock>, dim3(nlist), dim3(kBlock), 0, s, a);
     BVH_CHECK(hipGetLastError());
-    uint32_t hctl[5];
-    BVH_CHECK(hipMemcpyAsync(hctl, ctx, 20, hipMemcpyDeviceToHost, s));
+    uint32_t hctl[7];
+    BVH_CHECK(hipMemcpyAsync(hctl, ctx, 28, hipMemcpyDeviceToHost, s));
     BVH_CHECK(hipStreamSynchronize(s));
     if (hctl[3]) break;
     nlist = hctl[1];
     wide = hctl[4] > 0;
+    level_max = hctl[6];
     uint32_t* t = cur;
     cur = nxt;
     nxt = t;
