@@ -870,10 +870,14 @@ __global__ __launch_bounds__(256) void k_small(Args a) {
   int sp = 1;
   while (sp > 0) {
     sp--;
+    // the entry is the same for the whole wave: scalar values, so that the
+    // loops over the segment below are scalar loops with readlanes
     const uint32_t* e = stk[wave][sp];
-    const uint32_t off = e[0], cnt = e[1], id = e[2];
-    const int32_t parent = (int32_t)e[3];
-    const uint32_t bflags = e[4];
+    const uint32_t off = __builtin_amdgcn_readfirstlane(e[0]);
+    const uint32_t cnt = __builtin_amdgcn_readfirstlane(e[1]);
+    const uint32_t id = __builtin_amdgcn_readfirstlane(e[2]);
+    const int32_t parent = (int32_t)__builtin_amdgcn_readfirstlane(e[3]);
+    const uint32_t bflags = __builtin_amdgcn_readfirstlane(e[4]);
     float nbox[6];
     for (int k = 0; k < 6; k++) nbox[k] = u2f(e[5 + k]);
     __builtin_amdgcn_wave_barrier();
@@ -941,7 +945,7 @@ __global__ __launch_bounds__(256) void k_small(Args a) {
       int abest = -1;
       float acost = 0.0f;
       for (uint32_t k = off; k + 1 < off + cnt; k++) {
-        float ck = __shfl(cst, (int)k);
+        const float ck = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cst), k));
         if (abest < 0 || ck < acost) {
           acost = ck;
           abest = (int)(k - off + 1);
