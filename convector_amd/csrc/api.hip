@@ -66,7 +66,8 @@ cv_status validate_tuning(const cv_tuning& t) {
     return fail(CV_ERR_INVALID_ARGUMENT, "stack_entries must be 8, 16, 24 or 32");
   if (t.extend_waves_per_cu > 64 || t.refill_threshold > 65 || t.fetch_batches > 64 ||
       (t.block_waves && t.block_waves != 1 && t.block_waves != 2 && t.block_waves != 4) ||
-      (t.frames_per_step > 64 && t.frames_per_step != CV_FRAMES_AUTO) || t.refill_interval > 64)
+      (t.frames_per_step > 64 && t.frames_per_step != CV_FRAMES_AUTO) || t.refill_interval > 64 ||
+      t.leaf_batch > 65)
     return fail(CV_ERR_INVALID_ARGUMENT, "tuning value out of range");
   return CV_OK;
 }
@@ -79,6 +80,7 @@ void apply_tuning(LaunchCfg& c, const cv_tuning& t) {
   if (t.block_waves) c.block_waves = t.block_waves;
   if (t.frames_per_step) c.frames_per_step = t.frames_per_step;
   if (t.refill_interval) c.refill_interval = t.refill_interval;
+  if (t.leaf_batch) c.leaf_batch = t.leaf_batch;
 }
 
 cv_tuning tuning_of(const LaunchCfg& c) {
@@ -90,6 +92,7 @@ cv_tuning tuning_of(const LaunchCfg& c) {
   t.block_waves = c.block_waves;
   t.frames_per_step = c.frames_per_step;
   t.refill_interval = c.refill_interval;
+  t.leaf_batch = c.leaf_batch;
   return t;
 }
 

@@ -197,14 +197,25 @@ __device__ __forceinline__ void test_tri(const DevTri* __restrict__ tris, uint32
 // triangles.  The ray is finished when have == false and the stack is empty.
 // The root is the entry (index 0, len 0, tmin -inf), so that its pair is
 // visited first and counted like every other pair.
+//
+// With `waiting` (lane refill), a lane that reaches a leaf outside a leaf
+// round keeps the leaf's entry and waits (*waiting); it does nothing until
+// the wave's next leaf round, when it takes the same entry again (the same
+// cull test, the same best hit) and tests the triangles.  Each lane thus
+// performs the same operations in the same order, only at later steps, while
+// the wave runs its triangle tests for many lanes at once instead of for the
+// few that reach a leaf in any one step.
 template <bool COUNT, int S, bool PACKED>
 __device__ __forceinline__ void trav_step(const DevNode* __restrict__ nodes,
                                           const DevTri* __restrict__ tris, V3 o, V3 d, V3 inv,
                                           bool exact, Stack<S, PACKED>& st, TravState& ts,
-                                          entry_t& cur, bool& have) {
+                                          entry_t& cur, bool& have, bool leaf_round = true,
+                                          bool* waiting = nullptr) {
+  if (waiting && *waiting && !leaf_round) return;
   entry_t e = cur;
   if (!have) e = st.pop();
   have = false;
+  if (waiting) *waiting = false;
   if (!(entry_tmin(e) < ts.best_t)) return;
   const uint32_t x = entry_x(e);
   uint32_t idx, len;
@@ -223,6 +234,12 @@ __device__ __forceinline__ void trav_step(const DevNode* __restrict__ nodes,
     if (COUNT) ts.na += 2;
     visit_pair(nodes, idx, o, inv, exact, st, cur, have);
   } else {
+    if (waiting && !leaf_round) {
+      cur = e;
+      have = true;
+      *waiting = true;
+      return;
+    }
     for (uint32_t k = idx; k < idx + len; k++) {
       test_tri(tris, k, o, d, ts);
       if (COUNT) ts.nt++;
@@ -598,7 +615,7 @@ __global__ __launch_bounds__(256) void k_extend_refill(ExtendArgs a) {
   uint32_t ray = 0;
   entry_t cur = 0;
   bool have = false;
-  bool active = false, exact = false;
+  bool active = false, exact = false, waiting = false;
   // The wave's reserved rays [pool, pool_end): the cursor hands out chunks
   // of max(idle lanes, 64 * batches) rays, so most refills take no atomic
   // and a low refill threshold does not contend on the one cursor.
@@ -646,6 +663,7 @@ __global__ __launch_bounds__(256) void k_extend_refill(ExtendArgs a) {
           st.sp = 0;
           cur = root_entry<PACKED>();
           have = true;
+          waiting = false;
           ray = i;
           active = true;
         }
@@ -661,8 +679,16 @@ __global__ __launch_bounds__(256) void k_extend_refill(ExtendArgs a) {
     // idles for at most interval - 1 steps
 #pragma unroll 1
     for (uint32_t rep = 0; rep < a.interval; rep++) {
+      // a leaf round when a.leaf_batch lanes wait at leaves, or every
+      // active lane does (so that the wave always progresses)
+      bool leaf_round = true;
+      if (a.leaf_batch) {
+        const uint64_t wm = __ballot(active && waiting), am = __ballot(active);
+        leaf_round = __popcll(wm) >= a.leaf_batch || wm == am;
+      }
       if (active) {
-        trav_step<COUNT>(nodes, tris, o, d, inv, wave_exact, st, ts, cur, have);
+        trav_step<COUNT>(nodes, tris, o, d, inv, wave_exact, st, ts, cur, have, leaf_round,
+                         &waiting);
         if (!have && st.sp == 0) {
           a.hits[ray] = make_uint4(ts.best, f2u(ts.best_t), f2u(ts.bu), f2u(ts.bv));
           if (COUNT) {

@@ -69,14 +69,14 @@ struct RunOpts {
 struct GraphKey {
   int stack = 0;
   uint32_t blocks = 0, refill = 0, fetch_batches = 0, max_bounces = 0, block_waves = 0;
-  uint32_t interval = 0;
+  uint32_t interval = 0, leaf_batch = 0;
   bool packed = false, count = false, multi_job = false;
   const void* nodes = nullptr;
   const void* tris = nullptr;
   size_t capacity = 0;
   bool operator==(const GraphKey& o) const {
     return stack == o.stack && blocks == o.blocks && refill == o.refill &&
-           block_waves == o.block_waves && interval == o.interval &&
+           block_waves == o.block_waves && interval == o.interval && leaf_batch == o.leaf_batch &&
            fetch_batches == o.fetch_batches && max_bounces == o.max_bounces &&
            packed == o.packed && count == o.count && multi_job == o.multi_job &&
            nodes == o.nodes && tris == o.tris && capacity == o.capacity;
@@ -288,6 +288,7 @@ cv_status launch_step(Pipeline& p, const RunOpts& o, uint32_t slot, uint32_t pat
     x.batches = o.cfg.fetch_batches;
     x.refill = o.cfg.refill <= 64 ? o.cfg.refill : (multi_scene ? 64u : 0u);
     x.interval = o.cfg.refill_interval;
+    x.leaf_batch = o.cfg.leaf_batch <= 64 ? o.cfg.leaf_batch : 0u;
     ep = ev_pair(1);
     if (ep) (void)hipEventRecord(ep->first, st);
     HIPTRY(cvk::launch_extend(x, count, o.packed, multi_scene, o.cfg.stack, o.cfg.blocks,
@@ -332,6 +333,7 @@ cv_status pipeline_run(Pipeline& p, const std::vector<StepPlan>& steps, const Ru
   key.refill = o.cfg.refill;
   key.block_waves = o.cfg.block_waves;
   key.interval = o.cfg.refill_interval;
+  key.leaf_batch = o.cfg.leaf_batch;
   key.fetch_batches = o.cfg.fetch_batches;
   key.max_bounces = o.max_bounces;
   key.packed = o.packed;
@@ -1045,6 +1047,7 @@ cv_status cv_batch_autotune(cv_batch* b, const cv_job* jobs, size_t n, uint64_t 
   static const uint32_t refills[] = {16, 24, 32, 48, 64, 65};
   static const uint32_t block_waves[] = {1, 2, 4};
   static const uint32_t intervals[] = {1, 4, 16, 32};
+  static const uint32_t leaf_batches[] = {8, 16, 24, 65};
   cvh::LaunchCfg best = b->cfg;
   double best_ms = 0;
   cv_status cs = time_cfg(best, &best_ms);
@@ -1081,6 +1084,13 @@ cv_status cv_batch_autotune(cv_batch* b, const cv_job* jobs, size_t n, uint64_t 
     for (uint32_t v : intervals) {
       cvh::LaunchCfg c = best;
       c.refill_interval = v;
+      double ms;
+      if ((cs = time_cfg(c, &ms)) != CV_OK) return cs;
+      if (ms < best_ms) best = c, best_ms = ms;
+    }
+    for (uint32_t v : leaf_batches) {
+      cvh::LaunchCfg c = best;
+      c.leaf_batch = v;
       double ms;
       if ((cs = time_cfg(c, &ms)) != CV_OK) return cs;
       if (ms < best_ms) best = c, best_ms = ms;

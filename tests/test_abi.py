@@ -113,3 +113,31 @@ def test_config_scene_sizes():
     sd = scenes.config0()
     assert (sd.materials == scenes.MAT_SKY).sum() == 8
     assert scenes.MAT_GLOSSY_WHITE == 0x10FFFFFF
+
+
+def _header_fields(struct):
+    """Field names of `typedef struct <struct> { ... }` in declaration order."""
+    text = open(os.path.join(ROOT, "include", "convector.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    body = re.search(r"typedef struct " + struct + r" \{(.*?)\}", text, flags=re.S).group(1)
+    names = []
+    for decl in body.split(";"):
+        decl = decl.strip()
+        if not decl:
+            continue
+        for part in decl.split(","):
+            m = re.search(r"\**\s*([A-Za-z_][A-Za-z0-9_]*)\s*(\[[^\]]*\])?\s*$", part.strip())
+            names.append(m.group(1))
+    return names
+
+
+@pytest.mark.parametrize("struct,mirror", [
+    ("cv_tuning", "Tuning"), ("cv_render_opts", "RenderOpts"), ("cv_job", "Job"),
+    ("cv_request", "Request"), ("cv_server_stats", "ServerStats"),
+    ("cv_render_stats", "RenderStats"), ("cv_bvh_info", "BvhInfo"),
+    ("cv_scene_opts", "SceneOpts"), ("cv_material_entry", "MaterialEntry")])
+def test_struct_mirrors_match_header(cv, struct, mirror):
+    """The ctypes mirrors of the ABI's structs have the header's fields in
+    the header's order (a field added on one side only shifts the rest)."""
+    fields = [f[0] for f in getattr(cv._native, mirror)._fields_]
+    assert fields == _header_fields(struct), struct

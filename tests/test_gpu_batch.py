@@ -153,7 +153,9 @@ def test_batch_tuning_variants(cv, world):
     b = cv.Batch()
     for t in (dict(stack_entries=16, refill_threshold=8), dict(stack_entries=32, refill_threshold=65),
               dict(stack_entries=8, extend_waves_per_cu=4, refill_threshold=64),
-              dict(stack_entries=24, extend_waves_per_cu=8, refill_threshold=32, block_waves=4)):
+              dict(stack_entries=24, extend_waves_per_cu=8, refill_threshold=32, block_waves=4),
+              dict(stack_entries=8, refill_threshold=16, leaf_batch=65),
+              dict(stack_entries=16, refill_threshold=8, leaf_batch=3)):
         print("batch tuning", t, flush=True)
         b.set_tuning(**t)
         jobs = _jobs(cv, gs)

@@ -145,14 +145,17 @@ def test_full_size_properties(cv):
 
 def test_tuning_variants_exact(cv, oracle):
     """Launch shapes change only the schedule: every stack size, refill
-    threshold (lane refill and whole-batch fetch), refill interval and
+    threshold (lane refill and whole-batch fetch), refill interval, leaf
+    batch (lanes waiting at leaves for a common leaf round, or none) and
     fetch width renders the same bits."""
     sd = scenes.config0(subdivisions=4)
     g, o = _pair(cv, oracle, sd)
     W, H, n = 160, 96, 2
     ohdr, ogb = o.accumulate_frames(W, H, 1, n, oracle.RNG_PIXEL, 8)
-    for t in (dict(stack_entries=8, refill_threshold=1, refill_interval=1),
-              dict(stack_entries=16, refill_threshold=32, refill_interval=64),
+    for t in (dict(stack_entries=8, refill_threshold=1, refill_interval=1, leaf_batch=1),
+              dict(stack_entries=16, refill_threshold=32, refill_interval=64, leaf_batch=64),
+              dict(stack_entries=8, refill_threshold=16, refill_interval=16, leaf_batch=65),
+              dict(stack_entries=8, refill_threshold=16, refill_interval=4, leaf_batch=7),
               dict(stack_entries=24, refill_threshold=65, fetch_batches=1),
               dict(stack_entries=32, refill_threshold=65, fetch_batches=4),
               dict(stack_entries=8, extend_waves_per_cu=1, refill_threshold=64, refill_interval=5),

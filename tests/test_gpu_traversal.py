@@ -175,7 +175,7 @@ def test_tuning_validation(cv, models_dir):
     before = gsc.tuning()
     for bad in (dict(stack_entries=12), dict(extend_waves_per_cu=65),
                 dict(refill_threshold=66), dict(fetch_batches=65), dict(block_waves=3),
-                dict(frames_per_step=65), dict(refill_interval=65)):
+                dict(frames_per_step=65), dict(refill_interval=65), dict(leaf_batch=66)):
         with pytest.raises(cv.CvError) as e:
             gsc.set_tuning(**bad)
         assert e.value.name == "CV_ERR_INVALID_ARGUMENT"
