@@ -84,7 +84,7 @@ struct LaunchCfg {
   uint32_t block_waves = 1;   // waves per workgroup (lane refill)
   uint32_t frames_per_step = CV_FRAMES_AUTO;  // accumulation frames sharing launches
   uint32_t refill_interval = 16;              // traversal steps between refill checks
-  uint32_t leaf_batch = 16;                   // lanes waiting at leaves per leaf round; 65 = off
+  uint32_t leaf_batch = 16;                   // 64ths of the active lanes per leaf round; 65 = off
 };
 
 }  // namespace cvh

@@ -85,7 +85,8 @@ struct ExtendArgs {
   uint32_t batches;  // 64-ray batches per cursor fetch (batch mode)
   uint32_t refill;   // > 0: lane refill when this many lanes are idle
   uint32_t interval; // lane refill: traversal steps between refill checks (>= 1)
-  uint32_t leaf_batch;  // lane refill: > 0: lanes at leaves wait until this many do (or all)
+  uint32_t leaf_batch;  // lane refill: > 0: lanes at leaves wait until leaf_batch / 64 of
+                        // the active lanes do (or all)
 };
 
 struct ShadeArgs {

@@ -679,12 +679,15 @@ __global__ __launch_bounds__(256) void k_extend_refill(ExtendArgs a) {
     // idles for at most interval - 1 steps
 #pragma unroll 1
     for (uint32_t rep = 0; rep < a.interval; rep++) {
-      // a leaf round when a.leaf_batch lanes wait at leaves, or every
-      // active lane does (so that the wave always progresses)
+      // a leaf round when leaf_batch / 64 of the active lanes wait at
+      // leaves (a share, so that a wave with few rays left does not hold
+      // its leaves back), or every active lane does (so that the wave
+      // always progresses)
       bool leaf_round = true;
       if (a.leaf_batch) {
         const uint64_t wm = __ballot(active && waiting), am = __ballot(active);
-        leaf_round = __popcll(wm) >= a.leaf_batch || wm == am;
+        leaf_round = (uint32_t)__popcll(wm) * 64u >= a.leaf_batch * (uint32_t)__popcll(am) ||
+                     wm == am;
       }
       if (active) {
         trav_step<COUNT>(nodes, tris, o, d, inv, wave_exact, st, ts, cur, have, leaf_round,

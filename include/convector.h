@@ -140,9 +140,10 @@ typedef struct cv_tuning {
                                      launches (each into its own buffer, added in frame order),
                                      or CV_FRAMES_AUTO (about 2^25 paths per step) */
     uint32_t refill_interval;     /* lane refill: 1..64 traversal steps between refill checks */
-    uint32_t leaf_batch;          /* lane refill: 1..64: a lane at a leaf waits until this many
-                                     lanes wait (or all do) and they test their triangles
-                                     together; 65: every lane tests its leaf at once */
+    uint32_t leaf_batch;          /* lane refill: 1..64: a lane at a leaf waits until
+                                     leaf_batch / 64 of the wave's active lanes wait (or all
+                                     do) and they test their triangles together (16: a
+                                     quarter); 65: every lane tests its leaf at once */
 } cv_tuning;
 #define CV_FRAMES_AUTO 255u
 cv_status cv_scene_set_tuning(cv_scene* scene, const cv_tuning* tuning);
