@@ -618,7 +618,8 @@ def bench_serve(args):
     """The request loop: 4 resident scenes (about 20k, 200k, 1.3M and 2M
     triangles), Poisson arrivals at each rate, every request a random camera
     in the box, a resolution from {256x256, 1280x720, 1920x1080}, spp from
-    {1, 4, 16} and the denoiser on half of them.  Continuous batching (the
+    {1, 4, 16} and the denoiser on half of them (one seeded set of requests,
+    replayed at every rate).  Continuous batching (the
     server admits requests into shared steps as memory and queue capacity
     free up) against the same requests one at a time through the same
     server."""
@@ -635,13 +636,16 @@ def bench_serve(args):
     rng = np.random.default_rng(77)
 
     def make_requests(n):
+        # the same requests for every rate and for the one-at-a-time run
+        # (their own stream), so that the rates compare like with like
+        req_rng = np.random.default_rng(78)
         out = []
         for _ in range(n):
-            pos = (float(rng.uniform(-0.6, 0.6)), float(rng.uniform(0.2, 0.9)),
-                   float(rng.uniform(-0.6, 0.6)))
-            w, h = sizes[int(rng.integers(0, 3))]
-            out.append((int(rng.integers(0, 4)), w, h, int((1, 4, 16)[int(rng.integers(0, 3))]),
-                        pos, bool(rng.integers(0, 2))))
+            pos = (float(req_rng.uniform(-0.6, 0.6)), float(req_rng.uniform(0.2, 0.9)),
+                   float(req_rng.uniform(-0.6, 0.6)))
+            w, h = sizes[int(req_rng.integers(0, 3))]
+            out.append((int(req_rng.integers(0, 4)), w, h, int((1, 4, 16)[int(req_rng.integers(0, 3))]),
+                        pos, bool(req_rng.integers(0, 2))))
         return out
 
     Q0 = (0.70710677, -0.70710677, 0.0, 0.0)
@@ -692,6 +696,10 @@ def bench_serve(args):
     peak = sv.stats()["peak_bytes"]
     sv.close()
     best = max(rates, key=lambda r: r["frames_per_s"])
+    # the highest rate the server keeps up with (frames/s within 5 % of the
+    # arrival rate): its latency is service time, not queueing
+    kept = [r for r in rates if r["frames_per_s"] >= 0.95 * r["rate_req_per_s"]]
+    sustained = max(kept, key=lambda r: r["rate_req_per_s"]) if kept else None
     out = {
         "metric": "p99 frame latency and frames/s, continuous batching",
         "value": best["frames_per_s"],
@@ -710,6 +718,9 @@ def bench_serve(args):
                                "half, 200 GiB budget" % ([sd.ntris for sd in sds], args.rates,
                                                          args.requests)},
         "rates": rates,
+        "sustained": None if sustained is None else {
+            "rate_req_per_s": sustained["rate_req_per_s"], "p50_ms": sustained["p50_ms"],
+            "p99_ms": sustained["p99_ms"]},
         "one_at_a_time": serial,
         "speedup_vs_one_at_a_time": round(best["frames_per_s"] / serial["frames_per_s"], 3),
         "peak_bytes": peak,
@@ -726,7 +737,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--subdivisions", type=int, default=8)
-    ap.add_argument("--rates", type=float, nargs="+", default=[50, 100, 200, 400])
+    ap.add_argument("--rates", type=float, nargs="+", default=[10, 25, 50, 100, 200, 400])
     ap.add_argument("--requests", type=int, default=100)
     ap.add_argument("--workload", choices=["frame", "batch", "denoise", "denoise-kernels", "serve"],
                     default="frame",
