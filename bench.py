@@ -614,6 +614,175 @@ def bench_denoise_kernels(args):
     print(json.dumps(out), flush=True)
 
 
+def _hip_copy(hip, dst, src, nbytes, kind):
+    rc = hip.hipMemcpy(dst, src, ctypes.c_size_t(nbytes), kind)
+    if rc != 0:
+        raise RuntimeError("hipMemcpy failed: %d" % rc)
+
+
+def bench_rays(args):
+    """Batched Bvh::intersect_nearest (cv_bvh_intersect_nearest on device
+    buffers) on the reference's own intersect benches (bvh.rs:681-739):
+    suzanne and the Stanford bunny (the committed OBJ fixtures), rays from a
+    sphere of radius 10 toward the unit sphere (decoherent) or from one origin
+    per 8-ray packet toward a ball of radius 0.5 (coherent; bench.rs:195-223),
+    max distance 1e5.  One step = one call over --rays rays; each case runs
+    --warmup then --steps calls.  The host baseline is the same queries as
+    AVX2 8-ray packets (the reference's MRay shape)."""
+    import gzip
+    import shutil
+    import tempfile
+
+    import convector_amd as cv
+    from convector_amd import scenes
+
+    n = int(args.rays)
+    ray_sets = {False: scenes.rays_inward(n, 31, radius=10.0),
+                True: scenes.rays_inward(n, 32, radius=10.0, coherent=True)}
+    rays_d = {}
+    for coh, r in ray_sets.items():
+        rays_d[coh] = DeviceBuffer(r.nbytes)
+        _hip_copy(rays_d[coh].hip, rays_d[coh].ptr, r.ctypes.data_as(ctypes.c_void_p), r.nbytes, 1)
+    hits_d = DeviceBuffer(16 * n)
+    tmp = tempfile.mkdtemp()
+    cases, tot_rays, tot_s, tot_bytes, tot_launch_s = [], 0, 0.0, 0.0, 0.0
+    meshes = {}
+    kernel = None
+    try:
+        for name in ("suzanne", "stanford_bunny"):
+            path = os.path.join(tmp, name + ".obj")
+            with gzip.open(os.path.join(ROOT, "tests", "golden", "models", name + ".obj.gz"),
+                           "rb") as fi, open(path, "wb") as fo:
+                shutil.copyfileobj(fi, fo)
+            mesh = cv.Mesh.load(path)
+            meshes[name] = mesh
+            scene = cv.Scene.from_meshes([mesh])
+            info = scene.bvh_info()
+            tun = scene.tuning()
+            kernel = "k_intersect<false, %d, %s>" % (tun["stack_entries"],
+                                                    "true" if info["max_leaf"] <= 31 else "false")
+            for coh in (False, True):
+                label = "%s %s" % (name, "coherent" if coh else "decoherent")
+                # counting pass (box and triangle tests per ray, identical in
+                # the timed calls, which skip the counters)
+                isd = DeviceBuffer(64 * n)
+                scene.intersect_nearest_device(rays_d[coh].ptr.value, n, hits_d.ptr.value,
+                                               isd.ptr.value)
+                cv.device_synchronize()
+                iso = np.zeros((n, 16), np.uint32)
+                _hip_copy(isd.hip, iso.ctypes.data_as(ctypes.c_void_p), isd.ptr, iso.nbytes, 2)
+                del isd
+                sum_b, sum_t = int(iso[:, 13].sum(dtype=np.uint64)), int(iso[:, 14].sum(dtype=np.uint64))
+                hit_frac = float((iso[:, 10] != 0xFFFFFFFF).mean())
+                del iso
+                call = lambda: scene.intersect_nearest_device(rays_d[coh].ptr.value, n,
+                                                              hits_d.ptr.value)
+                for _ in range(args.warmup):
+                    call()
+                cv.device_synchronize()
+                t0 = time.perf_counter()
+                for _ in range(args.steps):
+                    call()
+                cv.device_synchronize()
+                wall = time.perf_counter() - t0
+                ms = wall * 1e3 / args.steps
+                nbytes = 32.0 * sum_b + 36.0 * sum_t
+                cases.append({
+                    "case": label,
+                    "triangles": mesh.num_triangles,
+                    "mrays_per_s": round(n * args.steps / wall / 1e6, 1),
+                    "ms_per_call": round(ms, 4),
+                    "box_tests_per_ray": round(sum_b / n, 2),
+                    "tri_tests_per_ray": round(sum_t / n, 3),
+                    "hit_fraction": round(hit_frac, 4),
+                    "algorithmic_gb_per_s": round(nbytes / (ms * 1e-3) / 1e9, 1),
+                })
+                log("rays %s: %.1f Mrays/s, %.3f ms per call of %d rays, %.1f box / %.2f "
+                    "triangle tests per ray" % (label, cases[-1]["mrays_per_s"], ms, n,
+                                                sum_b / n, sum_t / n))
+                tot_rays += n * args.steps
+                tot_s += wall
+                tot_bytes += nbytes * args.steps
+                tot_launch_s += wall
+            del scene
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    value = tot_rays / tot_s / 1e6
+    achieved = tot_bytes / tot_launch_s / 1e9
+    out = {
+        "metric": "Mrays/s, batched intersect_nearest",
+        "value": round(value, 1),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(tot_s * 1e3 / (4 * args.steps), 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "suzanne (968 triangles) and Stanford bunny (69,666 triangles), "
+                               "%d decoherent and %d coherent inward rays per call, max "
+                               "distance 1e5" % (n, n)},
+        "cases": cases,
+        "roofline": {
+            "bound": "hbm",
+            "kernel": kernel,
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "note": "algorithmic node and triangle bytes (32 B per box test, 36 B per triangle "
+                    "test) over the wall time of back-to-back calls (an upper bound on the "
+                    "launch time); both meshes fit in L2, so the bound is VALU issue, not HBM",
+        },
+    }
+    if not args.no_cpu:
+        try:
+            out["cpu_baseline"] = cpu_baseline_rays(meshes, ray_sets, hits_d, args)
+            out["gpu_over_cpu"] = round(value / out["cpu_baseline"]["value"], 1)
+        except Exception as e:  # the GPU figure stands without it
+            log("cpu baseline failed: %r" % (e,))
+            out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline_rays(meshes, ray_sets, hits_d, args):
+    """The four cases as AVX2 8-ray packets on the host cores, a leading
+    sample of each ray set for about a quarter of --cpu-seconds each."""
+    from oracle import oracle as O
+
+    if not O.avx_available():
+        raise RuntimeError("host without AVX2 + FMA")
+    cores = min(args.cpu_threads, os.cpu_count() or 1)
+    chunk = 1 << 16
+    done, t_used, parts = 0, 0.0, []
+    for name, mesh in meshes.items():
+        v, uv, m = mesh.triangles()
+        osc = O.Scene(v, m, uv)
+        for coh, rays in ray_sets.items():
+            k, t_case = 0, 0.0
+            while t_case < args.cpu_seconds / 4 and k + chunk <= len(rays):
+                t1 = time.perf_counter()
+                osc.avx_intersect_packets(rays[k:k + chunk], nthreads=cores)
+                t_case += time.perf_counter() - t1
+                k += chunk
+            done += k
+            t_used += t_case
+            parts.append("%s %s %d rays %.1f Mrays/s" % (name, "coherent" if coh else "decoherent",
+                                                         k, k / t_case / 1e6))
+    return {
+        "value": round(done / t_used / 1e6, 2),
+        "unit": "Mrays/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": "AVX2 8-ray packet traversal (bvh.rs:589-657) on %d threads, leading rays "
+                  "of each set: %s" % (cores, "; ".join(parts)),
+    }
+
+
 def bench_serve(args):
     """The request loop: 4 resident scenes (about 20k, 200k, 1.3M and 2M
     triangles), Poisson arrivals at each rate, every request a random camera
@@ -739,13 +908,16 @@ def main():
     ap.add_argument("--subdivisions", type=int, default=8)
     ap.add_argument("--rates", type=float, nargs="+", default=[10, 25, 50, 100, 200, 400])
     ap.add_argument("--requests", type=int, default=100)
-    ap.add_argument("--workload", choices=["frame", "batch", "denoise", "denoise-kernels", "serve"],
+    ap.add_argument("--workload", choices=["frame", "batch", "denoise", "denoise-kernels", "serve",
+                                           "rays"],
                     default="frame",
                     help="frame: the large 16-spp 1080p render (default); batch: 64 mixed "
                          "jobs in shared launches against sequential launches; denoise: train "
                          "the denoiser and measure its equal-RMSE speedup; denoise-kernels: the "
                          "network at 1280x736 and training steps at batch 16 of 256x256 on "
-                         "device buffers; serve: the request loop's latency and throughput")
+                         "device buffers; serve: the request loop's latency and throughput; "
+                         "rays: batched intersect_nearest on suzanne and the bunny")
+    ap.add_argument("--rays", type=int, default=1 << 24, help="rays: rays per call")
     ap.add_argument("--jobs", type=int, default=64)
     ap.add_argument("--train-scenes", type=int, default=256)
     ap.add_argument("--train-steps", type=int, default=18000)
@@ -779,6 +951,9 @@ def main():
         return
     if args.workload == "denoise-kernels":
         bench_denoise_kernels(args)
+        return
+    if args.workload == "rays":
+        bench_rays(args)
         return
 
     import convector_amd as cv

@@ -521,7 +521,7 @@ static cv_status scene_finish(cv_scene* s, const std::vector<float>& v9,
   // traversal configuration: packed stack entries need index < 2^27 and
   // leaves of at most 31 triangles
   s->packed = s->info.max_leaf <= 31 && N < (1u << 27) && s->info.num_nodes < (1u << 27);
-  HIPTRY(s->cursor.alloc(4), "alloc");
+  HIPTRY(s->cursor.alloc(cvk::kIsectGroups * cvk::kIsectCursorStride), "alloc");
   cv_tuning t{8, 32, 16, 1, 1};
   return scene_apply_tuning(s, t);
 }
@@ -677,7 +677,8 @@ cv_status cv_bvh_intersect_nearest(const cv_scene* sc, const cv_ray* rays, size_
   a.spill = s->spill.p;
   a.hits = hp;
   a.isects = ip;
-  HIPTRY(cvk::launch_zero_u32(s->cursor.p, 1, st), "zero");
+  HIPTRY(cvk::launch_zero_u32(s->cursor.p, cvk::kIsectGroups * cvk::kIsectCursorStride, st),
+         "zero");
   HIPTRY(cvk::launch_intersect(a, isects != nullptr, s->packed, s->cfg.stack, s->cfg.blocks, st),
          "intersect");
   if (!dev) {

@@ -10,7 +10,10 @@
 namespace cvk {
 
 constexpr int kMaxBounceSlots = 16;
-constexpr uint32_t kShadeBlock = 256;
+// Shade workgroup: 16 waves, so that the compaction's one atomic per
+// workgroup stays below the kernel's memory time (one cursor serialises at
+// about 12 ns an atomic).
+constexpr uint32_t kShadeBlock = 1024;
 
 // Per-step control block in device memory: the step's path count and job
 // count (written by the host before each step), queue sizes and work cursors
@@ -98,6 +101,11 @@ struct ShadeArgs {
   const uint4* hits;
   const JobDesc* jobs;
 };
+
+// cv_bvh_intersect_nearest's work cursors: kIsectGroups words,
+// kIsectCursorStride words (128 bytes) apart.
+constexpr uint32_t kIsectGroups = 64;
+constexpr uint32_t kIsectCursorStride = 32;
 
 struct IntersectArgs {
   const float* rays;

@@ -1022,6 +1022,7 @@ void or_scene_get_view(const or_scene* s, or_scene_view* out) {
     out->mat = s->mat;
     out->nodes = s->nodes;
     out->direct = s->direct;
+    out->src = s->src;
     out->ntris = s->ntris;
     out->nnodes = s->nnodes;
     out->ndirect = s->ndirect;

@@ -176,6 +176,7 @@ typedef struct {
     const uint32_t* mat;
     const or_node* nodes;
     const uint32_t* direct;  /* light triangles (BVH indices) */
+    const uint32_t* src;     /* source triangle index of each BVH position */
     uint32_t ntris, nnodes, ndirect;
     float cam_pos[3], cam_dpos[3], cam_q[4], cam_dq[4], screen_distance;
     int emission_override;
@@ -201,6 +202,13 @@ int or_avx_accumulate_frames(const or_scene* s, uint32_t width, uint32_t height,
 int or_avx_sample_patches(const or_scene* s, uint32_t width, uint32_t height, uint32_t frame,
                           int nthreads, const uint32_t* patch_ids, uint32_t n_ids,
                           uint64_t* n_queries);
+/* Bvh::intersect_nearest on 8-ray packets (bvh.rs:589-657), the shape of the
+ * reference's bench_intersect_* benches (bvh.rs:681-739): rays (n x 6, n a
+ * multiple of 8) taken 8 consecutive rays to a packet, max distance 1e5.
+ * tri_id (source index or 0xffffffff) and dist may each be NULL.  Returns 0
+ * or a negative status. */
+int or_avx_intersect_packets(const or_scene* s, const float* rays, size_t n, uint32_t* tri_id,
+                             float* dist, int nthreads);
 
 #ifdef __cplusplus
 }

@@ -668,6 +668,80 @@ int or_avx_sample_patches(const or_scene* s, uint32_t width, uint32_t height, ui
     return avx_run(&j, nthreads, n_queries);
 }
 
+/* ------------------------------------------------ batched packet queries */
+
+typedef struct {
+    const or_scene_view* view;
+    const float* rays;
+    size_t npackets;
+    uint32_t* tri_id;
+    float* dist;
+    volatile size_t next;
+    pthread_mutex_t lock;
+    int status;
+} pk_job;
+
+static void* pk_worker(void* arg) {
+    pk_job* j = (pk_job*)arg;
+    ctx c = {j->view, j->view->approx, 0, 0, 0};
+    const size_t chunk = 256;
+    for (;;) {
+        pthread_mutex_lock(&j->lock);
+        size_t p0 = j->next;
+        j->next += chunk;
+        pthread_mutex_unlock(&j->lock);
+        if (p0 >= j->npackets) break;
+        size_t p1 = p0 + chunk < j->npackets ? p0 + chunk : j->npackets;
+        for (size_t p = p0; p < p1 && !c.status; p++) {
+            float l[6][8];
+            const float* r = j->rays + 48 * p;
+            for (int k = 0; k < 8; k++)
+                for (int q = 0; q < 6; q++) l[q][k] = r[6 * k + q];
+            ray8 ray;
+            ray.o = mk(_mm256_loadu_ps(l[0]), _mm256_loadu_ps(l[1]), _mm256_loadu_ps(l[2]));
+            ray.d = mk(_mm256_loadu_ps(l[3]), _mm256_loadu_ps(l[4]), _mm256_loadu_ps(l[5]));
+            ray.active = _mm256_setzero_ps();
+            isect8 is = far_away(&ray); /* MIntersection::with_max_distance(1e5) */
+            intersect_nearest(&c, &ray, &is);
+            uint32_t tri[8];
+            _mm256_storeu_si256((i8*)tri, fbits(is.tri));
+            if (j->tri_id)
+                for (int k = 0; k < 8; k++)
+                    j->tri_id[8 * p + k] = tri[k] == 0xffffffffu ? tri[k] : j->view->src[tri[k]];
+            if (j->dist) _mm256_storeu_ps(j->dist + 8 * p, is.dist);
+        }
+    }
+    if (c.status) {
+        pthread_mutex_lock(&j->lock);
+        j->status = c.status;
+        pthread_mutex_unlock(&j->lock);
+    }
+    return NULL;
+}
+
+int or_avx_intersect_packets(const or_scene* s, const float* rays, size_t n, uint32_t* tri_id,
+                             float* dist, int nthreads) {
+    if (n & 7) return -14;
+    if (!or_avx_available()) return -30;
+    or_scene_view view;
+    or_scene_get_view(s, &view);
+    pk_job j;
+    memset(&j, 0, sizeof(j));
+    j.view = &view;
+    j.rays = rays;
+    j.npackets = n / 8;
+    j.tri_id = tri_id;
+    j.dist = dist;
+    pthread_mutex_init(&j.lock, NULL);
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 256) nthreads = 256;
+    pthread_t th[256];
+    for (int t = 0; t < nthreads; t++) pthread_create(&th[t], NULL, pk_worker, &j);
+    for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+    pthread_mutex_destroy(&j.lock);
+    return j.status;
+}
+
 #else /* no AVX2 / FMA at build time */
 
 int or_avx_available(void) { return 0; }
@@ -683,6 +757,12 @@ int or_avx_sample_patches(const or_scene* s, uint32_t width, uint32_t height, ui
                           uint64_t* n_queries) {
     (void)s; (void)width; (void)height; (void)frame; (void)nthreads; (void)patch_ids;
     (void)n_ids; (void)n_queries;
+    return -30;
+}
+
+int or_avx_intersect_packets(const or_scene* s, const float* rays, size_t n, uint32_t* tri_id,
+                             float* dist, int nthreads) {
+    (void)s; (void)rays; (void)n; (void)tri_id; (void)dist; (void)nthreads;
     return -30;
 }
 

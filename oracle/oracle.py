@@ -99,6 +99,7 @@ def _declare(L):
         "or_avx_available": (I32, []),
         "or_avx_accumulate_frames": (I32, [P, U32, U32, P, U32, U32, I32, I32, P]),
         "or_avx_sample_patches": (I32, [P, U32, U32, U32, I32, P, U32, P]),
+        "or_avx_intersect_packets": (I32, [P, P, SZ, P, P, I32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -301,6 +302,19 @@ class Scene:
         if st:
             raise RuntimeError("oracle avx_sample_patches status %d" % st)
         return int(q[0])
+
+    def avx_intersect_packets(self, rays, nthreads=8):
+        """Bvh::intersect_nearest on packets of 8 consecutive rays with AVX2
+        (bvh.rs:589-657; the bench shape of bvh.rs:681-739).  Returns
+        (source triangle index or 0xffffffff, distance) per ray."""
+        rays = f32(rays)
+        n = rays.shape[0]
+        tri = np.zeros(n, dtype=np.uint32)
+        dist = np.zeros(n, dtype=np.float32)
+        st = lib().or_avx_intersect_packets(self._h, ptr(rays), n, ptr(tri), ptr(dist), nthreads)
+        if st:
+            raise RuntimeError("oracle avx_intersect_packets status %d" % st)
+        return tri, dist
 
 
 def avx_available():

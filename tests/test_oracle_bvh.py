@@ -214,3 +214,24 @@ def test_packet_traversal_agrees(oracle, models_dir):
     assert (a["tuv"][same].view(np.uint32) == p["tuv"][same].view(np.uint32)).all()
     # the packet visits at least the boxes one ray of it visits
     assert (p["numi_aabb"] >= a["numi_aabb"]).all()
+
+
+@pytest.mark.parametrize("name", ["suzanne", "stanford_bunny"])
+def test_avx_packets_match_scalar_packets(oracle, models_dir, name):
+    """The AVX2 8-wide packet queries (the host baseline of the batched
+    intersect bench) and the scalar restatement of the same packet traversal
+    (bvh.rs:589-657) give the same triangle and distance bits, for decoherent
+    and coherent inward rays (bench.rs:195-223)."""
+    if not oracle.avx_available():
+        pytest.skip("host without AVX2 + FMA")
+    sd = model_scene(models_dir, name)
+    s = oracle.Scene(sd.tris, sd.materials)
+    rays = np.concatenate([scenes.rays_inward(4096, 6, radius=10.0),
+                           scenes.rays_inward(4096, 7, radius=10.0, coherent=True)])
+    tri, dist = s.avx_intersect_packets(rays, nthreads=4)
+    p = s.intersect_packet(rays)
+    assert (tri == p["tri"]).all()
+    assert (dist.view(np.uint32) == p["tuv"][:, 0].view(np.uint32)).all()
+    assert (tri != oracle.NO_HIT).mean() > 0.05
+    with pytest.raises(RuntimeError):
+        s.avx_intersect_packets(rays[:12])
