@@ -659,12 +659,14 @@ def bench_rays(args):
             scene = cv.Scene.from_meshes([mesh])
             info = scene.bvh_info()
             tun = scene.tuning()
-            kernel = "k_intersect<false, %d, %s>" % (tun["stack_entries"],
-                                                    "true" if info["max_leaf"] <= 31 else "false")
+            kernel = "k_extend_refill<false, %d, %s, false>" % (
+                tun["stack_entries"], "true" if info["max_leaf"] <= 31 else "false")
             for coh in (False, True):
                 label = "%s %s" % (name, "coherent" if coh else "decoherent")
-                # counting pass (box and triangle tests per ray, identical in
-                # the timed calls, which skip the counters)
+                # counting pass: full cv_isect output with per-ray box and
+                # triangle test counts (k_intersect); the timed hits-only
+                # calls run the lane-refill traversal, whose tests per ray
+                # are the same
                 isd = DeviceBuffer(64 * n)
                 scene.intersect_nearest_device(rays_d[coh].ptr.value, n, hits_d.ptr.value,
                                                isd.ptr.value)
@@ -736,7 +738,8 @@ def bench_rays(args):
             "traffic": None,
             "note": "algorithmic node and triangle bytes (32 B per box test, 36 B per triangle "
                     "test) over the wall time of back-to-back calls (an upper bound on the "
-                    "launch time); both meshes fit in L2, so the bound is VALU issue, not HBM",
+                    "traversal's launch time: it includes the ray packing and source-id "
+                    "kernels); both meshes fit in L2, so the bound is VALU issue, not HBM",
         },
     }
     if not args.no_cpu:

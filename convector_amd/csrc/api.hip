@@ -424,7 +424,9 @@ static cv_status scene_apply_tuning(cv_scene* s, const cv_tuning& t) {
   uint32_t need = s->info.depth + 4;
   size_t spill_entries = need > (uint32_t)s->cfg.stack ? (size_t)(need - s->cfg.stack) : 0;
   HIPTRY(hipStreamSynchronize(s->stream), "sync");
-  HIPTRY(s->spill.alloc(spill_entries * s->cfg.blocks * 64 + 1), "alloc spill");
+  // lanes of a launch: the waves rounded up to whole 4-wave workgroups
+  HIPTRY(s->spill.alloc(spill_entries * (((size_t)s->cfg.blocks + 3) / 4 * 4) * 64 + 1),
+         "alloc spill");
   return CV_OK;
 }
 
@@ -632,6 +634,9 @@ void cv_scene_destroy(cv_scene* s) {
   s->lights.reset();
   s->spill.reset();
   s->cursor.reset();
+  s->q_r0.reset();
+  s->q_r1.reset();
+  s->q_ctrl.reset();
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
 }
@@ -664,6 +669,46 @@ cv_status cv_bvh_intersect_nearest(const cv_scene* sc, const cv_ray* rays, size_
       HIPTRY(d_isects.alloc(16 * n), "alloc");
       ip = d_isects.p;
     }
+  }
+  if (hits && !isects) {
+    // Hits only: the renderer's lane-refill traversal (idle lanes take new
+    // rays, lanes at leaves wait for a common leaf round) over the rays as
+    // a one-bounce extend queue; each lane's operations are those of the
+    // single-ray traversal, so the hits are the same bits.
+    if (s->q_r0.n < n || s->q_r1.n < n || !s->q_ctrl.p) {
+      HIPTRY(hipStreamSynchronize(st), "sync");
+      HIPTRY(s->q_r0.reserve(n), "alloc");
+      HIPTRY(s->q_r1.reserve(n), "alloc");
+      HIPTRY(s->q_ctrl.reserve(1), "alloc");
+    }
+    HIPTRY(cvk::launch_isect_prepare(rp, (uint32_t)n, s->q_r0.p, s->q_r1.p, s->q_ctrl.p, st),
+           "prepare");
+    cvk::ExtendArgs x{};
+    x.ctrl = s->q_ctrl.p;
+    x.bounce = 0;
+    x.r0 = s->q_r0.p;
+    x.r1 = s->q_r1.p;
+    x.hits = reinterpret_cast<uint4*>(hp);
+    x.nodes = s->nodes.p;
+    x.tris = s->tris.p;
+    x.jobs = nullptr;
+    x.spill = s->spill.p;
+    // At least 4 batches of 64 rays per cursor fetch: queries end after a
+    // few dozen box tests, so one fetch per 64 rays serialises on the
+    // cursor (3.4 ms for 2^24 rays whatever the scene; 2.0 ms with 4 on
+    // 968 triangles).  Rendering keeps the tuned value (1 is fastest there).
+    x.batches = s->cfg.fetch_batches > 4 ? s->cfg.fetch_batches : 4u;
+    x.refill = s->cfg.refill <= 64 ? s->cfg.refill : 0u;
+    x.interval = s->cfg.refill_interval;
+    x.leaf_batch = s->cfg.leaf_batch <= 64 ? s->cfg.leaf_batch : 0u;
+    HIPTRY(cvk::launch_extend(x, false, s->packed, false, s->cfg.stack, s->cfg.blocks,
+                              s->cfg.block_waves, st),
+           "intersect");
+    HIPTRY(cvk::launch_isect_finish(reinterpret_cast<uint4*>(hp), s->tri_src.p, (uint32_t)n, st),
+           "finish");
+    if (!dev) HIPTRY(hipMemcpyAsync(hits, hp, 16 * n, hipMemcpyDeviceToHost, st), "download");
+    HIPTRY(hipStreamSynchronize(st), "sync");
+    return CV_OK;
   }
   cvk::IntersectArgs a{};
   a.rays = rp;

@@ -123,6 +123,12 @@ struct IntersectArgs {
 
 hipError_t launch_intersect(const IntersectArgs& a, bool count, bool packed, int stack,
                             uint32_t blocks, hipStream_t s);
+// Hits-only queries through launch_extend: rays (n x 6 floats) into an
+// extend queue and a control block with count[0] = n; afterwards BVH
+// positions in hits.x replaced by source indices.
+hipError_t launch_isect_prepare(const float* rays, uint32_t n, float4* r0, float4* r1, Ctrl* ctrl,
+                                hipStream_t s);
+hipError_t launch_isect_finish(uint4* hits, const uint32_t* tri_src, uint32_t n, hipStream_t s);
 // multi: rays of the step belong to more than one scene (per-ray node and
 // triangle pointers from the job table).
 // blocks = persistent waves; block_waves (1, 2, 4) waves per workgroup in

@@ -1170,6 +1170,46 @@ hipError_t launch_fold_hdr(float* hdr, float* slots, size_t slot_stride, uint32_
   return hipGetLastError();
 }
 
+// cv_bvh_intersect_nearest without cv_isect output runs the lane-refill
+// traversal of the extend step: the rays as a one-bounce extend queue
+// (origin, direction as float4; the w words unused) and its control block.
+__global__ __launch_bounds__(256) void k_isect_prepare(const float* rays, uint32_t n, float4* r0,
+                                                     float4* r1, Ctrl* ctrl) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i == 0) {
+    Ctrl c{};
+    c.count[0] = n;
+    *ctrl = c;
+  }
+  if (i < n) {
+    const float* r = rays + 6 * (size_t)i;
+    r0[i] = make_float4(r[0], r[1], r[2], 0.0f);
+    r1[i] = make_float4(r[3], r[4], r[5], 0.0f);
+  }
+}
+
+// The extend step's hits name BVH positions; the API's name source
+// triangles (the miss value passes through).
+__global__ __launch_bounds__(256) void k_isect_finish(uint4* hits, const uint32_t* tri_src,
+                                                    uint32_t n) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t best = hits[i].x;
+  if (best != kNoHit) hits[i].x = tri_src[best];
+}
+
+hipError_t launch_isect_prepare(const float* rays, uint32_t n, float4* r0, float4* r1, Ctrl* ctrl,
+                                hipStream_t s) {
+  hipLaunchKernelGGL(k_isect_prepare, dim3((n + 255) / 256), dim3(256), 0, s, rays, n, r0, r1,
+                     ctrl);
+  return hipGetLastError();
+}
+
+hipError_t launch_isect_finish(uint4* hits, const uint32_t* tri_src, uint32_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_isect_finish, dim3((n + 255) / 256), dim3(256), 0, s, hits, tri_src, n);
+  return hipGetLastError();
+}
+
 hipError_t launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t s) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_zero_u32, dim3((n + 255) / 256), dim3(256), 0, s, p, n);

@@ -180,3 +180,27 @@ def test_tuning_validation(cv, models_dir):
             gsc.set_tuning(**bad)
         assert e.value.name == "CV_ERR_INVALID_ARGUMENT"
     assert gsc.tuning() == before
+
+
+def test_hits_only_refill_path_exact(cv, oracle, models_dir):
+    """Calls without cv_isect output run the lane-refill traversal (rays as
+    a one-bounce extend queue, hits mapped to source indices): the same
+    hit bits as the oracle for every knob that traversal reads, on a tree
+    deep enough to spill the stack at 8 entries."""
+    sd = model_scene(models_dir, "stanford_dragon")
+    osc = oracle.Scene.from_scene_data(sd)
+    gsc = cv.Scene.from_scene_data(sd)
+    lo, hi = sd.tris.reshape(-1, 3).min(0), sd.tris.reshape(-1, 3).max(0)
+    rays = _ray_families((lo + hi) / 2, float((hi - lo).max()), 90)
+    ref = osc.intersect(rays)
+    cases = [{}, {"stack_entries": 16}, {"refill_threshold": 65},
+             {"refill_threshold": 1, "refill_interval": 1}, {"leaf_batch": 65},
+             {"leaf_batch": 1}, {"block_waves": 4, "extend_waves_per_cu": 7},
+             {"fetch_batches": 4, "refill_threshold": 64}]
+    for kw in cases:
+        print("hits-only case", kw, len(rays), flush=True)
+        gsc.set_tuning(**kw)
+        _check_exact(gsc.intersect_nearest(rays), None, ref)
+    # a smaller call after a larger one reuses the queue; a ragged tail
+    part = rays[:1001]
+    _check_exact(gsc.intersect_nearest(part), None, osc.intersect(part))
