@@ -659,7 +659,7 @@ def bench_rays(args):
             scene = cv.Scene.from_meshes([mesh])
             info = scene.bvh_info()
             tun = scene.tuning()
-            kernel = "k_extend_refill<false, %d, %s, false>" % (
+            kernel = "k_extend_refill<false, %d, %s, false, true>" % (
                 tun["stack_entries"], "true" if info["max_leaf"] <= 31 else "false")
             for coh in (False, True):
                 label = "%s %s" % (name, "coherent" if coh else "decoherent")

@@ -634,8 +634,6 @@ void cv_scene_destroy(cv_scene* s) {
   s->lights.reset();
   s->spill.reset();
   s->cursor.reset();
-  s->q_r0.reset();
-  s->q_r1.reset();
   s->q_ctrl.reset();
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
@@ -672,22 +670,17 @@ cv_status cv_bvh_intersect_nearest(const cv_scene* sc, const cv_ray* rays, size_
   }
   if (hits && !isects) {
     // Hits only: the renderer's lane-refill traversal (idle lanes take new
-    // rays, lanes at leaves wait for a common leaf round) over the rays as
-    // a one-bounce extend queue; each lane's operations are those of the
-    // single-ray traversal, so the hits are the same bits.
-    if (s->q_r0.n < n || s->q_r1.n < n || !s->q_ctrl.p) {
-      HIPTRY(hipStreamSynchronize(st), "sync");
-      HIPTRY(s->q_r0.reserve(n), "alloc");
-      HIPTRY(s->q_r1.reserve(n), "alloc");
-      HIPTRY(s->q_ctrl.reserve(1), "alloc");
-    }
-    HIPTRY(cvk::launch_isect_prepare(rp, (uint32_t)n, s->q_r0.p, s->q_r1.p, s->q_ctrl.p, st),
-           "prepare");
+    // rays, lanes at leaves wait for a common leaf round) as a one-bounce
+    // step over the caller's rays, writing source indices; each lane's
+    // operations are those of the single-ray traversal, so the hits are
+    // the same bits.
+    if (!s->q_ctrl.p) HIPTRY(s->q_ctrl.alloc(1), "alloc");
+    HIPTRY(cvk::launch_query_ctrl(s->q_ctrl.p, (uint32_t)n, st), "query control");
     cvk::ExtendArgs x{};
     x.ctrl = s->q_ctrl.p;
     x.bounce = 0;
-    x.r0 = s->q_r0.p;
-    x.r1 = s->q_r1.p;
+    x.rays6 = rp;
+    x.tri_src = s->tri_src.p;
     x.hits = reinterpret_cast<uint4*>(hp);
     x.nodes = s->nodes.p;
     x.tris = s->tris.p;
@@ -704,8 +697,6 @@ cv_status cv_bvh_intersect_nearest(const cv_scene* sc, const cv_ray* rays, size_
     HIPTRY(cvk::launch_extend(x, false, s->packed, false, s->cfg.stack, s->cfg.blocks,
                               s->cfg.block_waves, st),
            "intersect");
-    HIPTRY(cvk::launch_isect_finish(reinterpret_cast<uint4*>(hp), s->tri_src.p, (uint32_t)n, st),
-           "finish");
     if (!dev) HIPTRY(hipMemcpyAsync(hits, hp, 16 * n, hipMemcpyDeviceToHost, st), "download");
     HIPTRY(hipStreamSynchronize(st), "sync");
     return CV_OK;

@@ -124,8 +124,7 @@ struct cv_scene {
   cvh::LaunchCfg cfg;              // traversal launch shape (cv_scene_set_tuning)
   cvh::DevBuf<uint2> spill;        // stack spill area of cv_bvh_intersect_nearest
   cvh::DevBuf<uint32_t> cursor;
-  cvh::DevBuf<float4> q_r0, q_r1;  // hits-only queries: the rays as an extend queue
-  cvh::DevBuf<cvk::Ctrl> q_ctrl;
+  cvh::DevBuf<cvk::Ctrl> q_ctrl;   // hits-only queries: one-bounce step control block
   std::mutex mu;
 };
 

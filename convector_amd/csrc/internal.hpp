@@ -90,6 +90,8 @@ struct ExtendArgs {
   uint32_t interval; // lane refill: traversal steps between refill checks (>= 1)
   uint32_t leaf_batch;  // lane refill: > 0: lanes at leaves wait until leaf_batch / 64 of
                         // the active lanes do (or all)
+  const float* rays6;         // non-null (batched queries): the rays as n x 6 floats, not r0/r1
+  const uint32_t* tri_src;    // non-null (batched queries): hits name source triangles
 };
 
 struct ShadeArgs {
@@ -123,12 +125,9 @@ struct IntersectArgs {
 
 hipError_t launch_intersect(const IntersectArgs& a, bool count, bool packed, int stack,
                             uint32_t blocks, hipStream_t s);
-// Hits-only queries through launch_extend: rays (n x 6 floats) into an
-// extend queue and a control block with count[0] = n; afterwards BVH
-// positions in hits.x replaced by source indices.
-hipError_t launch_isect_prepare(const float* rays, uint32_t n, float4* r0, float4* r1, Ctrl* ctrl,
-                                hipStream_t s);
-hipError_t launch_isect_finish(uint4* hits, const uint32_t* tri_src, uint32_t n, hipStream_t s);
+// Hits-only queries through launch_extend: a zeroed control block with
+// count[0] = n.
+hipError_t launch_query_ctrl(Ctrl* ctrl, uint32_t n, hipStream_t s);
 // multi: rays of the step belong to more than one scene (per-ray node and
 // triangle pointers from the job table).
 // blocks = persistent waves; block_waves (1, 2, 4) waves per workgroup in

@@ -567,6 +567,27 @@ __global__ __launch_bounds__(256) void k_generate(GenerateArgs a) {
 // extend: nearest hit of every queued ray
 // ---------------------------------------------------------------------------
 
+// A queued ray: the extend queue's float4 pair or, for batched queries, the
+// caller's rays as 6 floats (QUERY: known at compile time; otherwise a
+// wave-uniform branch).
+template <bool QUERY = false>
+__device__ __forceinline__ void load_ray(const ExtendArgs& a, uint32_t i, V3& o, V3& d) {
+  if (QUERY || a.rays6) {
+    const float* r = a.rays6 + 6 * (size_t)i;
+    o = mk3(r[0], r[1], r[2]);
+    d = mk3(r[3], r[4], r[5]);
+  } else {
+    const float4 r0 = a.r0[i], r1 = a.r1[i];
+    o = mk3(r0.x, r0.y, r0.z);
+    d = mk3(r1.x, r1.y, r1.z);
+  }
+}
+// The hit's triangle: its BVH position, or its source index for queries.
+template <bool QUERY = false>
+__device__ __forceinline__ uint32_t hit_id(const ExtendArgs& a, uint32_t best) {
+  return (QUERY || a.tri_src) && best != kNoHit ? a.tri_src[best] : best;
+}
+
 template <bool COUNT, int S, bool PACKED>
 __global__ __launch_bounds__(64) void k_extend(ExtendArgs a) {
   __shared__ uint2 lds[S * 64];
@@ -586,15 +607,11 @@ __global__ __launch_bounds__(64) void k_extend(ExtendArgs a) {
     uint32_t i = base + lane;
     bool valid = i < n;
     V3 o = mk3(0, 0, 0), d = mk3(1, 0, 0);
-    if (valid) {
-      float4 r0 = a.r0[i], r1 = a.r1[i];
-      o = mk3(r0.x, r0.y, r0.z);
-      d = mk3(r1.x, r1.y, r1.z);
-    }
+    if (valid) load_ray(a, i, o, d);
     TravState ts = traverse<COUNT, S, PACKED>(a.nodes, a.tris, o, d, lds + lane,
                                               a.spill + gtid, stride, valid);
     if (valid) {
-      a.hits[i] = make_uint4(ts.best, f2u(ts.best_t), f2u(ts.bu), f2u(ts.bv));
+      a.hits[i] = make_uint4(hit_id(a, ts.best), f2u(ts.best_t), f2u(ts.bu), f2u(ts.bv));
       if (COUNT) {
         na += ts.na;
         nt += ts.nt;
@@ -621,7 +638,7 @@ __global__ __launch_bounds__(64) void k_extend(ExtendArgs a) {
 // traverse(), one trav_step per loop iteration.
 // Workgroups of 1, 2 or 4 independent waves (the launch's choice); each
 // wave owns S * 64 entries of the dynamic LDS.
-template <bool COUNT, int S, bool PACKED, bool MULTI>
+template <bool COUNT, int S, bool PACKED, bool MULTI, bool QUERY = false>
 __global__ __launch_bounds__(256) void k_extend_refill(ExtendArgs a) {
   extern __shared__ uint2 lds_waves[];
   const uint32_t lane = lane_id();
@@ -670,13 +687,19 @@ __global__ __launch_bounds__(256) void k_extend_refill(ExtendArgs a) {
         const uint32_t rank = __popcll(idle & lt);
         const uint32_t i = rank < from_pool ? base0 + rank : base1 + (rank - from_pool);
         if (rank < from_pool || i < end1) {
-          float4 r0 = a.r0[i], r1 = a.r1[i];
-          o = mk3(r0.x, r0.y, r0.z);
-          d = mk3(r1.x, r1.y, r1.z);
           if (MULTI) {
+            const float4 r0 = a.r0[i], r1 = a.r1[i];
+            o = mk3(r0.x, r0.y, r0.z);
+            d = mk3(r1.x, r1.y, r1.z);
             const JobDesc& J = a.jobs[f2u(r1.w)];
             nodes = J.nodes;
             tris = J.tris;
+          } else if (QUERY) {
+            load_ray<true>(a, i, o, d);
+          } else {
+            const float4 r0 = a.r0[i], r1 = a.r1[i];
+            o = mk3(r0.x, r0.y, r0.z);
+            d = mk3(r1.x, r1.y, r1.z);
           }
           inv = mk3(rcp(d.x), rcp(d.y), rcp(d.z));
           exact = needs_exact_nan(o, inv);
@@ -718,7 +741,8 @@ __global__ __launch_bounds__(256) void k_extend_refill(ExtendArgs a) {
         trav_step<COUNT>(nodes, tris, o, d, inv, wave_exact, st, ts, cur, have, leaf_round,
                          &waiting);
         if (!have && st.sp == 0) {
-          a.hits[ray] = make_uint4(ts.best, f2u(ts.best_t), f2u(ts.bu), f2u(ts.bv));
+          a.hits[ray] = make_uint4(QUERY ? hit_id<true>(a, ts.best) : ts.best, f2u(ts.best_t),
+                                   f2u(ts.bu), f2u(ts.bv));
           if (COUNT) {
             na += ts.na;
             nt += ts.nt;
@@ -1090,6 +1114,14 @@ static hipError_t launch_extend_s(const ExtendArgs& a, bool packed, bool multi, 
       hipLaunchKernelGGL((k_extend_refill<COUNT, S, true, true>), grid, block, shmem, s, a);
     else
       hipLaunchKernelGGL((k_extend_refill<COUNT, S, false, true>), grid, block, shmem, s, a);
+  } else if (a.refill && a.rays6 && !COUNT) {
+    // batched queries: the caller's rays and source ids, compiled apart so
+    // that the render instantiations carry neither
+    if (packed)
+      hipLaunchKernelGGL((k_extend_refill<false, S, true, false, true>), grid, block, shmem, s, a);
+    else
+      hipLaunchKernelGGL((k_extend_refill<false, S, false, false, true>), grid, block, shmem, s,
+                         a);
   } else if (a.refill) {
     if (packed)
       hipLaunchKernelGGL((k_extend_refill<COUNT, S, true, false>), grid, block, shmem, s, a);
@@ -1171,42 +1203,19 @@ hipError_t launch_fold_hdr(float* hdr, float* slots, size_t slot_stride, uint32_
 }
 
 // cv_bvh_intersect_nearest without cv_isect output runs the lane-refill
-// traversal of the extend step: the rays as a one-bounce extend queue
-// (origin, direction as float4; the w words unused) and its control block.
-__global__ __launch_bounds__(256) void k_isect_prepare(const float* rays, uint32_t n, float4* r0,
-                                                     float4* r1, Ctrl* ctrl) {
-  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-  if (i == 0) {
-    Ctrl c{};
-    c.count[0] = n;
-    *ctrl = c;
-  }
-  if (i < n) {
-    const float* r = rays + 6 * (size_t)i;
-    r0[i] = make_float4(r[0], r[1], r[2], 0.0f);
-    r1[i] = make_float4(r[3], r[4], r[5], 0.0f);
-  }
+// traversal of the extend step over the caller's rays: the control block of
+// a one-bounce step.
+// One word per lane: zeros, and n in count[0].
+__global__ __launch_bounds__(64) void k_query_ctrl(Ctrl* ctrl, uint32_t n) {
+  static_assert(sizeof(Ctrl) % 4 == 0, "Ctrl is whole words");
+  constexpr uint32_t kWords = sizeof(Ctrl) / 4;
+  constexpr uint32_t kCount0 = offsetof(Ctrl, count) / 4;
+  uint32_t* w = reinterpret_cast<uint32_t*>(ctrl);
+  for (uint32_t k = threadIdx.x; k < kWords; k += 64) w[k] = k == kCount0 ? n : 0u;
 }
 
-// The extend step's hits name BVH positions; the API's name source
-// triangles (the miss value passes through).
-__global__ __launch_bounds__(256) void k_isect_finish(uint4* hits, const uint32_t* tri_src,
-                                                    uint32_t n) {
-  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t best = hits[i].x;
-  if (best != kNoHit) hits[i].x = tri_src[best];
-}
-
-hipError_t launch_isect_prepare(const float* rays, uint32_t n, float4* r0, float4* r1, Ctrl* ctrl,
-                                hipStream_t s) {
-  hipLaunchKernelGGL(k_isect_prepare, dim3((n + 255) / 256), dim3(256), 0, s, rays, n, r0, r1,
-                     ctrl);
-  return hipGetLastError();
-}
-
-hipError_t launch_isect_finish(uint4* hits, const uint32_t* tri_src, uint32_t n, hipStream_t s) {
-  hipLaunchKernelGGL(k_isect_finish, dim3((n + 255) / 256), dim3(256), 0, s, hits, tri_src, n);
+hipError_t launch_query_ctrl(Ctrl* ctrl, uint32_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_query_ctrl, dim3(1), dim3(64), 0, s, ctrl, n);
   return hipGetLastError();
 }
 

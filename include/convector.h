@@ -190,10 +190,9 @@ typedef struct cv_isect {
  * (scene.rs:220-244, bvh.rs:589-669), batched, one ray per GPU lane.  Each
  * ray is traversed as the reference traverses a packet holding that ray in
  * every lane.  hits and isects may each be NULL.  A call with hits and no
- * isects keeps a device copy of the rays (32 bytes per ray, grown to the
- * largest call) with the scene until cv_scene_destroy, and traverses with
- * the scene's tuning, taking at least 4 batches of 64 rays per fetch.  hits
- * must be 16-byte aligned.  Returns after the results are written. */
+ * isects traverses with the scene's tuning, taking at least 4 batches of 64
+ * rays per fetch.  hits must be 16-byte aligned.  Returns after the results
+ * are written. */
 cv_status cv_bvh_intersect_nearest(const cv_scene* scene, const cv_ray* rays, size_t n,
                                    cv_hit* hits, cv_isect* isects, uint32_t flags);
 
