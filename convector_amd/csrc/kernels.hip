@@ -530,8 +530,24 @@ __global__ __launch_bounds__(256) void k_generate(GenerateArgs a) {
     const uint32_t j = job_of(a.jobs, njobs, p);
     const JobDesc& J = a.jobs[j];
     const uint32_t k = p - J.path_base;
-    uint32_t px = J.rect_x + k % J.rect_w;
-    uint32_t py = J.rect_y + k / J.rect_w;
+    // The rectangle in bands of 8 rows, each walked column by column, so
+    // that the 64 paths of a wave are an 8 x 8 tile of camera rays rather
+    // than a 64-pixel row strip; the last band holds the rect_h % 8 rows
+    // left over.  Images do not depend on the order: every result is
+    // keyed by the path's pixel.
+    const uint32_t full = J.rect_h & ~7u;
+    uint32_t tx, ty;
+    if (k < full * J.rect_w) {
+      const uint32_t b = k / (8u * J.rect_w), r = k - b * (8u * J.rect_w);
+      tx = r >> 3;
+      ty = b * 8u + (r & 7u);
+    } else {
+      const uint32_t hb = J.rect_h - full, r = k - full * J.rect_w;
+      tx = r / hb;
+      ty = full + r % hb;
+    }
+    uint32_t px = J.rect_x + tx;
+    uint32_t py = J.rect_y + ty;
     uint32_t pixel = py * J.width + px;
     uint32_t xb = px & ~15u, yb = py & ~3u, dx = px & 15u, dy = py & 3u;
     uint32_t g = (dx >> 2) * 2 + (dy >> 1);
