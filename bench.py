@@ -1018,8 +1018,8 @@ def main():
     achieved_gbs = (bytes_per_step / launches_per_step) / (avg_launch_ms * 1e-3) / 1e9
     achieved_tops = (ops_per_step / launches_per_step) / (avg_launch_ms * 1e-3) / 1e12
     tun = scene.tuning()
-    kernel = "k_extend_refill<false, %d, %s, false>" % (tun["stack_entries"],
-                                                       "true" if info["max_leaf"] <= 31 else "false")
+    kernel = "k_extend_refill<false, %d, %s, false, false>" % (
+        tun["stack_entries"], "true" if info["max_leaf"] <= 31 else "false")
     traffic = measured_traffic(kernel) if tun["refill_threshold"] <= 64 else None
     out = {
         "metric": "Mrays/s vs host AVX",
