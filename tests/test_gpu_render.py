@@ -253,4 +253,3 @@ def test_trace_export(cv, tmp_path):
         cv.trace_disable()
     # tracing changes nothing in the image
     assert hdr.tobytes() == plain.tobytes()
-
