@@ -1453,20 +1453,32 @@ static inline v3 microfacet_brdf(uint32_t material, v3 in_dir, v3 out_dir, v3 no
     return scale3(fc, d);
 }
 
-/* Scene::get_direct_sample for one lane (scene.rs:143-202) */
-static inline void direct_sample(const or_scene* s, uint32_t bits, float u, float v, v3* pos) {
+/* Scene::get_direct_sample for one lane (scene.rs:143-202).  normal and area
+ * (may be NULL) are computed as the reference computes them; its caller
+ * continue_path_direct_sample (material.rs:253-265) uses only the position. */
+static inline void direct_sample(const or_scene* s, uint32_t bits, float u, float v, v3* pos,
+                                 v3* normal, float* area) {
     uint32_t idx = bits >> 29;
     const float* tv = s->v + 9 * (size_t)s->direct[idx];
     v3 v0 = ld3(tv), v1 = ld3(tv + 3), v2 = ld3(tv + 6);
     v3 e1 = sub3(v0, v2);
     v3 e2 = sub3(v1, v0);
-    /* normal and area are computed by the reference but unused by
-     * continue_path_direct_sample (material.rs:253-265) */
+    if (normal || area) {
+        v3 nd = cross3(e1, e2);
+        float rn = or_rsqrt(dot3(nd, nd), s->approx);
+        if (normal) *normal = scale3(nd, rn);
+        if (area) *area = 0.5f * or_rcp(rn, s->approx);
+    }
     float sgn_s = (1.0f - u) - v;
     uint32_t m = f2u(sgn_s);
     float uu = pickf(u, 1.0f - u, m);
     float vv = pickf(v, 1.0f - v, m);
     *pos = muladd3(e2, vv, negmuladd3(e1, uu, v0));
+}
+
+/* pd_brdf (material.rs:245-250) */
+static inline float pd_brdf(v3 normal, v3 dir) {
+    return xmax(dot3(normal, dir), 0.0f) * (1.0f / 3.14159274f);
 }
 
 /* pd_direct_sample (material.rs:281-320) */
@@ -1499,15 +1511,14 @@ static void continue_path(const or_scene* s, const ray1* ray, const isect1* is,
     v3 org_b = muladd3(dir_b, 1.0e-4f, is->pos);
     /* continue_path_direct_sample (material.rs:253-265) */
     v3 p;
-    direct_sample(s, dr->d_u32, u32_unit(dr->d_u), u32_unit(dr->d_v), &p);
+    direct_sample(s, dr->d_u32, u32_unit(dr->d_u), u32_unit(dr->d_v), &p, NULL, NULL);
     v3 dir_d = normalized3(sub3(p, is->pos), approx);
     v3 org_d = muladd3(dir_d, 1.0e-4f, is->pos);
     float rr = fmaf(is->normal.z, 0.8f, u32_biunit(dr->d_rr));
     uint32_t rrm = f2u(rr);
     v3 n_org = pick3(org_b, org_d, rrm);
     v3 n_dir = pick3(dir_b, dir_d, rrm);
-    /* pd_brdf (material.rs:245-250) */
-    float pd_b = xmax(dot3(is->normal, n_dir), 0.0f) * (1.0f / 3.14159274f);
+    float pd_b = pd_brdf(is->normal, n_dir);
     float pd_d = pd_direct(s, n_org, n_dir, approx);
     float weight_denom = (pd_b + pd_d) + 0.01f;
     float p_brdf = fmaf(is->normal.z, 0.4f, 0.5f);
@@ -1523,6 +1534,105 @@ static void continue_path(const or_scene* s, const ray1* ray, const isect1* is,
     out->d = pick3(n_dir, ray->d, active);
     out->active = active;
     *color_mod = pick3(cm, mk3(1.0f, 1.0f, 1.0f), active);
+}
+
+/* ---- single-lane shading entry points (checked against independent FP64
+ *      restatements and closed forms by the tests) ---- */
+
+float or_microfacet_normal_dist(const float h[3], const float n[3], int32_t gloss) {
+    return normal_dist(ld3(h), ld3(n), gloss);
+}
+
+void or_microfacet_fresnel(const float incoming[3], const float h[3], const float color[3],
+                           float out[3], float* raw) {
+    v3 f = fresnel(ld3(incoming), ld3(h), ld3(color), raw);
+    out[0] = f.x;
+    out[1] = f.y;
+    out[2] = f.z;
+}
+
+void or_microfacet_brdf(uint32_t material, const float in_dir[3], const float out_dir[3],
+                        const float normal[3], int ignore_fresnel, int approx, float out[3],
+                        float* raw) {
+    v3 b = microfacet_brdf(material, ld3(in_dir), ld3(out_dir), ld3(normal), ignore_fresnel,
+                           approx, raw);
+    out[0] = b.x;
+    out[1] = b.y;
+    out[2] = b.z;
+}
+
+float or_pd_brdf(const float normal[3], const float dir[3]) {
+    return pd_brdf(ld3(normal), ld3(dir));
+}
+
+void or_sky_intensity_lane(const float dir[3], float out[3]) {
+    v3 c = sky_intensity(ld3(dir));
+    out[0] = c.x;
+    out[1] = c.y;
+    out[2] = c.z;
+}
+
+int or_pd_direct(const or_scene* s, const float* rays, size_t n, float* out) {
+    if (s->ndirect == 0) return -1;
+    for (size_t i = 0; i < n; i++)
+        out[i] = pd_direct(s, ld3(rays + 6 * i), ld3(rays + 6 * i + 3), s->approx);
+    return 0;
+}
+
+int or_direct_sample(const or_scene* s, const uint32_t* draws3, size_t n, float* pos,
+                     float* normal, float* area) {
+    if (s->ndirect != 8) return -1; /* the index bits >> 29 address exactly 8 (scene.rs:147) */
+    for (size_t i = 0; i < n; i++) {
+        const uint32_t* d = draws3 + 3 * i;
+        v3 p, nm;
+        float a;
+        direct_sample(s, d[0], u32_unit(d[1]), u32_unit(d[2]), &p, &nm, &a);
+        pos[3 * i] = p.x;
+        pos[3 * i + 1] = p.y;
+        pos[3 * i + 2] = p.z;
+        if (normal) {
+            normal[3 * i] = nm.x;
+            normal[3 * i + 1] = nm.y;
+            normal[3 * i + 2] = nm.z;
+        }
+        if (area) area[i] = a;
+    }
+    return 0;
+}
+
+int or_continue_path(const or_scene* s, size_t n, const float* ray7, const float* isect7,
+                     const uint32_t* draws6, int ignore_fresnel, float* out_ray7,
+                     float* color_mod3, float* fresnel_raw) {
+    if (s->ndirect != 8) return -1;
+    for (size_t i = 0; i < n; i++) {
+        const float* r = ray7 + 7 * i;
+        const float* is = isect7 + 7 * i;
+        const uint32_t* d = draws6 + 6 * i;
+        ray1 ray = {ld3(r), ld3(r + 3), f2u(r[6])};
+        isect1 hit;
+        memset(&hit, 0, sizeof(hit));
+        hit.pos = ld3(is);
+        hit.normal = ld3(is + 3);
+        hit.mat = f2u(is[6]);
+        path_draws dr = {d[0], d[1], d[2], d[3], d[4], d[5]};
+        ray1 nr;
+        v3 cm;
+        float fr;
+        continue_path(s, &ray, &hit, &dr, ignore_fresnel, &nr, &cm, &fr);
+        float* o = out_ray7 + 7 * i;
+        o[0] = nr.o.x;
+        o[1] = nr.o.y;
+        o[2] = nr.o.z;
+        o[3] = nr.d.x;
+        o[4] = nr.d.y;
+        o[5] = nr.d.z;
+        o[6] = u2f(nr.active);
+        color_mod3[3 * i] = cm.x;
+        color_mod3[3 * i + 1] = cm.y;
+        color_mod3[3 * i + 2] = cm.z;
+        if (fresnel_raw) fresnel_raw[i] = fr;
+    }
+    return 0;
 }
 
 /* Camera::get_ray (scene.rs:81-99) */

@@ -132,6 +132,39 @@ void or_intersect_nearest_packet(const or_scene* s, const float* rays, size_t n,
 void or_intersect_brute_f64(const or_scene* s, const float* rays, size_t n, uint32_t* tri_id,
                             double* tuv);
 
+/* ---- shading, one lane at a time (material.rs, scene.rs) ---- */
+/* microfacet_normal_dist (material.rs:475-513); gloss index 0..5 */
+float or_microfacet_normal_dist(const float h[3], const float n[3], int32_t gloss);
+/* microfacet_fresnel (material.rs:453-469): out = Schlick colour, raw = ct5 */
+void or_microfacet_fresnel(const float incoming[3], const float h[3], const float color[3],
+                           float out[3], float* raw);
+/* microfacet_brdf (material.rs:415-450) */
+void or_microfacet_brdf(uint32_t material, const float in_dir[3], const float out_dir[3],
+                        const float normal[3], int ignore_fresnel, int approx, float out[3],
+                        float* raw);
+/* pd_brdf (material.rs:245-250) */
+float or_pd_brdf(const float normal[3], const float dir[3]);
+/* sky_intensity (material.rs:205-215) */
+void or_sky_intensity_lane(const float dir[3], float out[3]);
+/* pd_direct_sample (material.rs:281-320) over the scene's direct-sample
+ * triangles for n rays (n x 6).  Returns -1 when the scene has none. */
+int or_pd_direct(const or_scene* s, const float* rays, size_t n, float* out);
+/* Scene::get_direct_sample (scene.rs:143-202) for n lanes; draws3 holds the
+ * lane's sample_u32 draw and its two sample_unit draws (raw u32).  normal
+ * and area may be NULL.  Returns -1 unless the scene has exactly 8
+ * direct-sample triangles. */
+int or_direct_sample(const or_scene* s, const uint32_t* draws3, size_t n, float* pos,
+                     float* normal, float* area);
+/* continue_path (material.rs:327-411) for n independent lanes.
+ * ray7: origin, direction, active bits; isect7: position, normal, material
+ * bits; draws6: the six draws in the reference's order (angle, unit, u32,
+ * unit, unit, biunit).  Outputs the new ray (same layout), the colour
+ * modulation and the raw Fresnel factor (may be NULL).  Returns -1 unless
+ * the scene has exactly 8 direct-sample triangles. */
+int or_continue_path(const or_scene* s, size_t n, const float* ray7, const float* isect7,
+                     const uint32_t* draws6, int ignore_fresnel, float* out_ray7,
+                     float* color_mod3, float* fresnel_raw);
+
 /* ---- renderer (renderer.rs) ---- */
 /* render_patch_u8 (renderer.rs:306-327). bitmap/gbuffer: width*height u32. */
 int or_render_patch_u8(const or_scene* s, uint32_t width, uint32_t height, uint32_t* bitmap,

@@ -100,6 +100,14 @@ def _declare(L):
         "or_avx_accumulate_frames": (I32, [P, U32, U32, P, U32, U32, I32, I32, P]),
         "or_avx_sample_patches": (I32, [P, U32, U32, U32, I32, P, U32, P]),
         "or_avx_intersect_packets": (I32, [P, P, SZ, P, P, I32]),
+        "or_microfacet_normal_dist": (F32, [P, P, ctypes.c_int32]),
+        "or_microfacet_fresnel": (None, [P, P, P, P, P]),
+        "or_microfacet_brdf": (None, [U32, P, P, P, I32, I32, P, P]),
+        "or_pd_brdf": (F32, [P, P]),
+        "or_sky_intensity_lane": (None, [P, P]),
+        "or_pd_direct": (I32, [P, P, SZ, P]),
+        "or_direct_sample": (I32, [P, P, SZ, P, P, P]),
+        "or_continue_path": (I32, [P, SZ, P, P, P, I32, P, P, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -277,6 +285,42 @@ class Scene:
             raise RuntimeError("oracle sample_patches status %d" % st)
         return int(out[0]), int(out[1]), int(out[2])
 
+    def pd_direct(self, rays):
+        """pd_direct_sample (material.rs:281-320) for n rays (n x 6)."""
+        rays = f32(rays)
+        out = np.zeros(rays.shape[0], dtype=np.float32)
+        if lib().or_pd_direct(self._h, ptr(rays), rays.shape[0], ptr(out)):
+            raise RuntimeError("scene has no direct-sample triangles")
+        return out
+
+    def direct_sample(self, draws3):
+        """Scene::get_direct_sample (scene.rs:143-202) per lane from its
+        (sample_u32, sample_unit, sample_unit) draws: (position, normal, area)."""
+        d = u32(draws3).reshape(-1, 3)
+        n = d.shape[0]
+        pos = np.zeros((n, 3), dtype=np.float32)
+        nrm = np.zeros((n, 3), dtype=np.float32)
+        area = np.zeros(n, dtype=np.float32)
+        if lib().or_direct_sample(self._h, ptr(d), n, ptr(pos), ptr(nrm), ptr(area)):
+            raise RuntimeError("direct sampling needs exactly 8 direct-sample triangles")
+        return pos, nrm, area
+
+    def continue_path(self, ray7, isect7, draws6, ignore_fresnel=False):
+        """continue_path (material.rs:327-411) for independent lanes:
+        returns (new ray n x 7, colour modulation n x 3, raw Fresnel n)."""
+        ray7 = f32(ray7).reshape(-1, 7)
+        isect7 = f32(isect7).reshape(-1, 7)
+        d = u32(draws6).reshape(-1, 6)
+        n = ray7.shape[0]
+        assert isect7.shape[0] == n and d.shape[0] == n
+        out = np.zeros((n, 7), dtype=np.float32)
+        cm = np.zeros((n, 3), dtype=np.float32)
+        fr = np.zeros(n, dtype=np.float32)
+        if lib().or_continue_path(self._h, n, ptr(ray7), ptr(isect7), ptr(d),
+                                  1 if ignore_fresnel else 0, ptr(out), ptr(cm), ptr(fr)):
+            raise RuntimeError("continue_path needs exactly 8 direct-sample triangles")
+        return out, cm, fr
+
     def avx_accumulate_frames(self, width, height, first_frame, n_frames, per_lane=False,
                               nthreads=8, hdr=None):
         """The 8-wide AVX2 path (reference Rng per patch, the scene's approx
@@ -315,6 +359,38 @@ class Scene:
         if st:
             raise RuntimeError("oracle avx_intersect_packets status %d" % st)
         return tri, dist
+
+
+def microfacet_normal_dist(h, n, gloss):
+    return float(lib().or_microfacet_normal_dist(ptr(f32(h)), ptr(f32(n)), int(gloss)))
+
+
+def microfacet_fresnel(incoming, h, color):
+    out = np.zeros(3, dtype=np.float32)
+    raw = np.zeros(1, dtype=np.float32)
+    lib().or_microfacet_fresnel(ptr(f32(incoming)), ptr(f32(h)), ptr(f32(color)), ptr(out),
+                                ptr(raw))
+    return out, float(raw[0])
+
+
+def microfacet_brdf(material, in_dir, out_dir, normal, ignore_fresnel=False,
+                    approx=APPROX_PORTABLE):
+    out = np.zeros(3, dtype=np.float32)
+    raw = np.zeros(1, dtype=np.float32)
+    lib().or_microfacet_brdf(int(material), ptr(f32(in_dir)), ptr(f32(out_dir)),
+                             ptr(f32(normal)), 1 if ignore_fresnel else 0, approx, ptr(out),
+                             ptr(raw))
+    return out, float(raw[0])
+
+
+def pd_brdf(normal, direction):
+    return float(lib().or_pd_brdf(ptr(f32(normal)), ptr(f32(direction))))
+
+
+def sky_intensity(direction):
+    out = np.zeros(3, dtype=np.float32)
+    lib().or_sky_intensity_lane(ptr(f32(direction)), ptr(out))
+    return out
 
 
 def avx_available():
