@@ -50,6 +50,27 @@ def _p(a):
     return a.ctypes.data_as(ctypes.c_void_p)
 
 
+def _host_buf(a, dtype, count: int, name: str):
+    """Pointer of a caller buffer the library reads or writes count elements
+    of: a device pointer given as int (taken as is), or a C-contiguous,
+    writeable numpy array of exactly `dtype` holding at least count
+    elements.  Anything else raises instead of letting the library read or
+    write past the array."""
+    if a is None:
+        return None
+    if isinstance(a, int):
+        return a
+    if not isinstance(a, np.ndarray):
+        raise TypeError("%s must be a numpy array or a device pointer (int)" % name)
+    if a.dtype != np.dtype(dtype):
+        raise TypeError("%s must have dtype %s, not %s" % (name, np.dtype(dtype), a.dtype))
+    if not a.flags.c_contiguous or not a.flags.writeable:
+        raise ValueError("%s must be C-contiguous and writeable" % name)
+    if a.size < count:
+        raise ValueError("%s holds %d elements, the call needs %d" % (name, a.size, count))
+    return a.ctypes.data
+
+
 class Mesh:
     """A triangle mesh (wavefront.rs:18-28) held by the library."""
 
@@ -259,13 +280,18 @@ class Renderer:
     def new_bitmap(self) -> np.ndarray:
         return np.zeros(self.width * self.height, np.uint32)
 
+    def _px(self, a, dtype, per_pixel, name):
+        return _p(_host_buf(a, dtype, self.width * self.height * per_pixel, name))
+
     def render_patch_u8(self, bitmap, gbuffer, patch_width, x, y, frame_number):
-        check(lib().cv_renderer_render_patch_u8(self._h, _p(bitmap), _p(gbuffer), patch_width, x,
-                                                y, frame_number, 0))
+        check(lib().cv_renderer_render_patch_u8(
+            self._h, self._px(bitmap, np.uint32, 1, "bitmap"),
+            self._px(gbuffer, np.uint32, 1, "gbuffer"), patch_width, x, y, frame_number, 0))
 
     def accumulate_patch_f32(self, hdr, gbuffer, patch_width, x, y, frame_number):
-        check(lib().cv_renderer_accumulate_patch_f32(self._h, _p(hdr), _p(gbuffer), patch_width,
-                                                     x, y, frame_number, 0))
+        check(lib().cv_renderer_accumulate_patch_f32(
+            self._h, self._px(hdr, np.float32, 3, "hdr"),
+            self._px(gbuffer, np.uint32, 1, "gbuffer"), patch_width, x, y, frame_number, 0))
 
     def render_frame_u8(self, frame_number, gbuffer=True):
         bitmap = self.new_bitmap()
@@ -274,8 +300,9 @@ class Renderer:
         return bitmap, gb
 
     def accumulate_frames_f32(self, hdr, first_frame, n_frames, gbuffer=None, flags=0):
-        check(lib().cv_renderer_accumulate_frames_f32(self._h, _p(hdr), _p(gbuffer), first_frame,
-                                                      n_frames, flags))
+        check(lib().cv_renderer_accumulate_frames_f32(
+            self._h, self._px(hdr, np.float32, 3, "hdr"),
+            self._px(gbuffer, np.uint32, 1, "gbuffer"), first_frame, n_frames, flags))
 
     def new_features(self) -> np.ndarray:
         """First-hit features: 8 f32 per pixel in rows (albedo rgb, normal
@@ -283,8 +310,9 @@ class Renderer:
         return np.zeros(self.width * self.height * 8, np.float32)
 
     def accumulate_aov_f32(self, hdr, features, first_frame, n_frames, flags=0):
-        check(lib().cv_renderer_accumulate_aov_f32(self._h, _p(hdr), _p(features), first_frame,
-                                                   n_frames, flags))
+        check(lib().cv_renderer_accumulate_aov_f32(
+            self._h, self._px(hdr, np.float32, 3, "hdr"),
+            self._px(features, np.float32, 8, "features"), first_frame, n_frames, flags))
 
     def accumulate_frames_device(self, hdr_ptr: int, first_frame, n_frames, flags=0):
         check(lib().cv_renderer_accumulate_frames_f32(self._h, ctypes.c_void_p(hdr_ptr), None,
@@ -293,8 +321,8 @@ class Renderer:
 
     def buffer_f32_into_render_buffer(self, hdr, num_samples) -> np.ndarray:
         bitmap = self.new_bitmap()
-        check(lib().cv_renderer_buffer_f32_into_bitmap(self._h, _p(hdr), _p(bitmap), num_samples,
-                                                       0))
+        check(lib().cv_renderer_buffer_f32_into_bitmap(
+            self._h, self._px(hdr, np.float32, 3, "hdr"), _p(bitmap), num_samples, 0))
         return bitmap
 
     def stats(self) -> dict:
@@ -323,12 +351,8 @@ class Job:
         self.gbuffer = gbuffer
         self.features = features
 
-    def _ptr(self, a):
-        if a is None:
-            return None
-        if isinstance(a, int):
-            return a
-        return a.ctypes.data
+    def _ptr(self, a, dtype=np.float32, per_pixel=3, name="hdr"):
+        return _host_buf(a, dtype, self.width * self.height * per_pixel, name)
 
 
 class Batch:
@@ -360,8 +384,8 @@ class Batch:
             arr[i].width, arr[i].height = j.width, j.height
             arr[i].first_frame, arr[i].num_frames = j.first_frame, j.num_frames
             arr[i].hdr = j._ptr(j.hdr)
-            arr[i].gbuffer = j._ptr(j.gbuffer)
-            arr[i].features = j._ptr(j.features)
+            arr[i].gbuffer = j._ptr(j.gbuffer, np.uint32, 1, "gbuffer")
+            arr[i].features = j._ptr(j.features, np.float32, 8, "features")
         return arr
 
     def render(self, jobs: Sequence[Job], flags: int = 0):
@@ -518,7 +542,7 @@ class Server:
         r.camera_pos[:] = [float(x) for x in camera_pos]
         r.camera_q[:] = [float(x) for x in camera_q]
         r.screen_distance = float(screen_distance)
-        r.hdr = hdr if isinstance(hdr, int) else hdr.ctypes.data
+        r.hdr = _host_buf(hdr, np.float32, width * height * 3, "hdr")
         t = ctypes.c_uint64()
         check(lib().cv_server_submit(self._h, ctypes.byref(r), ctypes.byref(t)))
         self._live[t.value] = hdr

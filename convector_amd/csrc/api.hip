@@ -60,6 +60,37 @@ uint32_t num_cus() { return g_num_cus ? g_num_cus : 256u; }
 
 const char* last_error() { return g_last_error.c_str(); }
 
+namespace {
+// One pinned bounce buffer for the whole process, allocated on first use and
+// kept until the process ends (freeing pinned memory at exit can run after
+// the HIP runtime is torn down).
+constexpr size_t kBounceBytes = 16u << 20;
+std::mutex g_bounce_mu;
+char* g_bounce = nullptr;
+}  // namespace
+
+cv_status copy_caller_memory(void* dst, const void* src, size_t bytes, bool to_device,
+                             hipStream_t st) {
+  if (bytes == 0) return CV_OK;
+  std::lock_guard<std::mutex> lk(g_bounce_mu);
+  if (!g_bounce) HIPTRY(hipHostMalloc((void**)&g_bounce, kBounceBytes, hipHostMallocDefault),
+                        "pinned alloc");
+  for (size_t off = 0; off < bytes; off += kBounceBytes) {
+    const size_t c = bytes - off < kBounceBytes ? bytes - off : kBounceBytes;
+    if (to_device) {
+      memcpy(g_bounce, (const char*)src + off, c);
+      HIPTRY(hipMemcpyAsync((char*)dst + off, g_bounce, c, hipMemcpyHostToDevice, st), "upload");
+      HIPTRY(hipStreamSynchronize(st), "upload");
+    } else {
+      HIPTRY(hipMemcpyAsync(g_bounce, (const char*)src + off, c, hipMemcpyDeviceToHost, st),
+             "download");
+      HIPTRY(hipStreamSynchronize(st), "download");
+      memcpy((char*)dst + off, g_bounce, c);
+    }
+  }
+  return CV_OK;
+}
+
 cv_status validate_tuning(const cv_tuning& t) {
   if (t.stack_entries && t.stack_entries != 8 && t.stack_entries != 16 && t.stack_entries != 24 &&
       t.stack_entries != 32)

@@ -22,6 +22,12 @@ cv_status hip_fail(hipError_t e, const char* what);
 cv_status ensure_device();
 uint32_t num_cus();
 const char* last_error();
+// Copies between caller host memory and the device through a pinned bounce
+// buffer, a chunk at a time, waiting for each chunk (and so for the work
+// queued before it on st): the runtime never copies from or to the caller's
+// pageable memory, and nothing touches that memory after the call returns.
+cv_status copy_caller_memory(void* dst, const void* src, size_t bytes, bool to_device,
+                             hipStream_t st);
 
 inline uint32_t grid_for(size_t n, uint32_t block, uint32_t cap = 1u << 20) {
   size_t b = (n + block - 1) / block;

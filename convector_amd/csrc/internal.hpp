@@ -125,6 +125,23 @@ struct IntersectArgs {
 
 hipError_t launch_intersect(const IntersectArgs& a, bool count, bool packed, int stack,
                             uint32_t blocks, hipStream_t s);
+
+// CV_RNG_REF_PATCH rendering: patches (x, y, width, 0) of the job's image,
+// frames first_frame .. first_frame + nframes - 1 of each in order, into
+// the job's hdr (added) or bitmap, and its gbuffer (the last frame's).
+struct RefPatchArgs {
+  JobDesc job;
+  const uint4* patches;
+  uint32_t npatches;
+  uint32_t first_frame;
+  uint32_t nframes;
+  uint32_t max_bounces;
+  uint2* spill;  // stack entries beyond the LDS part: stride gridDim.x * 64 lanes
+};
+// blocks: 64-lane workgroups, 8 patches each at a time (at most the
+// spill area's waves).
+hipError_t launch_ref_patch(const RefPatchArgs& a, bool packed, int stack, uint32_t blocks,
+                            hipStream_t s);
 // Hits-only queries through launch_extend: a zeroed control block with
 // count[0] = n.
 hipError_t launch_query_ctrl(Ctrl* ctrl, uint32_t n, hipStream_t s);

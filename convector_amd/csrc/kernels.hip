@@ -790,12 +790,18 @@ struct ShadeOut {
   float fresnel;
 };
 
+// The six draws of one continue_path call in the reference's order
+// (material.rs:342-354): hemisphere angle and unit, light index bits and
+// the two unit coordinates of the light sample, the biunit of the choice.
+struct PathDraws {
+  uint32_t angle, unit, bits, u, v, rr;
+};
+
 __device__ __forceinline__ ShadeOut continue_path(const JobDesc& J, V3 ray_d, V3 pos, V3 normal,
-                                                  uint32_t mat, uint64_t& st,
+                                                  uint32_t mat, const PathDraws& dr,
                                                   bool ignore_fresnel) {
-  const uint32_t d_angle = rng_next(st), d_unit = rng_next(st);
-  const uint32_t d_bits = rng_next(st), d_u = rng_next(st), d_v = rng_next(st);
-  const uint32_t d_rr = rng_next(st);
+  const uint32_t d_angle = dr.angle, d_unit = dr.unit, d_bits = dr.bits, d_u = dr.u, d_v = dr.v;
+  const uint32_t d_rr = dr.rr;
   // continue_path_brdf (material.rs:228-242), sample_hemisphere_vector (random.rs:140-159)
   float phi = u32_angle(d_angle), r_sqr = u32_unit(d_unit);
   float rr_ = sqrtf(r_sqr);
@@ -888,6 +894,20 @@ __device__ __forceinline__ ShadeOut continue_path(const JobDesc& J, V3 ray_d, V3
   r.d = nd;
   r.fresnel = ct5;
   return r;
+}
+
+// continue_path with the draws taken from a PIXEL stream.
+__device__ __forceinline__ ShadeOut continue_path(const JobDesc& J, V3 ray_d, V3 pos, V3 normal,
+                                                  uint32_t mat, uint64_t& st,
+                                                  bool ignore_fresnel) {
+  PathDraws dr;
+  dr.angle = rng_next(st);
+  dr.unit = rng_next(st);
+  dr.bits = rng_next(st);
+  dr.u = rng_next(st);
+  dr.v = rng_next(st);
+  dr.rr = rng_next(st);
+  return continue_path(J, ray_d, pos, normal, mat, dr, ignore_fresnel);
 }
 
 // One path per thread; blocks beyond the queue exit at once.  Survivors are
@@ -1096,6 +1116,169 @@ __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
 
 namespace cvk {
 
+// ---------------------------------------------------------------------------
+// REF_PATCH: the reference's own random streams (random.rs:33-136,
+// renderer.rs:292-468), one 8-lane group at a time
+// ---------------------------------------------------------------------------
+
+// Rng::with_seed (random.rs:43-70) and Rng::next with the Mu64 -> 8 x u32
+// view (random.rs:73-101): every lane of a group holds the whole 4 x u64
+// state and advances it identically; lane l reads the low (l even) or high
+// (l odd) half of u64 l / 2.
+struct RefRng {
+  uint64_t s0, s1, s2, s3;
+};
+
+__device__ __forceinline__ RefRng ref_rng_seed(uint32_t x, uint32_t y, uint32_t i) {
+  uint64_t seed = (uint64_t)x * 12276630456901467871ull + (uint64_t)y * 7661526868048087387ull +
+                  (uint64_t)i * 2268244495640532043ull;
+  seed = seed + seed % 9358246936573323101ull;
+  RefRng r;
+  r.s0 = seed * 14491630826648200009ull;
+  r.s1 = seed * 13149596372461506851ull;
+  r.s2 = seed * 6119410235796056053ull;
+  r.s3 = seed * 14990141545859273719ull;
+  return r;
+}
+
+__device__ __forceinline__ uint32_t ref_rng_next(RefRng& r, uint32_t l) {
+  const uint32_t k = l >> 1;
+  const uint64_t v = k == 0 ? r.s0 : k == 1 ? r.s1 : k == 2 ? r.s2 : r.s3;
+  r.s0 *= 3ull * 1073243692214514217ull;
+  r.s1 *= 5ull * 3335100457702756523ull;
+  r.s2 *= 7ull * 8789056573444181ull;
+  r.s3 *= 11ull * 781436371140792079ull;
+  return (l & 1u) ? (uint32_t)(v >> 32) : (uint32_t)v;
+}
+
+// One 8-lane group per patch: lanes 8k .. 8k+7 of a wave render patch
+// blockIdx.x * 8 + k (and every 8 * gridDim.x-th after it), frames in
+// order, blocks of 16 x 4 pixels column by column (renderer.rs:306-362),
+// the block's 16 coordinate draws, then its 8 groups one after the other,
+// each drawing t and six draws per bounce from the patch's stream until all
+// 8 of its lanes have hit emitters (renderer.rs:413-468).  A lane's own
+// traversal is the single-ray traversal of the other modes.  The 8 lanes
+// of a group take the same branches at every step (their loop counts are
+// the group's), so the all-emissive vote sees all 8.
+template <int S, bool PACKED>
+__global__ __launch_bounds__(64) void k_ref_patch(RefPatchArgs a) {
+  __shared__ uint2 lds[S * 64];
+  const uint32_t lane = lane_id(), sub = lane >> 3, l = lane & 7u;
+  const uint32_t gtid = blockIdx.x * 64 + lane;
+  const uint32_t stride = gridDim.x * 64;
+  const JobDesc& J = a.job;
+  const float scale = 2.0f / (float)J.width;
+  const V3 cam_pos = mk3(J.cam.pos[0], J.cam.pos[1], J.cam.pos[2]);
+  const V3 cam_dpos = mk3(J.cam.dpos[0], J.cam.dpos[1], J.cam.dpos[2]);
+  for (uint32_t pidx = blockIdx.x * 8 + sub; pidx < a.npatches; pidx += gridDim.x * 8) {
+    const uint4 P = a.patches[pidx];
+    const uint32_t x = P.x, y = P.y, pw = P.z;
+    for (uint32_t f = 0; f < a.nframes; f++) {
+      RefRng rng = ref_rng_seed(x, y, a.first_frame + f);
+      for (uint32_t bi = 0; bi < pw / 16; bi++)
+        for (uint32_t bj = 0; bj < pw / 4; bj++) {
+          const uint32_t xb = x + 16 * bi, yb = y + 4 * bj;
+          const float cx = (float)xb - (float)J.width * 0.5f;
+          const float cy = (float)yb - (float)J.height * 0.5f;
+          const float bx = scale * ((float)(l & 3u) + cx);
+          const float by = scale * ((float)(l >> 2) + cy);
+          float xs[8], ys[8];
+#pragma unroll
+          for (int g = 0; g < 8; g++) {
+            const int gx = g >> 1;
+            xs[g] = gx == 0 ? bx : gx == 1 ? bx + 4.0f * scale : gx == 2 ? bx + 8.0f * scale
+                                                                          : bx + 12.0f * scale;
+            ys[g] = (g & 1) ? by + 2.0f * scale : by;
+          }
+#pragma unroll
+          for (int g = 0; g < 8; g++) xs[g] = fmaf(u32_unit(ref_rng_next(rng, l)), scale, xs[g]);
+#pragma unroll
+          for (int g = 0; g < 8; g++) ys[g] = fmaf(u32_unit(ref_rng_next(rng, l)), scale, ys[g]);
+          for (int g = 0; g < 8; g++) {
+            // render_pixels for group g (renderer.rs:413-468)
+            const float t = u32_unit(ref_rng_next(rng, l));
+            V3 o = muladd3(cam_dpos, t, cam_pos);
+            V3 d = camera_dir(J.cam, xs[g], ys[g], t);
+            uint32_t active = 0;  // sign bit 1: inactive (ray.rs:21-33)
+            V3 color = mk3(1.0f, 1.0f, 1.0f);
+            uint32_t hit_em = 0;
+            bool term_tri = false;
+            uint32_t tex = 0;
+            float gtu = 0.0f, gtv = 0.0f, gfr = 0.0f;
+            for (uint32_t b = 0; b < a.max_bounces; b++) {
+              const bool act = (active >> 31) == 0;
+              TravState ts = traverse<false, S, PACKED>(J.nodes, J.tris, o, d, lds + lane,
+                                                        a.spill + gtid, stride, act);
+              V3 pos, normal;
+              uint32_t mat;
+              float tu = 0.0f, tv = 0.0f;
+              if (!act || ts.best == kNoHit) {
+                // Scene::intersect_nearest's far-away sky hit (scene.rs:220-230)
+                pos = muladd3(d, 1.0e5f, o);
+                normal = d;
+                mat = kMatSky;
+              } else {
+                const float4* tp = reinterpret_cast<const float4*>(J.tris + ts.best);
+                pos = muladd3(d, ts.best_t, o);
+                normal = normalized3(mk3(tp[0].w, tp[1].w, tp[2].w));
+                mat = J.tri_mat[ts.best];
+                const float* uv = J.tri_uv + 6 * (size_t)ts.best;
+                const float w = (1.0f - ts.bu) - ts.bv;
+                tu = fmaf(uv[0], w, fmaf(uv[2], ts.bv, uv[4] * ts.bu));
+                tv = fmaf(uv[1], w, fmaf(uv[3], ts.bv, uv[5] * ts.bu));
+              }
+              if (act && (mat >> 31)) term_tri = ts.best != kNoHit;
+              hit_em = mat;
+              // break when all 8 lanes of the group hit emitters (renderer.rs:433)
+              const uint64_t em = __ballot((mat >> 31) != 0);
+              if (((em >> (sub * 8)) & 0xffull) == 0xffull) break;
+              PathDraws dr;
+              dr.angle = ref_rng_next(rng, l);
+              dr.unit = ref_rng_next(rng, l);
+              dr.bits = ref_rng_next(rng, l);
+              dr.u = ref_rng_next(rng, l);
+              dr.v = ref_rng_next(rng, l);
+              dr.rr = ref_rng_next(rng, l);
+              const ShadeOut so = continue_path(J, d, pos, normal, mat, dr, b == 0);
+              // an emissive hit (or an inactive ray) keeps its ray and gets
+              // a white modulation (material.rs:338,401-408)
+              active |= mat;
+              if ((active >> 31) == 0) {
+                o = so.o;
+                d = so.d;
+                color = mulc3(color, so.cm);
+              }
+              if (b == 0) {
+                tex = ((int32_t)mat >> 24) & 3;
+                gtu = tu;
+                gtv = tv;
+                gfr = so.fresnel;
+              }
+            }
+            // renderer.rs:452-460: colour x emission of the last ray, zero
+            // unless the last intersection was emissive
+            V3 emis = (J.has_override && term_tri)
+                          ? mk3(J.emission[0], J.emission[1], J.emission[2])
+                          : sky_intensity(d);
+            V3 c = mulc3(color, emis);
+            if (!(hit_em >> 31)) c = mk3(0.0f, 0.0f, 0.0f);
+            const uint32_t px = xb + 4 * ((uint32_t)g >> 1) + (l & 3u);
+            const uint32_t py = yb + 2 * ((uint32_t)g & 1u) + (l >> 2);
+            const size_t pixel = (size_t)py * J.width + px;
+            if (J.hdr) {
+              float* h = J.hdr + hdr_offset(px, py, J.width);
+              h[0] += c.x;
+              h[8] += c.y;
+              h[16] += c.z;
+            }
+            if (J.bitmap) J.bitmap[pixel] = pack_color(c);
+            if (J.gbuffer) J.gbuffer[pixel] = pack_gbuffer(tex, gtu, gtv, gfr);
+          }
+        }
+    }
+  }
+}
+
 #define CVK_STACK_VARIANTS(X) X(8) X(16) X(24) X(32)
 
 template <bool COUNT, int S>
@@ -1233,6 +1416,26 @@ __global__ __launch_bounds__(64) void k_query_ctrl(Ctrl* ctrl, uint32_t n) {
 hipError_t launch_query_ctrl(Ctrl* ctrl, uint32_t n, hipStream_t s) {
   hipLaunchKernelGGL(k_query_ctrl, dim3(1), dim3(64), 0, s, ctrl, n);
   return hipGetLastError();
+}
+
+template <int S>
+static hipError_t launch_ref_patch_s(const RefPatchArgs& a, bool packed, uint32_t blocks,
+                                     hipStream_t s) {
+  if (packed)
+    hipLaunchKernelGGL((k_ref_patch<S, true>), dim3(blocks), dim3(64), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_ref_patch<S, false>), dim3(blocks), dim3(64), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_ref_patch(const RefPatchArgs& a, bool packed, int stack, uint32_t blocks,
+                            hipStream_t s) {
+  if (a.npatches == 0) return hipSuccess;
+#define CASE(S) \
+  if (stack == S) return launch_ref_patch_s<S>(a, packed, blocks, s);
+  CVK_STACK_VARIANTS(CASE)
+#undef CASE
+  return hipErrorInvalidValue;
 }
 
 hipError_t launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t s) {

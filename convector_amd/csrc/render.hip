@@ -343,12 +343,18 @@ cv_status pipeline_run(Pipeline& p, const std::vector<StepPlan>& steps, const Ru
   key.tris = o.tris;
   key.capacity = p.capacity;
   EventList evs;
-  hipEvent_t t0, t1;
+  // t0 / t1 bracket the steps; every exit below the loop destroys them and
+  // waits for the stream, also when a step fails part-way
+  hipEvent_t t0 = nullptr, t1 = nullptr;
   HIPTRY(hipEventCreate(&t0), "event");
-  HIPTRY(hipEventCreate(&t1), "event");
-  HIPTRY(hipEventRecord(t0, st), "event");
+  if (hipEventCreate(&t1) != hipSuccess) {
+    (void)hipEventDestroy(t0);
+    return fail(CV_ERR_DEVICE, "event");
+  }
   cv_status status = CV_OK;
+  if (hipEventRecord(t0, st) != hipSuccess) status = fail(CV_ERR_DEVICE, "event");
   for (const StepPlan& s : steps) {
+    if (status != CV_OK) break;
     if (s.paths == 0) continue;
     const uint32_t slot = p.next_slot;
     p.next_slot = (p.next_slot + 1) % kRingSlots;
@@ -374,7 +380,11 @@ cv_status pipeline_run(Pipeline& p, const std::vector<StepPlan>& steps, const Ru
         p.graph_defs[slot] = nullptr;
         std::unique_lock<std::mutex> cl;
         if (o.capture_mu) cl = std::unique_lock<std::mutex>(*o.capture_mu);
-        HIPTRY(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed), "graph capture");
+        hipError_t be = hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed);
+        if (be != hipSuccess) {
+          status = hip_fail(be, "graph capture");
+          break;
+        }
         cv_status ls = launch_step(p, o, slot, s.paths, true, nullptr);
         hipGraph_t gdef = nullptr;
         hipError_t ce = hipStreamEndCapture(st, &gdef);
@@ -388,21 +398,35 @@ cv_status pipeline_run(Pipeline& p, const std::vector<StepPlan>& steps, const Ru
           if (status != CV_OK) break;
         } else {
           p.graph_defs[slot] = gdef;
-          HIPTRY(hipGraphInstantiate(&p.graphs[slot], gdef, nullptr, nullptr, 0),
-                 "graph instantiate");
+          hipError_t ie = hipGraphInstantiate(&p.graphs[slot], gdef, nullptr, nullptr, 0);
+          if (ie != hipSuccess) {
+            p.graphs[slot] = nullptr;
+            status = hip_fail(ie, "graph instantiate");
+            break;
+          }
           p.keys[slot] = key;
         }
       }
-      if (p.graphs[slot]) HIPTRY(hipGraphLaunch(p.graphs[slot], st), "graph launch");
+      if (p.graphs[slot]) {
+        hipError_t le = hipGraphLaunch(p.graphs[slot], st);
+        if (le != hipSuccess) {
+          status = hip_fail(le, "graph launch");
+          break;
+        }
+      }
     } else {
       status = launch_step(p, o, slot, s.paths, false, timing ? &evs : nullptr,
                            s.jobs.empty() ? 0u : s.jobs[0].frame);
       if (status != CV_OK) break;
     }
-    for (const FoldDesc& fd : s.folds)
-      HIPTRY(cvk::launch_fold_hdr(fd.hdr, fd.slots, fd.stride, fd.count, fd.width, fd.rect[0],
-                                  fd.rect[1], fd.rect[2], fd.rect[3], st),
-             "fold");
+    for (const FoldDesc& fd : s.folds) {
+      hipError_t fe = cvk::launch_fold_hdr(fd.hdr, fd.slots, fd.stride, fd.count, fd.width,
+                                           fd.rect[0], fd.rect[1], fd.rect[2], fd.rect[3], st);
+      if (fe != hipSuccess) {
+        status = hip_fail(fe, "fold");
+        break;
+      }
+    }
   }
   (void)hipEventRecord(t1, st);
   hipError_t se = hipStreamSynchronize(st);
@@ -458,20 +482,26 @@ struct cv_renderer {
   DevBuf<float> slots;
   bool slots_clean = false;
   DevBuf<uint32_t> bitmap, gbuffer;
+  DevBuf<uint4> ref_patches;  // CV_RNG_REF_PATCH: the call's patches (x, y, width, 0)
   cv_render_stats stats{};
   std::mutex mu;
 };
 
 namespace {
 
-cv_status check_opts(const cv_render_opts* opts, cv_render_opts* o) {
+// allow_ref_patch: renderers render CV_RNG_REF_PATCH too (patch-serial, a
+// correctness mode); batches and servers share launches between jobs and
+// take CV_RNG_PIXEL only.
+cv_status check_opts(const cv_render_opts* opts, cv_render_opts* o, bool allow_ref_patch) {
   *o = cv_render_opts{CV_RNG_PIXEL, CV_APPROX_PORTABLE, 5};
   if (opts) *o = *opts;
   if (o->max_bounces == 0) o->max_bounces = 5;
   if (o->max_bounces >= (uint32_t)cvk::kMaxBounceSlots)
     return fail(CV_ERR_INVALID_ARGUMENT, "max_bounces too large");
-  if (o->rng_mode != CV_RNG_PIXEL)
-    return fail(CV_ERR_INVALID_ARGUMENT, "the GPU renderer implements CV_RNG_PIXEL");
+  if (o->rng_mode != CV_RNG_PIXEL && !(allow_ref_patch && o->rng_mode == CV_RNG_REF_PATCH))
+    return fail(CV_ERR_INVALID_ARGUMENT,
+                allow_ref_patch ? "rng_mode must be CV_RNG_PIXEL or CV_RNG_REF_PATCH"
+                                : "batches and servers render CV_RNG_PIXEL");
   if (o->approx_mode != CV_APPROX_PORTABLE)
     return fail(CV_ERR_INVALID_ARGUMENT, "the GPU renderer implements CV_APPROX_PORTABLE");
   return CV_OK;
@@ -490,9 +520,85 @@ cv_status check_frame(const cv_scene* s, uint32_t width, uint32_t height) {
   return CV_OK;
 }
 
+// The rectangle of a render call; patch > 0 for the patch calls (one
+// reference patch of that width at x, y), 0 for whole frames.
 struct Rect {
   uint32_t x, y, w, h;
+  uint32_t patch;
 };
+
+// CV_RNG_REF_PATCH: the reference's patch streams.  A patch call renders
+// its one patch; a frame call renders the patch loop of main.rs:174-214:
+// 32 x 32 patches, columns outer, rows inner, and where fewer than 32
+// columns or rows remain, that 32 x 32 cell tiled with 16 x 16 patches
+// (the reference needs frames of whole patches).  Each patch seeds its
+// stream with (x, y, frame), so the partition is part of the result.
+cv_status ref_patch_render(cv_renderer* r, const Rect& rc, uint32_t first, uint32_t nframes,
+                           const cvk::JobDesc& job, bool features) {
+  if (features)
+    return fail(CV_ERR_INVALID_ARGUMENT, "first-hit features are rendered with CV_RNG_PIXEL");
+  cv_scene* s = r->scene;
+  const uint32_t W = r->width, H = r->height;
+  std::vector<uint4> patches;
+  if (rc.patch) {
+    patches.push_back(make_uint4(rc.x, rc.y, rc.patch, 0u));
+  } else {
+    if ((H & 15) != 0)
+      return fail(CV_ERR_NOT_MULTIPLE_OF_16,
+                  "CV_RNG_REF_PATCH frames are whole 16 x 16 patches: height must be a multiple "
+                  "of 16");
+    const uint32_t npx = (W + 31) / 32, npy = (H + 31) / 32;
+    for (uint32_t p = 0; p < npx * npy; p++) {
+      const uint32_t x = (p / npy) * 32, y = (p % npy) * 32;
+      const uint32_t pw = (W - x >= 32 && H - y >= 32) ? 32u : 16u;
+      const uint32_t xe = std::min(x + 32, W), ye = std::min(y + 32, H);
+      for (uint32_t yy = y; yy < ye; yy += pw)
+        for (uint32_t xx = x; xx < xe; xx += pw) patches.push_back(make_uint4(xx, yy, pw, 0u));
+    }
+  }
+  hipStream_t st = r->pipe.stream;
+  cv_status cs = r->pipe.reserve_spill(s->cfg, s->info.depth);
+  if (cs != CV_OK) return cs;
+  if (r->ref_patches.n < patches.size()) HIPTRY(r->ref_patches.alloc(patches.size()), "alloc");
+  cs = cvh::copy_caller_memory(r->ref_patches.p, patches.data(), patches.size() * sizeof(uint4),
+                               true, st);
+  if (cs != CV_OK) return cs;
+  cvk::RefPatchArgs a{};
+  a.job = job;
+  a.patches = r->ref_patches.p;
+  a.npatches = (uint32_t)patches.size();
+  a.first_frame = first;
+  a.nframes = nframes;
+  a.max_bounces = r->opts.max_bounces;
+  a.spill = r->pipe.spill.p;
+  // 8 patches per 64-lane workgroup, within the spill area's waves
+  const uint32_t blocks = std::max<uint32_t>(
+      1u, std::min<uint32_t>((a.npatches + 7) / 8, s->cfg.blocks));
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  HIPTRY(hipEventCreate(&t0), "event");
+  if (hipEventCreate(&t1) != hipSuccess) {
+    (void)hipEventDestroy(t0);
+    return fail(CV_ERR_DEVICE, "event");
+  }
+  cv_status status = CV_OK;
+  (void)hipEventRecord(t0, st);
+  hipError_t le = cvk::launch_ref_patch(a, s->packed, s->cfg.stack, blocks, st);
+  if (le != hipSuccess) status = hip_fail(le, "ref patch render");
+  (void)hipEventRecord(t1, st);
+  hipError_t se = hipStreamSynchronize(st);
+  if (status == CV_OK && se != hipSuccess) status = hip_fail(se, "ref patch render");
+  float ms = 0.0f;
+  (void)hipEventElapsedTime(&ms, t0, t1);
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  cv_render_stats stt{};
+  stt.paths = (uint64_t)rc.w * rc.h * nframes;
+  stt.total_ms = ms;
+  stt.extend_ms = ms;
+  stt.extend_launches = 1;
+  r->stats = stt;
+  return status;
+}
 
 cv_status run_frames(cv_renderer* r, const Rect& rc, uint32_t first, uint32_t nframes, float* hdr,
                      uint32_t* bitmap, uint32_t* gbuffer, uint32_t flags, bool hdr_mode,
@@ -541,6 +647,10 @@ cv_status run_frames(cv_renderer* r, const Rect& rc, uint32_t first, uint32_t nf
   base.bitmap = bm_dev;
   base.features = ft_dev;
   base.path_base = 0;
+  cv_status status = CV_OK;
+  if (r->opts.rng_mode == CV_RNG_REF_PATCH) {
+    status = ref_patch_render(r, rc, first, nframes, base, features != nullptr);
+  } else {
   // Frames per step: an accumulation renders K frames in the same launches
   // (a launch's last rays keep most of the GPU idle, so larger launches are
   // faster per ray); frame k > 0 of a step accumulates into its own zeroed
@@ -601,8 +711,9 @@ cv_status run_frames(cv_renderer* r, const Rect& rc, uint32_t first, uint32_t nf
   o.multi_job = K > 1;
   o.max_depth = s->info.depth;
   o.graph = flags & CV_RENDER_GRAPH;
-  cv_status status = pipeline_run(r->pipe, steps, o, &r->stats);
+  status = pipeline_run(r->pipe, steps, o, &r->stats);
   if (status == CV_OK && K > 1) r->slots_clean = true;
+  }
   if (status == CV_OK && !dev) {
     const size_t row = (size_t)W * 4;
     const size_t off = ((size_t)rc.y * W + rc.x) * 4;
@@ -664,7 +775,7 @@ cv_status cv_renderer_create(cv_scene* s, uint32_t width, uint32_t height,
   cv_status cs = check_frame(s, width, height);
   if (cs != CV_OK) return cs;
   cv_render_opts o;
-  cs = check_opts(opts, &o);
+  cs = check_opts(opts, &o, true);
   if (cs != CV_OK) return cs;
   cv_renderer* r = new cv_renderer();
   r->scene = s;
@@ -692,6 +803,7 @@ void cv_renderer_destroy(cv_renderer* r) {
   r->features.reset();
   r->bitmap.reset();
   r->gbuffer.reset();
+  r->ref_patches.reset();
   delete r;
 }
 
@@ -702,7 +814,7 @@ cv_status cv_renderer_render_patch_u8(cv_renderer* r, uint32_t* bitmap, uint32_t
   if (st != CV_OK) return st;
   if (!bitmap) return fail(CV_ERR_INVALID_ARGUMENT, "null bitmap");
   std::lock_guard<std::mutex> g(r->mu);
-  return run_frames(r, Rect{x, y, patch_width, patch_width}, frame, 1, nullptr, bitmap, gbuffer,
+  return run_frames(r, Rect{x, y, patch_width, patch_width, patch_width}, frame, 1, nullptr, bitmap, gbuffer,
                     flags, false);
 }
 
@@ -713,7 +825,7 @@ cv_status cv_renderer_accumulate_patch_f32(cv_renderer* r, float* hdr, uint32_t*
   if (st != CV_OK) return st;
   if (!hdr) return fail(CV_ERR_INVALID_ARGUMENT, "null hdr");
   std::lock_guard<std::mutex> g(r->mu);
-  return run_frames(r, Rect{x, y, patch_width, patch_width}, frame, 1, hdr, nullptr, gbuffer,
+  return run_frames(r, Rect{x, y, patch_width, patch_width, patch_width}, frame, 1, hdr, nullptr, gbuffer,
                     flags, true);
 }
 
@@ -721,7 +833,7 @@ cv_status cv_renderer_render_frame_u8(cv_renderer* r, uint32_t* bitmap, uint32_t
                                       uint32_t frame, uint32_t flags) {
   if (!r || !bitmap) return fail(CV_ERR_INVALID_ARGUMENT, "null argument");
   std::lock_guard<std::mutex> g(r->mu);
-  return run_frames(r, Rect{0, 0, r->width, r->height}, frame, 1, nullptr, bitmap, gbuffer, flags,
+  return run_frames(r, Rect{0, 0, r->width, r->height, 0}, frame, 1, nullptr, bitmap, gbuffer, flags,
                     false);
 }
 
@@ -730,7 +842,7 @@ cv_status cv_renderer_accumulate_frames_f32(cv_renderer* r, float* hdr, uint32_t
                                             uint32_t flags) {
   if (!r || !hdr) return fail(CV_ERR_INVALID_ARGUMENT, "null argument");
   std::lock_guard<std::mutex> g(r->mu);
-  return run_frames(r, Rect{0, 0, r->width, r->height}, first_frame, n_frames, hdr, nullptr,
+  return run_frames(r, Rect{0, 0, r->width, r->height, 0}, first_frame, n_frames, hdr, nullptr,
                     gbuffer, flags, true);
 }
 
@@ -739,7 +851,7 @@ cv_status cv_renderer_accumulate_aov_f32(cv_renderer* r, float* hdr, float* feat
                                          uint32_t flags) {
   if (!r || !hdr || !features) return fail(CV_ERR_INVALID_ARGUMENT, "null argument");
   std::lock_guard<std::mutex> g(r->mu);
-  return run_frames(r, Rect{0, 0, r->width, r->height}, first_frame, n_frames, hdr, nullptr,
+  return run_frames(r, Rect{0, 0, r->width, r->height, 0}, first_frame, n_frames, hdr, nullptr,
                     nullptr, flags, true, features);
 }
 
@@ -778,7 +890,7 @@ cv_status cv_batch_create(const cv_render_opts* opts, uint64_t max_paths_per_ste
   cv_status cs = cvh::ensure_device();
   if (cs != CV_OK) return cs;
   cv_render_opts o;
-  if ((cs = check_opts(opts, &o)) != CV_OK) return cs;
+  if ((cs = check_opts(opts, &o, false)) != CV_OK) return cs;
   if (max_paths_per_step >= (1ull << 31))
     return fail(CV_ERR_INVALID_ARGUMENT, "max_paths_per_step must be below 2^31");
   cv_batch* b = new cv_batch();
@@ -861,8 +973,11 @@ cv_status cv_batch_render(cv_batch* b, const cv_job* jobs, size_t n, uint32_t fl
     d.features = J.features;
     d.bitmap = nullptr;
     if (!dev) {
+      // caller arrays: hdr and features npix * 12 and npix * 32 bytes (read
+      // and written), gbuffer npix * 4 (written)
       HIPTRY(d_hdr[i].alloc(npix * 3), "alloc hdr");
-      HIPTRY(hipMemcpyAsync(d_hdr[i].p, J.hdr, npix * 12, hipMemcpyHostToDevice, st), "upload hdr");
+      cv_status cs = cvh::copy_caller_memory(d_hdr[i].p, J.hdr, npix * 12, true, st);
+      if (cs != CV_OK) return cs;
       d.hdr = d_hdr[i].p;
       if (J.gbuffer) {
         HIPTRY(d_gb[i].alloc(npix), "alloc gbuffer");
@@ -870,8 +985,8 @@ cv_status cv_batch_render(cv_batch* b, const cv_job* jobs, size_t n, uint32_t fl
       }
       if (J.features) {
         HIPTRY(d_ft[i].alloc(npix * 8), "alloc features");
-        HIPTRY(hipMemcpyAsync(d_ft[i].p, J.features, npix * 32, hipMemcpyHostToDevice, st),
-               "upload features");
+        cs = cvh::copy_caller_memory(d_ft[i].p, J.features, npix * 32, true, st);
+        if (cs != CV_OK) return cs;
         d.features = d_ft[i].p;
       }
     }
@@ -970,18 +1085,14 @@ cv_status cv_batch_render(cv_batch* b, const cv_job* jobs, size_t n, uint32_t fl
   cs = pipeline_run(b->pipe, steps, o, &b->stats);
   b->last_steps = (uint32_t)steps.size();
   if (cs == CV_OK && !dev) {
-    for (size_t i = 0; i < n; i++) {
+    for (size_t i = 0; i < n && cs == CV_OK; i++) {
       const size_t npix = (size_t)jobs[i].width * jobs[i].height;
-      HIPTRY(hipMemcpyAsync(jobs[i].hdr, d_hdr[i].p, npix * 12, hipMemcpyDeviceToHost, st),
-             "download hdr");
-      if (jobs[i].gbuffer)
-        HIPTRY(hipMemcpyAsync(jobs[i].gbuffer, d_gb[i].p, npix * 4, hipMemcpyDeviceToHost, st),
-               "download gbuffer");
-      if (jobs[i].features)
-        HIPTRY(hipMemcpyAsync(jobs[i].features, d_ft[i].p, npix * 32, hipMemcpyDeviceToHost, st),
-               "download features");
+      cs = cvh::copy_caller_memory(jobs[i].hdr, d_hdr[i].p, npix * 12, false, st);
+      if (cs == CV_OK && jobs[i].gbuffer)
+        cs = cvh::copy_caller_memory(jobs[i].gbuffer, d_gb[i].p, npix * 4, false, st);
+      if (cs == CV_OK && jobs[i].features)
+        cs = cvh::copy_caller_memory(jobs[i].features, d_ft[i].p, npix * 32, false, st);
     }
-    HIPTRY(hipStreamSynchronize(st), "download");
   }
   return cs;
 }
@@ -1459,7 +1570,7 @@ cv_status cv_server_create(const cv_render_opts* opts, uint64_t hbm_budget_bytes
   cv_status cs = cvh::ensure_device();
   if (cs != CV_OK) return cs;
   cv_render_opts o;
-  if ((cs = check_opts(opts, &o)) != CV_OK) return cs;
+  if ((cs = check_opts(opts, &o, false)) != CV_OK) return cs;
   if (max_paths_per_step == 0) max_paths_per_step = 1ull << 23;
   if (max_paths_per_step >= (1ull << 31))
     return fail(CV_ERR_INVALID_ARGUMENT, "max_paths_per_step must be below 2^31");

@@ -1871,6 +1871,35 @@ static int patch_impl(const or_scene* s, uint32_t width, uint32_t height, uint32
     return 0;
 }
 
+/* PIXEL mode over a rectangle of whole 16x4 blocks.  Every pixel draws from
+ * its own stream, so the result does not depend on how the frame is cut
+ * into patches; this lets PIXEL frames have heights that are multiples of 4
+ * (an extension: the reference's RenderBuffer asks 16, renderer.rs:45-47). */
+static void pixel_rect_impl(const or_scene* s, uint32_t width, uint32_t height, uint32_t* bitmap,
+                            float* hdr, uint32_t* gbuffer, uint32_t x0, uint32_t y0, uint32_t w,
+                            uint32_t h, uint32_t frame, work_counts* wc) {
+    rng8 unused;
+    memset(&unused, 0, sizeof(unused));
+    unused.mode = OR_RNG_PIXEL;
+    for (uint32_t xb = x0; xb < x0 + w; xb += 16)
+        for (uint32_t yb = y0; yb < y0 + h; yb += 4) {
+            pixel_data8 data[8];
+            render_block(s, width, height, xb, yb, frame, &unused, OR_RNG_PIXEL, data, wc);
+            if (hdr) {
+                float* blk = hdr + ((size_t)(yb / 4) * (width / 16) + xb / 16) * 192;
+                for (int g = 0; g < 8; g++)
+                    for (int l = 0; l < 8; l++) {
+                        blk[g * 24 + 0 * 8 + l] += data[g].color[l].x;
+                        blk[g * 24 + 1 * 8 + l] += data[g].color[l].y;
+                        blk[g * 24 + 2 * 8 + l] += data[g].color[l].z;
+                    }
+                store_block(width, NULL, gbuffer, xb, yb, data, 0);
+            } else {
+                store_block(width, bitmap, gbuffer, xb, yb, data, 1);
+            }
+        }
+}
+
 int or_render_patch_u8(const or_scene* s, uint32_t width, uint32_t height, uint32_t* bitmap,
                        uint32_t* gbuffer, uint32_t patch_width, uint32_t x, uint32_t y,
                        uint32_t frame, int rng_mode) {
@@ -1960,6 +1989,11 @@ static void* frame_worker(void* arg) {
         uint32_t ye = y + 32 <= j->height ? y + 32 : j->height;
         for (uint32_t f = 0; f < j->n_frames; f++) {
             uint32_t frame = j->first_frame + f;
+            if (j->rng_mode == OR_RNG_PIXEL) {
+                pixel_rect_impl(j->s, j->width, j->height, j->bitmap, j->hdr, j->gbuffer, x, y,
+                                xe - x, ye - y, frame, &local);
+                continue;
+            }
             for (uint32_t yy = y; yy < ye; yy += pw)
                 for (uint32_t xx = x; xx < xe; xx += pw) {
                     int st = patch_impl(j->s, j->width, j->height, j->bitmap, j->hdr, j->gbuffer,
@@ -1977,7 +2011,8 @@ static void* frame_worker(void* arg) {
 }
 
 static int run_frames(frame_job* j, int nthreads) {
-    if ((j->width & 15) || (j->height & 15)) return -10;
+    /* REF_PATCH frames are whole 16x16 patches; PIXEL frames whole 16x4 blocks */
+    if ((j->width & 15) || (j->height & (j->rng_mode == OR_RNG_PIXEL ? 3u : 15u))) return -10;
     if (j->s->ndirect < 8) return -12;
     j->npx = (j->width + 31) / 32;
     j->npy = (j->height + 31) / 32;

@@ -141,3 +141,26 @@ def test_struct_mirrors_match_header(cv, struct, mirror):
     the header's order (a field added on one side only shifts the rest)."""
     fields = [f[0] for f in getattr(cv._native, mirror)._fields_]
     assert fields == _header_fields(struct), struct
+
+
+def test_host_buffers_are_checked_before_the_library_sees_them(cv):
+    """A wrong array (dtype, layout, size) raises in the binding; the C ABI
+    takes raw pointers and would read or write past it."""
+    j = cv.Job(None, 64, 16, hdr=np.zeros(64 * 16 * 3, np.float32))
+    assert j._ptr(j.hdr) == j.hdr.ctypes.data
+    assert j._ptr(12345) == 12345  # a device pointer passes as is
+    assert j._ptr(None) is None
+    with pytest.raises(ValueError, match="elements"):
+        j._ptr(np.zeros(64 * 16 * 3 - 1, np.float32))
+    with pytest.raises(TypeError, match="dtype"):
+        j._ptr(np.zeros(64 * 16 * 3, np.float64))
+    with pytest.raises(ValueError, match="contiguous"):
+        j._ptr(np.zeros(64 * 16 * 6, np.float32)[::2])
+    ro = np.zeros(64 * 16 * 3, np.float32)
+    ro.flags.writeable = False
+    with pytest.raises(ValueError, match="writeable"):
+        j._ptr(ro)
+    with pytest.raises(ValueError, match="elements"):
+        j._ptr(np.zeros(64 * 16 * 7, np.float32), np.float32, 8, "features")
+    with pytest.raises(TypeError, match="dtype"):
+        j._ptr(np.zeros(64 * 16, np.int32), np.uint32, 1, "gbuffer")

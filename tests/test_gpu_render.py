@@ -253,3 +253,59 @@ def test_trace_export(cv, tmp_path):
         cv.trace_disable()
     # tracing changes nothing in the image
     assert hdr.tobytes() == plain.tobytes()
+
+
+@pytest.mark.parametrize("W,H", [(64, 52), (80, 44), (48, 12)])
+def test_leftover_band_exact(cv, oracle, W, H):
+    """Heights of 4 mod 8: the camera paths' 8 x 8 tiles leave a last band
+    of 4 rows, rendered and compared pixel for pixel (the oracle's PIXEL
+    frames take heights that are multiples of 4)."""
+    sd = scenes.config0(subdivisions=4)
+    g, o = _pair(cv, oracle, sd)
+    r = cv.Renderer(g, W, H)
+    hdr = r.new_buffer_f32()
+    gb = np.full(W * H, 0xDEADBEEF, np.uint32)
+    r.accumulate_frames_f32(hdr, 1, 2, gbuffer=gb)
+    ohdr, ogb = o.accumulate_frames(W, H, 1, 2, oracle.RNG_PIXEL, 8)
+    assert hdr.tobytes() == ohdr.tobytes()
+    np.testing.assert_array_equal(gb, ogb)
+    # every pixel of the band received its paths
+    rows = oracle.hdr_blocks_to_rows(W, H, hdr)
+    assert (rows[H - 4:].max(axis=2) > 0).mean() > 0.3
+    bm, gbu = r.render_frame_u8(3)
+    obm, ogbu = o.render_frame_u8(W, H, 3, oracle.RNG_PIXEL, 8)
+    np.testing.assert_array_equal(bm, obm)
+    np.testing.assert_array_equal(gbu, ogbu)
+
+
+def test_config0_full_size_exact(cv, oracle):
+    """The small configuration at its own size: 1280 x 736, 1 spp."""
+    sd = scenes.config0()
+    g, o = _pair(cv, oracle, sd)
+    W, H = 1280, 736
+    r = cv.Renderer(g, W, H)
+    hdr = r.new_buffer_f32()
+    gb = np.zeros(W * H, np.uint32)
+    r.accumulate_frames_f32(hdr, 1, 1, gbuffer=gb)
+    ohdr, ogb = o.accumulate_frames(W, H, 1, 1, oracle.RNG_PIXEL, 16)
+    assert hdr.tobytes() == ohdr.tobytes()
+    np.testing.assert_array_equal(gb, ogb)
+
+
+def test_config1_full_size_exact(cv, oracle):
+    """The large configuration at its own size: 1,330,746 triangles at
+    1920 x 1080 (135 rows of 8), 1 spp, against the oracle traversing the
+    same tree (the GPU's BVH, itself pinned to the oracle's build)."""
+    sd = scenes.config1()
+    g = cv.Scene.from_scene_data(sd)
+    nodes, order = g.bvh()
+    o = oracle.Scene.from_scene_data(sd, nodes=nodes.view(oracle.NODE_DTYPE), order=order)
+    W, H = 1920, 1080
+    r = cv.Renderer(g, W, H)
+    hdr = r.new_buffer_f32()
+    gb = np.zeros(W * H, np.uint32)
+    r.accumulate_frames_f32(hdr, 1, 1, gbuffer=gb)
+    print("config1 full size: GPU done", flush=True)
+    ohdr, ogb = o.accumulate_frames(W, H, 1, 1, oracle.RNG_PIXEL, 16)
+    assert hdr.tobytes() == ohdr.tobytes()
+    np.testing.assert_array_equal(gb, ogb)
