@@ -14,7 +14,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libconvector_amd.so")
-SOURCES = ["api.hip", "render.hip", "kernels.hip", "build.hip", "denoise.hip"]
+SOURCES = ["api.hip", "render.hip", "kernels.hip", "build.hip", "denoise.hip", "wide.hip"]
 HEADERS = ["device_math.hpp", "internal.hpp", "host.hpp"]
 
 

@@ -100,6 +100,8 @@ cv_status validate_tuning(const cv_tuning& t) {
       (t.frames_per_step > 64 && t.frames_per_step != CV_FRAMES_AUTO) || t.refill_interval > 64 ||
       t.leaf_batch > 65)
     return fail(CV_ERR_INVALID_ARGUMENT, "tuning value out of range");
+  if (t.bvh_width && t.bvh_width != 2 && t.bvh_width != 4 && t.bvh_width != 8)
+    return fail(CV_ERR_INVALID_ARGUMENT, "bvh_width must be 2, 4 or 8");
   return CV_OK;
 }
 
@@ -112,6 +114,7 @@ void apply_tuning(LaunchCfg& c, const cv_tuning& t) {
   if (t.frames_per_step) c.frames_per_step = t.frames_per_step;
   if (t.refill_interval) c.refill_interval = t.refill_interval;
   if (t.leaf_batch) c.leaf_batch = t.leaf_batch;
+  if (t.bvh_width) c.width = t.bvh_width;
 }
 
 cv_tuning tuning_of(const LaunchCfg& c) {
@@ -124,12 +127,13 @@ cv_tuning tuning_of(const LaunchCfg& c) {
   t.frames_per_step = c.frames_per_step;
   t.refill_interval = c.refill_interval;
   t.leaf_batch = c.leaf_batch;
+  t.bvh_width = c.width;
   return t;
 }
 
-cvk::JobDesc job_for_scene(cv_scene* s) {
+cvk::JobDesc job_for_scene(cv_scene* s, uint32_t width) {
   cvk::JobDesc j{};
-  j.nodes = s->nodes.p;
+  j.nodes = traversal_nodes(s, width);
   j.tris = s->tris.p;
   j.tri_mat = s->tri_mat.p;
   j.tri_uv = s->tri_uv.p;
@@ -451,8 +455,9 @@ void cv_mesh_destroy(cv_mesh* mesh) { delete mesh; }
 static cv_status scene_apply_tuning(cv_scene* s, const cv_tuning& t) {
   cv_status v = cvh::validate_tuning(t);
   if (v != CV_OK) return v;
+  if (t.bvh_width > 2 && (v = cvh::ensure_wide_locked(s, t.bvh_width)) != CV_OK) return v;
   cvh::apply_tuning(s->cfg, t);
-  uint32_t need = s->info.depth + 4;
+  uint32_t need = cvh::stack_depth(s, s->cfg.width) + 4;
   size_t spill_entries = need > (uint32_t)s->cfg.stack ? (size_t)(need - s->cfg.stack) : 0;
   HIPTRY(hipStreamSynchronize(s->stream), "sync");
   // lanes of a launch: the waves rounded up to whole 4-wave workgroups
@@ -713,7 +718,9 @@ cv_status cv_bvh_intersect_nearest(const cv_scene* sc, const cv_ray* rays, size_
     x.rays6 = rp;
     x.tri_src = s->tri_src.p;
     x.hits = reinterpret_cast<uint4*>(hp);
-    x.nodes = s->nodes.p;
+    cv_status ws = cvh::ensure_wide_locked(s, s->cfg.width);
+    if (ws != CV_OK) return ws;
+    x.nodes = cvh::traversal_nodes(s, s->cfg.width);
     x.tris = s->tris.p;
     x.jobs = nullptr;
     x.spill = s->spill.p;
@@ -726,7 +733,7 @@ cv_status cv_bvh_intersect_nearest(const cv_scene* sc, const cv_ray* rays, size_
     x.interval = s->cfg.refill_interval;
     x.leaf_batch = s->cfg.leaf_batch <= 64 ? s->cfg.leaf_batch : 0u;
     HIPTRY(cvk::launch_extend(x, false, s->packed, false, s->cfg.stack, s->cfg.blocks,
-                              s->cfg.block_waves, st),
+                              s->cfg.block_waves, st, s->cfg.width),
            "intersect");
     if (!dev) HIPTRY(hipMemcpyAsync(hits, hp, 16 * n, hipMemcpyDeviceToHost, st), "download");
     HIPTRY(hipStreamSynchronize(st), "sync");

@@ -91,6 +91,7 @@ struct LaunchCfg {
   uint32_t frames_per_step = CV_FRAMES_AUTO;  // accumulation frames sharing launches
   uint32_t refill_interval = 16;              // traversal steps between refill checks
   uint32_t leaf_batch = 16;                   // 64ths of the active lanes per leaf round; 65 = off
+  uint32_t width = 2;                         // BVH children per node: 2 (the reference's), 4, 8
 };
 
 }  // namespace cvh
@@ -131,6 +132,10 @@ struct cv_scene {
   cvh::DevBuf<uint2> spill;        // stack spill area of cv_bvh_intersect_nearest
   cvh::DevBuf<uint32_t> cursor;
   cvh::DevBuf<cvk::Ctrl> q_ctrl;   // hits-only queries: one-bounce step control block
+  // the tree collapsed to 4 and 8 children per node (built on first use)
+  cvh::DevBuf<float4> wide[2];
+  uint32_t wide_nodes[2] = {0, 0};
+  uint32_t wide_depth[2] = {0, 0};
   std::mutex mu;
 };
 
@@ -141,8 +146,20 @@ cv_status validate_tuning(const cv_tuning& t);
 void apply_tuning(LaunchCfg& c, const cv_tuning& t);
 cv_tuning tuning_of(const LaunchCfg& c);
 
-// The job of a step for scene s (geometry, lights, camera, emission).
-cvk::JobDesc job_for_scene(cv_scene* s);
+// The job of a step for scene s (geometry, lights, camera, emission); its
+// nodes are the tree of the given width (built by ensure_wide beforehand).
+cvk::JobDesc job_for_scene(cv_scene* s, uint32_t width = 2);
+
+// Builds the scene's tree of `width` children per node (4 or 8; 2 is the
+// reference tree itself) once; ensure_wide_locked expects s->mu held.
+cv_status ensure_wide(cv_scene* s, uint32_t width);
+cv_status ensure_wide_locked(cv_scene* s, uint32_t width);
+// The node array the traversal of that width reads.
+const cvk::DevNode* traversal_nodes(const cv_scene* s, uint32_t width);
+// What the traversal of that width needs in place of the tree depth when
+// stack spill areas are sized (depth + 4 entries per lane); the wide tree
+// must have been built.
+uint32_t stack_depth(const cv_scene* s, uint32_t width);
 
 // Denoises one accumulated image (hdr and features on the device, in the
 // renderer's layouts) into dst (device or host, hdr layout) on the

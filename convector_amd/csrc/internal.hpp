@@ -148,9 +148,12 @@ hipError_t launch_query_ctrl(Ctrl* ctrl, uint32_t n, hipStream_t s);
 // multi: rays of the step belong to more than one scene (per-ray node and
 // triangle pointers from the job table).
 // blocks = persistent waves; block_waves (1, 2, 4) waves per workgroup in
-// lane-refill mode.
+// lane-refill mode.  width 2: the reference's node pairs (exact); 4 or 8:
+// the collapsed wide tree of build_wide_bvh in place of the nodes (nodes and
+// the jobs' node pointers then point at its float4 array).
 hipError_t launch_extend(const ExtendArgs& a, bool count, bool packed, bool multi, int stack,
-                         uint32_t blocks, uint32_t block_waves, hipStream_t s);
+                         uint32_t blocks, uint32_t block_waves, hipStream_t s,
+                         uint32_t width = 2);
 hipError_t launch_generate(const GenerateArgs& a, uint32_t blocks, hipStream_t s);
 // multi: the step has more than one job.
 // shared_scene: every job of the step has the same scene (tris, lights,

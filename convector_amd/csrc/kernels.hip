@@ -647,6 +647,133 @@ __global__ __launch_bounds__(64) void k_extend(ExtendArgs a) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Wide BVH (4 or 8 children per node): the reference tree with inner levels
+// collapsed (each wide node holds the leaves and the inner nodes up to W
+// that its binary subtree expands to, largest surface area first).  The
+// leaves and every triangle test are the reference's, and a node's children
+// are visited nearest first, so the nearest hit is the reference's except
+// where two triangles hit at the same t or a box test's last bit decides: a
+// tolerance mode (its counters are the wide tree's).
+// ---------------------------------------------------------------------------
+
+// Child references: an inner wide node's index, or kWideLeaf | count << 26 |
+// first triangle (count 1..31); kWideLeaf alone marks an empty slot.
+constexpr uint32_t kWideLeaf = 0x80000000u;
+
+// A node is W / 4 quads of children in float4s: min.x of children 4q..4q+3,
+// then max.x, min.y, max.y, min.z, max.z for every quad, then the child
+// references (uint4 per quad), padded to whole 64-byte halves of a 128-byte
+// line: 8 float4 (128 B) for W = 4, 16 (256 B) for W = 8.
+template <int W>
+struct Wide {
+  static constexpr int kQuads = W / 4;
+  static constexpr int kFloat4 = W == 4 ? 8 : 16;
+};
+
+// The reciprocal and origin-times-reciprocal of a direction for the
+// multiply-add slab test, zero components replaced by 1e-20 of their sign
+// (no infinities: no 0 * inf in the test).
+__device__ __forceinline__ void wide_ray(V3 d, V3 o, V3& inv, V3& oi) {
+  const float e = 1.0e-20f;
+  const float dx = fabsf(d.x) < e ? copysignf(e, d.x) : d.x;
+  const float dy = fabsf(d.y) < e ? copysignf(e, d.y) : d.y;
+  const float dz = fabsf(d.z) < e ? copysignf(e, d.z) : d.z;
+  inv = mk3(rcp(dx), rcp(dy), rcp(dz));
+  oi = mk3(o.x * inv.x, o.y * inv.y, o.z * inv.z);
+}
+
+__device__ __forceinline__ void wide_cas(float& ka, uint32_t& ra, float& kb, uint32_t& rb) {
+  const bool sw = kb < ka;
+  const float k = sw ? kb : ka;
+  const uint32_t r = sw ? rb : ra;
+  kb = sw ? ka : kb;
+  rb = sw ? ra : rb;
+  ka = k;
+  ra = r;
+}
+
+// One iteration of the wide traversal: take the next entry, cull it against
+// the best hit, then test an inner node's W child boxes and keep the nearest
+// hit child in registers while the others are pushed farthest first, or
+// test a leaf's triangles (waiting for the wave's leaf round as trav_step).
+template <bool COUNT, int S, int W>
+__device__ __forceinline__ void trav_step_wide(const float4* __restrict__ wn,
+                                               const DevTri* __restrict__ tris, V3 o, V3 d,
+                                               V3 inv, V3 oi, Stack<S, true>& st, TravState& ts,
+                                               entry_t& cur, bool& have, bool leaf_round,
+                                               bool* waiting) {
+  if (waiting && *waiting && !leaf_round) return;
+  entry_t e = cur;
+  if (!have) e = st.pop();
+  have = false;
+  if (waiting) *waiting = false;
+  if (!(entry_tmin(e) < ts.best_t)) return;
+  const uint32_t x = entry_x(e);
+  if (!(x & kWideLeaf)) {
+    const float4* p = wn + (size_t)x * Wide<W>::kFloat4;
+    constexpr int Q = Wide<W>::kQuads;
+    float key[W];
+    uint32_t ref[W];
+    uint32_t nhit = 0;
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+      const float4 lx = p[0 * Q + q], hx = p[1 * Q + q], ly = p[2 * Q + q], hy = p[3 * Q + q],
+                   lz = p[4 * Q + q], hz = p[5 * Q + q];
+      const uint4 cr = reinterpret_cast<const uint4*>(p)[6 * Q + q];
+      const float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, hxa[4] = {hx.x, hx.y, hx.z, hx.w};
+      const float lya[4] = {ly.x, ly.y, ly.z, ly.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w};
+      const float lza[4] = {lz.x, lz.y, lz.z, lz.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
+      const uint32_t cra[4] = {cr.x, cr.y, cr.z, cr.w};
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const float tx1 = fmaf(lxa[j], inv.x, -oi.x), tx2 = fmaf(hxa[j], inv.x, -oi.x);
+        const float ty1 = fmaf(lya[j], inv.y, -oi.y), ty2 = fmaf(hya[j], inv.y, -oi.y);
+        const float tz1 = fmaf(lza[j], inv.z, -oi.z), tz2 = fmaf(hza[j], inv.z, -oi.z);
+        const float tmin = fmaxf(fmaxf(fminf(tx1, tx2), fminf(ty1, ty2)), fminf(tz1, tz2));
+        const float tmax = fminf(fminf(fmaxf(tx1, tx2), fmaxf(ty1, ty2)), fmaxf(tz1, tz2));
+        const bool hit = tmax >= tmin && tmax >= 0.0f && tmin < ts.best_t && cra[j] != kWideLeaf;
+        key[4 * q + j] = hit ? tmin : INFINITY;
+        ref[4 * q + j] = cra[j];
+        nhit += hit ? 1u : 0u;
+        if (COUNT) ts.na += cra[j] != kWideLeaf ? 1u : 0u;
+      }
+    }
+    // ascending by entry distance (misses last): the sorting networks of
+    // 4 and 8 keys (Batcher's odd-even merge sort)
+    if (W == 4) {
+      wide_cas(key[0], ref[0], key[1], ref[1]);
+      wide_cas(key[2], ref[2], key[3], ref[3]);
+      wide_cas(key[0], ref[0], key[2], ref[2]);
+      wide_cas(key[1], ref[1], key[3], ref[3]);
+      wide_cas(key[1], ref[1], key[2], ref[2]);
+    } else {
+      constexpr int net[19][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}, {0, 2}, {1, 3}, {4, 6},
+                                  {5, 7}, {1, 2}, {5, 6}, {0, 4}, {1, 5}, {2, 6}, {3, 7},
+                                  {2, 4}, {3, 5}, {1, 2}, {3, 4}, {5, 6}};
+#pragma unroll
+      for (int c = 0; c < 19; c++)
+        wide_cas(key[net[c][0]], ref[net[c][0]], key[net[c][1]], ref[net[c][1]]);
+    }
+    have = nhit > 0;
+    cur = pack_entry(ref[0], key[0]);
+#pragma unroll
+    for (int j = W - 1; j >= 1; j--) st.push_if((uint32_t)j < nhit, pack_entry(ref[j], key[j]));
+  } else {
+    if (waiting && !leaf_round) {
+      cur = e;
+      have = true;
+      *waiting = true;
+      return;
+    }
+    const uint32_t first = x & 0x03ffffffu, count = (x >> 26) & 31u;
+    for (uint32_t k = first; k < first + count; k++) {
+      test_tri(tris, k, o, d, ts);
+      if (COUNT) ts.nt++;
+    }
+  }
+}
+
 // Traversal with lane refill: a lane whose ray is finished takes a new ray
 // from the queue while the other lanes continue, so a wave stays full until
 // the queue is empty.  A wave refills when at least `refill` lanes are idle
@@ -654,7 +781,7 @@ __global__ __launch_bounds__(64) void k_extend(ExtendArgs a) {
 // traverse(), one trav_step per loop iteration.
 // Workgroups of 1, 2 or 4 independent waves (the launch's choice); each
 // wave owns S * 64 entries of the dynamic LDS.
-template <bool COUNT, int S, bool PACKED, bool MULTI, bool QUERY = false>
+template <bool COUNT, int S, bool PACKED, bool MULTI, bool QUERY = false, int W = 2>
 __global__ __launch_bounds__(256) void k_extend_refill(ExtendArgs a) {
   extern __shared__ uint2 lds_waves[];
   const uint32_t lane = lane_id();
@@ -667,7 +794,7 @@ __global__ __launch_bounds__(256) void k_extend_refill(ExtendArgs a) {
   const uint64_t lt = (1ull << lane) - 1ull;
   Stack<S, PACKED> st(lds + lane, a.spill + gtid, stride, 0);
   TravState ts;
-  V3 o = mk3(0, 0, 0), d = mk3(1, 0, 0), inv = mk3(1, 0, 0);
+  V3 o = mk3(0, 0, 0), d = mk3(1, 0, 0), inv = mk3(1, 0, 0), oi = mk3(0, 0, 0);
   const DevNode* nodes = a.nodes;
   const DevTri* tris = a.tris;
   uint32_t ray = 0;
@@ -717,15 +844,19 @@ __global__ __launch_bounds__(256) void k_extend_refill(ExtendArgs a) {
             o = mk3(r0.x, r0.y, r0.z);
             d = mk3(r1.x, r1.y, r1.z);
           }
-          inv = mk3(rcp(d.x), rcp(d.y), rcp(d.z));
-          exact = needs_exact_nan(o, inv);
+          if (W == 2) {
+            inv = mk3(rcp(d.x), rcp(d.y), rcp(d.z));
+            exact = needs_exact_nan(o, inv);
+          } else {
+            wide_ray(d, o, inv, oi);
+          }
           ts.best_t = 1.0e5f;
           ts.best = kNoHit;
           ts.bu = ts.bv = 0.0f;
           ts.na = 0;
           ts.nt = 0;
           st.sp = 0;
-          cur = root_entry<PACKED>();
+          cur = W == 2 ? root_entry<PACKED>() : pack_entry(0u, -INFINITY);
           have = true;
           waiting = false;
           ray = i;
@@ -754,8 +885,12 @@ __global__ __launch_bounds__(256) void k_extend_refill(ExtendArgs a) {
                      wm == am;
       }
       if (active) {
-        trav_step<COUNT>(nodes, tris, o, d, inv, wave_exact, st, ts, cur, have, leaf_round,
-                         &waiting);
+        if constexpr (W == 2)
+          trav_step<COUNT>(nodes, tris, o, d, inv, wave_exact, st, ts, cur, have, leaf_round,
+                           &waiting);
+        else
+          trav_step_wide<COUNT, S, W>(reinterpret_cast<const float4*>(nodes), tris, o, d, inv, oi,
+                                      st, ts, cur, have, leaf_round, &waiting);
         if (!have && st.sp == 0) {
           a.hits[ray] = make_uint4(QUERY ? hit_id<true>(a, ts.best) : ts.best, f2u(ts.best_t),
                                    f2u(ts.bu), f2u(ts.bv));
@@ -1302,9 +1437,32 @@ hipError_t launch_intersect(const IntersectArgs& a, bool count, bool packed, int
   return hipErrorInvalidValue;
 }
 
+// Wide trees (4 or 8 children): lane refill only (whole-batch requests run
+// with refill at 64 lanes), stack entries are child references.
+template <bool COUNT, int S, int W>
+static hipError_t launch_extend_wide(const ExtendArgs& a, bool multi, uint32_t blocks,
+                                     uint32_t block_waves, hipStream_t s) {
+  const uint32_t bw = block_waves == 2 || block_waves == 4 ? block_waves : 1u;
+  const dim3 grid((blocks + bw - 1) / bw), block(64 * bw);
+  const size_t shmem = (size_t)S * 64 * sizeof(uint2) * bw;
+  ExtendArgs b = a;
+  if (!b.refill) b.refill = 64;
+  if (multi)
+    hipLaunchKernelGGL((k_extend_refill<COUNT, S, true, true, false, W>), grid, block, shmem, s, b);
+  else if (b.rays6 && !COUNT)
+    hipLaunchKernelGGL((k_extend_refill<false, S, true, false, true, W>), grid, block, shmem, s,
+                       b);
+  else
+    hipLaunchKernelGGL((k_extend_refill<COUNT, S, true, false, false, W>), grid, block, shmem, s,
+                       b);
+  return hipGetLastError();
+}
+
 template <bool COUNT, int S>
 static hipError_t launch_extend_s(const ExtendArgs& a, bool packed, bool multi, uint32_t blocks,
-                                  uint32_t block_waves, hipStream_t s) {
+                                  uint32_t block_waves, uint32_t width, hipStream_t s) {
+  if (width == 4) return launch_extend_wide<COUNT, S, 4>(a, multi, blocks, block_waves, s);
+  if (width == 8) return launch_extend_wide<COUNT, S, 8>(a, multi, blocks, block_waves, s);
   const uint32_t bw = block_waves == 2 || block_waves == 4 ? block_waves : 1u;
   const dim3 grid((blocks + bw - 1) / bw), block(64 * bw);
   const size_t shmem = (size_t)S * 64 * sizeof(uint2) * bw;
@@ -1335,12 +1493,13 @@ static hipError_t launch_extend_s(const ExtendArgs& a, bool packed, bool multi, 
 }
 
 hipError_t launch_extend(const ExtendArgs& a, bool count, bool packed, bool multi, int stack,
-                         uint32_t blocks, uint32_t block_waves, hipStream_t s) {
+                         uint32_t blocks, uint32_t block_waves, hipStream_t s, uint32_t width) {
   if (multi && !a.refill) return hipErrorInvalidValue;
-#define CASE(S)                                                                        \
-  if (stack == S)                                                                      \
-    return count ? launch_extend_s<true, S>(a, packed, multi, blocks, block_waves, s)  \
-                 : launch_extend_s<false, S>(a, packed, multi, blocks, block_waves, s);
+  if (width != 2 && width != 4 && width != 8) return hipErrorInvalidValue;
+#define CASE(S)                                                                               \
+  if (stack == S)                                                                             \
+    return count ? launch_extend_s<true, S>(a, packed, multi, blocks, block_waves, width, s)  \
+                 : launch_extend_s<false, S>(a, packed, multi, blocks, block_waves, width, s);
   CVK_STACK_VARIANTS(CASE)
 #undef CASE
   return hipErrorInvalidValue;

@@ -69,7 +69,7 @@ struct RunOpts {
 struct GraphKey {
   int stack = 0;
   uint32_t blocks = 0, refill = 0, fetch_batches = 0, max_bounces = 0, block_waves = 0;
-  uint32_t interval = 0, leaf_batch = 0;
+  uint32_t interval = 0, leaf_batch = 0, width = 2;
   bool packed = false, count = false, multi_job = false;
   const void* nodes = nullptr;
   const void* tris = nullptr;
@@ -79,7 +79,7 @@ struct GraphKey {
            block_waves == o.block_waves && interval == o.interval && leaf_batch == o.leaf_batch &&
            fetch_batches == o.fetch_batches && max_bounces == o.max_bounces &&
            packed == o.packed && count == o.count && multi_job == o.multi_job &&
-           nodes == o.nodes && tris == o.tris && capacity == o.capacity;
+           nodes == o.nodes && tris == o.tris && capacity == o.capacity && width == o.width;
   }
 };
 
@@ -292,7 +292,7 @@ cv_status launch_step(Pipeline& p, const RunOpts& o, uint32_t slot, uint32_t pat
     ep = ev_pair(1);
     if (ep) (void)hipEventRecord(ep->first, st);
     HIPTRY(cvk::launch_extend(x, count, o.packed, multi_scene, o.cfg.stack, o.cfg.blocks,
-                              o.cfg.block_waves, st),
+                              o.cfg.block_waves, st, o.cfg.width),
            "extend");
     if (ep) (void)hipEventRecord(ep->second, st);
     cvk::ShadeArgs sh{};
@@ -334,6 +334,7 @@ cv_status pipeline_run(Pipeline& p, const std::vector<StepPlan>& steps, const Ru
   key.block_waves = o.cfg.block_waves;
   key.interval = o.cfg.refill_interval;
   key.leaf_batch = o.cfg.leaf_batch;
+  key.width = o.cfg.width;
   key.fetch_batches = o.cfg.fetch_batches;
   key.max_bounces = o.max_bounces;
   key.packed = o.packed;
@@ -636,7 +637,9 @@ cv_status run_frames(cv_renderer* r, const Rect& rc, uint32_t first, uint32_t nf
     }
   }
   cv_scene* s = r->scene;
-  cvk::JobDesc base = cvh::job_for_scene(s);
+  cv_status ws = cvh::ensure_wide(s, s->cfg.width);
+  if (ws != CV_OK) return ws;
+  cvk::JobDesc base = cvh::job_for_scene(s, s->cfg.width);
   base.width = r->width;
   base.height = r->height;
   base.rect_x = rc.x;
@@ -649,7 +652,10 @@ cv_status run_frames(cv_renderer* r, const Rect& rc, uint32_t first, uint32_t nf
   base.path_base = 0;
   cv_status status = CV_OK;
   if (r->opts.rng_mode == CV_RNG_REF_PATCH) {
-    status = ref_patch_render(r, rc, first, nframes, base, features != nullptr);
+    // every frame writes the gbuffer; the last frame's remains
+    cvk::JobDesc job = base;
+    job.gbuffer = gb_dev;
+    status = ref_patch_render(r, rc, first, nframes, job, features != nullptr);
   } else {
   // Frames per step: an accumulation renders K frames in the same launches
   // (a launch's last rays keep most of the GPU idle, so larger launches are
@@ -706,10 +712,10 @@ cv_status run_frames(cv_renderer* r, const Rect& rc, uint32_t first, uint32_t nf
   o.max_bounces = r->opts.max_bounces;
   o.flags = flags;
   o.packed = s->packed;
-  o.nodes = s->nodes.p;
+  o.nodes = cvh::traversal_nodes(s, s->cfg.width);
   o.tris = s->tris.p;
   o.multi_job = K > 1;
-  o.max_depth = s->info.depth;
+  o.max_depth = cvh::stack_depth(s, s->cfg.width);
   o.graph = flags & CV_RENDER_GRAPH;
   status = pipeline_run(r->pipe, steps, o, &r->stats);
   if (status == CV_OK && K > 1) r->slots_clean = true;
@@ -942,8 +948,9 @@ cv_status cv_batch_render(cv_batch* b, const cv_job* jobs, size_t n, uint32_t fl
     cv_status cs = check_frame(J.scene, J.width, J.height);
     if (cs != CV_OK) return cs;
     if (J.num_frames == 0) return fail(CV_ERR_INVALID_ARGUMENT, "job with zero frames");
+    if ((cs = cvh::ensure_wide(J.scene, b->cfg.width)) != CV_OK) return cs;
     max_frames = std::max(max_frames, J.num_frames);
-    max_depth = std::max(max_depth, J.scene->info.depth);
+    max_depth = std::max(max_depth, cvh::stack_depth(J.scene, b->cfg.width));
     largest = std::max(largest, (size_t)J.width * J.height);
     total += (size_t)J.width * J.height * J.num_frames;
     packed = packed && J.scene->packed;
@@ -962,7 +969,7 @@ cv_status cv_batch_render(cv_batch* b, const cv_job* jobs, size_t n, uint32_t fl
   for (size_t i = 0; i < n; i++) {
     const cv_job& J = jobs[i];
     const size_t npix = (size_t)J.width * J.height;
-    cvk::JobDesc d = cvh::job_for_scene(J.scene);
+    cvk::JobDesc d = cvh::job_for_scene(J.scene, b->cfg.width);
     d.width = J.width;
     d.height = J.height;
     d.rect_x = d.rect_y = 0;
@@ -1075,7 +1082,7 @@ cv_status cv_batch_render(cv_batch* b, const cv_job* jobs, size_t n, uint32_t fl
   o.max_bounces = b->opts.max_bounces;
   o.flags = flags;
   o.packed = packed;
-  o.nodes = one_scene ? jobs[0].scene->nodes.p : nullptr;
+  o.nodes = one_scene ? cvh::traversal_nodes(jobs[0].scene, b->cfg.width) : nullptr;
   o.tris = one_scene ? jobs[0].scene->tris.p : nullptr;
   o.multi_job = multi_job;
   o.max_depth = max_depth;
@@ -1106,8 +1113,9 @@ cv_status cv_batch_autotune(cv_batch* b, const cv_job* jobs, size_t n, uint64_t 
   // sample: the first frame of the jobs in order, up to max_sample_paths
   // paths (at least one job), written to scratch outputs
   StepPlan step;
+  std::vector<cv_scene*> sample_scenes;  // the scene of each sample entry
   uint32_t max_depth = 0;
-  bool packed = true, one_scene = true;
+  bool packed = true, one_scene = true, wide_ok = true;
   for (size_t i = 0; i < n; i++) {
     const cv_job& J = jobs[i];
     if (!J.scene) return fail(CV_ERR_INVALID_ARGUMENT, "job without scene");
@@ -1115,7 +1123,7 @@ cv_status cv_batch_autotune(cv_batch* b, const cv_job* jobs, size_t n, uint64_t 
     if (cs != CV_OK) return cs;
     const uint32_t npix = J.width * J.height;
     if (!step.jobs.empty() && step.paths + (uint64_t)npix > max_sample_paths) continue;
-    cvk::JobDesc d = cvh::job_for_scene(J.scene);
+    cvk::JobDesc d = cvh::job_for_scene(J.scene, 2);
     d.width = J.width;
     d.height = J.height;
     d.rect_w = J.width;
@@ -1123,7 +1131,9 @@ cv_status cv_batch_autotune(cv_batch* b, const cv_job* jobs, size_t n, uint64_t 
     d.frame = J.first_frame;
     d.path_base = step.paths;
     step.jobs.push_back(d);
+    sample_scenes.push_back(J.scene);
     step.paths += npix;
+    wide_ok = wide_ok && J.scene->info.max_leaf <= 31 && J.scene->ntris < (1u << 26);
     max_depth = std::max(max_depth, J.scene->info.depth);
     packed = packed && J.scene->packed;
     one_scene = one_scene && J.scene == jobs[0].scene;
@@ -1136,12 +1146,24 @@ cv_status cv_batch_autotune(cv_batch* b, const cv_job* jobs, size_t n, uint64_t 
   o.max_bounces = b->opts.max_bounces;
   o.flags = 0;
   o.packed = packed;
-  o.nodes = one_scene ? jobs[0].scene->nodes.p : nullptr;
   o.tris = one_scene ? jobs[0].scene->tris.p : nullptr;
   o.multi_job = step.jobs.size() > 1;
   o.max_depth = max_depth;
+  if (wide_ok) {
+    // every candidate width: the spill area is sized for the deepest stack
+    for (cv_scene* sc : sample_scenes) {
+      cv_status ws;
+      if ((ws = cvh::ensure_wide(sc, 4)) != CV_OK || (ws = cvh::ensure_wide(sc, 8)) != CV_OK)
+        return ws;
+      o.max_depth = std::max({o.max_depth, cvh::stack_depth(sc, 4), cvh::stack_depth(sc, 8)});
+    }
+  }
   auto time_cfg = [&](const cvh::LaunchCfg& c, double* ms) -> cv_status {
     o.cfg = c;
+    // the tree of the candidate's width for every entry of the sample step
+    for (size_t k = 0; k < steps[0].jobs.size(); k++)
+      steps[0].jobs[k].nodes = cvh::traversal_nodes(sample_scenes[k], c.width);
+    o.nodes = one_scene ? cvh::traversal_nodes(jobs[0].scene, c.width) : nullptr;
     double best = 1e30;
     for (int rep = 0; rep < 2; rep++) {
       cv_render_stats st{};
@@ -1159,7 +1181,9 @@ cv_status cv_batch_autotune(cv_batch* b, const cv_job* jobs, size_t n, uint64_t 
   static const uint32_t block_waves[] = {1, 2, 4};
   static const uint32_t intervals[] = {1, 4, 16, 32};
   static const uint32_t leaf_batches[] = {8, 16, 24, 65};
+  static const uint32_t widths[] = {2, 4, 8};
   cvh::LaunchCfg best = b->cfg;
+  if (!wide_ok) best.width = 2;
   double best_ms = 0;
   cv_status cs = time_cfg(best, &best_ms);
   if (cs != CV_OK) return cs;
@@ -1206,6 +1230,14 @@ cv_status cv_batch_autotune(cv_batch* b, const cv_job* jobs, size_t n, uint64_t 
       if ((cs = time_cfg(c, &ms)) != CV_OK) return cs;
       if (ms < best_ms) best = c, best_ms = ms;
     }
+    for (uint32_t v : widths) {
+      if (v != 2 && !wide_ok) continue;
+      cvh::LaunchCfg c = best;
+      c.width = v;
+      double ms;
+      if ((cs = time_cfg(c, &ms)) != CV_OK) return cs;
+      if (ms < best_ms) best = c, best_ms = ms;
+    }
   }
   b->cfg = best;
   if (chosen) *chosen = cvh::tuning_of(best);
@@ -1248,7 +1280,8 @@ struct ServeReq {
 size_t scene_bytes(const cv_scene* s) {
   return (size_t)s->info.num_nodes * sizeof(cvk::DevNode) +
          (size_t)s->ntris * (sizeof(cvk::DevTri) + 4 + 24 + 4 + 4) +
-         (size_t)s->ndirect * sizeof(cvk::DevLight);
+         (size_t)s->ndirect * sizeof(cvk::DevLight) +
+         (s->wide[0].n + s->wide[1].n) * sizeof(float4);
 }
 
 size_t request_bytes(const cv_request& r) {
@@ -1428,11 +1461,18 @@ void serve_loop(cv_server* sv) {
     StepPlan step;
     uint32_t max_depth = 0;
     bool packed = true;
+    // the tree width of this step (every resident scene holds that tree:
+    // built when the scene is added or the width set)
+    uint32_t width = 2;
+    {
+      std::lock_guard<std::mutex> lk(sv->mu);
+      width = sv->cfg.width;
+    }
     for (size_t k = 0; k < act.size(); k++) {
       const ServeReq& r = *act[k];
       const cv_request& q = r.req;
       cv_scene* sc = sv->scenes[q.scene];
-      cvk::JobDesc d = cvh::job_for_scene(sc);
+      cvk::JobDesc d = cvh::job_for_scene(sc, width);
       memcpy(d.cam.pos, q.camera_pos, 12);
       memset(d.cam.dpos, 0, 12);
       memcpy(d.cam.q, q.camera_q, 16);
@@ -1462,11 +1502,12 @@ void serve_loop(cv_server* sv) {
         fd.rect[3] = q.height;
         step.folds.push_back(fd);
       }
-      max_depth = std::max(max_depth, sc->info.depth);
+      max_depth = std::max(max_depth, cvh::stack_depth(sc, width));
       packed = packed && sc->packed;
     }
     RunOpts o;
     o.cfg = sv->cfg;
+    o.cfg.width = width;
     o.max_bounces = sv->opts.max_bounces;
     o.packed = packed;
     o.multi_job = true;
@@ -1476,7 +1517,7 @@ void serve_loop(cv_server* sv) {
       bool one = true;
       for (auto& r : act) one = one && r->req.scene == s0;
       if (one) {
-        o.nodes = sv->scenes[s0]->nodes.p;
+        o.nodes = cvh::traversal_nodes(sv->scenes[s0], width);
         o.tris = sv->scenes[s0]->tris.p;
       }
     }
@@ -1624,6 +1665,8 @@ cv_status cv_server_add_scene(cv_server* sv, cv_scene* scene, uint32_t* id) {
   if (scene->ndirect < 8)
     return fail(CV_ERR_LIGHT_COUNT, "the scene has fewer than 8 direct-sample triangles");
   std::lock_guard<std::mutex> lk(sv->mu);
+  cv_status ws = cvh::ensure_wide(scene, sv->cfg.width);
+  if (ws != CV_OK) return ws;
   const size_t b = scene_bytes(scene);
   if (sv->fixed_bytes + sv->used_bytes + b > sv->budget)
     return fail(CV_ERR_OUT_OF_MEMORY, "the scene does not fit the memory budget");
@@ -1645,6 +1688,13 @@ cv_status cv_server_set_tuning(cv_server* sv, const cv_tuning* t) {
   cv_status v = cvh::validate_tuning(*t);
   if (v != CV_OK) return v;
   std::lock_guard<std::mutex> lk(sv->mu);
+  if (t->bvh_width)
+    for (cv_scene* sc : sv->scenes) {
+      const size_t before = scene_bytes(sc);
+      cv_status ws = cvh::ensure_wide(sc, t->bvh_width);
+      if (ws != CV_OK) return ws;
+      sv->fixed_bytes += scene_bytes(sc) - before;
+    }
   cvh::apply_tuning(sv->cfg, *t);
   return CV_OK;
 }

@@ -144,6 +144,12 @@ typedef struct cv_tuning {
                                      leaf_batch / 64 of the wave's active lanes wait (or all
                                      do) and they test their triangles together (16: a
                                      quarter); 65: every lane tests its leaf at once */
+    uint32_t bvh_width;           /* children per traversed node: 2, the reference's node
+                                     pairs (bvh.rs:20-30), the default; 4 or 8, the same tree
+                                     with inner levels collapsed, visited nearest child first:
+                                     hits equal the reference's except at exact ties and
+                                     last-bit box decisions, counters those of the wide tree
+                                     (needs leaves of at most 31 triangles) */
 } cv_tuning;
 #define CV_FRAMES_AUTO 255u
 cv_status cv_scene_set_tuning(cv_scene* scene, const cv_tuning* tuning);

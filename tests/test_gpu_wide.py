@@ -1,0 +1,155 @@
+"""BVH width 4 and 8: the reference tree with inner levels collapsed,
+traversed nearest child first (cv_tuning.bvh_width).  Leaves and triangle
+tests are the reference's, so against the oracle's single-ray traversal:
+hit triangle ids equal for >= 99.99 % of rays, every difference within
+1e-6 of a triangle edge or an exact tie in t; where the ids agree, t, u and
+v are the same bits; renders equal the reference-width renders pixel for
+pixel except where a path met such a tie."""
+import numpy as np
+import pytest
+
+from convector_amd import scenes
+from helpers import bits, model_scene
+
+pytestmark = pytest.mark.gpu
+
+
+def _edge_distance64(tris, tri_id, rays):
+    v = tris[tri_id].astype(np.float64).reshape(-1, 3, 3)
+    o, d = rays[:, :3].astype(np.float64), rays[:, 3:].astype(np.float64)
+    n = np.cross(v[:, 1] - v[:, 0], v[:, 2] - v[:, 0])
+    t = ((v[:, 0] - o) * n).sum(1) / (d * n).sum(1)
+    p = o + t[:, None] * d
+    dist = np.full(len(p), np.inf)
+    for i, j in ((0, 1), (1, 2), (2, 0)):
+        e = v[:, j] - v[:, i]
+        dist = np.minimum(dist, np.linalg.norm(np.cross(e, p - v[:, i]), axis=1)
+                          / np.linalg.norm(e, axis=1))
+    return dist
+
+
+def _check_tolerance(sd, hits, ref, rays):
+    gid, oid = hits["tri_id"], ref["tri"]
+    mism = np.nonzero(gid != oid)[0]
+    assert len(mism) <= max(1, len(rays) // 10000), len(mism)
+    for k in mism:
+        tg, to = hits["t"][k], ref["tuv"][k, 0]
+        if tg == to:
+            continue  # an exact tie: two triangles at the same distance
+        near = [float(_edge_distance64(sd.tris, np.array([t]), rays[k:k + 1])[0])
+                for t in (gid[k], oid[k]) if t != 0xFFFFFFFF]
+        assert near and min(near) <= 1e-6, (k, near)
+    same = gid == oid
+    assert (bits(hits["t"][same]) == bits(ref["tuv"][same, 0])).all()
+    assert (bits(hits["u"][same]) == bits(ref["tuv"][same, 1])).all()
+    assert (bits(hits["v"][same]) == bits(ref["tuv"][same, 2])).all()
+
+
+@pytest.mark.parametrize("width", [4, 8])
+@pytest.mark.parametrize("case", ["suzanne", "stanford_bunny", "stanford_dragon", "config0",
+                                  "config1-sample"])
+def test_wide_hits_against_oracle(cv, oracle, models_dir, case, width):
+    if case == "config0":
+        sd = scenes.config0()
+        rays = np.concatenate([scenes.rays_inward(20000, 21, radius=0.9, target_scale=0.45),
+                               scenes.rays_inward(20000, 22, radius=0.95, target_scale=0.95)])
+    elif case == "config1-sample":
+        sd = scenes.config1(subdivisions=7, n_free=5000)
+        rays = np.concatenate([scenes.rays_inward(30000, 31, radius=0.95, target_scale=0.45),
+                               scenes.rays_inward(30000, 32, radius=0.9, target_scale=0.95)])
+    else:
+        sd = model_scene(models_dir, case)
+        lo, hi = sd.tris.reshape(-1, 3).min(0), sd.tris.reshape(-1, 3).max(0)
+        c, r = (lo + hi) / 2, float((hi - lo).max())
+        rays = np.concatenate([
+            scenes.rays_inward(20000, 3, radius=3 * r, target_scale=0.5 * r, center=c),
+            scenes.rays_inward(20000, 4, radius=3 * r, coherent=True, target_scale=0.5 * r,
+                               center=c),
+            scenes.rays_inward(10000, 5, radius=0.3 * r, target_scale=0.9 * r, center=c)])
+    print("wide case %s width %d: %d triangles, %d rays" % (case, width, sd.ntris, len(rays)),
+          flush=True)
+    g = cv.Scene.from_scene_data(sd)
+    o = oracle.Scene.from_scene_data(sd)
+    g.set_tuning(bvh_width=width)
+    assert g.tuning()["bvh_width"] == width
+    hits = g.intersect_nearest(rays)
+    ref = o.intersect(rays)
+    _check_tolerance(sd, hits, ref, rays)
+    assert (hits["tri_id"] != cv.NO_HIT).mean() > 0.2
+    # the isect path keeps the reference tree (exact, with the reference's counters)
+    full, isects = g.intersect_nearest(rays[:4096], full=True)
+    assert (full["tri_id"] == ref["tri"][:4096]).all()
+
+
+@pytest.mark.parametrize("width", [4, 8])
+def test_wide_renders(cv, oracle, width):
+    sd = scenes.config0()
+    W, H, n = 320, 192, 3
+    g = cv.Scene.from_scene_data(sd)
+    r2 = cv.Renderer(g, W, H)
+    ref = r2.new_buffer_f32()
+    r2.accumulate_frames_f32(ref, 1, n)
+    g.set_tuning(bvh_width=width)
+    r = cv.Renderer(g, W, H)
+    hdr = r.new_buffer_f32()
+    r.accumulate_frames_f32(hdr, 1, n, flags=cv.RENDER_COUNT)
+    st = r.stats()
+    a = oracle.hdr_blocks_to_rows(W, H, hdr)
+    b = oracle.hdr_blocks_to_rows(W, H, ref)
+    same = (a == b).all(axis=2)
+    print("width %d: %.5f of the pixels bit-identical, %.2f box / %.2f triangle tests per query"
+          % (width, same.mean(), st["sum_numi_aabb"] / st["extend_rays"],
+             st["sum_numi_tri"] / st["extend_rays"]), flush=True)
+    assert same.mean() >= 0.999
+    assert abs(float(a.sum()) - float(b.sum())) <= 1e-3 * float(b.sum())
+    # frames per step and graphs give the same bits at the wide width too
+    g.set_tuning(frames_per_step=1)
+    h1 = r.new_buffer_f32()
+    r.accumulate_frames_f32(h1, 1, n, flags=cv.RENDER_GRAPH)
+    assert h1.tobytes() == hdr.tobytes()
+    g.set_tuning(frames_per_step=cv.FRAMES_AUTO, bvh_width=2)
+
+
+def test_wide_batch_and_autotune(cv):
+    sds = [scenes.config0(subdivisions=4), scenes.procedural_scene(30000, 9),
+           scenes.procedural_scene(5000, 10)]
+    gs = [cv.Scene.from_scene_data(sd) for sd in sds]
+    specs = [(0, 64, 48, 2), (1, 128, 64, 3), (2, 96, 32, 1)]
+    want = []
+    for (si, w, h, n) in specs:
+        gs[si].set_tuning(bvh_width=4)
+        r = cv.Renderer(gs[si], w, h)
+        hdr = r.new_buffer_f32()
+        r.accumulate_frames_f32(hdr, 1, n)
+        want.append(hdr)
+        gs[si].set_tuning(bvh_width=2)
+    b = cv.Batch()
+    b.set_tuning(bvh_width=4)
+    jobs = [cv.Job(gs[si], w, h, 1, n) for (si, w, h, n) in specs]
+    b.render(jobs)
+    for j, hdr in zip(jobs, want):
+        assert j.hdr.tobytes() == hdr.tobytes()
+    tun, ms = b.autotune(jobs, max_sample_paths=1 << 16)
+    assert tun["bvh_width"] in (2, 4, 8) and ms > 0
+    jobs2 = [cv.Job(gs[si], w, h, 1, n) for (si, w, h, n) in specs]
+    b.render(jobs2, cv.RENDER_GRAPH)
+    assert all(np.isfinite(j.hdr).all() for j in jobs2)
+
+
+def test_wide_server(cv):
+    sd = scenes.procedural_scene(20000, 11)
+    g = cv.Scene.from_scene_data(sd)
+    sv = cv.Server()
+    sid = sv.add_scene(g)
+    sv.set_tuning(bvh_width=8)
+    pos, q = (0.1, 0.7, 0.0), (0.70710677, -0.70710677, 0.0, 0.0)
+    t, hdr = sv.submit(sid, 64, 32, 3, pos, q)
+    sv.wait(t)
+    sv.close()
+    g.set_tuning(bvh_width=8)
+    g.camera.set_position(pos)
+    g.camera.set_orientation(q)
+    r = cv.Renderer(g, 64, 32)
+    ref = r.new_buffer_f32()
+    r.accumulate_frames_f32(ref, 1, 3)
+    assert ref.tobytes() == hdr.tobytes()
