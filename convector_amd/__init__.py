@@ -254,6 +254,14 @@ class Scene:
                                              ctypes.c_void_p(isects_ptr) if isects_ptr else None,
                                              PTR_DEVICE))
 
+    def intersect_nearest_noalloc(self, rays_ptr: int, n: int, hits_ptr=None, isects_ptr=None):
+        """cv_bvh_intersect_nearest_noalloc: device buffers only, no
+        allocation (the library's fixed per-GPU scratch array)."""
+        check(lib().cv_bvh_intersect_nearest_noalloc(
+            self._h, ctypes.c_void_p(rays_ptr), n,
+            ctypes.c_void_p(hits_ptr) if hits_ptr else None,
+            ctypes.c_void_p(isects_ptr) if isects_ptr else None))
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:

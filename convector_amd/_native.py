@@ -21,6 +21,8 @@ STATUS_NAMES = {
 }
 
 CV_PTR_DEVICE = 1
+CV_NOALLOC_MAX_DEPTH = 44
+CV_NOALLOC_SCRATCH_BYTES = 8192 + 16 * 1024 * 1024
 CV_RENDER_COUNT = 2
 CV_RENDER_TIMING = 4
 CV_RENDER_GRAPH = 8
@@ -35,7 +37,7 @@ EXPORTS = [
     "cv_mesh_load_obj", "cv_mesh_from_triangles", "cv_mesh_num_triangles",
     "cv_mesh_copy_triangles", "cv_mesh_destroy", "cv_scene_create", "cv_scene_create_with_bvh",
     "cv_scene_set_camera", "cv_scene_set_tuning", "cv_scene_get_tuning", "cv_scene_get_bvh_info", "cv_scene_copy_bvh", "cv_scene_destroy",
-    "cv_bvh_intersect_nearest", "cv_renderer_create", "cv_renderer_render_patch_u8",
+    "cv_bvh_intersect_nearest", "cv_bvh_intersect_nearest_noalloc", "cv_renderer_create", "cv_renderer_render_patch_u8",
     "cv_renderer_accumulate_patch_f32", "cv_renderer_render_frame_u8",
     "cv_renderer_accumulate_frames_f32", "cv_renderer_buffer_f32_into_bitmap",
     "cv_renderer_get_stats", "cv_renderer_destroy", "cv_batch_create", "cv_batch_set_tuning",
@@ -156,6 +158,7 @@ def _declare(L):
         "cv_scene_copy_bvh": (I, [P, P, P]),
         "cv_scene_destroy": (None, [P]),
         "cv_bvh_intersect_nearest": (I, [P, P, SZ, P, P, U32]),
+        "cv_bvh_intersect_nearest_noalloc": (I, [P, P, SZ, P, P]),
         "cv_renderer_create": (I, [P, U32, U32, P, PP]),
         "cv_renderer_render_patch_u8": (I, [P, P, P, U32, U32, U32, U32, U32]),
         "cv_renderer_accumulate_patch_f32": (I, [P, P, P, U32, U32, U32, U32, U32]),

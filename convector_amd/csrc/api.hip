@@ -763,6 +763,81 @@ cv_status cv_bvh_intersect_nearest(const cv_scene* sc, const cv_ray* rays, size_
   return CV_OK;
 }
 
+}  // extern "C"
+
+// ------------------------------------------------ allocation-free queries
+
+namespace {
+// k_intersect with a fixed launch shape: 2048 persistent 64-lane workgroups
+// (8 per CU), 32 LDS stack entries per lane, and the deeper entries of
+// trees up to CV_NOALLOC_MAX_DEPTH levels in the spill area.
+constexpr uint32_t kNoAllocWaves = 2048;
+constexpr int kNoAllocStack = 32;
+constexpr size_t kNoAllocSpillPerLane = CV_NOALLOC_MAX_DEPTH + 4 - kNoAllocStack;
+constexpr size_t kNoAllocCursorBytes =
+    (size_t)cvk::kIsectGroups * cvk::kIsectCursorStride * sizeof(uint32_t);
+constexpr size_t kNoAllocBytes =
+    kNoAllocCursorBytes + (size_t)kNoAllocWaves * 64 * kNoAllocSpillPerLane * sizeof(uint2);
+static_assert(kNoAllocBytes == CV_NOALLOC_SCRATCH_BYTES, "scratch size and header disagree");
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+// per device: the device address of `scratch` (each GPU holds its own copy)
+void* g_noalloc_ptr[kMaxDevices] = {};
+}  // namespace
+
+// The queries' scratch: work cursors, then the stack spill area.  In host
+// code `scratch` names a host-side stand-in; kernels get the device address
+// that hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+extern "C" {
+
+cv_status cv_bvh_intersect_nearest_noalloc(const cv_scene* sc, const cv_ray* rays, size_t n,
+                                           cv_hit* hits, cv_isect* isects) {
+  if (!sc || (n && !rays)) return fail(CV_ERR_INVALID_ARGUMENT, "null argument");
+  if (n == 0) return CV_OK;
+  if (n >= (1ull << 31)) return fail(CV_ERR_INVALID_ARGUMENT, "too many rays");
+  if (sc->info.depth > CV_NOALLOC_MAX_DEPTH)
+    return fail(CV_ERR_OUT_OF_RANGE,
+                "BVH depth " + std::to_string(sc->info.depth) + " exceeds the " +
+                    std::to_string(CV_NOALLOC_MAX_DEPTH) +
+                    " levels the fixed scratch of the allocation-free query holds");
+  int dev = 0;
+  HIPTRY(hipGetDevice(&dev), "hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices) return fail(CV_ERR_INVALID_ARGUMENT, "device index");
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (!g_noalloc_ptr[dev]) {
+    // once per device, with that device current
+    void* p = nullptr;
+    HIPTRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "scratch address");
+    g_noalloc_ptr[dev] = p;
+  }
+  char* base = static_cast<char*>(g_noalloc_ptr[dev]);
+  cv_scene* s = const_cast<cv_scene*>(sc);
+  std::lock_guard<std::mutex> g(s->mu);
+  hipStream_t st = s->stream;
+  cvk::IntersectArgs a{};
+  a.rays = reinterpret_cast<const float*>(rays);
+  a.n = (uint32_t)n;
+  a.cursor = reinterpret_cast<uint32_t*>(base);
+  a.nodes = s->nodes.p;
+  a.tris = s->tris.p;
+  a.tri_src = s->tri_src.p;
+  a.tri_mat = s->tri_mat.p;
+  a.tri_uv = s->tri_uv.p;
+  a.spill = reinterpret_cast<uint2*>(base + kNoAllocCursorBytes);
+  a.hits = reinterpret_cast<uint32_t*>(hits);
+  a.isects = reinterpret_cast<uint32_t*>(isects);
+  // the cursors start at zero for every call (a fill kernel, no memset node)
+  HIPTRY(cvk::launch_zero_u32(a.cursor, cvk::kIsectGroups * cvk::kIsectCursorStride, st),
+         "zero");
+  HIPTRY(cvk::launch_intersect(a, isects != nullptr, s->packed, kNoAllocStack, kNoAllocWaves,
+                               st),
+         "intersect");
+  HIPTRY(hipStreamSynchronize(st), "sync");
+  return CV_OK;
+}
+
 cv_status cv_device_synchronize(void) {
   cv_status ds = cvh::ensure_device();
   if (ds != CV_OK) return ds;

@@ -202,6 +202,24 @@ typedef struct cv_isect {
 cv_status cv_bvh_intersect_nearest(const cv_scene* scene, const cv_ray* rays, size_t n,
                                    cv_hit* hits, cv_isect* isects, uint32_t flags);
 
+/* The same query with no allocation at all.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  rays, hits and isects are device pointers (hits and isects
+ * may each be NULL); the results equal cv_bvh_intersect_nearest's with
+ * CV_PTR_DEVICE bit for bit.  The traversal's scratch (work cursors and the
+ * stack spill area) is a fixed array of CV_NOALLOC_SCRATCH_BYTES in the
+ * device memory of every GPU the library runs on, held until the process
+ * ends; scenes whose BVH is deeper than CV_NOALLOC_MAX_DEPTH levels are
+ * refused with CV_ERR_OUT_OF_RANGE.  Every call on a GPU shares that GPU's
+ * array and overwrites it: calls are serialised inside the library and
+ * return after the results are written, so calls that overlapped on one GPU
+ * would race on the array. */
+#define CV_NOALLOC_MAX_DEPTH 44u
+#define CV_NOALLOC_SCRATCH_BYTES (8192u + 16u * 1024u * 1024u)
+cv_status cv_bvh_intersect_nearest_noalloc(const cv_scene* scene, const cv_ray* rays, size_t n,
+                                           cv_hit* hits, cv_isect* isects);
+
 /* --------------------------------------------------------------- renderer */
 
 typedef struct cv_renderer cv_renderer;
