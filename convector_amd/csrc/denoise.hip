@@ -33,6 +33,7 @@ namespace cvd {
 
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ uint16_t to_bf16(float f) {
   // round to nearest even (NaN kept quiet)
@@ -190,6 +191,190 @@ __global__ __launch_bounds__(256) void k_conv_lds(ConvArgs a) {
         if (a.yf) {
           float* o = a.yf + (size_t)p * a.yfcs + j;
           *o = a.accumulate ? *o + v : v;
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// The same convolution from LDS halo tiles, for images whose rows are a
+// multiple of 32 pixels wide.  A block of 4 waves computes an 8 x 32 pixel
+// tile for NS * 16 outputs (blockIdx.y picks the slice of NS output tiles):
+// each wave two rows, 4 fragments of 16 pixels.  The input channels run in
+// chunks of 32 (16 when KC == 16).  A chunk's tile with its 1-pixel halo
+// (zeros outside the image) and the chunk's weights for all 9 taps sit in
+// LDS while the next chunk's global loads are in flight into registers, so
+// the MFMAs wait only on LDS.  Per tap a wave reads 4 input and NS weight
+// fragments and runs 4 * NS MFMAs.  With KC == 16 a 32-wide K step covers
+// two taps (lanes 0-31 the even tap, 32-63 the odd one, zeros for the
+// tenth).  The weights are the MFMA's A operand and the pixels its B
+// operand, so each lane ends with 4 consecutive output channels of one
+// pixel and stores them with one 8-byte (bf16) or 16-byte (f32) access.
+// FLIP mirrors the taps (backward to the input, weights Wt).
+// ---------------------------------------------------------------------------
+constexpr uint32_t kHaloW = 32, kHaloH = 8;
+
+template <int KC, int NS, bool FLIP>
+__global__ __launch_bounds__(256, 2) void k_conv_halo(ConvArgs a) {
+  constexpr int MF = 4;
+  constexpr int HH = (int)kHaloH + 2, HW = (int)kHaloW + 2;
+  constexpr int CC = KC == 16 ? 16 : 32;  // channels per chunk
+  constexpr int NCH = KC / CC;
+  constexpr int RS = CC + 8;  // LDS pixel stride (bf16): 16 bytes of padding
+  constexpr int PP = CC / 8;  // 16-byte pieces per pixel and chunk
+  constexpr int XP = HH * HW * PP;  // input pieces per chunk
+  constexpr int KS = KC == 16 ? 5 : 9;  // 32-wide K steps per chunk
+  constexpr int WR = NS * 16;           // weight rows of the slice
+  constexpr int WS = 40;                // LDS weight row stride (bf16)
+  constexpr int WP = KS * WR * 4;       // weight pieces per chunk
+  constexpr int XPT = (XP + 255) / 256, WPT = (WP + 255) / 256;
+  static_assert(KC % CC == 0, "channels per chunk");
+  __shared__ __attribute__((aligned(16))) uint16_t xs[HH * HW * RS];
+  __shared__ __attribute__((aligned(16))) uint16_t ws[KS * WR * WS];
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const uint32_t ntx = a.w_ / kHaloW, nty = (a.h + kHaloH - 1) / kHaloH;
+  const uint32_t txi = blockIdx.x % ntx, tyi = (blockIdx.x / ntx) % nty;
+  const uint32_t img = blockIdx.x / (ntx * nty);
+  const int x0 = (int)(txi * kHaloW), y0 = (int)(tyi * kHaloH);
+  const size_t img_base = (size_t)img * a.h * a.w_;
+  const uint32_t row0 = blockIdx.y * WR;  // first output of the slice
+  // this thread's input pieces: element offset of chunk 0, or ~0u outside
+  // the image; its weight pieces: element offset of chunk 0
+  uint32_t xoff[XPT], woff[WPT];
+#pragma unroll
+  for (int i = 0; i < XPT; i++) {
+    const int q = (int)tid + 256 * i;
+    const int pix = q / PP, part = q % PP;
+    const int sy = y0 + pix / HW - 1, sx = x0 + pix % HW - 1;
+    xoff[i] = (q < XP && sy >= 0 && sy < (int)a.h && sx >= 0 && sx < (int)a.w_)
+                  ? (uint32_t)((img_base + (size_t)sy * a.w_ + (uint32_t)sx) * a.xcs) + part * 8
+                  : ~0u;
+  }
+#pragma unroll
+  for (int i = 0; i < WPT; i++) {
+    const int q = min((int)tid + 256 * i, WP - 1);
+    const int st = q / (WR * 4), r = (q / 4) % WR, g = q % 4;
+    // K step st of the chunk: tap st (KC >= 32) or taps 2 st, 2 st + 1 (KC == 16)
+    const int k = KC == 16 ? st * 32 + 8 * g : st * KC + 8 * g;
+    woff[i] = (row0 + (uint32_t)r) * a.kpad + (uint32_t)k;
+  }
+  u32x4 xr[XPT], wr[WPT];
+#define CV_HALO_LOAD(c)                                                                   \
+  {                                                                                       \
+    _Pragma("unroll") for (int i = 0; i < XPT; i++) xr[i] =                               \
+        xoff[i] != ~0u ? *reinterpret_cast<const u32x4*>(a.x + xoff[i] + (c) * CC)         \
+                       : u32x4{0u, 0u, 0u, 0u};                                           \
+    _Pragma("unroll") for (int i = 0; i < WPT; i++) wr[i] =                               \
+        *reinterpret_cast<const u32x4*>(a.w + woff[i] + (c) * 32);                        \
+  }
+#define CV_HALO_STORE()                                                                   \
+  {                                                                                       \
+    _Pragma("unroll") for (int i = 0; i < XPT; i++) {                                     \
+      const int q = (int)tid + 256 * i;                                                   \
+      if (q < XP) *reinterpret_cast<u32x4*>(&xs[(q / PP) * RS + (q % PP) * 8]) = xr[i];   \
+    }                                                                                     \
+    _Pragma("unroll") for (int i = 0; i < WPT; i++) {                                     \
+      const int q = (int)tid + 256 * i;                                                   \
+      if (q < WP) *reinterpret_cast<u32x4*>(&ws[(q / 4) * WS + (q % 4) * 8]) = wr[i];     \
+    }                                                                                     \
+  }
+  f4 acc[MF][NS];
+#pragma unroll
+  for (int m = 0; m < MF; m++)
+#pragma unroll
+    for (int t = 0; t < NS; t++) acc[m][t] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+  const uint32_t col = lane & 15, kq = lane >> 4;
+  // the wave's fragments: m -> row 2 * wave + m / 2, pixels 16 * (m % 2) + col
+  const int wrow = 2 * (int)wave;
+  const bf8 zero8 = {};
+  CV_HALO_LOAD(0);
+  CV_HALO_STORE();
+  __syncthreads();
+#pragma unroll 1
+  for (int c = 0; c < NCH; c++) {
+    if (c + 1 < NCH) CV_HALO_LOAD(c + 1);
+#pragma unroll
+    for (int st = 0; st < KS; st++) {
+      // the lane's tap: with KC == 16 the upper 32 lanes take the odd one
+      // (the tenth reads the centre and is zeroed)
+      const int tp = KC == 16 ? 2 * st + (int)(kq >> 1) : st;
+      const int tq = tp > 8 ? 4 : tp;
+      int dy = tq / 3 - 1, dx = tq % 3 - 1;
+      if (FLIP) {
+        dy = -dy;
+        dx = -dx;
+      }
+      const int coff = KC == 16 ? 8 * (int)(kq & 1) : 8 * (int)kq;
+      bf8 Wf[NS];
+#pragma unroll
+      for (int t = 0; t < NS; t++)
+        Wf[t] = *reinterpret_cast<const bf8*>(&ws[(st * WR + t * 16 + (int)col) * WS + 8 * kq]);
+#pragma unroll
+      for (int m = 0; m < MF; m++) {
+        const int hy = wrow + m / 2 + 1 + dy, hx = 16 * (m % 2) + (int)col + 1 + dx;
+        bf8 X = *reinterpret_cast<const bf8*>(&xs[(hy * HW + hx) * RS + coff]);
+        if (KC == 16 && tp > 8) X = zero8;
+#pragma unroll
+        for (int t = 0; t < NS; t++)
+          acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Wf[t], X, acc[m][t], 0, 0, 0);
+      }
+    }
+    if (c + 1 < NCH) {
+      __syncthreads();
+      CV_HALO_STORE();
+      __syncthreads();
+    }
+  }
+#undef CV_HALO_LOAD
+#undef CV_HALO_STORE
+  // lane: output channels row0 + t * 16 + 4 * kq + r (r = 0..3) of pixel
+  // col of each fragment
+  const bool vec_y = a.y && ((uintptr_t)a.y % 8 == 0) && (a.ycs % 4 == 0);
+  const bool vec_f = a.yf && ((uintptr_t)a.yf % 16 == 0) && (a.yfcs % 4 == 0);
+#pragma unroll
+  for (int m = 0; m < MF; m++) {
+    const int py = y0 + wrow + m / 2;
+    if (py >= (int)a.h) continue;
+    const size_t p = img_base + (size_t)py * a.w_ + (uint32_t)x0 + 16 * (m % 2) + col;
+#pragma unroll
+    for (int t = 0; t < NS; t++) {
+      const uint32_t j0 = row0 + t * 16 + 4 * kq;
+      if (j0 >= a.nout) continue;
+      float v[4];
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const uint32_t j = j0 + r;
+        v[r] = acc[m][t][r] + ((a.bias && j < a.nout) ? a.bias[j] : 0.0f);
+        if (a.relu) v[r] = fmaxf(v[r], 0.0f);
+      }
+      const bool whole = j0 + 4 <= a.nout;
+      if (a.y) {
+        if (whole && vec_y) {
+          uint2 pk;
+          pk.x = (uint32_t)to_bf16(v[0]) | ((uint32_t)to_bf16(v[1]) << 16);
+          pk.y = (uint32_t)to_bf16(v[2]) | ((uint32_t)to_bf16(v[3]) << 16);
+          *reinterpret_cast<uint2*>(a.y + p * a.ycs + j0) = pk;
+        } else {
+          for (int r = 0; r < 4; r++)
+            if (j0 + r < a.nout) a.y[p * a.ycs + j0 + r] = to_bf16(v[r]);
+        }
+      }
+      if (a.yf) {
+        float* o = a.yf + p * a.yfcs + j0;
+        if (whole && vec_f) {
+          float4 w4 = make_float4(v[0], v[1], v[2], v[3]);
+          if (a.accumulate) {
+            const float4 old = *reinterpret_cast<const float4*>(o);
+            w4.x += old.x;
+            w4.y += old.y;
+            w4.z += old.z;
+            w4.w += old.w;
+          }
+          *reinterpret_cast<float4*>(o) = w4;
+        } else {
+          for (int r = 0; r < 4; r++)
+            if (j0 + r < a.nout) o[r] = a.accumulate ? o[r] + v[r] : v[r];
         }
       }
     }
@@ -743,12 +928,47 @@ struct cv_denoiser {
       g_d0a, g_d0b, g_up1, g_up0, loss;
   DevBuf<float> in_f32, tgt_f32;
   DevBuf<float> wpart;  // weight-gradient partials of the pixel splits
+  // convolutions from LDS halo tiles where the shape allows (else k_conv_lds)
+  bool halo_convs = true;
   std::mutex mu;
 };
 
 namespace {
 
+// The halo-tile kernel for the network's shapes (rows a multiple of 32
+// pixels, the input channels a multiple of 16 in a dense channel range):
+// NT output tiles in slices of NS per block (grid.y = NT / NS), at most 4,
+// so that a chunk's weights and tile fit in LDS beside other blocks.
+template <int KC, int NS, bool FLIP>
+bool try_halo(cv_denoiser* d, const cvd::ConvArgs& a, uint32_t nt, uint32_t ns) {
+  if (a.kc != (uint32_t)KC || ns != (uint32_t)NS || (a.flip != 0) != FLIP) return false;
+  const uint32_t imgs = a.npix / (a.h * a.w_);
+  const uint32_t blocks =
+      imgs * (a.w_ / cvd::kHaloW) * ((a.h + cvd::kHaloH - 1) / cvd::kHaloH);
+  hipLaunchKernelGGL((cvd::k_conv_halo<KC, NS, FLIP>), dim3(blocks, nt / NS), dim3(256), 0,
+                     d->stream, a);
+  return true;
+}
+
+bool launch_conv_halo(cv_denoiser* d, const cvd::ConvArgs& a, uint32_t nt) {
+  // whole 32-pixel rows, whole images, and every pixel's KC channels within
+  // its stride (the tile loads read channels [0, KC) of each pixel)
+  if (a.w_ % cvd::kHaloW || a.npix % (a.h * a.w_) || a.xcs % 8 || a.xcs < a.kc) return false;
+  const uint32_t ns = nt % 4 == 0 ? 4 : nt % 3 == 0 ? 3 : nt % 2 == 0 ? 2 : 1;
+  return try_halo<16, 2, false>(d, a, nt, ns) || try_halo<32, 1, false>(d, a, nt, ns) ||
+         try_halo<32, 2, false>(d, a, nt, ns) || try_halo<32, 4, false>(d, a, nt, ns) ||
+         try_halo<64, 4, false>(d, a, nt, ns) || try_halo<96, 2, false>(d, a, nt, ns) ||
+         try_halo<128, 4, false>(d, a, nt, ns) || try_halo<192, 4, false>(d, a, nt, ns) ||
+         try_halo<16, 2, true>(d, a, nt, ns) || try_halo<32, 2, true>(d, a, nt, ns) ||
+         try_halo<32, 3, true>(d, a, nt, ns) || try_halo<64, 2, true>(d, a, nt, ns) ||
+         try_halo<64, 4, true>(d, a, nt, ns) || try_halo<128, 4, true>(d, a, nt, ns);
+}
+
 cv_status launch_conv(cv_denoiser* d, const cvd::ConvArgs& a, uint32_t nt) {
+  if (d->halo_convs && launch_conv_halo(d, a, nt)) {
+    HIPTRY(hipGetLastError(), "conv");
+    return CV_OK;
+  }
   const uint32_t blocks = (a.npix + cvd::kConvPix - 1) / cvd::kConvPix;
   hipStream_t s = d->stream;
   switch (nt) {
