@@ -232,7 +232,8 @@ def run_batch(args):
             for i, (g, (_, _, w, h, spp)) in enumerate(zip(gs, specs))]
     paths = sum(w * h * spp for (_, _, w, h, spp) in specs)
     b = cv.Batch()
-    tuning, tune_ms = b.autotune(jobs, max_sample_paths=1 << 22)
+    # the configuration's tolerance admits the wide trees: sweep them too
+    tuning, tune_ms = b.autotune(jobs, max_sample_paths=1 << 22, tolerance=True)
     log("batch: autotune chose %s (sample step %.2f ms)" % (tuning, tune_ms))
     flags = cv.PTR_DEVICE | cv.RENDER_GRAPH
     b.render(jobs, flags)  # warm-up (captures the graphs)
@@ -990,6 +991,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true",
                     help="frame workload: leave out the nested 64-job batch result")
+    ap.add_argument("--no-wide", action="store_true",
+                    help="frame workload: leave out the width-4 (tolerance mode) step")
     ap.add_argument("--batch-steps", type=int, default=3,
                     help="frame workload: timed renders of the nested 64-job batch")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -1166,6 +1169,8 @@ def main():
     # the same step with the tree collapsed to 4 children (a tolerance mode:
     # hits as the reference's except at exact ties), reported beside
     try:
+        if args.no_wide:
+            raise RuntimeError("skipped (--no-wide)")
         scene.set_tuning(bvh_width=4)
         step(cv.RENDER_TIMING)
         cv.device_synchronize()
@@ -1182,6 +1187,27 @@ def main():
             "note": "tolerance mode; value above is the exact width-2 traversal"}
     except Exception as e:
         log("width-4 step failed: %r" % (e,))
+    # the same step with the bounce rays traversed in ray-sort key order
+    # (exact: the same bits), 12 and 24 key bits
+    if not args.no_wide:
+        sorted_steps = {}
+        try:
+            for bits in (12, 24):
+                scene.set_tuning(ray_sort_bits=bits)
+                step(cv.RENDER_TIMING)
+                cv.device_synchronize()
+                t1 = time.perf_counter()
+                ss = [step(cv.RENDER_TIMING) for _ in range(min(args.steps, 3))]
+                cv.device_synchronize()
+                walls = (time.perf_counter() - t1) / len(ss)
+                sorted_steps[str(bits)] = {
+                    "ms_per_step": round(walls * 1e3, 3),
+                    "extend_ms_per_step": round(sum(s["extend_ms"] for s in ss) / len(ss), 3),
+                    "sort_ms_per_step": round(sum(s["sort_ms"] for s in ss) / len(ss), 3)}
+            out["ray_sort"] = sorted_steps
+        except Exception as e:
+            log("ray-sort steps failed: %r" % (e,))
+        scene.set_tuning(ray_sort_bits=cv.RAY_SORT_NONE)
     if not args.no_cpu:
         try:
             cb = cpu_baseline(sd, scene, args)

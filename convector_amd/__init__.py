@@ -31,6 +31,7 @@ PTR_DEVICE = _native.CV_PTR_DEVICE
 RENDER_COUNT = _native.CV_RENDER_COUNT
 RENDER_TIMING = _native.CV_RENDER_TIMING
 FRAMES_AUTO = 255  # cv_tuning.frames_per_step: about 2^25 paths per step
+RAY_SORT_NONE = _native.CV_RAY_SORT_NONE  # cv_tuning.ray_sort_bits: queue order
 RENDER_GRAPH = _native.CV_RENDER_GRAPH
 
 NODE_DTYPE = np.dtype([("min", "<f4", 3), ("max", "<f4", 3), ("index", "<u4"), ("len", "<u4")])
@@ -214,11 +215,11 @@ class Scene:
 
     def set_tuning(self, stack_entries=0, extend_waves_per_cu=0, refill_threshold=0,
                    fetch_batches=0, block_waves=0, frames_per_step=0, refill_interval=0,
-                   leaf_batch=0, bvh_width=0):
+                   leaf_batch=0, bvh_width=0, ray_sort_bits=0):
         """Launch shape of the traversal (the autotuner's knobs)."""
         t = _native.Tuning(stack_entries, extend_waves_per_cu, refill_threshold, fetch_batches,
                            block_waves, frames_per_step, refill_interval, leaf_batch,
-                           bvh_width)
+                           bvh_width, ray_sort_bits)
         check(lib().cv_scene_set_tuning(self._h, ctypes.byref(t)))
 
     def tuning(self) -> dict:
@@ -375,10 +376,10 @@ class Batch:
 
     def set_tuning(self, stack_entries=0, extend_waves_per_cu=0, refill_threshold=0,
                    fetch_batches=0, block_waves=0, frames_per_step=0, refill_interval=0,
-                   leaf_batch=0, bvh_width=0):
+                   leaf_batch=0, bvh_width=0, ray_sort_bits=0):
         t = _native.Tuning(stack_entries, extend_waves_per_cu, refill_threshold, fetch_batches,
                            block_waves, frames_per_step, refill_interval, leaf_batch,
-                           bvh_width)
+                           bvh_width, ray_sort_bits)
         check(lib().cv_batch_set_tuning(self._h, ctypes.byref(t)))
 
     def tuning(self) -> dict:
@@ -401,12 +402,15 @@ class Batch:
     def render(self, jobs: Sequence[Job], flags: int = 0):
         check(lib().cv_batch_render(self._h, self._jobs(jobs), len(jobs), flags))
 
-    def autotune(self, jobs: Sequence[Job], max_sample_paths: int = 0):
+    def autotune(self, jobs: Sequence[Job], max_sample_paths: int = 0, tolerance: bool = False):
         """Times launch shapes on a sample step of these jobs and keeps the
-        fastest; returns (tuning dict, step ms)."""
+        fastest; returns (tuning dict, step ms).  tolerance=True also sweeps
+        the BVH widths 4 and 8 (hits as the reference's except at exact
+        ties)."""
         t = _native.Tuning()
         ms = ctypes.c_double()
         check(lib().cv_batch_autotune(self._h, self._jobs(jobs), len(jobs), max_sample_paths,
+                                      _native.CV_TUNE_TOLERANCE if tolerance else 0,
                                       ctypes.byref(t), ctypes.byref(ms)))
         return {k: getattr(t, k) for k, _ in t._fields_}, ms.value
 
@@ -535,10 +539,10 @@ class Server:
 
     def set_tuning(self, stack_entries=0, extend_waves_per_cu=0, refill_threshold=0,
                    fetch_batches=0, block_waves=0, frames_per_step=0, refill_interval=0,
-                   leaf_batch=0, bvh_width=0):
+                   leaf_batch=0, bvh_width=0, ray_sort_bits=0):
         t = _native.Tuning(stack_entries, extend_waves_per_cu, refill_threshold, fetch_batches,
                            block_waves, frames_per_step, refill_interval, leaf_batch,
-                           bvh_width)
+                           bvh_width, ray_sort_bits)
         check(lib().cv_server_set_tuning(self._h, ctypes.byref(t)))
 
     def submit(self, scene_id: int, width: int, height: int, spp: int, camera_pos, camera_q,

@@ -26,6 +26,8 @@ CV_NOALLOC_SCRATCH_BYTES = 8192 + 16 * 1024 * 1024
 CV_RENDER_COUNT = 2
 CV_RENDER_TIMING = 4
 CV_RENDER_GRAPH = 8
+CV_RAY_SORT_NONE = 255
+CV_TUNE_TOLERANCE = 1
 CV_RNG_PIXEL = 0
 CV_RNG_REF_PATCH = 1
 CV_APPROX_PORTABLE = 0
@@ -81,7 +83,7 @@ class Tuning(ctypes.Structure):
                 ("refill_threshold", ctypes.c_uint32), ("fetch_batches", ctypes.c_uint32),
                 ("block_waves", ctypes.c_uint32), ("frames_per_step", ctypes.c_uint32),
                 ("refill_interval", ctypes.c_uint32), ("leaf_batch", ctypes.c_uint32),
-                ("bvh_width", ctypes.c_uint32)]
+                ("bvh_width", ctypes.c_uint32), ("ray_sort_bits", ctypes.c_uint32)]
 
 
 class RenderOpts(ctypes.Structure):
@@ -117,7 +119,7 @@ class RenderStats(ctypes.Structure):
                 ("extend_launches", ctypes.c_uint64), ("extend_ms", ctypes.c_double),
                 ("shade_ms", ctypes.c_double), ("generate_ms", ctypes.c_double),
                 ("total_ms", ctypes.c_double), ("sum_numi_aabb", ctypes.c_uint64),
-                ("sum_numi_tri", ctypes.c_uint64)]
+                ("sum_numi_tri", ctypes.c_uint64), ("sort_ms", ctypes.c_double)]
 
 
 _lib = None
@@ -171,7 +173,7 @@ def _declare(L):
         "cv_batch_set_tuning": (I, [P, P]),
         "cv_batch_get_tuning": (I, [P, P]),
         "cv_batch_render": (I, [P, P, SZ, U32]),
-        "cv_batch_autotune": (I, [P, P, SZ, ctypes.c_uint64, P, P]),
+        "cv_batch_autotune": (I, [P, P, SZ, ctypes.c_uint64, U32, P, P]),
         "cv_batch_get_stats": (I, [P, P]),
         "cv_batch_destroy": (None, [P]),
         "cv_renderer_accumulate_aov_f32": (I, [P, P, P, U32, U32, U32]),

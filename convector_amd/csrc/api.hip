@@ -102,6 +102,10 @@ cv_status validate_tuning(const cv_tuning& t) {
     return fail(CV_ERR_INVALID_ARGUMENT, "tuning value out of range");
   if (t.bvh_width && t.bvh_width != 2 && t.bvh_width != 4 && t.bvh_width != 8)
     return fail(CV_ERR_INVALID_ARGUMENT, "bvh_width must be 2, 4 or 8");
+  if (t.ray_sort_bits && t.ray_sort_bits != CV_RAY_SORT_NONE &&
+      (t.ray_sort_bits < 3 || t.ray_sort_bits > 24 || t.ray_sort_bits % 3))
+    return fail(CV_ERR_INVALID_ARGUMENT,
+                "ray_sort_bits must be a multiple of 3 from 3 to 24, or CV_RAY_SORT_NONE");
   return CV_OK;
 }
 
@@ -115,6 +119,7 @@ void apply_tuning(LaunchCfg& c, const cv_tuning& t) {
   if (t.refill_interval) c.refill_interval = t.refill_interval;
   if (t.leaf_batch) c.leaf_batch = t.leaf_batch;
   if (t.bvh_width) c.width = t.bvh_width;
+  if (t.ray_sort_bits) c.sort_bits = t.ray_sort_bits == CV_RAY_SORT_NONE ? 0u : t.ray_sort_bits;
 }
 
 cv_tuning tuning_of(const LaunchCfg& c) {
@@ -128,6 +133,7 @@ cv_tuning tuning_of(const LaunchCfg& c) {
   t.refill_interval = c.refill_interval;
   t.leaf_batch = c.leaf_batch;
   t.bvh_width = c.width;
+  t.ray_sort_bits = c.sort_bits ? c.sort_bits : CV_RAY_SORT_NONE;
   return t;
 }
 
@@ -141,6 +147,10 @@ cvk::JobDesc job_for_scene(cv_scene* s, uint32_t width) {
   j.nlights = s->ndirect;
   j.has_override = s->has_override ? 1u : 0u;
   memcpy(j.emission, s->emission, 12);
+  for (int k = 0; k < 3; k++) {
+    j.lo[k] = s->lo[k];
+    j.inv_ext[k] = 1.0f / std::max(s->hi[k] - s->lo[k], 1e-30f);
+  }
   std::lock_guard<std::mutex> g(s->mu);
   j.cam = s->cam;
   return j;
@@ -556,6 +566,15 @@ static cv_status scene_finish(cv_scene* s, const std::vector<float>& v9,
   HIPTRY(cvk::launch_prepare_lights(s->tris.p, s->direct.p, s->ndirect, s->lights.p, st),
          "prepare lights");
   HIPTRY(hipStreamSynchronize(st), "sync");
+  // the scene's bounds: the union of the root's two children
+  {
+    cv_bvh_node root[2];
+    HIPTRY(hipMemcpy(root, s->nodes.p, sizeof(root), hipMemcpyDeviceToHost), "download");
+    for (int k = 0; k < 3; k++) {
+      s->lo[k] = std::min(root[0].min[k], root[1].min[k]);
+      s->hi[k] = std::max(root[0].max[k], root[1].max[k]);
+    }
+  }
   // traversal configuration: packed stack entries need index < 2^27 and
   // leaves of at most 31 triangles
   s->packed = s->info.max_leaf <= 31 && N < (1u << 27) && s->info.num_nodes < (1u << 27);

@@ -92,6 +92,7 @@ struct LaunchCfg {
   uint32_t refill_interval = 16;              // traversal steps between refill checks
   uint32_t leaf_batch = 16;                   // 64ths of the active lanes per leaf round; 65 = off
   uint32_t width = 2;                         // BVH children per node: 2 (the reference's), 4, 8
+  uint32_t sort_bits = 0;                     // ray-sort key bits after the camera rays (0: none)
 };
 
 }  // namespace cvh
@@ -124,6 +125,7 @@ struct cv_scene {
   cvh::DevBuf<cvk::DevLight> lights;
   uint32_t ndirect = 0;
   cv_bvh_info info{};
+  float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};  // bounds of the triangles (the root's box)
   bool packed = true;
   cvk::CameraParams cam{};
   bool has_override = false;

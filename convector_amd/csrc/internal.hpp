@@ -67,6 +67,8 @@ struct JobDesc {
   uint32_t rect_x, rect_y, rect_w, rect_h;
   uint32_t frame;
   uint32_t path_base;  // index of the job's first path in the step
+  float lo[3];         // the scene's bounds: lower corner and 1 / extent per
+  float inv_ext[3];    // axis (ray-sort keys)
 };
 
 struct GenerateArgs {
@@ -92,7 +94,28 @@ struct ExtendArgs {
                         // the active lanes do (or all)
   const float* rays6;         // non-null (batched queries): the rays as n x 6 floats, not r0/r1
   const uint32_t* tri_src;    // non-null (batched queries): hits name source triangles
+  const uint32_t* order;      // non-null: queue position k traverses path order[k]
 };
+
+// The ray sort before a bounce's traversal: a counting sort of the paths
+// by key (job index, direction octant, Morton code of the origin) into
+// `order`.  Buckets: 1 << key_bits counters; block_sums: one word per 4096
+// buckets.
+struct SortArgs {
+  Ctrl* ctrl;
+  uint32_t bounce;
+  const float4* r0;
+  const float4* r1;
+  const JobDesc* jobs;
+  uint32_t multi_job;   // job index in r1.w (else every path is job 0's)
+  uint32_t axis_bits;   // Morton bits per axis
+  uint32_t key_bits;    // job bits + 3 + 3 * axis_bits
+  uint32_t* keys;
+  uint32_t* buckets;
+  uint32_t* block_sums;
+  uint32_t* order;
+};
+constexpr uint32_t kSortScanBlock = 4096;  // buckets per scan workgroup
 
 struct ShadeArgs {
   Ctrl* ctrl;
@@ -155,6 +178,9 @@ hipError_t launch_extend(const ExtendArgs& a, bool count, bool packed, bool mult
                          uint32_t blocks, uint32_t block_waves, hipStream_t s,
                          uint32_t width = 2);
 hipError_t launch_generate(const GenerateArgs& a, uint32_t blocks, hipStream_t s);
+// The ray sort of one bounce (four kernels: clear, keys, scan, scatter);
+// `paths` bounds the step's path count (the grid of the per-path kernels).
+hipError_t launch_sort(const SortArgs& a, uint32_t paths, hipStream_t s);
 // multi: the step has more than one job.
 // shared_scene: every job of the step has the same scene (tris, lights,
 // emission), read once per wave from job 0.

@@ -620,8 +620,8 @@ __global__ __launch_bounds__(64) void k_extend(ExtendArgs a) {
     for (uint32_t b = 0; b < a.batches; b++) {
     uint32_t base = cbase + 64u * b;
     if (base >= n) break;
-    uint32_t i = base + lane;
-    bool valid = i < n;
+    bool valid = base + lane < n;
+    const uint32_t i = valid && a.order ? a.order[base + lane] : base + lane;
     V3 o = mk3(0, 0, 0), d = mk3(1, 0, 0);
     if (valid) load_ray(a, i, o, d);
     TravState ts = traverse<COUNT, S, PACKED>(a.nodes, a.tris, o, d, lds + lane,
@@ -828,8 +828,9 @@ __global__ __launch_bounds__(256) void k_extend_refill(ExtendArgs a) {
       }
       if (!active) {
         const uint32_t rank = __popcll(idle & lt);
-        const uint32_t i = rank < from_pool ? base0 + rank : base1 + (rank - from_pool);
-        if (rank < from_pool || i < end1) {
+        const uint32_t k = rank < from_pool ? base0 + rank : base1 + (rank - from_pool);
+        if (rank < from_pool || k < end1) {
+          const uint32_t i = a.order ? a.order[k] : k;
           if (MULTI) {
             const float4 r0 = a.r0[i], r1 = a.r1[i];
             o = mk3(r0.x, r0.y, r0.z);
@@ -1243,6 +1244,105 @@ __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
   if (i < n) p[i] = 0u;
 }
 
+// ---------------------------------------------------------------------------
+// Ray sort: a counting sort of a bounce's paths by key into a traversal
+// order.  The key of a path is its job index (multi-job steps), the octant
+// of its direction and the Morton code of its origin quantised to
+// 2^axis_bits cells per axis of the job's scene bounds (outside: clamped).
+// Paths of one key keep no particular order (atomics), which changes only
+// which lane traverses which ray.
+// ---------------------------------------------------------------------------
+
+// The low 8 bits of v moved to every third bit (bit k to bit 3k).
+__device__ __forceinline__ uint32_t spread3(uint32_t v) {
+  v &= 0xffu;
+  v = (v | (v << 8)) & 0x0000f00fu;
+  v = (v | (v << 4)) & 0x000c30c3u;
+  v = (v | (v << 2)) & 0x00249249u;
+  return v;
+}
+
+__device__ __forceinline__ uint32_t ray_key(const SortArgs& a, float4 r0, float4 r1) {
+  const uint32_t job = a.multi_job ? f2u(r1.w) : 0u;
+  const JobDesc& J = a.jobs[job];
+  const float cells = (float)(1u << a.axis_bits);
+  const float p[3] = {r0.x, r0.y, r0.z};
+  uint32_t m = 0;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    // NaN and points below the bounds go to cell 0 (fmaxf keeps the number)
+    const float u = fminf(fmaxf((p[k] - J.lo[k]) * J.inv_ext[k] * cells, 0.0f), cells - 1.0f);
+    m |= spread3((uint32_t)u) << k;
+  }
+  const uint32_t oct =
+      (r1.x < 0.0f ? 1u : 0u) | (r1.y < 0.0f ? 2u : 0u) | (r1.z < 0.0f ? 4u : 0u);
+  const uint32_t mb = 3 * a.axis_bits;
+  return (job << (mb + 3)) | (oct << mb) | m;
+}
+
+__global__ __launch_bounds__(256) void k_sort_clear(uint32_t* p, uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    p[i] = 0u;
+}
+
+__global__ __launch_bounds__(256) void k_sort_keys(SortArgs a) {
+  const uint32_t n = a.ctrl->count[a.bounce];
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t key = ray_key(a, a.r0[i], a.r1[i]);
+    a.keys[i] = key;
+    atomicAdd(&a.buckets[key], 1u);
+  }
+}
+
+// In-place exclusive prefix sums of b[0, nb) within blocks of
+// kSortScanBlock words (1024 threads, 4 words each); each block's total
+// goes to sums[blockIdx.x].
+__global__ __launch_bounds__(1024) void k_sort_scan(uint32_t* b, uint32_t nb, uint32_t* sums) {
+  __shared__ uint32_t wsum[16];
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t base = blockIdx.x * kSortScanBlock + threadIdx.x * 4;
+  uint32_t v[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) v[k] = base + k < nb ? b[base + k] : 0u;
+  const uint32_t t = v[0] + v[1] + v[2] + v[3];
+  uint32_t inc = t;
+#pragma unroll
+  for (uint32_t off = 1; off < 64; off <<= 1) {
+    const uint32_t y = __shfl_up(inc, off);
+    if (lane >= off) inc += y;
+  }
+  if (lane == 63) wsum[wave] = inc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t run = 0;
+    for (int w = 0; w < 16; w++) {
+      const uint32_t x = wsum[w];
+      wsum[w] = run;
+      run += x;
+    }
+    sums[blockIdx.x] = run;
+  }
+  __syncthreads();
+  uint32_t run = wsum[wave] + inc - t;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    if (base + k < nb) b[base + k] = run;
+    run += v[k];
+  }
+}
+
+// order[position] = path: the position is the key's start (its block's
+// prefix plus the prefix within the block) plus the path's rank among the
+// key's paths.
+__global__ __launch_bounds__(256) void k_sort_scatter(SortArgs a) {
+  const uint32_t n = a.ctrl->count[a.bounce];
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t key = a.keys[i];
+    const uint32_t pos = a.block_sums[key / kSortScanBlock] + atomicAdd(&a.buckets[key], 1u);
+    a.order[pos] = i;
+  }
+}
+
 }  // namespace cvk
 
 // ---------------------------------------------------------------------------
@@ -1507,6 +1607,22 @@ hipError_t launch_extend(const ExtendArgs& a, bool count, bool packed, bool mult
 
 hipError_t launch_generate(const GenerateArgs& a, uint32_t blocks, hipStream_t s) {
   hipLaunchKernelGGL(k_generate, dim3(blocks), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_sort(const SortArgs& a, uint32_t paths, hipStream_t s) {
+  if (a.key_bits > 24 || a.key_bits < 3) return hipErrorInvalidValue;
+  const uint32_t nb = 1u << a.key_bits;
+  const uint32_t nblocks = (nb + kSortScanBlock - 1) / kSortScanBlock;  // <= 4096
+  const uint32_t pgrid = std::max(1u, std::min((paths + 255) / 256, 8192u));
+  hipLaunchKernelGGL(k_sort_clear, dim3(std::min((nb + 255) / 256, 8192u)), dim3(256), 0, s,
+                     a.buckets, nb);
+  hipLaunchKernelGGL(k_sort_keys, dim3(pgrid), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_sort_scan, dim3(nblocks), dim3(1024), 0, s, a.buckets, nb, a.block_sums);
+  // the block totals' own prefix (one block; its total lands past them)
+  hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, s, a.block_sums, nblocks,
+                     a.block_sums + kSortScanBlock);
+  hipLaunchKernelGGL(k_sort_scatter, dim3(pgrid), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 

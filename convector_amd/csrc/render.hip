@@ -69,7 +69,7 @@ struct RunOpts {
 struct GraphKey {
   int stack = 0;
   uint32_t blocks = 0, refill = 0, fetch_batches = 0, max_bounces = 0, block_waves = 0;
-  uint32_t interval = 0, leaf_batch = 0, width = 2;
+  uint32_t interval = 0, leaf_batch = 0, width = 2, sort_bits = 0;
   bool packed = false, count = false, multi_job = false;
   const void* nodes = nullptr;
   const void* tris = nullptr;
@@ -79,7 +79,8 @@ struct GraphKey {
            block_waves == o.block_waves && interval == o.interval && leaf_batch == o.leaf_batch &&
            fetch_batches == o.fetch_batches && max_bounces == o.max_bounces &&
            packed == o.packed && count == o.count && multi_job == o.multi_job &&
-           nodes == o.nodes && tris == o.tris && capacity == o.capacity && width == o.width;
+           nodes == o.nodes && tris == o.tris && capacity == o.capacity && width == o.width &&
+           sort_bits == o.sort_bits;
   }
 };
 
@@ -107,6 +108,9 @@ struct Pipeline {
   hipEvent_t slot_done[kRingSlots] = {};
   uint32_t next_slot = 0;
   DevBuf<uint2> spill;
+  // the ray sort: keys and traversal order per path, key buckets and the
+  // scan's block totals
+  DevBuf<uint32_t> sort_keys, sort_order, sort_buckets, sort_sums;
   // one captured step per job-table slot (the slot's table address is part
   // of the recorded launches)
   hipGraphExec_t graphs[kRingSlots] = {};
@@ -134,6 +138,10 @@ struct Pipeline {
     hits.reset();
     ctrl.reset();
     spill.reset();
+    sort_keys.reset();
+    sort_order.reset();
+    sort_buckets.reset();
+    sort_sums.reset();
     if (stream) (void)hipStreamDestroy(stream);
     stream = nullptr;
   }
@@ -182,6 +190,34 @@ struct Pipeline {
     return CV_OK;
   }
 
+  // Key bits of the ray sort for sort_bits requested bits: the job index
+  // (multi-job steps, up to max_jobs jobs) above the octant and Morton code,
+  // at most 24 bits in all (fewer Morton bits per axis when jobs need them).
+  static void sort_key_bits(uint32_t sort_bits, bool multi_job, size_t max_jobs,
+                            uint32_t* axis_bits, uint32_t* key_bits) {
+    uint32_t jb = 0;
+    while (multi_job && (1ull << jb) < max_jobs) jb++;
+    uint32_t ab = (sort_bits - 3) / 3;
+    while (ab > 0 && jb + 3 + 3 * ab > 24) ab--;
+    *axis_bits = ab;
+    *key_bits = jb + 3 + 3 * ab;
+  }
+
+  cv_status reserve_sort(uint32_t key_bits) {
+    const size_t nb = (size_t)1 << key_bits;
+    if (sort_keys.n < capacity || sort_order.n < capacity || sort_buckets.n < nb ||
+        sort_sums.n < cvk::kSortScanBlock + 1) {
+      HIPTRY(hipStreamSynchronize(stream), "sync");
+      drop_graphs();
+      if (sort_keys.n < capacity) HIPTRY(sort_keys.alloc(capacity), "alloc sort");
+      if (sort_order.n < capacity) HIPTRY(sort_order.alloc(capacity), "alloc sort");
+      if (sort_buckets.n < nb) HIPTRY(sort_buckets.alloc(nb), "alloc sort");
+      if (sort_sums.n < cvk::kSortScanBlock + 1)
+        HIPTRY(sort_sums.alloc(cvk::kSortScanBlock + 1), "alloc sort");
+    }
+    return CV_OK;
+  }
+
   cv_status reserve_spill(const cvh::LaunchCfg& cfg, uint32_t max_depth) {
     uint32_t need = max_depth + 4;
     size_t per_lane = need > (uint32_t)cfg.stack ? (size_t)(need - cfg.stack) : 0;
@@ -198,7 +234,7 @@ struct Pipeline {
 
 namespace {
 
-// A recorded launch: kind (0 generate, 1 extend, 2 shade), bounce, the
+// A recorded launch: kind (0 generate, 1 extend, 2 shade, 3 ray sort), bounce, the
 // frame of the step's first job, and its start / end events.
 struct LaunchEvent {
   int kind;
@@ -289,6 +325,27 @@ cv_status launch_step(Pipeline& p, const RunOpts& o, uint32_t slot, uint32_t pat
     x.refill = o.cfg.refill <= 64 ? o.cfg.refill : (multi_scene ? 64u : 0u);
     x.interval = o.cfg.refill_interval;
     x.leaf_batch = o.cfg.leaf_batch <= 64 ? o.cfg.leaf_batch : 0u;
+    if (o.cfg.sort_bits && b > 0) {
+      // after the camera rays: traverse the bounce's paths in key order
+      cvk::SortArgs so{};
+      so.ctrl = p.ctrl.p;
+      so.bounce = b;
+      so.r0 = x.r0;
+      so.r1 = x.r1;
+      so.jobs = jobs;
+      so.multi_job = o.multi_job ? 1u : 0u;
+      Pipeline::sort_key_bits(o.cfg.sort_bits, o.multi_job, p.max_jobs, &so.axis_bits,
+                              &so.key_bits);
+      so.keys = p.sort_keys.p;
+      so.buckets = p.sort_buckets.p;
+      so.block_sums = p.sort_sums.p;
+      so.order = p.sort_order.p;
+      ep = ev_pair(3);
+      if (ep) (void)hipEventRecord(ep->first, st);
+      HIPTRY(cvk::launch_sort(so, (uint32_t)grid_paths, st), "ray sort");
+      if (ep) (void)hipEventRecord(ep->second, st);
+      x.order = p.sort_order.p;
+    }
     ep = ev_pair(1);
     if (ep) (void)hipEventRecord(ep->first, st);
     HIPTRY(cvk::launch_extend(x, count, o.packed, multi_scene, o.cfg.stack, o.cfg.blocks,
@@ -323,6 +380,11 @@ cv_status pipeline_run(Pipeline& p, const std::vector<StepPlan>& steps, const Ru
   if (cs != CV_OK) return cs;
   cs = p.reserve_spill(o.cfg, o.max_depth);
   if (cs != CV_OK) return cs;
+  if (o.cfg.sort_bits && o.max_bounces > 1) {
+    uint32_t ab = 0, kb = 0;
+    Pipeline::sort_key_bits(o.cfg.sort_bits, o.multi_job, p.max_jobs, &ab, &kb);
+    if ((cs = p.reserve_sort(kb)) != CV_OK) return cs;
+  }
   hipStream_t st = p.stream;
   const bool timing = (o.flags & CV_RENDER_TIMING) || trace_enabled();
   const bool use_graph = o.graph && !timing;
@@ -335,6 +397,7 @@ cv_status pipeline_run(Pipeline& p, const std::vector<StepPlan>& steps, const Ru
   key.interval = o.cfg.refill_interval;
   key.leaf_batch = o.cfg.leaf_batch;
   key.width = o.cfg.width;
+  key.sort_bits = o.cfg.sort_bits;
   key.fetch_batches = o.cfg.fetch_batches;
   key.max_bounces = o.max_bounces;
   key.packed = o.packed;
@@ -451,6 +514,7 @@ cv_status pipeline_run(Pipeline& p, const std::vector<StepPlan>& steps, const Ru
       stt.extend_launches++;
     }
     if (e.kind == 2) stt.shade_ms += m;
+    if (e.kind == 3) stt.sort_ms += m;
     (void)hipEventDestroy(e.start);
     (void)hipEventDestroy(e.end);
   }
@@ -1105,9 +1169,11 @@ cv_status cv_batch_render(cv_batch* b, const cv_job* jobs, size_t n, uint32_t fl
 }
 
 cv_status cv_batch_autotune(cv_batch* b, const cv_job* jobs, size_t n, uint64_t max_sample_paths,
-                            cv_tuning* chosen, double* chosen_ms) {
+                            uint32_t flags, cv_tuning* chosen, double* chosen_ms) {
   if (!b || !jobs || n == 0) return fail(CV_ERR_INVALID_ARGUMENT, "null argument");
   if (n > 65536) return fail(CV_ERR_INVALID_ARGUMENT, "too many jobs");
+  if (flags & ~CV_TUNE_TOLERANCE) return fail(CV_ERR_INVALID_ARGUMENT, "unknown autotune flag");
+  const bool sweep_width = flags & CV_TUNE_TOLERANCE;
   std::lock_guard<std::mutex> g(b->mu);
   if (max_sample_paths == 0) max_sample_paths = 1ull << 22;
   // sample: the first frame of the jobs in order, up to max_sample_paths
@@ -1133,7 +1199,8 @@ cv_status cv_batch_autotune(cv_batch* b, const cv_job* jobs, size_t n, uint64_t 
     step.jobs.push_back(d);
     sample_scenes.push_back(J.scene);
     step.paths += npix;
-    wide_ok = wide_ok && J.scene->info.max_leaf <= 31 && J.scene->ntris < (1u << 26);
+    wide_ok = wide_ok && J.scene->info.max_leaf <= 31 && J.scene->ntris < (1u << 26) &&
+              (sweep_width || b->cfg.width != 2);
     max_depth = std::max(max_depth, J.scene->info.depth);
     packed = packed && J.scene->packed;
     one_scene = one_scene && J.scene == jobs[0].scene;
@@ -1182,6 +1249,7 @@ cv_status cv_batch_autotune(cv_batch* b, const cv_job* jobs, size_t n, uint64_t 
   static const uint32_t intervals[] = {1, 4, 16, 32};
   static const uint32_t leaf_batches[] = {8, 16, 24, 65};
   static const uint32_t widths[] = {2, 4, 8};
+  static const uint32_t sort_bits[] = {0, 12, 24};
   cvh::LaunchCfg best = b->cfg;
   if (!wide_ok) best.width = 2;
   double best_ms = 0;
@@ -1230,8 +1298,15 @@ cv_status cv_batch_autotune(cv_batch* b, const cv_job* jobs, size_t n, uint64_t 
       if ((cs = time_cfg(c, &ms)) != CV_OK) return cs;
       if (ms < best_ms) best = c, best_ms = ms;
     }
+    for (uint32_t v : sort_bits) {
+      cvh::LaunchCfg c = best;
+      c.sort_bits = v;
+      double ms;
+      if ((cs = time_cfg(c, &ms)) != CV_OK) return cs;
+      if (ms < best_ms) best = c, best_ms = ms;
+    }
     for (uint32_t v : widths) {
-      if (v != 2 && !wide_ok) continue;
+      if (!sweep_width || (v != 2 && !wide_ok)) continue;
       cvh::LaunchCfg c = best;
       c.width = v;
       double ms;
@@ -1816,7 +1891,7 @@ cv_status cv_trace_export(const char* path) {
   std::lock_guard<std::mutex> lk(g_trace_mu);
   FILE* f = fopen(path, "w");
   if (!f) return fail(CV_ERR_IO, std::string("cannot open ") + path);
-  static const char* names[3] = {"generate", "extend", "shade"};
+  static const char* names[4] = {"generate", "extend", "shade", "sort"};
   fputs("{\"traceEvents\":[", f);
   bool first = true;
   for (const TraceSpan& e : g_trace) {

@@ -150,8 +150,17 @@ typedef struct cv_tuning {
                                      hits equal the reference's except at exact ties and
                                      last-bit box decisions, counters those of the wide tree
                                      (needs leaves of at most 31 triangles) */
+    uint32_t ray_sort_bits;       /* rendering: before each bounce after the camera rays, the
+                                     paths are traversed in the order of a key of this many
+                                     bits (3 to 24, a multiple of 3): the direction octant and
+                                     a Morton code of the origin in the scene's bounds
+                                     ((bits - 3) / 3 bits per axis), under the job index in a
+                                     multi-job step; CV_RAY_SORT_NONE (the default): queue
+                                     order.  Only the order of the work changes: results are
+                                     the same bits */
 } cv_tuning;
 #define CV_FRAMES_AUTO 255u
+#define CV_RAY_SORT_NONE 255u
 cv_status cv_scene_set_tuning(cv_scene* scene, const cv_tuning* tuning);
 cv_status cv_scene_get_tuning(const cv_scene* scene, cv_tuning* tuning);
 /* Copies the BVH out: nodes (num_nodes) and order (num_triangles). */
@@ -244,6 +253,7 @@ typedef struct cv_render_stats {
     double total_ms;           /* device time from first to last launch */
     uint64_t sum_numi_aabb;    /* with CV_RENDER_COUNT: summed box tests */
     uint64_t sum_numi_tri;     /* with CV_RENDER_COUNT: summed triangle tests */
+    double sort_ms;            /* summed device time of the ray-sort launches */
 } cv_render_stats;
 
 #define CV_RENDER_COUNT 2u /* count box/triangle tests (slower) */
@@ -332,13 +342,18 @@ cv_status cv_batch_get_tuning(const cv_batch* b, cv_tuning* tuning);
 cv_status cv_batch_render(cv_batch* b, const cv_job* jobs, size_t n, uint32_t flags);
 /* The autotuner: times launch shapes (stack entries 8/16/24/32, 16-48
  * persistent waves per CU, refill thresholds 16-64 and whole batches, 1/2/4
- * waves per workgroup, 1/4/16/32 steps between refill checks) on a
- * sample step -- the first frame of the jobs in order, up to
- * max_sample_paths paths (0: 2^22), rendered into scratch buffers -- by
- * two rounds of coordinate descent, and keeps the fastest on the batch.
- * chosen and chosen_ms (may be NULL) receive it and its step time. */
+ * waves per workgroup, 1/4/16/32 steps between refill checks, leaf rounds,
+ * ray-sort keys of 0/12/24 bits) on a sample step -- the first frame of the
+ * jobs in order, up to max_sample_paths paths (0: 2^22), rendered into
+ * scratch buffers -- by two rounds of coordinate descent, and keeps the
+ * fastest on the batch.  Every one of these knobs leaves the results
+ * unchanged.  With CV_TUNE_TOLERANCE in flags it also sweeps BVH width 2/4/8,
+ * whose wide trees give the reference's hits except at exact ties (see
+ * cv_tuning.bvh_width); without it the batch's width is kept.  chosen and
+ * chosen_ms (may be NULL) receive the choice and its step time. */
+#define CV_TUNE_TOLERANCE 1u
 cv_status cv_batch_autotune(cv_batch* b, const cv_job* jobs, size_t n, uint64_t max_sample_paths,
-                            cv_tuning* chosen, double* chosen_ms);
+                            uint32_t flags, cv_tuning* chosen, double* chosen_ms);
 /* Counters and device times of the last cv_batch_render. */
 cv_status cv_batch_get_stats(const cv_batch* b, cv_render_stats* stats);
 void cv_batch_destroy(cv_batch* b);

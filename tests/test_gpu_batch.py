@@ -186,6 +186,8 @@ def test_batch_autotune(cv, world):
     t, ms = b.autotune(jobs, max_sample_paths=20000)
     print("autotune", t, ms, flush=True)
     assert t["stack_entries"] in (8, 16, 24, 32) and ms > 0
+    # without the tolerance flag the reference's tree is kept
+    assert t["bvh_width"] == 2 and t["ray_sort_bits"] in (cv.RAY_SORT_NONE, 12, 24)
     assert b.tuning() == t
     b.render(jobs)
     _check(jobs, expect)

@@ -129,7 +129,7 @@ def test_wide_batch_and_autotune(cv):
     b.render(jobs)
     for j, hdr in zip(jobs, want):
         assert j.hdr.tobytes() == hdr.tobytes()
-    tun, ms = b.autotune(jobs, max_sample_paths=1 << 16)
+    tun, ms = b.autotune(jobs, max_sample_paths=1 << 16, tolerance=True)
     assert tun["bvh_width"] in (2, 4, 8) and ms > 0
     jobs2 = [cv.Job(gs[si], w, h, 1, n) for (si, w, h, n) in specs]
     b.render(jobs2, cv.RENDER_GRAPH)
