@@ -168,9 +168,10 @@ class Scene:
         self.camera = Camera(self)
 
     @staticmethod
-    def _opts(emission_override):
+    def _opts(emission_override, bins=64):
         o = _native.SceneOpts()
-        o.builder = _native.CV_BUILDER_REFERENCE
+        # 64 bins: the reference's build; 32 or 16: the same build with fewer bins
+        o.builder = _native.CV_BUILDER_REFERENCE if bins == 64 else bins
         if emission_override is not None:
             o.has_emission_override = 1
             for k in range(3):
@@ -178,9 +179,9 @@ class Scene:
         return o
 
     @classmethod
-    def from_meshes(cls, meshes: Sequence[Mesh], emission_override=None) -> "Scene":
+    def from_meshes(cls, meshes: Sequence[Mesh], emission_override=None, bins=64) -> "Scene":
         arr = (ctypes.c_void_p * len(meshes))(*[m._h for m in meshes])
-        o = cls._opts(emission_override)
+        o = cls._opts(emission_override, bins)
         h = ctypes.c_void_p()
         check(lib().cv_scene_create(ctypes.cast(arr, ctypes.c_void_p), len(meshes),
                                     ctypes.byref(o), ctypes.byref(h)))
@@ -200,11 +201,12 @@ class Scene:
         return cls(h)
 
     @classmethod
-    def from_scene_data(cls, sd, with_bvh=None) -> "Scene":
-        """Scene from a convector_amd.scenes.SceneData (camera included)."""
+    def from_scene_data(cls, sd, with_bvh=None, bins=64) -> "Scene":
+        """Scene from a convector_amd.scenes.SceneData (camera included);
+        bins: centroid bins of the GPU build (64, the reference's; 32; 16)."""
         mesh = Mesh.from_triangles(sd.tris, sd.materials, sd.uv)
         if with_bvh is None:
-            s = cls.from_meshes([mesh], emission_override=sd.emission_override)
+            s = cls.from_meshes([mesh], emission_override=sd.emission_override, bins=bins)
         else:
             s = cls.from_meshes_with_bvh([mesh], with_bvh[0], with_bvh[1],
                                          emission_override=sd.emission_override)

@@ -32,6 +32,8 @@ CV_RNG_PIXEL = 0
 CV_RNG_REF_PATCH = 1
 CV_APPROX_PORTABLE = 0
 CV_BUILDER_REFERENCE = 0
+CV_BUILDER_BINS_32 = 32
+CV_BUILDER_BINS_16 = 16
 CV_NO_HIT = 0xFFFFFFFF
 
 # every function the header declares
@@ -75,7 +77,8 @@ class BvhInfo(ctypes.Structure):
     _fields_ = [("num_nodes", ctypes.c_uint32), ("num_triangles", ctypes.c_uint32),
                 ("depth", ctypes.c_uint32), ("max_leaf", ctypes.c_uint32),
                 ("num_direct", ctypes.c_uint32), ("avg_area_ratio", ctypes.c_float),
-                ("avg_tris_per_leaf", ctypes.c_float), ("build_ms", ctypes.c_float)]
+                ("avg_tris_per_leaf", ctypes.c_float), ("build_ms", ctypes.c_float),
+                ("bins", ctypes.c_uint32)]
 
 
 class Tuning(ctypes.Structure):

@@ -526,7 +526,8 @@ static cv_status scene_finish(cv_scene* s, const std::vector<float>& v9,
   } else {
     HIPTRY(s->nodes.alloc(2ull * N + 2), "alloc");
     cvk::BuildResult br{};
-    HIPTRY(cvk::build_reference_bvh(d_v9.p, N, s->nodes.p, s->tri_src.p, &br, st), "bvh build");
+    HIPTRY(cvk::build_reference_bvh(d_v9.p, N, s->nodes.p, s->tri_src.p, &br, st, s->info.bins),
+           "bvh build");
     if (br.error) {
       return fail((cv_status)br.error, br.error == CV_ERR_NAN
                                            ? "NaN in triangle coordinates"
@@ -590,7 +591,8 @@ static cv_status scene_create_impl(const cv_mesh* const* meshes, size_t n,
   *out = nullptr;
   cv_status ds = cvh::ensure_device();
   if (ds != CV_OK) return ds;
-  if (opts && opts->builder != CV_BUILDER_REFERENCE)
+  if (opts && opts->builder != CV_BUILDER_REFERENCE && opts->builder != CV_BUILDER_BINS_32 &&
+      opts->builder != CV_BUILDER_BINS_16)
     return fail(CV_ERR_INVALID_ARGUMENT, "unknown builder");
   std::vector<float> v9, uv6;
   std::vector<uint32_t> mat;
@@ -604,6 +606,7 @@ static cv_status scene_create_impl(const cv_mesh* const* meshes, size_t n,
   if (mat.size() >= (1ull << 31)) return fail(CV_ERR_INVALID_ARGUMENT, "too many triangles");
   cv_scene* s = new cv_scene();
   s->ntris = (uint32_t)mat.size();
+  s->info.bins = nodes ? 0u : (opts && opts->builder ? opts->builder : 64u);
   // Camera::new (scene.rs:33-42)
   s->cam.q[0] = 1.0f;
   s->cam.screen_distance = 1.0f / sinf(3.14159274f / 5.0f);

@@ -6,7 +6,10 @@
 // triangles; the cheapest split by the TreeSurfaceAreaHeuristic, ties to the
 // first axis and the first position; no split when n*120 is cheaper; the
 // child with the smaller surface area first.  crystallize's depth-first
-// layout of sibling pairs is reproduced exactly.
+// layout of sibling pairs is reproduced exactly.  The build also runs with
+// 16 or 32 bins (bins.len() changed: at most bins/2 triangles or more than
+// n/8 of n > bins in one bin take the sorted bins, in chunks of
+// ceil(n/bins)); below, "32" stands for bins/2 and "64" for bins.
 //
 // Level-synchronous:
 //   k_decide (one workgroup per node of more than 32 triangles) bins the
@@ -93,6 +96,8 @@ struct Args {
   const float* tpm1;  // powf(0.8f, log2f(n) - 1.0f)
   uint32_t mode;
   uint32_t ntris;
+  uint32_t nbins;      // centroid bins per axis: 64 (the reference), 32 or 16
+  uint32_t small_max;  // nodes of at most nbins / 2 triangles go to k_small
   // chunked wide nodes: per chunk the node, the bins (kBinWords), the first
   // output position per bin of the split axis (64) and the children's
   // bounds (2 sides x 12 okeys)
@@ -139,10 +144,12 @@ __device__ __forceinline__ float tree_sah(const Args& a, float parent_area, cons
   return aabb_term * 40.0f + tri_term * 120.0f;
 }
 
-// centroid bin of bin_triangles (bvh.rs:149-156); NaN and 64 clamp to 63.
-__device__ __forceinline__ uint32_t centroid_bin(float c, float mn, float size) {
-  float f = floorf((64.0f * (c - mn)) / size);
-  return (f >= 0.0f && f < 64.0f) ? (uint32_t)f : 63u;
+// centroid bin of bin_triangles (bvh.rs:149-156) with nb bins; NaN and nb
+// clamp to nb - 1.
+__device__ __forceinline__ uint32_t centroid_bin(float c, float mn, float size, uint32_t nb) {
+  const float fb = (float)nb;
+  float f = floorf((fb * (c - mn)) / size);
+  return (f >= 0.0f && f < fb) ? (uint32_t)f : nb - 1u;
 }
 
 __device__ __forceinline__ uint32_t load_coherent(const uint32_t* p) {
@@ -237,7 +244,7 @@ __device__ __forceinline__ void bin_centroids(const Args& a, const uint32_t* ref
     uint32_t kmx[3] = {okey(B.x), okey(B.y), okey(B.z)};
     for (uint32_t axis = 0; axis < 3; axis++) {
       if ((umask >> axis) & 1u) continue;
-      uint32_t b = centroid_bin(c[axis], cmn[axis], csz[axis]);
+      uint32_t b = centroid_bin(c[axis], cmn[axis], csz[axis], a.nbins);
       atomicAdd(&cnt[axis][b], 1u);
       for (int j = 0; j < 3; j++) {
         atomicMin(&bx[axis][b][j], kmn[j]);
@@ -372,9 +379,9 @@ __global__ __launch_bounds__(BS) void k_decide(Args a) {
     bin_centroids<BS>(a, refs, n, cmn, csz, umask, cnt, bx);
   }
   if (umask) {
-    // bin_triangles_uniform (bvh.rs:176-201): chunks of ceil(n/64) of the
+    // bin_triangles_uniform (bvh.rs:176-201): chunks of ceil(n/bins) of the
     // stably sorted references.
-    const uint32_t tpb = (n + 63) / 64;
+    const uint32_t tpb = (n + a.nbins - 1) / a.nbins;
     for (uint32_t axis = 0; axis < 3; axis++) {
       if (!((umask >> axis) & 1u)) continue;
       const uint32_t* srt = a.sorted[axis] + begin;
@@ -394,13 +401,13 @@ __global__ __launch_bounds__(BS) void k_decide(Args a) {
   }
   __syncthreads();
   if (a.mode == 0 && tid < 64) {
-    // fallback test (bvh.rs:159-169): some bin above n/8 with n > 64
+    // fallback test (bvh.rs:159-169): some bin above n/8 with n > bins
     uint32_t lane = tid;
     uint32_t fb = 0;
     for (uint32_t axis = 0; axis < 3; axis++) {
       uint32_t c = cnt[axis][lane];
       uint64_t over = __ballot(c > n / 8);
-      if (over && n > 64) fb |= 1u << axis;
+      if (over && n > a.nbins) fb |= 1u << axis;
     }
     if (fb) {
       if (lane == 0) {
@@ -681,7 +688,9 @@ __global__ __launch_bounds__(kBlock) void k_part_chunks(Args a) {
   if (tid < 24) (&cb[0][0])[tid] = ((tid % 12) % 6) < 3 ? 0xffffffffu : 0u;
   __syncthreads();
   const float mn = nd->cbox[axis], size = nd->cbox[3 + axis] - mn;
-  auto bin_of = [&](uint32_t r) { return centroid_bin(centroid(a, r, axis), mn, size); };
+  auto bin_of = [&](uint32_t r) {
+    return centroid_bin(centroid(a, r, axis), mn, size, a.nbins);
+  };
   if (uniform) {
     const uint32_t* srt = a.sorted[axis] + begin;
     for (uint32_t s = lo + tid; s < hi; s += kBlock) dst[s] = srt[s];
@@ -715,7 +724,9 @@ __global__ __launch_bounds__(BS) void k_partition(Args a) {
   if (tid < 24) (&cb[0][0])[tid] = ((tid % 12) % 6) < 3 ? 0xffffffffu : 0u;
   __syncthreads();
   const float mn = nd->cbox[axis], size = nd->cbox[3 + axis] - mn;
-  auto bin_of = [&](uint32_t r) { return centroid_bin(centroid(a, r, axis), mn, size); };
+  auto bin_of = [&](uint32_t r) {
+    return centroid_bin(centroid(a, r, axis), mn, size, a.nbins);
+  };
   if (nd->nchunks) {
     // a wide node: k_part_chunks reordered it; its chunks' bounds are added
     // up here
@@ -777,7 +788,7 @@ __global__ __launch_bounds__(BS) void k_partition(Args a) {
       uint32_t cid = c + s;
       if (ch[s].count == 1) ch[s].flags |= kLeaf;
       a.nodes[cid] = ch[s];
-      if (ch[s].count > 32) {
+      if (ch[s].count > a.small_max) {
         uint32_t slot = atomicAdd(&a.ctx->next_large, 1u);
         a.next_list[slot] = cid;
         if (ch[s].count > kWide) atomicAdd(&a.ctx->next_wide, 1u);
@@ -1109,9 +1120,11 @@ __global__ __launch_bounds__(256) void k_layout(Args a, uint32_t ninodes, DevNod
   } while (0)
 
 hipError_t build_reference_bvh(const float* d_v9, uint32_t ntris, DevNode* nodes_out,
-                               uint32_t* order_out, BuildResult* res, hipStream_t s) {
+                               uint32_t* order_out, BuildResult* res, hipStream_t s,
+                               uint32_t nbins) {
   using namespace bvh;
   res->error = 0;
+  if (nbins != 16 && nbins != 32 && nbins != 64) return hipErrorInvalidValue;
   std::vector<void*> allocs;
   auto cleanup = [&]() {
     for (void* p : allocs) (void)hipFree(p);
@@ -1125,6 +1138,8 @@ hipError_t build_reference_bvh(const float* d_v9, uint32_t ntris, DevNode* nodes
   const size_t N = ntris;
   Args a{};
   a.ntris = ntris;
+  a.nbins = nbins;
+  a.small_max = nbins / 2;
   float4 *A, *B;
   float* C;
   BVH_CHECK(alloc((void**)&A, N * sizeof(float4)));
@@ -1149,7 +1164,7 @@ hipError_t build_reference_bvh(const float* d_v9, uint32_t ntris, DevNode* nodes
   Ctx* ctx;
   BVH_CHECK(alloc((void**)&ctx, sizeof(Ctx)));
   a.ctx = ctx;
-  const size_t list_cap = N / 33 + 2;
+  const size_t list_cap = N / (a.small_max + 1) + 2;
   uint32_t *listA, *listB, *small;
   BVH_CHECK(alloc((void**)&listA, list_cap * 4));
   BVH_CHECK(alloc((void**)&listB, list_cap * 4));
@@ -1200,7 +1215,7 @@ hipError_t build_reference_bvh(const float* d_v9, uint32_t ntris, DevNode* nodes
   }
   uint32_t zero = 0, one = 1;
   uint32_t nlist = 0;
-  if (ntris > 32) {
+  if (ntris > a.small_max) {
     BVH_CHECK(hipMemcpyAsync(listA, &zero, 4, hipMemcpyHostToDevice, s));
     nlist = 1;
   } else if (ntris >= 2) {

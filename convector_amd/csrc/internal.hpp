@@ -213,10 +213,13 @@ struct BuildResult {
 };
 
 // Builds the reference BVH (bvh.rs:91-545) of ntris triangles (device v9,
-// source order) on the GPU.  nodes_out (device, 2*ntris DevNode) receives
-// the depth-first pair layout of crystallize (bvh.rs:389-428), order_out
-// (device, ntris) the BVH-position -> source-index permutation.
+// source order) on the GPU, with nbins centroid bins per axis: 64 is the
+// reference's build, 16 and 32 the same steps with bins.len() changed.
+// nodes_out (device, 2*ntris DevNode) receives the depth-first pair layout
+// of crystallize (bvh.rs:389-428), order_out (device, ntris) the
+// BVH-position -> source-index permutation.
 hipError_t build_reference_bvh(const float* d_v9, uint32_t ntris, DevNode* nodes_out,
-                               uint32_t* order_out, BuildResult* res, hipStream_t s);
+                               uint32_t* order_out, BuildResult* res, hipStream_t s,
+                               uint32_t nbins = 64);
 
 }  // namespace cvk

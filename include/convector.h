@@ -90,9 +90,16 @@ typedef struct cv_bvh_node {
 } cv_bvh_node;
 
 #define CV_BUILDER_REFERENCE 0u /* 64 bins, TreeSAH, sorted fallback: bvh.rs:91-545 */
+/* The same build with 32 or 16 centroid bins per axis (bvh.rs:275 with
+ * bins.len() changed: sorted bins for at most bins/2 triangles or more than
+ * n/8 of n > bins in one bin, chunks of ceil(n/bins)).  A different tree:
+ * nearest hits equal the reference's except at exact ties in t; the
+ * traversal counters are this tree's. */
+#define CV_BUILDER_BINS_32 32u
+#define CV_BUILDER_BINS_16 16u
 
 typedef struct cv_scene_opts {
-    uint32_t builder;               /* CV_BUILDER_REFERENCE */
+    uint32_t builder;               /* CV_BUILDER_REFERENCE, CV_BUILDER_BINS_32 or _16 */
     uint32_t has_emission_override; /* 0: emissive hits give sky_intensity(dir) */
     float emission_override[3];     /* radiance of emissive triangle hits when set */
 } cv_scene_opts;
@@ -106,6 +113,8 @@ typedef struct cv_bvh_info {
     float avg_area_ratio;     /* bvh.rs:536-544 */
     float avg_tris_per_leaf;
     float build_ms;           /* device time of the GPU build */
+    uint32_t bins;            /* centroid bins of the build (64: the reference's; 0: a
+                                 tree given by the caller) */
 } cv_bvh_info;
 
 /* Scene::from_meshes + Bvh::from_meshes + Bvh::build (scene.rs:114-129,

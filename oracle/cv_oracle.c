@@ -594,6 +594,7 @@ typedef struct {
     uint32_t* perm;
     float* keys;
     int error;
+    uint32_t nbins; /* bins.len() of InterimNode::split (bvh.rs:275): 64, or 16 / 32 */
 } builder;
 
 static int32_t new_inode(builder* B, const tref* refs, uint32_t begin, uint32_t count) {
@@ -651,14 +652,17 @@ typedef struct {
     box6 box;
 } binacc;
 
-/* Bins one axis.  Returns 1 when uniform (sorted) binning was used; fills
- * bins[64] and, for every position of the segment, its bin (bin_of) and for
- * uniform binning the sorted permutation (perm). */
+/* Bins one axis into B->nbins bins (the reference's 64, or 16 / 32: the
+ * same steps with bins.len() changed).  Returns 1 when uniform (sorted)
+ * binning was used; fills bins[0, nbins) and, for every position of the
+ * segment, its bin (bin_of) and for uniform binning the sorted permutation
+ * (perm).  bins[nbins, 64) stay empty. */
 static int bin_axis(builder* B, const inode* nd, int axis, binacc bins[64], uint32_t* bin_of,
                     uint32_t* perm, uint32_t* perm_tmp) {
     const uint32_t n = nd->count;
+    const uint32_t nb = B->nbins;
     const tref* R = B->refs + nd->begin;
-    int uniform = n <= 32; /* bvh.rs:145-147: len <= bins/2 */
+    int uniform = n <= nb / 2; /* bvh.rs:145-147: len <= bins/2 */
     if (!uniform) {
         /* bin_triangles (bvh.rs:142-171) */
         float mn = nd->inner.mn[axis];
@@ -666,16 +670,16 @@ static int bin_axis(builder* B, const inode* nd, int axis, binacc bins[64], uint
         uint32_t counts[64] = {0};
         for (uint32_t k = 0; k < n; k++) {
             float coord = R[k].c[axis];
-            float f = floorf((64.0f * (coord - mn)) / size);
+            float f = floorf(((float)nb * (coord - mn)) / size);
             /* `as usize`: NaN becomes out of range on x86-64, then clamped */
-            uint32_t idx = (f >= 0.0f && f < 64.0f) ? (uint32_t)f : 63u;
-            if (f != f) idx = 63u;
+            uint32_t idx = (f >= 0.0f && f < (float)nb) ? (uint32_t)f : nb - 1u;
+            if (f != f) idx = nb - 1u;
             bin_of[k] = idx;
             counts[idx]++;
         }
         uint32_t mx = 0;
-        for (int b = 0; b < 64; b++) mx = counts[b] > mx ? counts[b] : mx;
-        if (mx > n / 8 && n > 64) uniform = 1; /* bvh.rs:159-169 */
+        for (uint32_t b = 0; b < nb; b++) mx = counts[b] > mx ? counts[b] : mx;
+        if (mx > n / 8 && n > nb) uniform = 1; /* bvh.rs:159-169 */
     }
     for (int b = 0; b < 64; b++) {
         bins[b].count = 0;
@@ -684,7 +688,7 @@ static int bin_axis(builder* B, const inode* nd, int axis, binacc bins[64], uint
     if (uniform) {
         for (uint32_t k = 0; k < n; k++) B->keys[k] = R[k].c[axis];
         stable_sort_by_key(perm, B->keys, n, perm_tmp);
-        uint32_t tpb = (n + 63) / 64;
+        uint32_t tpb = (n + nb - 1) / nb;
         if (tpb == 0) tpb = 1;
         for (uint32_t s = 0; s < n; s++) {
             uint32_t b = s / tpb;
@@ -867,9 +871,18 @@ static void crystallize(builder* B, int32_t ni, or_node* nodes, uint32_t* nn, ui
 
 int or_bvh_build(const float* v, uint32_t ntris, or_node* nodes_out, uint32_t* nnodes_out,
                  uint32_t* order_out, float* avg_area_ratio, float* avg_tris_per_leaf) {
+    return or_bvh_build_bins(v, ntris, 64, nodes_out, nnodes_out, order_out, avg_area_ratio,
+                             avg_tris_per_leaf);
+}
+
+int or_bvh_build_bins(const float* v, uint32_t ntris, uint32_t nbins, or_node* nodes_out,
+                      uint32_t* nnodes_out, uint32_t* order_out, float* avg_area_ratio,
+                      float* avg_tris_per_leaf) {
     if (ntris == 0) return -1;
+    if (nbins != 16 && nbins != 32 && nbins != 64) return -4;
     builder B;
     memset(&B, 0, sizeof(B));
+    B.nbins = nbins;
     B.refs = (tref*)malloc(sizeof(tref) * ntris);
     B.tmp = (tref*)malloc(sizeof(tref) * ntris);
     B.bin_of = (uint32_t*)malloc(sizeof(uint32_t) * ntris);

@@ -99,6 +99,13 @@ void or_triangle_intersect_direct(const float tri[9], const float ray[6], int ap
  * -2 no valid split axis in a node (all centroids coincide), -3 NaN input. */
 int or_bvh_build(const float* v, uint32_t ntris, or_node* nodes_out, uint32_t* nnodes_out,
                  uint32_t* order_out, float* avg_area_ratio, float* avg_tris_per_leaf);
+/* The same build with nbins (16, 32 or 64) centroid bins per axis in place
+ * of the reference's 64 (bvh.rs:275 with bins.len() changed: the uniform
+ * threshold bins/2, the fallback test n > bins and the chunk size
+ * ceil(n/bins) follow it).  -4 for another nbins. */
+int or_bvh_build_bins(const float* v, uint32_t ntris, uint32_t nbins, or_node* nodes_out,
+                      uint32_t* nnodes_out, uint32_t* order_out, float* avg_area_ratio,
+                      float* avg_tris_per_leaf);
 /* TreeSurfaceAreaHeuristic::aabb_cost (bvh.rs:452-469) */
 float or_tree_sah_cost(const float parent_box[6], const float box[6], uint32_t num_tris);
 

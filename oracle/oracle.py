@@ -76,6 +76,7 @@ def _declare(L):
         "or_triangle_intersect": (I32, [P, P, F32, P]),
         "or_triangle_intersect_direct": (None, [P, P, I32, P]),
         "or_bvh_build": (I32, [P, U32, P, P, P, P, P]),
+        "or_bvh_build_bins": (I32, [P, U32, U32, P, P, P, P, P]),
         "or_tree_sah_cost": (F32, [P, P, U32]),
         "or_scene_create": (P, [P, P, P, U32, P, U32, P, I32, P]),
         "or_scene_destroy": (None, [P]),
@@ -129,8 +130,9 @@ def u32(a):
     return np.ascontiguousarray(a, dtype=np.uint32)
 
 
-def bvh_build(tris: np.ndarray):
-    """Reference BVH (bvh.rs:487-545).  Returns (status, nodes, order, stats)."""
+def bvh_build(tris: np.ndarray, bins: int = 64):
+    """Reference BVH (bvh.rs:487-545); bins 16 or 32: the same build with
+    that many centroid bins.  Returns (status, nodes, order, stats)."""
     tris = f32(tris)
     n = tris.shape[0]
     nodes = np.zeros(max(2 * n, 2), dtype=NODE_DTYPE)
@@ -138,7 +140,8 @@ def bvh_build(tris: np.ndarray):
     nn = np.zeros(1, dtype=np.uint32)
     ar = np.zeros(1, dtype=np.float32)
     tl = np.zeros(1, dtype=np.float32)
-    st = lib().or_bvh_build(ptr(tris), n, ptr(nodes), ptr(nn), ptr(order), ptr(ar), ptr(tl))
+    st = lib().or_bvh_build_bins(ptr(tris), n, bins, ptr(nodes), ptr(nn), ptr(order), ptr(ar),
+                                 ptr(tl))
     return st, nodes[: int(nn[0])].copy(), order[:n].copy(), (float(ar[0]), float(tl[0]))
 
 
@@ -162,7 +165,11 @@ class Scene:
         self.ntris = self.tris.shape[0]
 
     @classmethod
-    def from_scene_data(cls, sd, approx=APPROX_PORTABLE, nodes=None, order=None):
+    def from_scene_data(cls, sd, approx=APPROX_PORTABLE, nodes=None, order=None, bins=64):
+        if nodes is None and bins != 64:
+            st, nodes, order, _ = bvh_build(sd.tris, bins)
+            if st != 0:
+                raise RuntimeError("oracle %d-bin build failed with status %d" % (bins, st))
         s = cls(sd.tris, sd.materials, sd.uv, nodes=nodes, order=order, approx=approx)
         s.set_camera(*sd.camera_args())
         if sd.emission_override is not None:

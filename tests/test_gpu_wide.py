@@ -9,40 +9,9 @@ import numpy as np
 import pytest
 
 from convector_amd import scenes
-from helpers import bits, model_scene
+from helpers import check_tolerance, model_scene
 
 pytestmark = pytest.mark.gpu
-
-
-def _edge_distance64(tris, tri_id, rays):
-    v = tris[tri_id].astype(np.float64).reshape(-1, 3, 3)
-    o, d = rays[:, :3].astype(np.float64), rays[:, 3:].astype(np.float64)
-    n = np.cross(v[:, 1] - v[:, 0], v[:, 2] - v[:, 0])
-    t = ((v[:, 0] - o) * n).sum(1) / (d * n).sum(1)
-    p = o + t[:, None] * d
-    dist = np.full(len(p), np.inf)
-    for i, j in ((0, 1), (1, 2), (2, 0)):
-        e = v[:, j] - v[:, i]
-        dist = np.minimum(dist, np.linalg.norm(np.cross(e, p - v[:, i]), axis=1)
-                          / np.linalg.norm(e, axis=1))
-    return dist
-
-
-def _check_tolerance(sd, hits, ref, rays):
-    gid, oid = hits["tri_id"], ref["tri"]
-    mism = np.nonzero(gid != oid)[0]
-    assert len(mism) <= max(1, len(rays) // 10000), len(mism)
-    for k in mism:
-        tg, to = hits["t"][k], ref["tuv"][k, 0]
-        if tg == to:
-            continue  # an exact tie: two triangles at the same distance
-        near = [float(_edge_distance64(sd.tris, np.array([t]), rays[k:k + 1])[0])
-                for t in (gid[k], oid[k]) if t != 0xFFFFFFFF]
-        assert near and min(near) <= 1e-6, (k, near)
-    same = gid == oid
-    assert (bits(hits["t"][same]) == bits(ref["tuv"][same, 0])).all()
-    assert (bits(hits["u"][same]) == bits(ref["tuv"][same, 1])).all()
-    assert (bits(hits["v"][same]) == bits(ref["tuv"][same, 2])).all()
 
 
 @pytest.mark.parametrize("width", [4, 8])
@@ -74,7 +43,7 @@ def test_wide_hits_against_oracle(cv, oracle, models_dir, case, width):
     assert g.tuning()["bvh_width"] == width
     hits = g.intersect_nearest(rays)
     ref = o.intersect(rays)
-    _check_tolerance(sd, hits, ref, rays)
+    check_tolerance(sd, hits, ref, rays)
     assert (hits["tri_id"] != cv.NO_HIT).mean() > 0.2
     # the isect path keeps the reference tree (exact, with the reference's counters)
     full, isects = g.intersect_nearest(rays[:4096], full=True)
