@@ -220,10 +220,11 @@ def run_batch(args):
     log("batch: %d scenes, %d triangles generated in %.1f s"
         % (len(sds), sum(sd.ntris for sd in sds), time.perf_counter() - t0))
     t0 = time.perf_counter()
-    gs = [cv.Scene.from_scene_data(sd) for sd in sds]
+    # the configuration's builder: binned SAH with args.batch_bins bins
+    gs = [cv.Scene.from_scene_data(sd, bins=args.batch_bins) for sd in sds]
     build_ms = sum(g.bvh_info()["build_ms"] for g in gs)
-    log("batch: GPU BVH builds %.1f ms device (%.1f s with uploads)"
-        % (build_ms, time.perf_counter() - t0))
+    log("batch: GPU BVH builds (%d bins) %.1f ms device (%.1f s with uploads)"
+        % (args.batch_bins, build_ms, time.perf_counter() - t0))
     npix = [w * h for (_, _, w, h, _) in specs]
     offs = np.concatenate([[0], np.cumsum(npix)]).astype(np.int64)
     hdr = DeviceBuffer(int(offs[-1]) * 12)
@@ -245,21 +246,30 @@ def run_batch(args):
     cv.device_synchronize()
     wall = (time.perf_counter() - t0) / args.steps
     st = b.stats()
-    # BVH widths 4 and 8 are a tolerance mode (hits as the reference's except
-    # at exact ties); the same batch at the reference's width 2, bit for bit
-    # the reference's images, with the tuned launch shape otherwise
+    # Trees of fewer than 64 bins and BVH widths 4 and 8 are tolerance modes
+    # (hits as the reference's except at exact ties); the same batch on the
+    # reference's 64-bin trees at width 2, bit for bit the reference's
+    # images, with the tuned launch shape otherwise
     exact_wall = wall
-    if tuning["bvh_width"] != 2:
+    exact_build_ms = build_ms
+    gs_exact, jobs_exact = gs, jobs
+    if args.batch_bins != 64:
+        gs_exact = [cv.Scene.from_scene_data(sd) for sd in sds]
+        exact_build_ms = sum(g.bvh_info()["build_ms"] for g in gs_exact)
+        jobs_exact = [cv.Job(g, w, h, 1, spp, hdr=hdr.ptr.value + int(offs[i]) * 12)
+                      for i, (g, (_, _, w, h, spp)) in enumerate(zip(gs_exact, specs))]
+    if tuning["bvh_width"] != 2 or args.batch_bins != 64:
         b.set_tuning(bvh_width=2)
-        b.render(jobs, flags)
+        b.render(jobs_exact, flags)
         cv.device_synchronize()
         t0 = time.perf_counter()
         for k in range(args.steps):
-            b.render(jobs, flags)
+            b.render(jobs_exact, flags)
         cv.device_synchronize()
         exact_wall = (time.perf_counter() - t0) / args.steps
-        log("batch at width 2: %.1f ms" % (exact_wall * 1e3))
+        log("batch on 64-bin trees at width 2: %.1f ms" % (exact_wall * 1e3))
         b.set_tuning(bvh_width=tuning["bvh_width"])
+    del jobs_exact, gs_exact
     # the same jobs one after the other, one renderer per job
     rs = [cv.Renderer(g, w, h) for g, (_, _, w, h, _) in zip(gs, specs)]
     for r, j in zip(rs, jobs):  # warm-up
@@ -297,14 +307,16 @@ def run_batch(args):
                                "the other"},
         "speedup_vs_sequential": round(seq / wall, 3),
         "bvh_width": tuning["bvh_width"],
-        "exact": {"bvh_width": 2, "paths_per_s": round(paths / exact_wall, 1),
+        "exact": {"bvh_width": 2, "bins": 64, "paths_per_s": round(paths / exact_wall, 1),
                   "ms": round(exact_wall * 1e3, 3),
                   "speedup_vs_sequential": round(seq / exact_wall, 3),
-                  "note": "the reference's tree and visiting order: every job's image bit "
-                          "for bit the oracle's; widths 4 and 8 give hits equal to the "
-                          "reference's except at exact ties in t (99.96 % of pixels "
-                          "bit-identical in tests/test_gpu_wide.py)"},
+                  "bvh_build_ms_total": round(exact_build_ms, 1),
+                  "note": "the reference's 64-bin trees and visiting order: every job's "
+                          "image bit for bit the oracle's; other bin counts and widths 4 "
+                          "and 8 give hits equal to the reference's except at exact ties in "
+                          "t (tests/test_gpu_wide.py, tests/test_gpu_bins.py)"},
         "autotune": {"chosen": tuning, "sample_step_ms": round(tune_ms, 3)},
+        "bins": args.batch_bins,
         "bvh_build_ms_total": round(build_ms, 1),
     }
     if not args.no_cpu:
@@ -993,6 +1005,9 @@ def main():
                     help="frame workload: leave out the nested 64-job batch result")
     ap.add_argument("--no-wide", action="store_true",
                     help="frame workload: leave out the width-4 (tolerance mode) step")
+    ap.add_argument("--batch-bins", type=int, default=32, choices=(16, 32, 64),
+                    help="batch workload: centroid bins of the GPU BVH builds (64: the "
+                         "reference's)")
     ap.add_argument("--batch-steps", type=int, default=3,
                     help="frame workload: timed renders of the nested 64-job batch")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
