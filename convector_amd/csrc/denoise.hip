@@ -396,6 +396,7 @@ struct WGradArgs {
   uint32_t npix, h, w_;
   uint32_t pix_per_block;
   float* part;         // k_wgrad_t: per pixel split [z][cout * 9 * cin + cout]
+  uint32_t tiles_per_split;  // k_wgrad_halo: 8 x 32 pixel tiles per split
 };
 
 // 8 pixels x 8 bf16 channels (in[i] = pixel i) to 8 channels x 8 pixels
@@ -578,6 +579,173 @@ __global__ __launch_bounds__(256) void k_wgrad_t(WGradArgs a) {
     }
   }
 }
+
+// ---------------------------------------------------------------------------
+// Backward to the weights from transposed LDS tiles, for images whose rows
+// are a multiple of 32 pixels wide: dW[co][tap][ci] = sum_p dZ[p][co] *
+// X[p + offset(tap)][ci] as MFMAs with K = 32 pixels of one row.  A block
+// takes NS output tiles and CT input tiles of 16 channels (one wave per
+// pair) and a split of the 8 x 32 pixel tiles.  Per tile it stages dZ
+// (8 rows x 32 pixels) and X with its halo (10 rows x 34 pixels, zeros
+// outside the image) channel-major into LDS -- pixel pairs transposed into
+// 32-bit words -- while the next tile's loads are in flight.  A wave keeps
+// its 8 dZ fragments (one per row) in registers and reads each halo row of
+// its input channels once: three aligned 8-pixel windows per lane, from
+// which funnel shifts make the windows one pixel left and right; every
+// such row feeds the 3 output rows and 3 column offsets it belongs to, 72
+// MFMAs per tile.  The bias gradient is an MFMA of dZ against ones.  Each
+// split stores its partial tile, which k_wgrad_reduce adds in split order.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kWhCols = 48;                    // X^T row: pixels x0 - 8 .. x0 + 39
+constexpr uint32_t kWhCi = 10 * kWhCols + 8;        // X^T stride per input channel (bf16)
+constexpr uint32_t kWhCo = 8 * 32 + 8;              // dZ^T stride per output channel
+
+template <int NS, int CT>
+__global__ __launch_bounds__(64 * NS * CT) void k_wgrad_halo(WGradArgs a) {
+  constexpr int BS = 64 * NS * CT;
+  constexpr int CC = 16 * CT, CO = 16 * NS;
+  constexpr int XG = CC / 8, ZG = CO / 8;  // 8-channel groups
+  constexpr int XP = 10 * 18 * XG;         // X pixel-pair pieces per tile
+  constexpr int ZP = 8 * 16 * ZG;          // dZ pixel-pair pieces per tile
+  constexpr int XPT = (XP + BS - 1) / BS, ZPT = (ZP + BS - 1) / BS;
+  __shared__ __attribute__((aligned(16))) uint16_t xs[CC * kWhCi];
+  __shared__ __attribute__((aligned(16))) uint16_t zs[CO * kWhCo];
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const uint32_t ct = wave / CT, it = wave % CT;
+  const uint32_t ln = lane & 15, g = lane >> 4;
+  const uint32_t ci0 = blockIdx.x * CC, co0 = blockIdx.y * CO;
+  const uint32_t ncols = 9 * a.cin;
+  const uint32_t ntx = a.w_ / 32, nty = (a.h + 7) / 8;
+  const uint32_t ntiles = (a.npix / (a.h * a.w_)) * ntx * nty;
+  const uint32_t tb = blockIdx.z * a.tiles_per_split;
+  const uint32_t te = min(ntiles, tb + a.tiles_per_split);
+  u32x4 xr[XPT][2], zr[ZPT][2];
+  // the tile's pieces into registers: a piece is 8 channels of two
+  // neighbouring pixels (X: columns 6 + 2 k of the halo row, i.e. pixels
+  // x0 - 2 + 2 k; dZ: pixels x0 + 2 k)
+  auto load_tile = [&](uint32_t t) {
+    const uint32_t tx = t % ntx, ty = (t / ntx) % nty, img = t / (ntx * nty);
+    const int x0 = (int)(tx * 32), y0 = (int)(ty * 8);
+    const size_t base = (size_t)img * a.h * a.w_;
+#pragma unroll
+    for (int i = 0; i < XPT; i++) {
+      const int q = (int)tid + BS * i;
+      const int g8 = q % XG, pr = (q / XG) % 18, row = q / XG / 18;
+      const int sy = y0 + row - 1;
+#pragma unroll
+      for (int e = 0; e < 2; e++) {
+        const int px = x0 - 2 + 2 * pr + e;
+        xr[i][e] = (q < XP && sy >= 0 && sy < (int)a.h && px >= 0 && px < (int)a.w_)
+                       ? *reinterpret_cast<const u32x4*>(
+                             a.x + (base + (size_t)sy * a.w_ + (uint32_t)px) * a.xcs + ci0 + 8 * g8)
+                       : u32x4{0u, 0u, 0u, 0u};
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < ZPT; i++) {
+      const int q = (int)tid + BS * i;
+      const int g8 = q % ZG, pr = (q / ZG) % 16, row = q / ZG / 16;
+      const int sy = y0 + row;
+#pragma unroll
+      for (int e = 0; e < 2; e++) {
+        const int px = x0 + 2 * pr + e;
+        zr[i][e] = (q < ZP && sy < (int)a.h)
+                       ? *reinterpret_cast<const u32x4*>(
+                             a.dz + (base + (size_t)sy * a.w_ + (uint32_t)px) * a.zcs + co0 + 8 * g8)
+                       : u32x4{0u, 0u, 0u, 0u};
+      }
+    }
+  };
+  // transposed into LDS: word (channel c, pixel pair) = c of the left pixel
+  // | c of the right pixel << 16
+  auto store_tile = [&]() {
+#pragma unroll
+    for (int i = 0; i < XPT; i++) {
+      const int q = (int)tid + BS * i;
+      if (q >= XP) continue;
+      const int g8 = q % XG, pr = (q / XG) % 18, row = q / XG / 18;
+      uint32_t* d = reinterpret_cast<uint32_t*>(xs + (8 * g8) * kWhCi + row * kWhCols + 6 + 2 * pr);
+#pragma unroll
+      for (int e = 0; e < 8; e++) {
+        const uint32_t lo = xr[i][0][e / 2], hi = xr[i][1][e / 2];
+        d[e * (kWhCi / 2)] = (e & 1) ? (lo >> 16) | (hi & 0xffff0000u) : (lo & 0xffffu) | (hi << 16);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < ZPT; i++) {
+      const int q = (int)tid + BS * i;
+      if (q >= ZP) continue;
+      const int g8 = q % ZG, pr = (q / ZG) % 16, row = q / ZG / 16;
+      uint32_t* d = reinterpret_cast<uint32_t*>(zs + (8 * g8) * kWhCo + row * 32 + 2 * pr);
+#pragma unroll
+      for (int e = 0; e < 8; e++) {
+        const uint32_t lo = zr[i][0][e / 2], hi = zr[i][1][e / 2];
+        d[e * (kWhCo / 2)] = (e & 1) ? (lo >> 16) | (hi & 0xffff0000u) : (lo & 0xffffu) | (hi << 16);
+      }
+    }
+  };
+  f4 acc[9], accb = f4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int k = 0; k < 9; k++) acc[k] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+  bf8 ones;
+#pragma unroll
+  for (int k = 0; k < 8; k++) ones[k] = (__bf16)1.0f;
+  if (tb < te) load_tile(tb);
+#pragma unroll 1
+  for (uint32_t t = tb; t < te; t++) {
+    store_tile();
+    __syncthreads();
+    if (t + 1 < te) load_tile(t + 1);
+    bf8 A[8];
+#pragma unroll
+    for (int r = 0; r < 8; r++)
+      A[r] = *reinterpret_cast<const bf8*>(&zs[(ct * 16 + ln) * kWhCo + r * 32 + 8 * g]);
+    if (it == 0) {
+#pragma unroll
+      for (int r = 0; r < 8; r++) accb = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[r], ones, accb, 0, 0, 0);
+    }
+    const uint16_t* xrow0 = &xs[(it * 16 + ln) * kWhCi + 8 * g];
+#pragma unroll
+    for (int hr = 0; hr < 10; hr++) {
+      // windows of pixels x0 + 8 g - 8, x0 + 8 g, x0 + 8 g + 8 (8 each)
+      const u32x4 w0 = *reinterpret_cast<const u32x4*>(xrow0 + hr * kWhCols);
+      const u32x4 w1 = *reinterpret_cast<const u32x4*>(xrow0 + hr * kWhCols + 8);
+      const u32x4 w2 = *reinterpret_cast<const u32x4*>(xrow0 + hr * kWhCols + 16);
+      // one pixel left (dx = -1) and right (dx = +1) of the centre window
+      const uint32_t s0 = __builtin_amdgcn_alignbit(w1[0], w0[3], 16);
+      const uint32_t s1 = __builtin_amdgcn_alignbit(w1[1], w1[0], 16);
+      const uint32_t s2 = __builtin_amdgcn_alignbit(w1[2], w1[1], 16);
+      const uint32_t s3 = __builtin_amdgcn_alignbit(w1[3], w1[2], 16);
+      const uint32_t s4 = __builtin_amdgcn_alignbit(w2[0], w1[3], 16);
+      const u32x4 fl = {s0, s1, s2, s3}, fr = {s1, s2, s3, s4};
+      const bf8 F[3] = {__builtin_bit_cast(bf8, fl), __builtin_bit_cast(bf8, w1),
+                        __builtin_bit_cast(bf8, fr)};
+#pragma unroll
+      for (int dyi = 0; dyi < 3; dyi++) {
+        // halo row hr is row hr - dyi of the tile for dy = dyi - 1
+        const int r = hr - dyi;
+        if (r < 0 || r >= 8) continue;
+#pragma unroll
+        for (int dxi = 0; dxi < 3; dxi++)
+          acc[dyi * 3 + dxi] =
+              __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[r], F[dxi], acc[dyi * 3 + dxi], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+  }
+  // the split's partial: lane holds output channels 4 g + r of input channel ln
+  float* part = a.part + (size_t)blockIdx.z * (a.cout * ncols + a.cout);
+  const uint32_t ci = ci0 + it * 16 + ln;
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const uint32_t co = co0 + ct * 16 + 4 * g + r;
+    if (co >= a.cout) continue;
+#pragma unroll
+    for (int tap = 0; tap < 9; tap++) part[(size_t)co * ncols + tap * a.cin + ci] = acc[tap][r];
+    if (it == 0 && blockIdx.x == 0 && ln == 0) part[(size_t)a.cout * ncols + co] = accb[r];
+  }
+}
+
 
 // dW += sum over the pixel splits of the partial slices, and db likewise
 // from the slices' tails.  A block takes 64 consecutive entries; its 16
@@ -1007,6 +1175,9 @@ cv_status conv_fwd(cv_denoiser* d, const Conv& L, const uint16_t* x, uint32_t xc
 }
 
 // Backward of one convolution given dY (f32 at y's shape): dZ (ReLU mask),
+cv_status conv_bwd_input(cv_denoiser* d, const Conv& L, float* gx, uint32_t gxcs,
+                         bool accumulate, uint32_t npix, uint32_t h, uint32_t w);
+
 // dW and db, and dX (+= into gx when gx != null).
 cv_status conv_bwd(cv_denoiser* d, const Conv& L, const uint16_t* x, uint32_t xcs,
                    const uint16_t* y, uint32_t ycs, const float* gy, uint32_t gycs, float* gx,
@@ -1034,12 +1205,39 @@ cv_status conv_bwd(cv_denoiser* d, const Conv& L, const uint16_t* x, uint32_t xc
   g.npix = npix;
   g.h = h;
   g.w_ = w;
+  const uint32_t nw = L.cout * 9 * L.cin;
+  if (w % 32 == 0 && npix % (h * w) == 0 && L.cin % 16 == 0 && L.cout_pad % 16 == 0 &&
+      xcs % 8 == 0 && (uintptr_t)x % 16 == 0) {
+    // transposed LDS tiles: NS output and CT input tiles of 16 channels per
+    // block, about 2048 blocks over pixel splits of at least 2 tiles
+    const uint32_t NS = L.cout_pad % 32 == 0 ? 2u : 1u, CT = L.cin % 32 == 0 ? 2u : 1u;
+    const uint32_t bps = (L.cin / (16 * CT)) * (L.cout_pad / (16 * NS));
+    const uint32_t tiles = (npix / (h * w)) * (w / 32) * ((h + 7) / 8);
+    uint32_t splits = std::max(1u, std::min(tiles / 2, std::max(1u, 2048u / bps)));
+    g.tiles_per_split = (tiles + splits - 1) / splits;
+    splits = (tiles + g.tiles_per_split - 1) / g.tiles_per_split;
+    HIPTRY(d->wpart.reserve((size_t)splits * (nw + L.cout)), "alloc");
+    g.part = d->wpart.p;
+    const dim3 grid(L.cin / (16 * CT), L.cout_pad / (16 * NS), splits), block(64 * NS * CT);
+    if (NS == 2 && CT == 2)
+      hipLaunchKernelGGL((cvd::k_wgrad_halo<2, 2>), grid, block, 0, s, g);
+    else if (NS == 2)
+      hipLaunchKernelGGL((cvd::k_wgrad_halo<2, 1>), grid, block, 0, s, g);
+    else if (CT == 2)
+      hipLaunchKernelGGL((cvd::k_wgrad_halo<1, 2>), grid, block, 0, s, g);
+    else
+      hipLaunchKernelGGL((cvd::k_wgrad_halo<1, 1>), grid, block, 0, s, g);
+    HIPTRY(hipGetLastError(), "wgrad");
+    hipLaunchKernelGGL(cvd::k_wgrad_reduce, dim3((nw + L.cout + 63) / 64), dim3(1024), 0, s,
+                       d->wpart.p, splits, nw, L.cout, g.dw, g.db);
+    HIPTRY(hipGetLastError(), "wgrad reduce");
+    return conv_bwd_input(d, L, gx, gxcs, accumulate, npix, h, w);
+  }
   // about 2048 blocks: pixel splits of a multiple of 64 pixels
   const uint32_t col_tiles = (9 * L.cin + cvd::kWgCols - 1) / cvd::kWgCols;
   uint32_t splits = std::max(1u, std::min(npix / 2048u, 2048u / col_tiles));
   g.pix_per_block = (npix / splits + cvd::kWgPix - 1) / cvd::kWgPix * cvd::kWgPix;
   splits = (npix + g.pix_per_block - 1) / g.pix_per_block;
-  const uint32_t nw = L.cout * 9 * L.cin;
   HIPTRY(d->wpart.reserve((size_t)splits * (nw + L.cout)), "alloc");
   g.part = d->wpart.p;
   dim3 grid(col_tiles, 1, splits);
@@ -1054,6 +1252,13 @@ cv_status conv_bwd(cv_denoiser* d, const Conv& L, const uint16_t* x, uint32_t xc
   hipLaunchKernelGGL(cvd::k_wgrad_reduce, dim3((nw + L.cout + 63) / 64), dim3(1024), 0, s,
                      d->wpart.p, splits, nw, L.cout, g.dw, g.db);
   HIPTRY(hipGetLastError(), "wgrad reduce");
+  return conv_bwd_input(d, L, gx, gxcs, accumulate, npix, h, w);
+}
+
+// dX of one convolution (+= into gx when accumulate): the convolution of
+// dZ with the flipped weights.
+cv_status conv_bwd_input(cv_denoiser* d, const Conv& L, float* gx, uint32_t gxcs,
+                         bool accumulate, uint32_t npix, uint32_t h, uint32_t w) {
   if (gx) {
     cvd::ConvArgs a{};
     a.x = d->dz.p;
