@@ -455,6 +455,8 @@ def bench_denoise(args):
         g = cv.Scene.from_scene_data(scenes.procedural_scene(n, 20000 + k))
         _lit_camera(cv, g, rng)
         gs.append(g)
+        if (k + 1) % 128 == 0:
+            log("denoise: %d training scenes built" % (k + 1))
     log("denoise: %d training scenes built in %.1f s" % (len(gs), time.perf_counter() - t0))
     # 1-sample inputs with features and the references, each set in one batch
     b = cv.Batch()
@@ -463,7 +465,10 @@ def bench_denoise(args):
     jref = [cv.Job(g, TW, TH, 100000, args.ref_spp) for g in gs]
     t1 = time.perf_counter()
     b.render(jin)
-    b.render(jref, cv.RENDER_GRAPH)
+    # the references in groups of 64 jobs (a progress line per group)
+    for k in range(0, len(jref), 64):
+        b.render(jref[k:k + 64], cv.RENDER_GRAPH)
+        log("denoise: references %d / %d rendered" % (min(k + 64, len(jref)), len(jref)))
     render_s = time.perf_counter() - t1
     xs = np.stack([cv.Denoiser.prepare_input(j.hdr, j.features, TW, TH, 1) for j in jin])
     ys = np.stack([np.expm1(cv.Denoiser.prepare_input(jr.hdr, j.features, TW, TH,
@@ -596,6 +601,22 @@ def bench_denoise(args):
                       "speedup": round(ratio, 2)})
         log("denoise eval %d: %s" % (e, evals[-1]))
     value = float(np.exp(np.mean(np.log(ratios))))
+    lr_ = np.log(np.array(ratios))
+    spread = {
+        "seeds": len(ratios),
+        "min": round(float(np.min(ratios)), 2),
+        "p10": round(float(np.percentile(ratios, 10)), 2),
+        "median": round(float(np.median(ratios)), 2),
+        "p90": round(float(np.percentile(ratios, 90)), 2),
+        "max": round(float(np.max(ratios)), 2),
+        "geometric_std": round(float(np.exp(np.std(lr_))), 3),
+        # 95 % interval of the geometric mean (t-based, log space)
+        "geomean_ci95": [round(float(np.exp(lr_.mean() - 1.96 * lr_.std(ddof=1)
+                                            / np.sqrt(len(lr_)))), 2) if len(lr_) > 1 else None,
+                         round(float(np.exp(lr_.mean() + 1.96 * lr_.std(ddof=1)
+                                            / np.sqrt(len(lr_)))), 2) if len(lr_) > 1 else None],
+        "fraction_at_least_8x": round(float(np.mean(np.array(ratios) >= 8.0)), 3),
+    }
     out = {
         "metric": "equal-RMSE time, denoised 1 spp vs plain accumulation",
         "value": round(value, 3),
@@ -617,6 +638,8 @@ def bench_denoise(args):
                                   args.train_scenes, n_aug,
                                   args.eval_scenes, W, H, args.eval_ref_spp)},
         "train_loss_first_last": [round(losses[0], 5), round(float(np.mean(losses[-20:])), 5)],
+        "spread": spread,
+        "phases_s": {"training_pairs_render": round(render_s, 1), "training": round(train_s, 1)},
         "evals": evals,
     }
     print(json.dumps(out), flush=True)
@@ -709,6 +732,54 @@ def _hip_copy(hip, dst, src, nbytes, kind):
     rc = hip.hipMemcpy(dst, src, ctypes.c_size_t(nbytes), kind)
     if rc != 0:
         raise RuntimeError("hipMemcpy failed: %d" % rc)
+
+
+def bench_build(args):
+    """GPU BVH builds of the 64 many-jobs scenes (seeds 1000..1063, 1e3..2e6
+    triangles): device time per build with args.batch_bins bins, and the
+    same scenes with the reference's 64 bins."""
+    import convector_amd as cv
+    from convector_amd import scenes
+
+    specs = scenes.batch_jobs(args.jobs, 1000)
+    sds = [scenes.procedural_scene(n, seed) for (n, seed, _, _, _) in specs]
+    ntris = sum(sd.ntris for sd in sds)
+    big = max(range(len(sds)), key=lambda k: sds[k].ntris)
+    res = {}
+    for bins in sorted({64, args.batch_bins}):
+        ms, depth_big, ms_big = 0.0, 0, 0.0
+        for k, sd in enumerate(sds):
+            g = cv.Scene.from_scene_data(sd, bins=bins)
+            info = g.bvh_info()
+            ms += info["build_ms"]
+            if k == big:
+                depth_big, ms_big = info["depth"], info["build_ms"]
+            del g
+        res[bins] = (ms, ms_big, depth_big)
+        log("build: %d bins, %d scenes, %.1f ms device; largest (%d triangles) %.1f ms, depth %d"
+            % (bins, len(sds), ms, sds[big].ntris, ms_big, depth_big))
+    ms, ms_big, depth_big = res[args.batch_bins]
+    out = {
+        "metric": "GPU BVH build, triangles/s",
+        "value": round(ntris / (ms * 1e-3), 1),
+        "unit": "triangles/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 0,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "%d procedural scenes (seeds 1000..%d), %d triangles in total, "
+                               "binned SAH with %d bins" % (len(sds), 999 + len(sds), ntris,
+                                                            args.batch_bins)},
+        "largest_scene": {"triangles": sds[big].ntris, "ms": round(ms_big, 3), "depth": depth_big},
+        "by_bins": {str(b): {"ms": round(v[0], 3), "largest_ms": round(v[1], 3)}
+                    for b, v in res.items()},
+    }
+    print(json.dumps(out), flush=True)
 
 
 def bench_rays(args):
@@ -1017,14 +1088,15 @@ def main():
     ap.add_argument("--rates", type=float, nargs="+", default=[10, 25, 50, 100, 200, 400])
     ap.add_argument("--requests", type=int, default=100)
     ap.add_argument("--workload", choices=["frame", "batch", "denoise", "denoise-kernels", "serve",
-                                           "rays"],
+                                           "rays", "build"],
                     default="frame",
                     help="frame: the large 16-spp 1080p render (default); batch: 64 mixed "
                          "jobs in shared launches against sequential launches; denoise: train "
                          "the denoiser and measure its equal-RMSE speedup; denoise-kernels: the "
                          "network at 1280x736 and training steps at batch 16 of 256x256 on "
                          "device buffers; serve: the request loop's latency and throughput; "
-                         "rays: batched intersect_nearest on suzanne and the bunny")
+                         "rays: batched intersect_nearest on suzanne and the bunny; build: GPU "
+                         "BVH builds of the 64 many-jobs scenes")
     ap.add_argument("--rays", type=int, default=1 << 24, help="rays: rays per call")
     ap.add_argument("--jobs", type=int, default=64)
     ap.add_argument("--train-scenes", type=int, default=256)
@@ -1043,9 +1115,8 @@ def main():
                          "rocprofv3 --pmc counters can be scaled by a copy of known size")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: the path is built for one GPU's share of the work "
-              "(see the sharding notes in DESIGN.md); --gpus %d is not run." % args.gpus,
-              file=sys.stderr)
+        print("bench.py measures one GPU: the path is built for one GPU's share of the work; "
+              "--gpus %d is not run." % args.gpus, file=sys.stderr)
         sys.exit(2)
 
     if args.workload == "batch":
@@ -1062,6 +1133,9 @@ def main():
         return
     if args.workload == "rays":
         bench_rays(args)
+        return
+    if args.workload == "build":
+        bench_build(args)
         return
 
     import convector_amd as cv

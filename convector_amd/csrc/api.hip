@@ -169,29 +169,51 @@ __global__ void k_gather_u32(const uint32_t* src, const uint32_t* order, uint32_
 }
 
 // Scene::from_meshes's direct_sample list (scene.rs:114-129): indices in BVH
-// order whose material has bit 30 set, ascending.  One workgroup, in order.
+// order whose material has bit 30 set, ascending.  One workgroup, in order,
+// 16 consecutive triangles per thread and tile (a prefix sum of the
+// threads' counts places each thread's indices).
 __global__ __launch_bounds__(1024) void k_direct_list(const uint32_t* mat, uint32_t n,
                                                       uint32_t* out, uint32_t* count) {
+  constexpr uint32_t E = 16;
   __shared__ uint32_t wsum[16];
   __shared__ uint32_t base;
   if (threadIdx.x == 0) base = 0;
   __syncthreads();
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (uint32_t t0 = 0; t0 < n; t0 += 1024) {
-    uint32_t k = t0 + threadIdx.x;
-    bool f = k < n && (mat[k] & 0x40000000u) == 0x40000000u;
-    uint64_t b = __ballot(f);
-    if (lane == 0) wsum[wave] = __popcll(b);
-    __syncthreads();
-    uint32_t prior = 0;
-    for (uint32_t w = 0; w < wave; w++) prior += wsum[w];
-    if (f) out[base + prior + __popcll(b & ((1ull << lane) - 1ull))] = k;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      uint32_t s = 0;
-      for (int w = 0; w < 16; w++) s += wsum[w];
-      base += s;
+  for (uint32_t t0 = 0; t0 < n; t0 += 1024 * E) {
+    const uint32_t k0 = t0 + threadIdx.x * E;
+    uint32_t bits = 0;
+    if (k0 + E <= n) {
+#pragma unroll
+      for (uint32_t q = 0; q < E / 4; q++) {
+        const uint4 m = reinterpret_cast<const uint4*>(mat + k0)[q];
+        const uint32_t mm[4] = {m.x, m.y, m.z, m.w};
+#pragma unroll
+        for (uint32_t e = 0; e < 4; e++)
+          bits |= ((mm[e] & 0x40000000u) ? 1u : 0u) << (4 * q + e);
+      }
+    } else {
+      for (uint32_t e = 0; e < E; e++)
+        if (k0 + e < n && (mat[k0 + e] & 0x40000000u)) bits |= 1u << e;
     }
+    const uint32_t c = __popc(bits);
+    uint32_t inc = c;
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+      const uint32_t y = __shfl_up(inc, off);
+      if (lane >= off) inc += y;
+    }
+    if (lane == 63) wsum[wave] = inc;
+    __syncthreads();
+    uint32_t prior = 0, tile = 0;
+    for (uint32_t w = 0; w < 16; w++) {
+      prior += w < wave ? wsum[w] : 0u;
+      tile += wsum[w];
+    }
+    uint32_t pos = base + prior + inc - c;
+    for (uint32_t e = 0; e < E; e++)
+      if ((bits >> e) & 1u) out[pos++] = k0 + e;
+    __syncthreads();
+    if (threadIdx.x == 0) base += tile;
     __syncthreads();
   }
   if (threadIdx.x == 0) *count = base;

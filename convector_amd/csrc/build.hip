@@ -26,8 +26,9 @@
 // children, so that the top levels, where one node holds most triangles,
 // use the whole GPU.
 // Nodes of at most 32 triangles are finished by k_small, one wave per node,
-// with the triangles in the lanes.  k_depth and k_bottom_up (one launch per
-// depth, deepest first) count leaves and the area ratio statistic; k_layout
+// with the triangles in the lanes.  k_depth (which also lists the inner
+// nodes by depth) and k_bottom_up (one launch per depth over that depth's
+// list, deepest first) count leaves and the area ratio statistic; k_layout
 // places every node where crystallize puts it.
 //
 // The TreeSAH cost uses powf/log2f; their values for every triangle count are
@@ -47,7 +48,7 @@ constexpr uint32_t kLeaf = 1, kParity = 2, kIsC1 = 4, kPending = 8, kSplit = 16;
 constexpr uint32_t kHole = 32;
 constexpr uint32_t kBlock = 512;
 constexpr uint32_t kChunk = 2048;   // references per chunk of a wide node
-constexpr uint32_t kWide = 16384;   // nodes above this many triangles are chunked
+constexpr uint32_t kWide = 4096;    // nodes above this many triangles are chunked
 constexpr uint32_t kNarrow = 2048;  // levels with no larger node use 64-thread blocks
 constexpr uint32_t kBinWords = 3 * 64 * 7;  // counts, then outer bounds, of 3 x 64 bins
 
@@ -264,20 +265,45 @@ __device__ __forceinline__ void clear_bins(uint32_t (*cnt)[64], uint32_t (*bx)[6
 
 // Chunk list of the level's wide nodes (one thread; only levels that have
 // wide nodes run it, and those lists are short).
-__global__ void k_wide_plan(Args a) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  uint32_t total = 0;
-  for (uint32_t i = 0; i < a.nlist; i++) {
-    const uint32_t id = a.list[i];
-    INode* nd = &a.nodes[id];
-    const uint32_t n = nd->count;
-    const uint32_t nc = n > kWide ? (n + kChunk - 1) / kChunk : 0u;
-    nd->chunk_base = total;
-    nd->nchunks = nc;
-    for (uint32_t c = 0; c < nc; c++) a.chunk_node[total + c] = id;
-    total += nc;
+// One workgroup: the list in tiles of 1024 nodes, a prefix sum of the chunk
+// counts per tile; each node's thread writes its own chunk entries.
+__global__ __launch_bounds__(1024) void k_wide_plan(Args a) {
+  __shared__ uint32_t wsum[16];
+  __shared__ uint32_t s_base;
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (threadIdx.x == 0) s_base = 0;
+  __syncthreads();
+  for (uint32_t i0 = 0; i0 < a.nlist; i0 += 1024) {
+    const uint32_t i = i0 + threadIdx.x;
+    uint32_t id = 0, nc = 0;
+    if (i < a.nlist) {
+      id = a.list[i];
+      const uint32_t n = a.nodes[id].count;
+      nc = n > kWide ? (n + kChunk - 1) / kChunk : 0u;
+    }
+    uint32_t inc = nc;
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+      const uint32_t y = __shfl_up(inc, off);
+      if (lane >= off) inc += y;
+    }
+    if (lane == 63) wsum[wave] = inc;
+    __syncthreads();
+    uint32_t prior = 0, tile = 0;
+    for (uint32_t w = 0; w < 16; w++) {
+      prior += w < wave ? wsum[w] : 0u;
+      tile += wsum[w];
+    }
+    const uint32_t base = s_base + prior + inc - nc;
+    if (i < a.nlist) {
+      a.nodes[id].chunk_base = base;
+      a.nodes[id].nchunks = nc;
+      for (uint32_t c = 0; c < nc; c++) a.chunk_node[base + c] = id;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) s_base += tile;
+    __syncthreads();
   }
-  a.ctx->total_chunks = total;
+  if (threadIdx.x == 0) a.ctx->total_chunks = s_base;
 }
 
 // Centroid bins of one chunk of a wide node into its partial.
@@ -1029,28 +1055,72 @@ __global__ __launch_bounds__(256) void k_small(Args a) {
 
 // Depth of every node (a walk to the root), the tree's depth, and the
 // leaves' statistics (one leaf, summed_area_ratio 0 + 0).
-__global__ __launch_bounds__(256) void k_depth(Args a, uint32_t ninodes) {
+// Depth of every node; leaves start the bottom-up statistics; inner nodes
+// are counted per depth (dcount holds dcap >= max depth + 1 words), in LDS
+// per workgroup first (one global atomic per workgroup and depth: a global
+// counter per depth shared by all nodes serialises).
+constexpr uint32_t kDepthLds = 512;
+__global__ __launch_bounds__(256) void k_depth(Args a, uint32_t ninodes, uint32_t* dcount,
+                                               uint32_t dcap) {
+  __shared__ uint32_t cnt[kDepthLds];
+  const bool local = dcap <= kDepthLds;
+  if (local)
+    for (uint32_t k = threadIdx.x; k < dcap; k += 256) cnt[k] = 0;
+  __syncthreads();
   const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
-  if (x >= ninodes) return;
   INode* nodes = a.nodes;
-  if (nodes[x].flags & kHole) return;
-  uint32_t d = 0;
-  for (int32_t u = nodes[x].parent; u >= 0; u = nodes[u].parent) d++;
-  nodes[x].depth = d;
-  if (nodes[x].flags & kLeaf) {
-    nodes[x].leaves = 1;
-    nodes[x].sar = 0.0f + 0.0f;
-    atomicMax(&a.ctx->max_depth, d);
+  if (x < ninodes && !(nodes[x].flags & kHole)) {
+    uint32_t d = 0;
+    for (int32_t u = nodes[x].parent; u >= 0; u = nodes[u].parent) d++;
+    nodes[x].depth = d;
+    if (nodes[x].flags & kLeaf) {
+      nodes[x].leaves = 1;
+      nodes[x].sar = 0.0f + 0.0f;
+      atomicMax(&a.ctx->max_depth, d);
+    } else if (d < dcap) {
+      atomicAdd(local ? &cnt[d] : &dcount[d], 1u);
+    }
   }
+  __syncthreads();
+  if (local)
+    for (uint32_t k = threadIdx.x; k < dcap; k += 256)
+      if (cnt[k]) atomicAdd(&dcount[k], cnt[k]);
 }
 
-// Leaf counts and summed_area_ratio (bvh.rs:380-386) of the inner nodes at
-// one depth, from their children (one launch per depth, deepest first).
-__global__ __launch_bounds__(256) void k_bottom_up(Args a, uint32_t ninodes, uint32_t depth) {
+// The inner nodes listed by depth: cursor[d] starts at depth d's offset;
+// a workgroup reserves one range per depth and ranks its nodes in LDS.
+__global__ __launch_bounds__(256) void k_depth_lists(Args a, uint32_t ninodes, uint32_t* cursor,
+                                                     uint32_t dcap, uint32_t* list) {
+  __shared__ uint32_t cnt[kDepthLds];
+  const bool local = dcap <= kDepthLds;
+  if (local)
+    for (uint32_t k = threadIdx.x; k < dcap; k += 256) cnt[k] = 0;
+  __syncthreads();
   const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
-  if (x >= ninodes) return;
+  bool inner = false;
+  uint32_t d = 0, rank = 0;
+  if (x < ninodes) {
+    const INode& n = a.nodes[x];
+    inner = !(n.flags & (kLeaf | kHole)) && n.depth < dcap;
+    d = n.depth;
+  }
+  if (inner && local) rank = atomicAdd(&cnt[d], 1u);
+  __syncthreads();
+  if (local)
+    for (uint32_t k = threadIdx.x; k < dcap; k += 256)
+      if (cnt[k]) cnt[k] = atomicAdd(&cursor[k], cnt[k]);
+  __syncthreads();
+  if (inner) list[local ? cnt[d] + rank : atomicAdd(&cursor[d], 1u)] = x;
+}
+
+// Leaf counts and summed_area_ratio (bvh.rs:380-386) of the inner nodes of
+// one depth (listed by k_depth_lists), from their children: one launch per
+// depth, deepest first, each over that depth's nodes only.
+__global__ __launch_bounds__(256) void k_bottom_up(Args a, const uint32_t* list, uint32_t n) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
   INode* nodes = a.nodes;
-  if (nodes[x].depth != depth || (nodes[x].flags & (kLeaf | kHole))) return;
+  const uint32_t x = list[k];
   const int32_t c0 = nodes[x].child[0], c1 = nodes[x].child[1];
   const float s0 = nodes[c0].sar, s1 = nodes[c1].sar;
   const float A = area6(nodes[x].box);
@@ -1058,6 +1128,15 @@ __global__ __launch_bounds__(256) void k_bottom_up(Args a, uint32_t ninodes, uin
   const float cs = (0.0f + area6(nodes[c0].box) / A) + area6(nodes[c1].box) / A;
   nodes[x].leaves = nodes[c0].leaves + nodes[c1].leaves;
   nodes[x].sar = cs + cc;
+}
+
+// Counters of the next level's node lists back to zero.
+__global__ void k_level_reset(Ctx* ctx) {
+  if (threadIdx.x == 0) {
+    ctx->next_large = 0;
+    ctx->next_wide = 0;
+    ctx->next_max = 0;
+  }
 }
 
 // crystallize (bvh.rs:389-428): node x is written at L(parent) + [x is c1],
@@ -1226,14 +1305,16 @@ hipError_t build_reference_bvh(const float* d_v9, uint32_t ntris, DevNode* nodes
   uint32_t* nxt = listB;
   bool wide = ntris > kWide;
   uint32_t level_max = ntris;
+  uint32_t levels = 0;  // iterations of the level phase
   while (nlist > 0) {
+    levels++;
     const bool narrow = level_max <= kNarrow;
     a.list = cur;
     a.nlist = nlist;
     a.next_list = nxt;
     a.mode = 0;
     if (wide) {
-      hipLaunchKernelGGL(k_wide_plan, dim3(1), dim3(64), 0, s, a);
+      hipLaunchKernelGGL(k_wide_plan, dim3(1), dim3(1024), 0, s, a);
       BVH_CHECK(hipGetLastError());
       hipLaunchKernelGGL(k_bin_chunks, dim3((uint32_t)max_chunks), dim3(kBlock), 0, s, a);
       BVH_CHECK(hipGetLastError());
@@ -1251,9 +1332,8 @@ hipError_t build_reference_bvh(const float* d_v9, uint32_t ntris, DevNode* nodes
     a.mode = 1;
     decide();
     BVH_CHECK(hipGetLastError());
-    BVH_CHECK(hipMemcpyAsync(&ctx->next_large, &zero, 4, hipMemcpyHostToDevice, s));
-    BVH_CHECK(hipMemcpyAsync(&ctx->next_wide, &zero, 4, hipMemcpyHostToDevice, s));
-    BVH_CHECK(hipMemcpyAsync(&ctx->next_max, &zero, 4, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_level_reset, dim3(1), dim3(64), 0, s, ctx);
+    BVH_CHECK(hipGetLastError());
     if (wide) {
       hipLaunchKernelGGL(k_part_chunks, dim3((uint32_t)max_chunks), dim3(kBlock), 0, s, a);
       BVH_CHECK(hipGetLastError());
@@ -1301,12 +1381,36 @@ hipError_t build_reference_bvh(const float* d_v9, uint32_t ntris, DevNode* nodes
     return hipSuccess;
   }
   uint32_t blocks = (ninodes + 255) / 256;
-  hipLaunchKernelGGL(k_depth, dim3(blocks), dim3(256), 0, s, a, ninodes);
+  // depths: the level phase's nodes lie at most `levels` deep and a small
+  // subtree of at most 32 triangles adds at most 31 levels
+  const uint32_t dcap = levels + 34;
+  uint32_t *dcount, *dlist;
+  BVH_CHECK(alloc((void**)&dcount, (size_t)dcap * 4));
+  BVH_CHECK(alloc((void**)&dlist, (size_t)ninodes * 4));
+  BVH_CHECK(hipMemsetAsync(dcount, 0, (size_t)dcap * 4, s));
+  hipLaunchKernelGGL(k_depth, dim3(blocks), dim3(256), 0, s, a, ninodes, dcount, dcap);
   BVH_CHECK(hipGetLastError());
+  std::vector<uint32_t> hcount(dcap), hoff(dcap);
+  BVH_CHECK(hipMemcpyAsync(hcount.data(), dcount, (size_t)dcap * 4, hipMemcpyDeviceToHost, s));
   BVH_CHECK(hipMemcpyAsync(&hc, ctx, sizeof(Ctx), hipMemcpyDeviceToHost, s));
   BVH_CHECK(hipStreamSynchronize(s));
+  if (hc.max_depth >= dcap) {
+    cleanup();
+    return hipErrorInvalidValue;
+  }
+  uint32_t run = 0;
+  for (uint32_t d = 0; d < dcap; d++) {
+    hoff[d] = run;
+    run += hcount[d];
+  }
+  BVH_CHECK(hipMemcpyAsync(dcount, hoff.data(), (size_t)dcap * 4, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_depth_lists, dim3(blocks), dim3(256), 0, s, a, ninodes, dcount, dcap,
+                     dlist);
+  BVH_CHECK(hipGetLastError());
   for (uint32_t d = hc.max_depth; d-- > 0;) {
-    hipLaunchKernelGGL(k_bottom_up, dim3(blocks), dim3(256), 0, s, a, ninodes, d);
+    if (!hcount[d]) continue;
+    hipLaunchKernelGGL(k_bottom_up, dim3((hcount[d] + 255) / 256), dim3(256), 0, s, a,
+                       dlist + hoff[d], hcount[d]);
     BVH_CHECK(hipGetLastError());
   }
   hipLaunchKernelGGL(k_layout, dim3(blocks), dim3(256), 0, s, a, ninodes, nodes_out, order_out);

@@ -60,7 +60,7 @@ def test_boundary_sizes(cv, oracle, n, bins):
 @pytest.mark.parametrize("n", [10000, 40000])
 def test_sorted_fallback(cv, oracle, n, bins):
     """A dense cluster and outliers: bins overflow n/8 (more often with
-    fewer bins) and nodes take the sorted chunks, chunked above 16384."""
+    fewer bins) and nodes take the sorted chunks, chunked above 4096."""
     rng = np.random.default_rng(n + bins)
     k = n * 9 // 10
     c = np.concatenate([rng.normal(0, 0.001, (k, 3)), rng.uniform(-50, 50, (n - k, 3))])

@@ -72,7 +72,7 @@ def test_sorted_fallback_large(cv, oracle):
 
 def test_sorted_fallback_wide(cv, oracle):
     """The same cluster and outliers at 40000 triangles: the top nodes are
-    binned and partitioned in chunks (nodes of more than 16384 triangles)
+    binned and partitioned in chunks (nodes of more than 4096 triangles)
     and fall back to sorted bins there."""
     rng = np.random.default_rng(9)
     c = np.concatenate([rng.normal(0, 0.001, (36000, 3)), rng.uniform(-50, 50, (4000, 3))])
