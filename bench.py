@@ -1324,8 +1324,10 @@ def main():
                 "host_cores": None if not cb else cb["cores"],
                 "gpu_over_host": bo.get("gpu_over_cpu"),
                 "bvh_width": bo["bvh_width"],
+                "bins": bo["bins"],
                 "exact_width2": {k: bo["exact"].get(k) for k in
-                                 ("paths_per_s", "ms", "speedup_vs_sequential", "gpu_over_cpu")},
+                                 ("bins", "paths_per_s", "ms", "speedup_vs_sequential",
+                                  "gpu_over_cpu", "bvh_build_ms_total")},
                 "bvh_build_ms_total": bo["bvh_build_ms_total"],
                 "autotune": bo["autotune"]["chosen"],
             }
