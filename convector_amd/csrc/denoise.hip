@@ -44,6 +44,14 @@ __device__ __forceinline__ uint16_t to_bf16(float f) {
 }
 __device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
+typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
+typedef float f2 __attribute__((ext_vector_type(2)));
+// Two floats to bf16 (round to nearest even) in one v_cvt_pk_bf16_f32:
+// a in the low half.
+__device__ __forceinline__ uint32_t pack_bf16x2(float a, float b) {
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(f2{a, b}, bf2));
+}
+
 
 // ---------------------------------------------------------------------------
 // Implicit-GEMM 3x3 convolution (stride 1, zero padding 1), NHWC.
@@ -71,6 +79,10 @@ struct ConvArgs {
   uint32_t relu;
   uint32_t flip;        // backward to the input: offsets mirrored
   uint32_t npix, h, w_;
+  // non-null: the ReLU gradient of the layer whose output `mask` (bf16,
+  // pixel stride mcs) is: outputs where mask <= 0 become 0 (written to y)
+  const uint16_t* mask;
+  uint32_t mcs;
 };
 
 // The convolution with the operands staged in LDS: a block of 4 waves
@@ -187,6 +199,7 @@ __global__ __launch_bounds__(256) void k_conv_lds(ConvArgs a) {
         if (!jok || p >= a.npix) continue;
         float v = acc[f][t][r] + bj;
         if (a.relu) v = fmaxf(v, 0.0f);
+        if (a.mask && !(from_bf16(a.mask[(size_t)p * a.mcs + j]) > 0.0f)) v = 0.0f;
         if (a.y) a.y[(size_t)p * a.ycs + j] = to_bf16(v);
         if (a.yf) {
           float* o = a.yf + (size_t)p * a.yfcs + j;
@@ -215,7 +228,7 @@ __global__ __launch_bounds__(256) void k_conv_lds(ConvArgs a) {
 // ---------------------------------------------------------------------------
 constexpr uint32_t kHaloW = 32, kHaloH = 8;
 
-template <int KC, int NS, bool FLIP>
+template <int KC, int NS, bool FLIP, bool MASK = false>
 __global__ __launch_bounds__(256, 2) void k_conv_halo(ConvArgs a) {
   constexpr int MF = 4;
   constexpr int HH = (int)kHaloH + 2, HW = (int)kHaloW + 2;
@@ -349,11 +362,25 @@ __global__ __launch_bounds__(256, 2) void k_conv_halo(ConvArgs a) {
         if (a.relu) v[r] = fmaxf(v[r], 0.0f);
       }
       const bool whole = j0 + 4 <= a.nout;
+      if constexpr (MASK) {
+        // the ReLU gradient of the layer below: 4 bf16 of its output at once
+        // (the host checks the mask's alignment and stride)
+        if (whole) {
+          const uint2 mk = *reinterpret_cast<const uint2*>(a.mask + p * a.mcs + j0);
+          const uint32_t mw[4] = {mk.x << 16, mk.x & 0xffff0000u, mk.y << 16, mk.y & 0xffff0000u};
+#pragma unroll
+          for (int r = 0; r < 4; r++)
+            if (!(__uint_as_float(mw[r]) > 0.0f)) v[r] = 0.0f;
+        } else {
+          for (int r = 0; r < 4; r++)
+            if (j0 + r < a.nout && !(from_bf16(a.mask[p * a.mcs + j0 + r]) > 0.0f)) v[r] = 0.0f;
+        }
+      }
       if (a.y) {
         if (whole && vec_y) {
           uint2 pk;
-          pk.x = (uint32_t)to_bf16(v[0]) | ((uint32_t)to_bf16(v[1]) << 16);
-          pk.y = (uint32_t)to_bf16(v[2]) | ((uint32_t)to_bf16(v[3]) << 16);
+          pk.x = pack_bf16x2(v[0], v[1]);
+          pk.y = pack_bf16x2(v[2], v[3]);
           *reinterpret_cast<uint2*>(a.y + p * a.ycs + j0) = pk;
         } else {
           for (int r = 0; r < 4; r++)
@@ -1091,9 +1118,12 @@ struct cv_denoiser {
   // activations for the current size
   uint32_t nb = 0, h = 0, w = 0;
   size_t cap_pixels = 0;
-  DevBuf<uint16_t> x0, e0a, cat0, p0, e1a, cat1, p1, b2a, b2b, d1a, d1b, d0a, d0b, dz;
-  DevBuf<float> out, dout, g_e0a, g_cat0, g_p0, g_e1a, g_cat1, g_p1, g_b2a, g_b2b, g_d1a, g_d1b,
-      g_d0a, g_d0b, g_up1, g_up0, loss;
+  // dZ of the layer in backward and of the layer below it (two buffers, so
+  // that one layer's dX convolution can write the next layer's dZ)
+  DevBuf<uint16_t> x0, e0a, cat0, p0, e1a, cat1, p1, b2a, b2b, d1a, d1b, d0a, d0b, dz, dz2;
+  // f32 gradients where they are summed or pooled / upsampled (the other
+  // layers hand their dZ on directly, see backward)
+  DevBuf<float> out, dout, g_cat0, g_p0, g_cat1, g_p1, g_b2b, g_d1b, loss;
   DevBuf<float> in_f32, tgt_f32;
   DevBuf<float> wpart;  // weight-gradient partials of the pixel splits
   // convolutions from LDS halo tiles where the shape allows (else k_conv_lds)
@@ -1107,13 +1137,15 @@ namespace {
 // pixels, the input channels a multiple of 16 in a dense channel range):
 // NT output tiles in slices of NS per block (grid.y = NT / NS), at most 4,
 // so that a chunk's weights and tile fit in LDS beside other blocks.
-template <int KC, int NS, bool FLIP>
+template <int KC, int NS, bool FLIP, bool MASK = false>
 bool try_halo(cv_denoiser* d, const cvd::ConvArgs& a, uint32_t nt, uint32_t ns) {
-  if (a.kc != (uint32_t)KC || ns != (uint32_t)NS || (a.flip != 0) != FLIP) return false;
+  if (a.kc != (uint32_t)KC || ns != (uint32_t)NS || (a.flip != 0) != FLIP ||
+      (a.mask != nullptr) != MASK)
+    return false;
   const uint32_t imgs = a.npix / (a.h * a.w_);
   const uint32_t blocks =
       imgs * (a.w_ / cvd::kHaloW) * ((a.h + cvd::kHaloH - 1) / cvd::kHaloH);
-  hipLaunchKernelGGL((cvd::k_conv_halo<KC, NS, FLIP>), dim3(blocks, nt / NS), dim3(256), 0,
+  hipLaunchKernelGGL((cvd::k_conv_halo<KC, NS, FLIP, MASK>), dim3(blocks, nt / NS), dim3(256), 0,
                      d->stream, a);
   return true;
 }
@@ -1122,6 +1154,8 @@ bool launch_conv_halo(cv_denoiser* d, const cvd::ConvArgs& a, uint32_t nt) {
   // whole 32-pixel rows, whole images, and every pixel's KC channels within
   // its stride (the tile loads read channels [0, KC) of each pixel)
   if (a.w_ % cvd::kHaloW || a.npix % (a.h * a.w_) || a.xcs % 8 || a.xcs < a.kc) return false;
+  // a fused ReLU mask is read 4 channels at a time
+  if (a.mask && (a.mcs % 4 || (uintptr_t)a.mask % 8)) return false;
   const uint32_t ns = nt % 4 == 0 ? 4 : nt % 3 == 0 ? 3 : nt % 2 == 0 ? 2 : 1;
   return try_halo<16, 2, false>(d, a, nt, ns) || try_halo<32, 1, false>(d, a, nt, ns) ||
          try_halo<32, 2, false>(d, a, nt, ns) || try_halo<32, 4, false>(d, a, nt, ns) ||
@@ -1129,7 +1163,9 @@ bool launch_conv_halo(cv_denoiser* d, const cvd::ConvArgs& a, uint32_t nt) {
          try_halo<128, 4, false>(d, a, nt, ns) || try_halo<192, 4, false>(d, a, nt, ns) ||
          try_halo<16, 2, true>(d, a, nt, ns) || try_halo<32, 2, true>(d, a, nt, ns) ||
          try_halo<32, 3, true>(d, a, nt, ns) || try_halo<64, 2, true>(d, a, nt, ns) ||
-         try_halo<64, 4, true>(d, a, nt, ns) || try_halo<128, 4, true>(d, a, nt, ns);
+         try_halo<64, 4, true>(d, a, nt, ns) || try_halo<128, 4, true>(d, a, nt, ns) ||
+         try_halo<16, 2, true, true>(d, a, nt, ns) || try_halo<32, 2, true, true>(d, a, nt, ns) ||
+         try_halo<64, 4, true, true>(d, a, nt, ns) || try_halo<128, 4, true, true>(d, a, nt, ns);
 }
 
 cv_status launch_conv(cv_denoiser* d, const cvd::ConvArgs& a, uint32_t nt) {
@@ -1174,27 +1210,35 @@ cv_status conv_fwd(cv_denoiser* d, const Conv& L, const uint16_t* x, uint32_t xc
   return launch_conv(d, a, L.cout_pad / 16);
 }
 
-// Backward of one convolution given dY (f32 at y's shape): dZ (ReLU mask),
-cv_status conv_bwd_input(cv_denoiser* d, const Conv& L, float* gx, uint32_t gxcs,
-                         bool accumulate, uint32_t npix, uint32_t h, uint32_t w);
+cv_status conv_bwd_input(cv_denoiser* d, const Conv& L, const uint16_t* dz, float* gx,
+                         uint32_t gxcs, bool accumulate, const uint16_t* x, uint32_t xcs,
+                         uint16_t* dz_next, uint32_t npix, uint32_t h, uint32_t w);
 
-// dW and db, and dX (+= into gx when gx != null).
+// Backward of one convolution: dZ (dY at y's shape, f32, through the ReLU
+// mask; or, with dz_ready, already in dz), dW and db, and dX (+= into gx
+// when accumulate).  With dz_next the layer below -- whose ReLU output is
+// this layer's input x, with L.cin channels -- gets its dZ from the dX
+// convolution directly: masked by x and written as bf16 to dz_next, in
+// place of dX.
 cv_status conv_bwd(cv_denoiser* d, const Conv& L, const uint16_t* x, uint32_t xcs,
-                   const uint16_t* y, uint32_t ycs, const float* gy, uint32_t gycs, float* gx,
-                   uint32_t gxcs, bool accumulate, uint32_t h, uint32_t w) {
+                   const uint16_t* y, uint32_t ycs, const float* gy, uint32_t gycs, uint16_t* dz,
+                   bool dz_ready, float* gx, uint32_t gxcs, bool accumulate, uint16_t* dz_next,
+                   uint32_t h, uint32_t w) {
   const uint32_t npix = d->nb * h * w;
   hipStream_t s = d->stream;
-  if (L.cout == L.cout_pad && L.cout % 8 == 0 && gycs % 8 == 0 && (!y || ycs % 8 == 0))
-    hipLaunchKernelGGL(cvd::k_relu_grad8, dim3(grid1((size_t)npix * L.cout / 8)), dim3(256), 0, s,
-                       gy, gycs, y, ycs, L.cout, d->dz.p, npix, L.relu ? 1u : 0u);
-  else
-    hipLaunchKernelGGL(cvd::k_relu_grad, dim3(grid1((size_t)npix * L.cout_pad)), dim3(256), 0, s,
-                       gy, gycs, y, ycs, L.cout, d->dz.p, L.cout_pad, npix, L.relu ? 1u : 0u);
-  HIPTRY(hipGetLastError(), "relu grad");
+  if (!dz_ready) {
+    if (L.cout == L.cout_pad && L.cout % 8 == 0 && gycs % 8 == 0 && (!y || ycs % 8 == 0))
+      hipLaunchKernelGGL(cvd::k_relu_grad8, dim3(grid1((size_t)npix * L.cout / 8)), dim3(256), 0,
+                         s, gy, gycs, y, ycs, L.cout, dz, npix, L.relu ? 1u : 0u);
+    else
+      hipLaunchKernelGGL(cvd::k_relu_grad, dim3(grid1((size_t)npix * L.cout_pad)), dim3(256), 0, s,
+                         gy, gycs, y, ycs, L.cout, dz, L.cout_pad, npix, L.relu ? 1u : 0u);
+    HIPTRY(hipGetLastError(), "relu grad");
+  }
   if ((uint64_t)npix * std::max(xcs, L.cout_pad) >= (1ull << 31))
     return fail(CV_ERR_INVALID_ARGUMENT, "training batch too large for 32-bit offsets");
   cvd::WGradArgs g{};
-  g.dz = d->dz.p;
+  g.dz = dz;
   g.zcs = L.cout_pad;
   g.cout = L.cout;
   g.x = x;
@@ -1231,7 +1275,7 @@ cv_status conv_bwd(cv_denoiser* d, const Conv& L, const uint16_t* x, uint32_t xc
     hipLaunchKernelGGL(cvd::k_wgrad_reduce, dim3((nw + L.cout + 63) / 64), dim3(1024), 0, s,
                        d->wpart.p, splits, nw, L.cout, g.dw, g.db);
     HIPTRY(hipGetLastError(), "wgrad reduce");
-    return conv_bwd_input(d, L, gx, gxcs, accumulate, npix, h, w);
+    return conv_bwd_input(d, L, dz, gx, gxcs, accumulate, x, xcs, dz_next, npix, h, w);
   }
   // about 2048 blocks: pixel splits of a multiple of 64 pixels
   const uint32_t col_tiles = (9 * L.cin + cvd::kWgCols - 1) / cvd::kWgCols;
@@ -1252,24 +1296,34 @@ cv_status conv_bwd(cv_denoiser* d, const Conv& L, const uint16_t* x, uint32_t xc
   hipLaunchKernelGGL(cvd::k_wgrad_reduce, dim3((nw + L.cout + 63) / 64), dim3(1024), 0, s,
                      d->wpart.p, splits, nw, L.cout, g.dw, g.db);
   HIPTRY(hipGetLastError(), "wgrad reduce");
-  return conv_bwd_input(d, L, gx, gxcs, accumulate, npix, h, w);
+  return conv_bwd_input(d, L, dz, gx, gxcs, accumulate, x, xcs, dz_next, npix, h, w);
 }
 
 // dX of one convolution (+= into gx when accumulate): the convolution of
 // dZ with the flipped weights.
-cv_status conv_bwd_input(cv_denoiser* d, const Conv& L, float* gx, uint32_t gxcs,
-                         bool accumulate, uint32_t npix, uint32_t h, uint32_t w) {
-  if (gx) {
+cv_status conv_bwd_input(cv_denoiser* d, const Conv& L, const uint16_t* dz, float* gx,
+                         uint32_t gxcs, bool accumulate, const uint16_t* x, uint32_t xcs,
+                         uint16_t* dz_next, uint32_t npix, uint32_t h, uint32_t w) {
+  if (gx || dz_next) {
     cvd::ConvArgs a{};
-    a.x = d->dz.p;
+    a.x = dz;
     a.xcs = L.cout_pad;
     a.kc = L.cout_pad;
     a.w = L.Wt.p;
     a.kpad = L.kpad_t;
     a.bias = nullptr;
     a.nout = L.cin;
-    a.yf = gx;
-    a.yfcs = gxcs;
+    if (dz_next) {
+      if (accumulate || L.cin % 16)
+        return fail(CV_ERR_INVALID_ARGUMENT, "fused ReLU gradient needs a plain dX");
+      a.y = dz_next;
+      a.ycs = L.cin;
+      a.mask = x;
+      a.mcs = xcs;
+    } else {
+      a.yf = gx;
+      a.yfcs = gxcs;
+    }
     a.accumulate = accumulate ? 1u : 0u;
     a.flip = 1;
     a.npix = npix;
@@ -1306,20 +1360,15 @@ cv_status reserve_acts(cv_denoiser* d, uint32_t nb, uint32_t h, uint32_t w) {
   HIPTRY(d->d0a.alloc(P0 * 32), "alloc");
   HIPTRY(d->d0b.alloc(P0 * 32), "alloc");
   HIPTRY(d->dz.alloc(P0 * 128), "alloc");
+  HIPTRY(d->dz2.alloc(P0 * 128), "alloc");
   HIPTRY(d->out.alloc(P0 * 3), "alloc");
   HIPTRY(d->dout.alloc(P0 * 3), "alloc");
-  HIPTRY(d->g_e0a.alloc(P0 * 32), "alloc");
   HIPTRY(d->g_cat0.alloc(P0 * 96), "alloc");
   HIPTRY(d->g_p0.alloc(P1 * 32), "alloc");
-  HIPTRY(d->g_e1a.alloc(P1 * 64), "alloc");
   HIPTRY(d->g_cat1.alloc(P1 * 192), "alloc");
   HIPTRY(d->g_p1.alloc(P2 * 64), "alloc");
-  HIPTRY(d->g_b2a.alloc(P2 * 128), "alloc");
   HIPTRY(d->g_b2b.alloc(P2 * 128), "alloc");
-  HIPTRY(d->g_d1a.alloc(P1 * 64), "alloc");
   HIPTRY(d->g_d1b.alloc(P1 * 64), "alloc");
-  HIPTRY(d->g_d0a.alloc(P0 * 32), "alloc");
-  HIPTRY(d->g_d0b.alloc(P0 * 32), "alloc");
   HIPTRY(d->loss.alloc(1), "alloc");
   d->cap_pixels = need;
   d->nb = nb;
@@ -1378,38 +1427,45 @@ cv_status backward(cv_denoiser* d) {
   cv_status cs;
   hipLaunchKernelGGL(cvd::k_fill, dim3(grid1(d->nparams)), dim3(256), 0, s, d->grads.p,
                      d->nparams, 0.0f);
+  // Where a layer's input is the ReLU output of the layer below (same
+  // buffer, same channels), the dX convolution writes that layer's dZ
+  // directly (dz_next); dZ alternates between the two buffers.
+  uint16_t* const dzA = d->dz.p;
+  uint16_t* const dzB = d->dz2.p;
   // out <- d0b
-  CVD_TRY(conv_bwd(d, L[10], d->d0b.p, 32, nullptr, 0, d->dout.p, 3, d->g_d0b.p, 32, false, H, W));
-  CVD_TRY(conv_bwd(d, L[9], d->d0a.p, 32, d->d0b.p, 32, d->g_d0b.p, 32, d->g_d0a.p, 32, false, H,
-                   W));
-  CVD_TRY(conv_bwd(d, L[8], d->cat0.p, 96, d->d0a.p, 32, d->g_d0a.p, 32, d->g_cat0.p, 96, false,
-                   H, W));
+  CVD_TRY(conv_bwd(d, L[10], d->d0b.p, 32, nullptr, 0, d->dout.p, 3, dzA, false, nullptr, 0,
+                   false, dzB, H, W));
+  CVD_TRY(conv_bwd(d, L[9], d->d0a.p, 32, d->d0b.p, 32, nullptr, 0, dzB, true, nullptr, 0, false,
+                   dzA, H, W));
+  CVD_TRY(conv_bwd(d, L[8], d->cat0.p, 96, d->d0a.p, 32, nullptr, 0, dzA, true, d->g_cat0.p, 96,
+                   false, nullptr, H, W));
   // cat0[0:64] = up(d1b)
   hipLaunchKernelGGL(cvd::k_upsample_grad, dim3(grid1((size_t)nb * H1 * W1 * 16)), dim3(256), 0, s,
                      d->g_cat0.p, 96u, d->g_d1b.p, 64u, 64u, nb, H1, W1);
-  CVD_TRY(conv_bwd(d, L[7], d->d1a.p, 64, d->d1b.p, 64, d->g_d1b.p, 64, d->g_d1a.p, 64, false,
-                   H1, W1));
-  CVD_TRY(conv_bwd(d, L[6], d->cat1.p, 192, d->d1a.p, 64, d->g_d1a.p, 64, d->g_cat1.p, 192, false,
-                   H1, W1));
+  CVD_TRY(conv_bwd(d, L[7], d->d1a.p, 64, d->d1b.p, 64, d->g_d1b.p, 64, dzB, false, nullptr, 0,
+                   false, dzA, H1, W1));
+  CVD_TRY(conv_bwd(d, L[6], d->cat1.p, 192, d->d1a.p, 64, nullptr, 0, dzA, true, d->g_cat1.p, 192,
+                   false, nullptr, H1, W1));
   // cat1[0:128] = up(b2b)
   hipLaunchKernelGGL(cvd::k_upsample_grad, dim3(grid1((size_t)nb * H2 * W2 * 32)), dim3(256), 0,
                      s, d->g_cat1.p, 192u, d->g_b2b.p, 128u, 128u, nb, H2, W2);
-  CVD_TRY(conv_bwd(d, L[5], d->b2a.p, 128, d->b2b.p, 128, d->g_b2b.p, 128, d->g_b2a.p, 128, false,
-                   H2, W2));
-  CVD_TRY(conv_bwd(d, L[4], d->p1.p, 64, d->b2a.p, 128, d->g_b2a.p, 128, d->g_p1.p, 64, false, H2,
-                   W2));
+  CVD_TRY(conv_bwd(d, L[5], d->b2a.p, 128, d->b2b.p, 128, d->g_b2b.p, 128, dzB, false, nullptr, 0,
+                   false, dzA, H2, W2));
+  CVD_TRY(conv_bwd(d, L[4], d->p1.p, 64, d->b2a.p, 128, nullptr, 0, dzA, true, d->g_p1.p, 64,
+                   false, nullptr, H2, W2));
   // p1 = pool(cat1[128:192]): its gradient joins the skip gradient in g_cat1
   hipLaunchKernelGGL(cvd::k_pool_grad, dim3(grid1((size_t)nb * H2 * W2 * 16)), dim3(256), 0, s,
                      d->g_p1.p, 64u, d->g_cat1.p + 128, 192u, 64u, nb, H2, W2);
-  CVD_TRY(conv_bwd(d, L[3], d->e1a.p, 64, d->cat1.p + 128, 192, d->g_cat1.p + 128, 192,
-                   d->g_e1a.p, 64, false, H1, W1));
-  CVD_TRY(conv_bwd(d, L[2], d->p0.p, 32, d->e1a.p, 64, d->g_e1a.p, 64, d->g_p0.p, 32, false, H1,
-                   W1));
+  CVD_TRY(conv_bwd(d, L[3], d->e1a.p, 64, d->cat1.p + 128, 192, d->g_cat1.p + 128, 192, dzB,
+                   false, nullptr, 0, false, dzA, H1, W1));
+  CVD_TRY(conv_bwd(d, L[2], d->p0.p, 32, d->e1a.p, 64, nullptr, 0, dzA, true, d->g_p0.p, 32,
+                   false, nullptr, H1, W1));
   hipLaunchKernelGGL(cvd::k_pool_grad, dim3(grid1((size_t)nb * H1 * W1 * 8)), dim3(256), 0, s,
                      d->g_p0.p, 32u, d->g_cat0.p + 64, 96u, 32u, nb, H1, W1);
-  CVD_TRY(conv_bwd(d, L[1], d->e0a.p, 32, d->cat0.p + 64, 96, d->g_cat0.p + 64, 96, d->g_e0a.p,
-                   32, false, H, W));
-  CVD_TRY(conv_bwd(d, L[0], d->x0.p, 16, d->e0a.p, 32, d->g_e0a.p, 32, nullptr, 0, false, H, W));
+  CVD_TRY(conv_bwd(d, L[1], d->e0a.p, 32, d->cat0.p + 64, 96, d->g_cat0.p + 64, 96, dzB, false,
+                   nullptr, 0, false, dzA, H, W));
+  CVD_TRY(conv_bwd(d, L[0], d->x0.p, 16, d->e0a.p, 32, nullptr, 0, dzA, true, nullptr, 0, false,
+                   nullptr, H, W));
   HIPTRY(hipGetLastError(), "backward");
   return CV_OK;
 #undef CVD_TRY
