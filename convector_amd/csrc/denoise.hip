@@ -1156,7 +1156,7 @@ bool launch_conv_halo(cv_denoiser* d, const cvd::ConvArgs& a, uint32_t nt) {
   if (a.w_ % cvd::kHaloW || a.npix % (a.h * a.w_) || a.xcs % 8 || a.xcs < a.kc) return false;
   // a fused ReLU mask is read 4 channels at a time
   if (a.mask && (a.mcs % 4 || (uintptr_t)a.mask % 8)) return false;
-  const uint32_t ns = nt % 4 == 0 ? 4 : nt % 3 == 0 ? 3 : nt % 2 == 0 ? 2 : 1;
+  const uint32_t ns = nt % 4 == 0 ? 4 : nt % 2 == 0 ? 2 : nt % 3 == 0 ? 3 : 1;
   return try_halo<16, 2, false>(d, a, nt, ns) || try_halo<32, 1, false>(d, a, nt, ns) ||
          try_halo<32, 2, false>(d, a, nt, ns) || try_halo<32, 4, false>(d, a, nt, ns) ||
          try_halo<64, 4, false>(d, a, nt, ns) || try_halo<96, 2, false>(d, a, nt, ns) ||
