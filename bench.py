@@ -258,17 +258,22 @@ def run_batch(args):
         exact_build_ms = sum(g.bvh_info()["build_ms"] for g in gs_exact)
         jobs_exact = [cv.Job(g, w, h, 1, spp, hdr=hdr.ptr.value + int(offs[i]) * 12)
                       for i, (g, (_, _, w, h, spp)) in enumerate(zip(gs_exact, specs))]
+    exact_tuning = tuning
     if tuning["bvh_width"] != 2 or args.batch_bins != 64:
-        b.set_tuning(bvh_width=2)
-        b.render(jobs_exact, flags)
+        # its own launch shape: the autotuner without the tolerance flag
+        # keeps width 2 and tunes the rest for it
+        be = cv.Batch()
+        exact_tuning, _ = be.autotune(jobs_exact, max_sample_paths=1 << 22)
+        log("batch exact: autotune chose %s" % (exact_tuning,))
+        be.render(jobs_exact, flags)
         cv.device_synchronize()
         t0 = time.perf_counter()
         for k in range(args.steps):
-            b.render(jobs_exact, flags)
+            be.render(jobs_exact, flags)
         cv.device_synchronize()
         exact_wall = (time.perf_counter() - t0) / args.steps
         log("batch on 64-bin trees at width 2: %.1f ms" % (exact_wall * 1e3))
-        b.set_tuning(bvh_width=tuning["bvh_width"])
+        del be
     del jobs_exact, gs_exact
     # the same jobs one after the other, one renderer per job
     rs = [cv.Renderer(g, w, h) for g, (_, _, w, h, _) in zip(gs, specs)]
@@ -308,7 +313,7 @@ def run_batch(args):
         "speedup_vs_sequential": round(seq / wall, 3),
         "bvh_width": tuning["bvh_width"],
         "exact": {"bvh_width": 2, "bins": 64, "paths_per_s": round(paths / exact_wall, 1),
-                  "ms": round(exact_wall * 1e3, 3),
+                  "ms": round(exact_wall * 1e3, 3), "autotune": exact_tuning,
                   "speedup_vs_sequential": round(seq / exact_wall, 3),
                   "bvh_build_ms_total": round(exact_build_ms, 1),
                   "note": "the reference's 64-bin trees and visiting order: every job's "

@@ -965,7 +965,7 @@ cv_status cv_batch_create(const cv_render_opts* opts, uint64_t max_paths_per_ste
     return fail(CV_ERR_INVALID_ARGUMENT, "max_paths_per_step must be below 2^31");
   cv_batch* b = new cv_batch();
   b->opts = o;
-  b->max_paths_per_step = max_paths_per_step ? max_paths_per_step : (1ull << 24);
+  b->max_paths_per_step = max_paths_per_step ? max_paths_per_step : (1ull << 27);
   cv_tuning t{8, 32, 16, 1, 1};
   cvh::apply_tuning(b->cfg, t);
   if ((cs = b->pipe.init()) != CV_OK) {

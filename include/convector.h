@@ -337,9 +337,12 @@ typedef struct cv_job {
                               normal xyz, hit distance (1e5 for sky), count) */
 } cv_job;
 
-/* max_paths_per_step bounds the path queues (0: 2^24); a step whose jobs
- * hold more pixels is split into several steps by jobs (a job larger than
- * the bound gets a step of its own).  opts may be NULL. */
+/* max_paths_per_step bounds the path queues (0: 2^27, about 17 GB of queues
+ * when a batch needs that many; queues grow only to the largest step a call
+ * renders); a step whose jobs hold more pixels is split into several steps
+ * by jobs (a job larger than the bound gets a step of its own).  Fewer,
+ * larger steps mean fewer traversal launches, each of which ends with a
+ * drain.  opts may be NULL. */
 cv_status cv_batch_create(const cv_render_opts* opts, uint64_t max_paths_per_step,
                           cv_batch** out);
 /* Launch shape of the batch's traversal (see cv_tuning). */
