@@ -217,11 +217,11 @@ class Scene:
 
     def set_tuning(self, stack_entries=0, extend_waves_per_cu=0, refill_threshold=0,
                    fetch_batches=0, block_waves=0, frames_per_step=0, refill_interval=0,
-                   leaf_batch=0, bvh_width=0, ray_sort_bits=0):
+                   leaf_batch=0, bvh_width=0, ray_sort_bits=0, queue_parts=0):
         """Launch shape of the traversal (the autotuner's knobs)."""
         t = _native.Tuning(stack_entries, extend_waves_per_cu, refill_threshold, fetch_batches,
                            block_waves, frames_per_step, refill_interval, leaf_batch,
-                           bvh_width, ray_sort_bits)
+                           bvh_width, ray_sort_bits, queue_parts)
         check(lib().cv_scene_set_tuning(self._h, ctypes.byref(t)))
 
     def tuning(self) -> dict:
@@ -378,10 +378,10 @@ class Batch:
 
     def set_tuning(self, stack_entries=0, extend_waves_per_cu=0, refill_threshold=0,
                    fetch_batches=0, block_waves=0, frames_per_step=0, refill_interval=0,
-                   leaf_batch=0, bvh_width=0, ray_sort_bits=0):
+                   leaf_batch=0, bvh_width=0, ray_sort_bits=0, queue_parts=0):
         t = _native.Tuning(stack_entries, extend_waves_per_cu, refill_threshold, fetch_batches,
                            block_waves, frames_per_step, refill_interval, leaf_batch,
-                           bvh_width, ray_sort_bits)
+                           bvh_width, ray_sort_bits, queue_parts)
         check(lib().cv_batch_set_tuning(self._h, ctypes.byref(t)))
 
     def tuning(self) -> dict:
@@ -541,10 +541,10 @@ class Server:
 
     def set_tuning(self, stack_entries=0, extend_waves_per_cu=0, refill_threshold=0,
                    fetch_batches=0, block_waves=0, frames_per_step=0, refill_interval=0,
-                   leaf_batch=0, bvh_width=0, ray_sort_bits=0):
+                   leaf_batch=0, bvh_width=0, ray_sort_bits=0, queue_parts=0):
         t = _native.Tuning(stack_entries, extend_waves_per_cu, refill_threshold, fetch_batches,
                            block_waves, frames_per_step, refill_interval, leaf_batch,
-                           bvh_width, ray_sort_bits)
+                           bvh_width, ray_sort_bits, queue_parts)
         check(lib().cv_server_set_tuning(self._h, ctypes.byref(t)))
 
     def submit(self, scene_id: int, width: int, height: int, spp: int, camera_pos, camera_q,

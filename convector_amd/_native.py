@@ -86,7 +86,8 @@ class Tuning(ctypes.Structure):
                 ("refill_threshold", ctypes.c_uint32), ("fetch_batches", ctypes.c_uint32),
                 ("block_waves", ctypes.c_uint32), ("frames_per_step", ctypes.c_uint32),
                 ("refill_interval", ctypes.c_uint32), ("leaf_batch", ctypes.c_uint32),
-                ("bvh_width", ctypes.c_uint32), ("ray_sort_bits", ctypes.c_uint32)]
+                ("bvh_width", ctypes.c_uint32), ("ray_sort_bits", ctypes.c_uint32),
+                ("queue_parts", ctypes.c_uint32)]
 
 
 class RenderOpts(ctypes.Structure):

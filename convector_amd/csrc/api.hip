@@ -106,6 +106,8 @@ cv_status validate_tuning(const cv_tuning& t) {
       (t.ray_sort_bits < 3 || t.ray_sort_bits > 24 || t.ray_sort_bits % 3))
     return fail(CV_ERR_INVALID_ARGUMENT,
                 "ray_sort_bits must be a multiple of 3 from 3 to 24, or CV_RAY_SORT_NONE");
+  if (t.queue_parts && t.queue_parts != 1 && t.queue_parts != cvk::kParts)
+    return fail(CV_ERR_INVALID_ARGUMENT, "queue_parts must be 1 or 8");
   return CV_OK;
 }
 
@@ -120,6 +122,7 @@ void apply_tuning(LaunchCfg& c, const cv_tuning& t) {
   if (t.leaf_batch) c.leaf_batch = t.leaf_batch;
   if (t.bvh_width) c.width = t.bvh_width;
   if (t.ray_sort_bits) c.sort_bits = t.ray_sort_bits == CV_RAY_SORT_NONE ? 0u : t.ray_sort_bits;
+  if (t.queue_parts) c.parts = t.queue_parts;
 }
 
 cv_tuning tuning_of(const LaunchCfg& c) {
@@ -134,6 +137,7 @@ cv_tuning tuning_of(const LaunchCfg& c) {
   t.leaf_batch = c.leaf_batch;
   t.bvh_width = c.width;
   t.ray_sort_bits = c.sort_bits ? c.sort_bits : CV_RAY_SORT_NONE;
+  t.queue_parts = c.parts;
   return t;
 }
 
@@ -776,6 +780,7 @@ cv_status cv_bvh_intersect_nearest(const cv_scene* sc, const cv_ray* rays, size_
     x.refill = s->cfg.refill <= 64 ? s->cfg.refill : 0u;
     x.interval = s->cfg.refill_interval;
     x.leaf_batch = s->cfg.leaf_batch <= 64 ? s->cfg.leaf_batch : 0u;
+    x.parts = s->cfg.parts;
     HIPTRY(cvk::launch_extend(x, false, s->packed, false, s->cfg.stack, s->cfg.blocks,
                               s->cfg.block_waves, st, s->cfg.width),
            "intersect");

@@ -93,6 +93,7 @@ struct LaunchCfg {
   uint32_t leaf_batch = 16;                   // 64ths of the active lanes per leaf round; 65 = off
   uint32_t width = 2;                         // BVH children per node: 2 (the reference's), 4, 8
   uint32_t sort_bits = 0;                     // ray-sort key bits after the camera rays (0: none)
+  uint32_t parts = 8;                         // queue partitions of the traversal: 1 or 8
 };
 
 }  // namespace cvh

@@ -18,6 +18,11 @@ constexpr uint32_t kShadeBlock = 1024;
 // Per-step control block in device memory: the step's path count and job
 // count (written by the host before each step), queue sizes and work cursors
 // per bounce, plus the optional traversal counters.
+// Queue partitions of the lane-refill traversal: one per XCD.  Each
+// partition's cursor has a 128-byte line of its own.
+constexpr uint32_t kParts = 8;
+constexpr uint32_t kPartStride = 32;
+
 struct Ctrl {
   uint32_t step_paths;
   uint32_t step_jobs;
@@ -27,6 +32,7 @@ struct Ctrl {
   unsigned long long sum_aabb;
   unsigned long long sum_tri;
   unsigned long long total_rays;
+  alignas(128) uint32_t part_cursor[kMaxBounceSlots][kParts * kPartStride];
 };
 
 // Paths in HBM as structure of arrays; every field is one 16-B or 8-B load.
@@ -95,6 +101,7 @@ struct ExtendArgs {
   const float* rays6;         // non-null (batched queries): the rays as n x 6 floats, not r0/r1
   const uint32_t* tri_src;    // non-null (batched queries): hits name source triangles
   const uint32_t* order;      // non-null: queue position k traverses path order[k]
+  uint32_t parts;             // lane refill: 1 (one cursor) or kParts queue partitions
 };
 
 // The ray sort before a bounce's traversal: a counting sort of the paths

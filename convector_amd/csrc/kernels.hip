@@ -285,11 +285,12 @@ __device__ __forceinline__ TravState traverse(const DevNode* __restrict__ nodes,
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
-// Wave-wide work fetch: lane 0 takes `count` entries from the cursor.
+// Wave-wide work fetch: lane 0 takes `count` entries from the cursor.  The
+// result is wave-uniform and held in a scalar register.
 __device__ __forceinline__ uint32_t fetch(uint32_t* cursor, uint32_t count) {
   uint32_t base = 0;
   if (lane_id() == 0) base = atomicAdd(cursor, count);
-  return __shfl(base, 0);
+  return __builtin_amdgcn_readfirstlane(__shfl(base, 0));
 }
 
 // One index below `limit` from a cursor that only grows, or `limit` when it
@@ -526,6 +527,8 @@ __global__ __launch_bounds__(256) void k_generate(GenerateArgs a) {
     a.ctrl->ext_cursor[threadIdx.x] = 0u;
     a.ctrl->shade_cursor[threadIdx.x] = 0u;
   }
+  if (blockIdx.x == 0 && threadIdx.x < kMaxBounceSlots * kParts)
+    a.ctrl->part_cursor[threadIdx.x / kParts][(threadIdx.x % kParts) * kPartStride] = 0u;
   for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < npaths; p += gridDim.x * blockDim.x) {
     const uint32_t j = job_of(a.jobs, njobs, p);
     const JobDesc& J = a.jobs[j];
@@ -790,8 +793,31 @@ __global__ __launch_bounds__(256) void k_extend_refill(ExtendArgs a) {
   const uint32_t stride = gridDim.x * blockDim.x;
   const uint32_t n = a.ctrl->count[a.bounce];
   if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&a.ctrl->total_rays, (unsigned long long)n);
-  uint32_t* cursor = &a.ctrl->ext_cursor[a.bounce];
   const uint64_t lt = (1ull << lane) - 1ull;
+  // Queue partitions (parts == kParts): the queue is cut into chunks of
+  // 2^glog rays, chunk c belonging to partition c % parts.  A wave takes
+  // rays from the partition of its XCD first, so that the rays in flight on
+  // one XCD, which has an L2 of its own, are neighbours in the queue (an
+  // image band, one job of a batch); when that partition is used up, from
+  // the others in turn.  Local position l of partition x is queue position
+  // ((l >> glog) * parts + x) << glog | (l & (2^glog - 1)).  parts == 1: one
+  // cursor, local position = queue position.
+  const uint32_t parts = a.parts == kParts ? kParts : 1u;
+  uint32_t glog = 31u - (uint32_t)__builtin_clz(max(n / parts, 1u));
+  glog = min(max(glog, 12u), 18u);
+  const uint32_t gmask = (1u << glog) - 1u;
+  auto part_size = [&](uint32_t x) -> uint32_t {
+    const uint32_t full = n >> glog;
+    const uint32_t c = full > x ? (full - x + parts - 1) / parts : 0u;
+    return (c << glog) + ((full & (parts - 1)) == x ? (n & gmask) : 0u);
+  };
+  auto part_cursor = [&](uint32_t x) -> uint32_t* {
+    return parts == 1 ? &a.ctrl->ext_cursor[a.bounce] : &a.ctrl->part_cursor[a.bounce][x * kPartStride];
+  };
+  uint32_t part = 0;
+  if (parts > 1)
+    part = __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & (parts - 1);  // HW_REG_XCC_ID
+  uint32_t visited = 0;
   Stack<S, PACKED> st(lds + lane, a.spill + gtid, stride, 0);
   TravState ts;
   V3 o = mk3(0, 0, 0), d = mk3(1, 0, 0), inv = mk3(1, 0, 0), oi = mk3(0, 0, 0);
@@ -801,11 +827,12 @@ __global__ __launch_bounds__(256) void k_extend_refill(ExtendArgs a) {
   entry_t cur = 0;
   bool have = false;
   bool active = false, exact = false, waiting = false;
-  // The wave's reserved rays [pool, pool_end): the cursor hands out chunks
-  // of max(idle lanes, 64 * batches) rays, so most refills take no atomic
-  // and a low refill threshold does not contend on the one cursor.
-  // `drained`: the cursor has passed the last ray.  (Wave-uniform values.)
-  uint32_t pool = 0, pool_end = 0;
+  // The wave's reserved rays [pool, pool_end) of partition pool_part: a
+  // cursor hands out chunks of max(idle lanes, 64 * batches) rays, so most
+  // refills take no atomic and a low refill threshold does not contend on
+  // a cursor.  `drained`: the cursors have passed the last ray of every
+  // partition.  (Wave-uniform values.)
+  uint32_t pool = 0, pool_end = 0, pool_part = 0;
   bool drained = false;
   const uint32_t chunk = 64u * (a.batches ? a.batches : 1u);
   uint64_t na = 0, nt = 0;
@@ -814,22 +841,39 @@ __global__ __launch_bounds__(256) void k_extend_refill(ExtendArgs a) {
     const uint32_t nidle = __popcll(idle);
     if ((pool < pool_end || !drained) && (nidle >= a.refill || nidle == 64)) {
       const uint32_t from_pool = min(nidle, pool_end - pool);
-      const uint32_t base0 = pool;
+      const uint32_t base0 = pool, part0 = pool_part;
       pool += from_pool;
       const uint32_t need = nidle - from_pool;
       uint32_t base1 = 0, end1 = 0;
       if (need && !drained) {
         const uint32_t sz = max(need, chunk);
-        base1 = fetch(cursor, sz);
-        end1 = base1 >= n ? base1 : min(base1 + sz, n);
-        if (base1 + sz >= n) drained = true;
+        // at most one fetch per partition finds it used up
+        for (;;) {
+          const uint32_t size = part_size(part);
+          base1 = size ? fetch(part_cursor(part), sz) : 0u;
+          const bool got = base1 < size;
+          if (got) {
+            end1 = min(base1 + sz, size);
+            pool_part = part;
+          }
+          if (!got || base1 + sz >= size) {
+            // this partition is used up: the next fetch goes to the next one
+            if (++visited == parts) drained = true;
+            part = (part + 1) & (parts - 1);
+          }
+          if (got || drained) break;
+        }
+        if (end1 == 0) base1 = 0;
         pool = min(base1 + need, end1);
         pool_end = end1;
       }
       if (!active) {
         const uint32_t rank = __popcll(idle & lt);
-        const uint32_t k = rank < from_pool ? base0 + rank : base1 + (rank - from_pool);
-        if (rank < from_pool || k < end1) {
+        const bool old = rank < from_pool;
+        const uint32_t l = old ? base0 + rank : base1 + (rank - from_pool);
+        if (old || l < end1) {
+          const uint32_t x = old ? part0 : pool_part;
+          const uint32_t k = parts == 1 ? l : (((l >> glog) * parts + x) << glog) | (l & gmask);
           const uint32_t i = a.order ? a.order[k] : k;
           if (MULTI) {
             const float4 r0 = a.r0[i], r1 = a.r1[i];

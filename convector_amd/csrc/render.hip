@@ -69,7 +69,7 @@ struct RunOpts {
 struct GraphKey {
   int stack = 0;
   uint32_t blocks = 0, refill = 0, fetch_batches = 0, max_bounces = 0, block_waves = 0;
-  uint32_t interval = 0, leaf_batch = 0, width = 2, sort_bits = 0;
+  uint32_t interval = 0, leaf_batch = 0, width = 2, sort_bits = 0, parts = 1;
   bool packed = false, count = false, multi_job = false;
   const void* nodes = nullptr;
   const void* tris = nullptr;
@@ -80,7 +80,7 @@ struct GraphKey {
            fetch_batches == o.fetch_batches && max_bounces == o.max_bounces &&
            packed == o.packed && count == o.count && multi_job == o.multi_job &&
            nodes == o.nodes && tris == o.tris && capacity == o.capacity && width == o.width &&
-           sort_bits == o.sort_bits;
+           sort_bits == o.sort_bits && parts == o.parts;
   }
 };
 
@@ -325,6 +325,7 @@ cv_status launch_step(Pipeline& p, const RunOpts& o, uint32_t slot, uint32_t pat
     x.refill = o.cfg.refill <= 64 ? o.cfg.refill : (multi_scene ? 64u : 0u);
     x.interval = o.cfg.refill_interval;
     x.leaf_batch = o.cfg.leaf_batch <= 64 ? o.cfg.leaf_batch : 0u;
+    x.parts = o.cfg.parts;
     if (o.cfg.sort_bits && b > 0) {
       // after the camera rays: traverse the bounce's paths in key order
       cvk::SortArgs so{};
@@ -398,6 +399,7 @@ cv_status pipeline_run(Pipeline& p, const std::vector<StepPlan>& steps, const Ru
   key.leaf_batch = o.cfg.leaf_batch;
   key.width = o.cfg.width;
   key.sort_bits = o.cfg.sort_bits;
+  key.parts = o.cfg.parts;
   key.fetch_batches = o.cfg.fetch_batches;
   key.max_bounces = o.max_bounces;
   key.packed = o.packed;
@@ -1250,6 +1252,7 @@ cv_status cv_batch_autotune(cv_batch* b, const cv_job* jobs, size_t n, uint64_t 
   static const uint32_t leaf_batches[] = {8, 16, 24, 65};
   static const uint32_t widths[] = {2, 4, 8};
   static const uint32_t sort_bits[] = {0, 12, 24};
+  static const uint32_t parts[] = {1, cvk::kParts};
   cvh::LaunchCfg best = b->cfg;
   if (!wide_ok) best.width = 2;
   double best_ms = 0;
@@ -1301,6 +1304,13 @@ cv_status cv_batch_autotune(cv_batch* b, const cv_job* jobs, size_t n, uint64_t 
     for (uint32_t v : sort_bits) {
       cvh::LaunchCfg c = best;
       c.sort_bits = v;
+      double ms;
+      if ((cs = time_cfg(c, &ms)) != CV_OK) return cs;
+      if (ms < best_ms) best = c, best_ms = ms;
+    }
+    for (uint32_t v : parts) {
+      cvh::LaunchCfg c = best;
+      c.parts = v;
       double ms;
       if ((cs = time_cfg(c, &ms)) != CV_OK) return cs;
       if (ms < best_ms) best = c, best_ms = ms;

@@ -167,6 +167,13 @@ typedef struct cv_tuning {
                                      multi-job step; CV_RAY_SORT_NONE (the default): queue
                                      order.  Only the order of the work changes: results are
                                      the same bits */
+    uint32_t queue_parts;         /* traversal: 1, every wave takes rays from one queue
+                                     cursor; 8, the queue is cut into chunks of 2^12..2^18
+                                     rays dealt to 8 partitions in turn, and a wave takes rays
+                                     from the partition of its XCD (each XCD has its own L2)
+                                     until it is used up, then from the others.  Only which
+                                     lane traverses which ray changes: results are the same
+                                     bits */
 } cv_tuning;
 #define CV_FRAMES_AUTO 255u
 #define CV_RAY_SORT_NONE 255u
