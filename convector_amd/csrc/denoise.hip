@@ -225,6 +225,11 @@ __global__ __launch_bounds__(256) void k_conv_lds(ConvArgs a) {
 // operand, so each lane ends with 4 consecutive output channels of one
 // pixel and stores them with one 8-byte (bf16) or 16-byte (f32) access.
 // FLIP mirrors the taps (backward to the input, weights Wt).
+// When the input is one chunk (KC <= 32) nothing would be in flight during
+// the MFMAs, so a block is persistent (unless the epilogue reads a mask):
+// it loads the weights and bias once, then walks tiles blockIdx.x,
+// blockIdx.x + gridDim.x, ..., loading the next tile's halo into registers
+// while it computes the current one.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kHaloW = 32, kHaloH = 8;
 
@@ -245,25 +250,38 @@ __global__ __launch_bounds__(256, 2) void k_conv_halo(ConvArgs a) {
   static_assert(KC % CC == 0, "channels per chunk");
   __shared__ __attribute__((aligned(16))) uint16_t xs[HH * HW * RS];
   __shared__ __attribute__((aligned(16))) uint16_t ws[KS * WR * WS];
+  // (a masked epilogue reads the mask from global memory: behind the
+  // prefetch, whose loads complete first, it would wait for all of them)
+  constexpr bool PERSIST = NCH == 1 && !MASK;
+  // persistent blocks: the slice's bias, read in the epilogue from LDS so
+  // that it does not wait behind the next tile's global loads
+  __shared__ float bias_s[PERSIST ? WR : 1];
   const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const uint32_t ntx = a.w_ / kHaloW, nty = (a.h + kHaloH - 1) / kHaloH;
-  const uint32_t txi = blockIdx.x % ntx, tyi = (blockIdx.x / ntx) % nty;
-  const uint32_t img = blockIdx.x / (ntx * nty);
-  const int x0 = (int)(txi * kHaloW), y0 = (int)(tyi * kHaloH);
-  const size_t img_base = (size_t)img * a.h * a.w_;
+  const uint32_t ntiles = (a.npix / (a.h * a.w_)) * ntx * nty;
+  uint32_t tile = blockIdx.x;
+  int x0 = 0, y0 = 0;
+  size_t img_base = 0;
   const uint32_t row0 = blockIdx.y * WR;  // first output of the slice
   // this thread's input pieces: element offset of chunk 0, or ~0u outside
   // the image; its weight pieces: element offset of chunk 0
   uint32_t xoff[XPT], woff[WPT];
+  // tile t's corner and its input pieces' offsets
+  auto tile_offsets = [&](uint32_t t, int& tx0, int& ty0, size_t& tbase) {
+    tx0 = (int)((t % ntx) * kHaloW);
+    ty0 = (int)(((t / ntx) % nty) * kHaloH);
+    tbase = (size_t)(t / (ntx * nty)) * a.h * a.w_;
 #pragma unroll
-  for (int i = 0; i < XPT; i++) {
-    const int q = (int)tid + 256 * i;
-    const int pix = q / PP, part = q % PP;
-    const int sy = y0 + pix / HW - 1, sx = x0 + pix % HW - 1;
-    xoff[i] = (q < XP && sy >= 0 && sy < (int)a.h && sx >= 0 && sx < (int)a.w_)
-                  ? (uint32_t)((img_base + (size_t)sy * a.w_ + (uint32_t)sx) * a.xcs) + part * 8
-                  : ~0u;
-  }
+    for (int i = 0; i < XPT; i++) {
+      const int q = (int)tid + 256 * i;
+      const int pix = q / PP, part = q % PP;
+      const int sy = ty0 + pix / HW - 1, sx = tx0 + pix % HW - 1;
+      xoff[i] = (q < XP && sy >= 0 && sy < (int)a.h && sx >= 0 && sx < (int)a.w_)
+                    ? (uint32_t)((tbase + (size_t)sy * a.w_ + (uint32_t)sx) * a.xcs) + part * 8
+                    : ~0u;
+    }
+  };
+  tile_offsets(tile, x0, y0, img_base);
 #pragma unroll
   for (int i = 0; i < WPT; i++) {
     const int q = min((int)tid + 256 * i, WP - 1);
@@ -281,131 +299,173 @@ __global__ __launch_bounds__(256, 2) void k_conv_halo(ConvArgs a) {
     _Pragma("unroll") for (int i = 0; i < WPT; i++) wr[i] =                               \
         *reinterpret_cast<const u32x4*>(a.w + woff[i] + (c) * 32);                        \
   }
-#define CV_HALO_STORE()                                                                   \
+#define CV_HALO_LOAD_X()                                                                  \
+  {                                                                                       \
+    _Pragma("unroll") for (int i = 0; i < XPT; i++) xr[i] =                               \
+        xoff[i] != ~0u ? *reinterpret_cast<const u32x4*>(a.x + xoff[i])                    \
+                       : u32x4{0u, 0u, 0u, 0u};                                           \
+  }
+#define CV_HALO_STORE_X()                                                                 \
   {                                                                                       \
     _Pragma("unroll") for (int i = 0; i < XPT; i++) {                                     \
       const int q = (int)tid + 256 * i;                                                   \
       if (q < XP) *reinterpret_cast<u32x4*>(&xs[(q / PP) * RS + (q % PP) * 8]) = xr[i];   \
     }                                                                                     \
+  }
+#define CV_HALO_STORE()                                                                   \
+  {                                                                                       \
+    CV_HALO_STORE_X();                                                                    \
     _Pragma("unroll") for (int i = 0; i < WPT; i++) {                                     \
       const int q = (int)tid + 256 * i;                                                   \
       if (q < WP) *reinterpret_cast<u32x4*>(&ws[(q / 4) * WS + (q % 4) * 8]) = wr[i];     \
     }                                                                                     \
   }
-  f4 acc[MF][NS];
-#pragma unroll
-  for (int m = 0; m < MF; m++)
-#pragma unroll
-    for (int t = 0; t < NS; t++) acc[m][t] = f4{0.0f, 0.0f, 0.0f, 0.0f};
   const uint32_t col = lane & 15, kq = lane >> 4;
   // the wave's fragments: m -> row 2 * wave + m / 2, pixels 16 * (m % 2) + col
   const int wrow = 2 * (int)wave;
   const bf8 zero8 = {};
   CV_HALO_LOAD(0);
   CV_HALO_STORE();
-  __syncthreads();
-#pragma unroll 1
-  for (int c = 0; c < NCH; c++) {
-    if (c + 1 < NCH) CV_HALO_LOAD(c + 1);
-#pragma unroll
-    for (int st = 0; st < KS; st++) {
-      // the lane's tap: with KC == 16 the upper 32 lanes take the odd one
-      // (the tenth reads the centre and is zeroed)
-      const int tp = KC == 16 ? 2 * st + (int)(kq >> 1) : st;
-      const int tq = tp > 8 ? 4 : tp;
-      int dy = tq / 3 - 1, dx = tq % 3 - 1;
-      if (FLIP) {
-        dy = -dy;
-        dx = -dx;
-      }
-      const int coff = KC == 16 ? 8 * (int)(kq & 1) : 8 * (int)kq;
-      bf8 Wf[NS];
-#pragma unroll
-      for (int t = 0; t < NS; t++)
-        Wf[t] = *reinterpret_cast<const bf8*>(&ws[(st * WR + t * 16 + (int)col) * WS + 8 * kq]);
-#pragma unroll
-      for (int m = 0; m < MF; m++) {
-        const int hy = wrow + m / 2 + 1 + dy, hx = 16 * (m % 2) + (int)col + 1 + dx;
-        bf8 X = *reinterpret_cast<const bf8*>(&xs[(hy * HW + hx) * RS + coff]);
-        if (KC == 16 && tp > 8) X = zero8;
-#pragma unroll
-        for (int t = 0; t < NS; t++)
-          acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Wf[t], X, acc[m][t], 0, 0, 0);
-      }
-    }
-    if (c + 1 < NCH) {
-      __syncthreads();
-      CV_HALO_STORE();
-      __syncthreads();
-    }
+  if constexpr (PERSIST) {
+    if (tid < (uint32_t)WR)
+      bias_s[tid] = (a.bias && row0 + tid < a.nout) ? a.bias[row0 + tid] : 0.0f;
   }
-#undef CV_HALO_LOAD
-#undef CV_HALO_STORE
+  __syncthreads();
   // lane: output channels row0 + t * 16 + 4 * kq + r (r = 0..3) of pixel
   // col of each fragment
   const bool vec_y = a.y && ((uintptr_t)a.y % 8 == 0) && (a.ycs % 4 == 0);
   const bool vec_f = a.yf && ((uintptr_t)a.yf % 16 == 0) && (a.yfcs % 4 == 0);
+#pragma unroll 1
+  for (;;) {
+    f4 acc[MF][NS];
 #pragma unroll
-  for (int m = 0; m < MF; m++) {
-    const int py = y0 + wrow + m / 2;
-    if (py >= (int)a.h) continue;
-    const size_t p = img_base + (size_t)py * a.w_ + (uint32_t)x0 + 16 * (m % 2) + col;
+    for (int m = 0; m < MF; m++)
 #pragma unroll
-    for (int t = 0; t < NS; t++) {
-      const uint32_t j0 = row0 + t * 16 + 4 * kq;
-      if (j0 >= a.nout) continue;
-      float v[4];
+      for (int t = 0; t < NS; t++) acc[m][t] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+    // persistent blocks: the next tile's halo is loaded during this one's MFMAs
+    const uint32_t next = tile + gridDim.x;
+    const bool more = PERSIST && next < ntiles;
+    int nx0 = 0, ny0 = 0;
+    size_t nbase = 0;
+    if (more) {
+      tile_offsets(next, nx0, ny0, nbase);
+      CV_HALO_LOAD_X();
+    }
+#pragma unroll 1
+    for (int c = 0; c < NCH; c++) {
+      if (c + 1 < NCH) CV_HALO_LOAD(c + 1);
 #pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const uint32_t j = j0 + r;
-        v[r] = acc[m][t][r] + ((a.bias && j < a.nout) ? a.bias[j] : 0.0f);
-        if (a.relu) v[r] = fmaxf(v[r], 0.0f);
-      }
-      const bool whole = j0 + 4 <= a.nout;
-      if constexpr (MASK) {
-        // the ReLU gradient of the layer below: 4 bf16 of its output at once
-        // (the host checks the mask's alignment and stride)
-        if (whole) {
-          const uint2 mk = *reinterpret_cast<const uint2*>(a.mask + p * a.mcs + j0);
-          const uint32_t mw[4] = {mk.x << 16, mk.x & 0xffff0000u, mk.y << 16, mk.y & 0xffff0000u};
+      for (int st = 0; st < KS; st++) {
+        // the lane's tap: with KC == 16 the upper 32 lanes take the odd one
+        // (the tenth reads the centre and is zeroed)
+        const int tp = KC == 16 ? 2 * st + (int)(kq >> 1) : st;
+        const int tq = tp > 8 ? 4 : tp;
+        int dy = tq / 3 - 1, dx = tq % 3 - 1;
+        if (FLIP) {
+          dy = -dy;
+          dx = -dx;
+        }
+        const int coff = KC == 16 ? 8 * (int)(kq & 1) : 8 * (int)kq;
+        bf8 Wf[NS];
 #pragma unroll
-          for (int r = 0; r < 4; r++)
-            if (!(__uint_as_float(mw[r]) > 0.0f)) v[r] = 0.0f;
-        } else {
-          for (int r = 0; r < 4; r++)
-            if (j0 + r < a.nout && !(from_bf16(a.mask[p * a.mcs + j0 + r]) > 0.0f)) v[r] = 0.0f;
+        for (int t = 0; t < NS; t++)
+          Wf[t] = *reinterpret_cast<const bf8*>(
+              &ws[(st * WR + t * 16 + (int)col) * WS + 8 * kq]);
+#pragma unroll
+        for (int m = 0; m < MF; m++) {
+          const int hy = wrow + m / 2 + 1 + dy, hx = 16 * (m % 2) + (int)col + 1 + dx;
+          bf8 X = *reinterpret_cast<const bf8*>(&xs[(hy * HW + hx) * RS + coff]);
+          if (KC == 16 && tp > 8) X = zero8;
+#pragma unroll
+          for (int t = 0; t < NS; t++)
+            acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Wf[t], X, acc[m][t], 0, 0, 0);
         }
       }
-      if (a.y) {
-        if (whole && vec_y) {
-          uint2 pk;
-          pk.x = pack_bf16x2(v[0], v[1]);
-          pk.y = pack_bf16x2(v[2], v[3]);
-          *reinterpret_cast<uint2*>(a.y + p * a.ycs + j0) = pk;
-        } else {
-          for (int r = 0; r < 4; r++)
-            if (j0 + r < a.nout) a.y[p * a.ycs + j0 + r] = to_bf16(v[r]);
-        }
+      if (c + 1 < NCH) {
+        __syncthreads();
+        CV_HALO_STORE();
+        __syncthreads();
       }
-      if (a.yf) {
-        float* o = a.yf + p * a.yfcs + j0;
-        if (whole && vec_f) {
-          float4 w4 = make_float4(v[0], v[1], v[2], v[3]);
-          if (a.accumulate) {
-            const float4 old = *reinterpret_cast<const float4*>(o);
-            w4.x += old.x;
-            w4.y += old.y;
-            w4.z += old.z;
-            w4.w += old.w;
+    }
+#pragma unroll
+    for (int m = 0; m < MF; m++) {
+      const int py = y0 + wrow + m / 2;
+      if (py >= (int)a.h) continue;
+      const size_t p = img_base + (size_t)py * a.w_ + (uint32_t)x0 + 16 * (m % 2) + col;
+#pragma unroll
+      for (int t = 0; t < NS; t++) {
+        const uint32_t j0 = row0 + t * 16 + 4 * kq;
+        if (j0 >= a.nout) continue;
+        float v[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const uint32_t j = j0 + r;
+          if constexpr (PERSIST)
+            v[r] = acc[m][t][r] + bias_s[j - row0];
+          else
+            v[r] = acc[m][t][r] + ((a.bias && j < a.nout) ? a.bias[j] : 0.0f);
+          if (a.relu) v[r] = fmaxf(v[r], 0.0f);
+        }
+        const bool whole = j0 + 4 <= a.nout;
+        if constexpr (MASK) {
+          // the ReLU gradient of the layer below: 4 bf16 of its output at once
+          // (the host checks the mask's alignment and stride)
+          if (whole) {
+            const uint2 mk = *reinterpret_cast<const uint2*>(a.mask + p * a.mcs + j0);
+            const uint32_t mw[4] = {mk.x << 16, mk.x & 0xffff0000u, mk.y << 16,
+                                    mk.y & 0xffff0000u};
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+              if (!(__uint_as_float(mw[r]) > 0.0f)) v[r] = 0.0f;
+          } else {
+            for (int r = 0; r < 4; r++)
+              if (j0 + r < a.nout && !(from_bf16(a.mask[p * a.mcs + j0 + r]) > 0.0f))
+                v[r] = 0.0f;
           }
-          *reinterpret_cast<float4*>(o) = w4;
-        } else {
-          for (int r = 0; r < 4; r++)
-            if (j0 + r < a.nout) o[r] = a.accumulate ? o[r] + v[r] : v[r];
+        }
+        if (a.y) {
+          if (whole && vec_y) {
+            uint2 pk;
+            pk.x = pack_bf16x2(v[0], v[1]);
+            pk.y = pack_bf16x2(v[2], v[3]);
+            *reinterpret_cast<uint2*>(a.y + p * a.ycs + j0) = pk;
+          } else {
+            for (int r = 0; r < 4; r++)
+              if (j0 + r < a.nout) a.y[p * a.ycs + j0 + r] = to_bf16(v[r]);
+          }
+        }
+        if (a.yf) {
+          float* o = a.yf + p * a.yfcs + j0;
+          if (whole && vec_f) {
+            float4 w4 = make_float4(v[0], v[1], v[2], v[3]);
+            if (a.accumulate) {
+              const float4 old = *reinterpret_cast<const float4*>(o);
+              w4.x += old.x;
+              w4.y += old.y;
+              w4.z += old.z;
+              w4.w += old.w;
+            }
+            *reinterpret_cast<float4*>(o) = w4;
+          } else {
+            for (int r = 0; r < 4; r++)
+              if (j0 + r < a.nout) o[r] = a.accumulate ? o[r] + v[r] : v[r];
+          }
         }
       }
     }
+    if (!more) break;
+    __syncthreads();
+    CV_HALO_STORE_X();
+    __syncthreads();
+    tile = next;
+    x0 = nx0;
+    y0 = ny0;
+    img_base = nbase;
   }
+#undef CV_HALO_LOAD
+#undef CV_HALO_LOAD_X
+#undef CV_HALO_STORE
+#undef CV_HALO_STORE_X
 }
 
 // ---------------------------------------------------------------------------
@@ -1137,14 +1197,39 @@ namespace {
 // pixels, the input channels a multiple of 16 in a dense channel range):
 // NT output tiles in slices of NS per block (grid.y = NT / NS), at most 4,
 // so that a chunk's weights and tile fit in LDS beside other blocks.
+// Workgroups of a halo-tile kernel resident on the device at once (CUs x
+// blocks per CU), taken once per instantiation.
+template <int KC, int NS, bool FLIP, bool MASK>
+uint32_t halo_resident_blocks() {
+  static const uint32_t n = [] {
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per, cvd::k_conv_halo<KC, NS, FLIP, MASK>, 256, 0) != hipSuccess ||
+        cus <= 0 || per <= 0)
+      return 512u;
+    return (uint32_t)(cus * per);
+  }();
+  return n;
+}
+
 template <int KC, int NS, bool FLIP, bool MASK = false>
 bool try_halo(cv_denoiser* d, const cvd::ConvArgs& a, uint32_t nt, uint32_t ns) {
   if (a.kc != (uint32_t)KC || ns != (uint32_t)NS || (a.flip != 0) != FLIP ||
       (a.mask != nullptr) != MASK)
     return false;
   const uint32_t imgs = a.npix / (a.h * a.w_);
-  const uint32_t blocks =
+  const uint32_t tiles =
       imgs * (a.w_ / cvd::kHaloW) * ((a.h + cvd::kHaloH - 1) / cvd::kHaloH);
+  uint32_t blocks = tiles;
+  if (KC <= 32 && !MASK) {
+    // persistent blocks (one input chunk): as many as are resident with
+    // every slice's, each taking an equal share of the tiles
+    const uint32_t cap = std::max(1u, halo_resident_blocks<KC, NS, FLIP, MASK>() / (nt / NS));
+    const uint32_t per = (tiles + cap - 1) / cap;
+    blocks = (tiles + per - 1) / per;
+  }
   hipLaunchKernelGGL((cvd::k_conv_halo<KC, NS, FLIP, MASK>), dim3(blocks, nt / NS), dim3(256), 0,
                      d->stream, a);
   return true;
