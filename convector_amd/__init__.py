@@ -456,13 +456,16 @@ class Denoiser:
         assert p.size == self.num_params
         check(lib().cv_denoiser_set_params(self._h, _p(p)))
 
-    def forward(self, inp: np.ndarray) -> np.ndarray:
-        """inp: (nb, h, w, 10) float32 -> (nb, h, w, 3) log1p radiance."""
+    def forward(self, inp: np.ndarray, materialize: bool = False) -> np.ndarray:
+        """inp: (nb, h, w, 10) float32 -> (nb, h, w, 3) log1p radiance.
+        materialize: write the upsampled decoder inputs out first, as a
+        training step does (CV_DENOISE_MATERIALIZE; the same outputs)."""
         inp = np.ascontiguousarray(inp, np.float32)
         nb, h, w, c = inp.shape
         assert c == self.NUM_INPUT
         out = np.zeros((nb, h, w, 3), np.float32)
-        check(lib().cv_denoiser_forward(self._h, _p(inp), _p(out), nb, h, w, 0))
+        flags = _native.CV_DENOISE_MATERIALIZE if materialize else 0
+        check(lib().cv_denoiser_forward(self._h, _p(inp), _p(out), nb, h, w, flags))
         return out
 
     def train_step(self, inp: np.ndarray, target: np.ndarray, lr: float) -> float:

@@ -21,6 +21,7 @@ STATUS_NAMES = {
 }
 
 CV_PTR_DEVICE = 1
+CV_DENOISE_MATERIALIZE = 2
 CV_NOALLOC_MAX_DEPTH = 44
 CV_NOALLOC_SCRATCH_BYTES = 8192 + 16 * 1024 * 1024
 CV_RENDER_COUNT = 2

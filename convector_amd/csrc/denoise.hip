@@ -83,6 +83,11 @@ struct ConvArgs {
   // pixel stride mcs) is: outputs where mask <= 0 become 0 (written to y)
   const uint16_t* mask;
   uint32_t mcs;
+  // non-null (k_conv_halo<..., UP>): input channels [0, 32 * up_chunks) are
+  // the nearest 2x upsampling of xup (bf16, pixel stride xupcs, h / 2 x
+  // w / 2 per image), read from it directly; the rest come from x
+  const uint16_t* xup;
+  uint32_t xupcs, up_chunks;
 };
 
 // The convolution with the operands staged in LDS: a block of 4 waves
@@ -233,7 +238,7 @@ __global__ __launch_bounds__(256) void k_conv_lds(ConvArgs a) {
 // ---------------------------------------------------------------------------
 constexpr uint32_t kHaloW = 32, kHaloH = 8;
 
-template <int KC, int NS, bool FLIP, bool MASK = false>
+template <int KC, int NS, bool FLIP, bool MASK = false, bool UP = false>
 __global__ __launch_bounds__(256, 2) void k_conv_halo(ConvArgs a) {
   constexpr int MF = 4;
   constexpr int HH = (int)kHaloH + 2, HW = (int)kHaloW + 2;
@@ -282,6 +287,15 @@ __global__ __launch_bounds__(256, 2) void k_conv_halo(ConvArgs a) {
     }
   };
   tile_offsets(tile, x0, y0, img_base);
+  // UP: offset in xup of input piece i of the current tile (inside the
+  // image), recomputed at each load rather than held in registers
+  auto up_offset = [&](int i) -> uint32_t {
+    const int q = (int)tid + 256 * i;
+    const int pix = q / PP, part = q % PP;
+    const uint32_t sy = (uint32_t)(y0 + pix / HW - 1), sx = (uint32_t)(x0 + pix % HW - 1);
+    return (uint32_t)((img_base / 4 + (size_t)(sy >> 1) * (a.w_ >> 1) + (sx >> 1)) * a.xupcs) +
+           part * 8;
+  };
 #pragma unroll
   for (int i = 0; i < WPT; i++) {
     const int q = min((int)tid + 256 * i, WP - 1);
@@ -293,9 +307,15 @@ __global__ __launch_bounds__(256, 2) void k_conv_halo(ConvArgs a) {
   u32x4 xr[XPT], wr[WPT];
 #define CV_HALO_LOAD(c)                                                                   \
   {                                                                                       \
-    _Pragma("unroll") for (int i = 0; i < XPT; i++) xr[i] =                               \
-        xoff[i] != ~0u ? *reinterpret_cast<const u32x4*>(a.x + xoff[i] + (c) * CC)         \
-                       : u32x4{0u, 0u, 0u, 0u};                                           \
+    if (UP && (uint32_t)(c) < a.up_chunks) {                                              \
+      _Pragma("unroll") for (int i = 0; i < XPT; i++) xr[i] =                             \
+          xoff[i] != ~0u ? *reinterpret_cast<const u32x4*>(a.xup + up_offset(i) + (c) * CC) \
+                         : u32x4{0u, 0u, 0u, 0u};                                         \
+    } else {                                                                              \
+      _Pragma("unroll") for (int i = 0; i < XPT; i++) xr[i] =                             \
+          xoff[i] != ~0u ? *reinterpret_cast<const u32x4*>(a.x + xoff[i] + (c) * CC)       \
+                         : u32x4{0u, 0u, 0u, 0u};                                         \
+    }                                                                                     \
     _Pragma("unroll") for (int i = 0; i < WPT; i++) wr[i] =                               \
         *reinterpret_cast<const u32x4*>(a.w + woff[i] + (c) * 32);                        \
   }
@@ -1199,14 +1219,14 @@ namespace {
 // so that a chunk's weights and tile fit in LDS beside other blocks.
 // Workgroups of a halo-tile kernel resident on the device at once (CUs x
 // blocks per CU), taken once per instantiation.
-template <int KC, int NS, bool FLIP, bool MASK>
+template <int KC, int NS, bool FLIP, bool MASK, bool UP>
 uint32_t halo_resident_blocks() {
   static const uint32_t n = [] {
     int dev = 0, cus = 0, per = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
         hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per, cvd::k_conv_halo<KC, NS, FLIP, MASK>, 256, 0) != hipSuccess ||
+            &per, cvd::k_conv_halo<KC, NS, FLIP, MASK, UP>, 256, 0) != hipSuccess ||
         cus <= 0 || per <= 0)
       return 512u;
     return (uint32_t)(cus * per);
@@ -1214,10 +1234,10 @@ uint32_t halo_resident_blocks() {
   return n;
 }
 
-template <int KC, int NS, bool FLIP, bool MASK = false>
+template <int KC, int NS, bool FLIP, bool MASK = false, bool UP = false>
 bool try_halo(cv_denoiser* d, const cvd::ConvArgs& a, uint32_t nt, uint32_t ns) {
   if (a.kc != (uint32_t)KC || ns != (uint32_t)NS || (a.flip != 0) != FLIP ||
-      (a.mask != nullptr) != MASK)
+      (a.mask != nullptr) != MASK || (a.xup != nullptr) != UP)
     return false;
   const uint32_t imgs = a.npix / (a.h * a.w_);
   const uint32_t tiles =
@@ -1226,12 +1246,13 @@ bool try_halo(cv_denoiser* d, const cvd::ConvArgs& a, uint32_t nt, uint32_t ns) 
   if (KC <= 32 && !MASK) {
     // persistent blocks (one input chunk): as many as are resident with
     // every slice's, each taking an equal share of the tiles
-    const uint32_t cap = std::max(1u, halo_resident_blocks<KC, NS, FLIP, MASK>() / (nt / NS));
+    const uint32_t cap =
+        std::max(1u, halo_resident_blocks<KC, NS, FLIP, MASK, UP>() / (nt / NS));
     const uint32_t per = (tiles + cap - 1) / cap;
     blocks = (tiles + per - 1) / per;
   }
-  hipLaunchKernelGGL((cvd::k_conv_halo<KC, NS, FLIP, MASK>), dim3(blocks, nt / NS), dim3(256), 0,
-                     d->stream, a);
+  hipLaunchKernelGGL((cvd::k_conv_halo<KC, NS, FLIP, MASK, UP>), dim3(blocks, nt / NS), dim3(256),
+                     0, d->stream, a);
   return true;
 }
 
@@ -1241,6 +1262,10 @@ bool launch_conv_halo(cv_denoiser* d, const cvd::ConvArgs& a, uint32_t nt) {
   if (a.w_ % cvd::kHaloW || a.npix % (a.h * a.w_) || a.xcs % 8 || a.xcs < a.kc) return false;
   // a fused ReLU mask is read 4 channels at a time
   if (a.mask && (a.mcs % 4 || (uintptr_t)a.mask % 8)) return false;
+  // a fused upsampling: even sizes, whole 16-byte pieces of xup
+  if (a.xup && (a.h % 2 || a.w_ % 2 || a.xupcs % 8 || a.xupcs < 32 * a.up_chunks ||
+                32 * a.up_chunks > a.kc))
+    return false;
   const uint32_t ns = nt % 4 == 0 ? 4 : nt % 2 == 0 ? 2 : nt % 3 == 0 ? 3 : 1;
   return try_halo<16, 2, false>(d, a, nt, ns) || try_halo<32, 1, false>(d, a, nt, ns) ||
          try_halo<32, 2, false>(d, a, nt, ns) || try_halo<32, 4, false>(d, a, nt, ns) ||
@@ -1250,7 +1275,9 @@ bool launch_conv_halo(cv_denoiser* d, const cvd::ConvArgs& a, uint32_t nt) {
          try_halo<32, 3, true>(d, a, nt, ns) || try_halo<64, 2, true>(d, a, nt, ns) ||
          try_halo<64, 4, true>(d, a, nt, ns) || try_halo<128, 4, true>(d, a, nt, ns) ||
          try_halo<16, 2, true, true>(d, a, nt, ns) || try_halo<32, 2, true, true>(d, a, nt, ns) ||
-         try_halo<64, 4, true, true>(d, a, nt, ns) || try_halo<128, 4, true, true>(d, a, nt, ns);
+         try_halo<64, 4, true, true>(d, a, nt, ns) || try_halo<128, 4, true, true>(d, a, nt, ns) ||
+         try_halo<96, 2, false, false, true>(d, a, nt, ns) ||
+         try_halo<192, 4, false, false, true>(d, a, nt, ns);
 }
 
 cv_status launch_conv(cv_denoiser* d, const cvd::ConvArgs& a, uint32_t nt) {
@@ -1258,6 +1285,8 @@ cv_status launch_conv(cv_denoiser* d, const cvd::ConvArgs& a, uint32_t nt) {
     HIPTRY(hipGetLastError(), "conv");
     return CV_OK;
   }
+  // the general kernel reads its whole input from x
+  if (a.xup) return fail(CV_ERR_INVALID_ARGUMENT, "fused upsampling needs the halo-tile kernel");
   const uint32_t blocks = (a.npix + cvd::kConvPix - 1) / cvd::kConvPix;
   hipStream_t s = d->stream;
   switch (nt) {
@@ -1275,10 +1304,14 @@ cv_status launch_conv(cv_denoiser* d, const cvd::ConvArgs& a, uint32_t nt) {
 
 // y = relu(conv(x) + b) at level size (h, w).
 cv_status conv_fwd(cv_denoiser* d, const Conv& L, const uint16_t* x, uint32_t xcs, uint16_t* y,
-                   uint32_t ycs, float* yf, uint32_t yfcs, uint32_t h, uint32_t w) {
+                   uint32_t ycs, float* yf, uint32_t yfcs, uint32_t h, uint32_t w,
+                   const uint16_t* xup = nullptr, uint32_t xupcs = 0, uint32_t up_chunks = 0) {
   cvd::ConvArgs a{};
   a.x = x;
   a.xcs = xcs;
+  a.xup = xup;
+  a.xupcs = xupcs;
+  a.up_chunks = up_chunks;
   a.kc = L.cin;
   a.w = L.W.p;
   a.kpad = L.kpad;
@@ -1472,8 +1505,11 @@ cv_status pack_weights(cv_denoiser* d) {
   return CV_OK;
 }
 
-// Forward pass from d->x0 to d->out (f32 log1p radiance, [p][3]).
-cv_status forward(cv_denoiser* d) {
+// Forward pass from d->x0 to d->out (f32 log1p radiance, [p][3]).  With
+// `materialize` (training: the backward reads them) the upsampled halves of
+// cat0 / cat1 are written out; otherwise the decoder convolutions read them
+// from the lower level directly, the same values.
+cv_status forward(cv_denoiser* d, bool materialize) {
   const uint32_t H = d->h, W = d->w, H1 = H / 2, W1 = W / 2, H2 = H / 4, W2 = W / 4;
   auto& L = d->conv;
   hipStream_t s = d->stream;
@@ -1490,13 +1526,22 @@ cv_status forward(cv_denoiser* d) {
                      d->cat1.p + 128, 192u, d->p1.p, 64u, 64u, d->nb, H2, W2);
   CVD_TRY(conv_fwd(d, L[4], d->p1.p, 64, d->b2a.p, 128, nullptr, 0, H2, W2));
   CVD_TRY(conv_fwd(d, L[5], d->b2a.p, 128, d->b2b.p, 128, nullptr, 0, H2, W2));
-  hipLaunchKernelGGL(cvd::k_upsample, dim3(grid1((size_t)d->nb * H1 * W1 * 16)), dim3(256), 0, s,
-                     d->b2b.p, 128u, d->cat1.p, 192u, 128u, d->nb, H1, W1);
-  CVD_TRY(conv_fwd(d, L[6], d->cat1.p, 192, d->d1a.p, 64, nullptr, 0, H1, W1));
+  const bool fuse_up = !materialize && d->halo_convs && W1 % cvd::kHaloW == 0;
+  if (fuse_up) {
+    CVD_TRY(conv_fwd(d, L[6], d->cat1.p, 192, d->d1a.p, 64, nullptr, 0, H1, W1, d->b2b.p, 128, 4));
+  } else {
+    hipLaunchKernelGGL(cvd::k_upsample, dim3(grid1((size_t)d->nb * H1 * W1 * 16)), dim3(256), 0,
+                       s, d->b2b.p, 128u, d->cat1.p, 192u, 128u, d->nb, H1, W1);
+    CVD_TRY(conv_fwd(d, L[6], d->cat1.p, 192, d->d1a.p, 64, nullptr, 0, H1, W1));
+  }
   CVD_TRY(conv_fwd(d, L[7], d->d1a.p, 64, d->d1b.p, 64, nullptr, 0, H1, W1));
-  hipLaunchKernelGGL(cvd::k_upsample, dim3(grid1((size_t)d->nb * H * W * 8)), dim3(256), 0, s,
-                     d->d1b.p, 64u, d->cat0.p, 96u, 64u, d->nb, H, W);
-  CVD_TRY(conv_fwd(d, L[8], d->cat0.p, 96, d->d0a.p, 32, nullptr, 0, H, W));
+  if (fuse_up) {
+    CVD_TRY(conv_fwd(d, L[8], d->cat0.p, 96, d->d0a.p, 32, nullptr, 0, H, W, d->d1b.p, 64, 2));
+  } else {
+    hipLaunchKernelGGL(cvd::k_upsample, dim3(grid1((size_t)d->nb * H * W * 8)), dim3(256), 0, s,
+                       d->d1b.p, 64u, d->cat0.p, 96u, 64u, d->nb, H, W);
+    CVD_TRY(conv_fwd(d, L[8], d->cat0.p, 96, d->d0a.p, 32, nullptr, 0, H, W));
+  }
   CVD_TRY(conv_fwd(d, L[9], d->d0a.p, 32, d->d0b.p, 32, nullptr, 0, H, W));
   CVD_TRY(conv_fwd(d, L[10], d->d0b.p, 32, nullptr, 0, d->out.p, 3, H, W));
   HIPTRY(hipGetLastError(), "forward");
@@ -1596,7 +1641,7 @@ cv_status denoise_image(cv_denoiser* d, const float* hdr, const float* features,
                      height, spp, d->in_f32.p);
   hipLaunchKernelGGL(cvd::k_to_input, dim3(grid1(P)), dim3(256), 0, s, d->in_f32.p, d->x0.p,
                      (uint32_t)P);
-  if ((cs = forward(d)) != CV_OK) return cs;
+  if ((cs = forward(d, false)) != CV_OK) return cs;
   hipLaunchKernelGGL(cvd::k_output, dim3(grid1(P)), dim3(256), 0, s, d->out.p, width, height, spp,
                      d->tgt_f32.p);
   HIPTRY(hipGetLastError(), "denoise");
@@ -1726,7 +1771,7 @@ cv_status cv_denoiser_forward(cv_denoiser* d, const float* input, float* output,
   if ((cs = stage_in(d, d->in_f32, input, P * 10, dev, &in)) != CV_OK) return cs;
   hipLaunchKernelGGL(cvd::k_to_input, dim3(grid1(P)), dim3(256), 0, d->stream, in, d->x0.p,
                      (uint32_t)P);
-  if ((cs = forward(d)) != CV_OK) return cs;
+  if ((cs = forward(d, flags & CV_DENOISE_MATERIALIZE)) != CV_OK) return cs;
   HIPTRY(hipMemcpyAsync(output, d->out.p, P * 12, dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
                         d->stream),
          "download");
@@ -1750,7 +1795,7 @@ cv_status cv_denoiser_train_step(cv_denoiser* d, const float* input, const float
   hipStream_t s = d->stream;
   hipLaunchKernelGGL(cvd::k_to_input, dim3(grid1(P)), dim3(256), 0, s, in, d->x0.p,
                      (uint32_t)P);
-  if ((cs = forward(d)) != CV_OK) return cs;
+  if ((cs = forward(d, true)) != CV_OK) return cs;
   hipLaunchKernelGGL(cvd::k_fill, dim3(1), dim3(64), 0, s, d->loss.p, (size_t)1, 0.0f);
   hipLaunchKernelGGL(cvd::k_l1_loss, dim3(std::min<size_t>(grid1(P * 3), 1024)), dim3(256), 0, s,
                      d->out.p, tg, d->dout.p, (uint32_t)P, d->loss.p);

@@ -455,6 +455,11 @@ cv_status cv_denoiser_create(uint64_t seed, cv_denoiser** out);
 cv_status cv_denoiser_get_params(cv_denoiser* d, float* params, float* grads);
 /* Replaces the parameters and resets the Adam state. */
 cv_status cv_denoiser_set_params(cv_denoiser* d, const float* params);
+/* flags: CV_PTR_DEVICE; CV_DENOISE_MATERIALIZE writes the upsampled
+ * decoder inputs out before their convolutions, as a training step does,
+ * instead of reading them from the lower level inside the convolution (the
+ * same values and outputs; slower). */
+#define CV_DENOISE_MATERIALIZE 2u
 cv_status cv_denoiser_forward(cv_denoiser* d, const float* input, float* output, uint32_t nb,
                               uint32_t height, uint32_t width, uint32_t flags);
 /* Forward, L1 loss against log1p(target) (3 f32 per pixel, linear

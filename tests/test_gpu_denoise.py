@@ -47,6 +47,19 @@ def test_forward_matches_torch(cv):
     assert np.abs(ref).mean() > 1e-3
 
 
+@pytest.mark.parametrize("nb,h,w", [(2, 16, 64), (1, 36, 128), (1, 736, 1280)])
+def test_fused_upsampling_forward_is_exact(cv, nb, h, w):
+    """The decoder convolutions reading the upsampled halves of their inputs
+    from the lower level give the bits of the forward that writes them out
+    first (borders, odd tile rows at 36, the configuration's 1280x736)."""
+    d = cv.Denoiser(11)
+    inp, _ = _random_batch(12, nb=nb, h=h, w=w)
+    fused = d.forward(inp)
+    ref = d.forward(inp, materialize=True)
+    assert np.isfinite(fused).all()
+    assert fused.tobytes() == ref.tobytes()
+
+
 def test_gradients_match_torch(cv):
     d = cv.Denoiser(5)
     p = d.params()
