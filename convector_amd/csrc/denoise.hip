@@ -88,6 +88,11 @@ struct ConvArgs {
   // w / 2 per image), read from it directly; the rest come from x
   const uint16_t* xup;
   uint32_t xupcs, up_chunks;
+  // non-null (k_conv_halo<..., POOL>): the 2 x 2 average of the bf16
+  // outputs y is written here too (pixel stride poolcs, h / 2 x w / 2 per
+  // image), summed in the order of k_pool
+  uint16_t* pool;
+  uint32_t poolcs;
 };
 
 // The convolution with the operands staged in LDS: a block of 4 waves
@@ -238,7 +243,7 @@ __global__ __launch_bounds__(256) void k_conv_lds(ConvArgs a) {
 // ---------------------------------------------------------------------------
 constexpr uint32_t kHaloW = 32, kHaloH = 8;
 
-template <int KC, int NS, bool FLIP, bool MASK = false, bool UP = false>
+template <int KC, int NS, bool FLIP, bool MASK = false, bool UP = false, bool POOL = false>
 __global__ __launch_bounds__(256, 2) void k_conv_halo(ConvArgs a) {
   constexpr int MF = 4;
   constexpr int HH = (int)kHaloH + 2, HW = (int)kHaloW + 2;
@@ -469,6 +474,49 @@ __global__ __launch_bounds__(256, 2) void k_conv_halo(ConvArgs a) {
           } else {
             for (int r = 0; r < 4; r++)
               if (j0 + r < a.nout) o[r] = a.accumulate ? o[r] + v[r] : v[r];
+          }
+        }
+      }
+    }
+    if constexpr (POOL) {
+      // fragment m (upper row of the wave's pair) and m + 2 (lower row):
+      // the even lane of each column pair holds pixel (2Y, 2X) and reads
+      // its neighbour (2Y, 2X + 1) across the lane pair (DPP quad_perm
+      // [1, 0, 3, 2])
+#pragma unroll
+      for (int m = 0; m < 2; m++) {
+        const int py = y0 + wrow;
+        if (py >= (int)a.h) continue;
+        const uint32_t px = (uint32_t)x0 + 16 * m + col;
+        const size_t q = img_base / 4 + (size_t)(py >> 1) * (a.w_ >> 1) + (px >> 1);
+#pragma unroll
+        for (int t = 0; t < NS; t++) {
+          const uint32_t j0 = row0 + t * 16 + 4 * kq;
+          if (j0 + 4 > a.nout) continue;
+          float s[4];
+#pragma unroll
+          for (int r = 0; r < 4; r++) {
+            const float b = PERSIST ? bias_s[t * 16 + 4 * kq + r]
+                                    : (a.bias ? a.bias[j0 + r] : 0.0f);
+            float u = acc[m][t][r] + b, l = acc[m + 2][t][r] + b;
+            if (a.relu) {
+              u = fmaxf(u, 0.0f);
+              l = fmaxf(l, 0.0f);
+            }
+            // the stored bf16 values
+            u = from_bf16(to_bf16(u));
+            l = from_bf16(to_bf16(l));
+            const float un = __int_as_float(
+                __builtin_amdgcn_mov_dpp(__float_as_int(u), 0xB1, 0xF, 0xF, false));
+            const float ln = __int_as_float(
+                __builtin_amdgcn_mov_dpp(__float_as_int(l), 0xB1, 0xF, 0xF, false));
+            s[r] = 0.25f * (((u + un) + l) + ln);
+          }
+          if ((col & 1) == 0) {
+            uint2 pk;
+            pk.x = (uint32_t)to_bf16(s[0]) | ((uint32_t)to_bf16(s[1]) << 16);
+            pk.y = (uint32_t)to_bf16(s[2]) | ((uint32_t)to_bf16(s[3]) << 16);
+            *reinterpret_cast<uint2*>(a.pool + q * a.poolcs + j0) = pk;
           }
         }
       }
@@ -1219,14 +1267,14 @@ namespace {
 // so that a chunk's weights and tile fit in LDS beside other blocks.
 // Workgroups of a halo-tile kernel resident on the device at once (CUs x
 // blocks per CU), taken once per instantiation.
-template <int KC, int NS, bool FLIP, bool MASK, bool UP>
+template <int KC, int NS, bool FLIP, bool MASK, bool UP, bool POOL>
 uint32_t halo_resident_blocks() {
   static const uint32_t n = [] {
     int dev = 0, cus = 0, per = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
         hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per, cvd::k_conv_halo<KC, NS, FLIP, MASK, UP>, 256, 0) != hipSuccess ||
+            &per, cvd::k_conv_halo<KC, NS, FLIP, MASK, UP, POOL>, 256, 0) != hipSuccess ||
         cus <= 0 || per <= 0)
       return 512u;
     return (uint32_t)(cus * per);
@@ -1234,10 +1282,10 @@ uint32_t halo_resident_blocks() {
   return n;
 }
 
-template <int KC, int NS, bool FLIP, bool MASK = false, bool UP = false>
+template <int KC, int NS, bool FLIP, bool MASK = false, bool UP = false, bool POOL = false>
 bool try_halo(cv_denoiser* d, const cvd::ConvArgs& a, uint32_t nt, uint32_t ns) {
   if (a.kc != (uint32_t)KC || ns != (uint32_t)NS || (a.flip != 0) != FLIP ||
-      (a.mask != nullptr) != MASK || (a.xup != nullptr) != UP)
+      (a.mask != nullptr) != MASK || (a.xup != nullptr) != UP || (a.pool != nullptr) != POOL)
     return false;
   const uint32_t imgs = a.npix / (a.h * a.w_);
   const uint32_t tiles =
@@ -1247,12 +1295,12 @@ bool try_halo(cv_denoiser* d, const cvd::ConvArgs& a, uint32_t nt, uint32_t ns) 
     // persistent blocks (one input chunk): as many as are resident with
     // every slice's, each taking an equal share of the tiles
     const uint32_t cap =
-        std::max(1u, halo_resident_blocks<KC, NS, FLIP, MASK, UP>() / (nt / NS));
+        std::max(1u, halo_resident_blocks<KC, NS, FLIP, MASK, UP, POOL>() / (nt / NS));
     const uint32_t per = (tiles + cap - 1) / cap;
     blocks = (tiles + per - 1) / per;
   }
-  hipLaunchKernelGGL((cvd::k_conv_halo<KC, NS, FLIP, MASK, UP>), dim3(blocks, nt / NS), dim3(256),
-                     0, d->stream, a);
+  hipLaunchKernelGGL((cvd::k_conv_halo<KC, NS, FLIP, MASK, UP, POOL>), dim3(blocks, nt / NS),
+                     dim3(256), 0, d->stream, a);
   return true;
 }
 
@@ -1262,6 +1310,10 @@ bool launch_conv_halo(cv_denoiser* d, const cvd::ConvArgs& a, uint32_t nt) {
   if (a.w_ % cvd::kHaloW || a.npix % (a.h * a.w_) || a.xcs % 8 || a.xcs < a.kc) return false;
   // a fused ReLU mask is read 4 channels at a time
   if (a.mask && (a.mcs % 4 || (uintptr_t)a.mask % 8)) return false;
+  // a fused pooling: even sizes, whole groups of 4 outputs, 8-byte stores
+  if (a.pool && (a.h % 2 || a.w_ % 2 || a.nout % 4 || a.poolcs % 4 || (uintptr_t)a.pool % 8 ||
+                 !a.y))
+    return false;
   // a fused upsampling: even sizes, whole 16-byte pieces of xup
   if (a.xup && (a.h % 2 || a.w_ % 2 || a.xupcs % 8 || a.xupcs < 32 * a.up_chunks ||
                 32 * a.up_chunks > a.kc))
@@ -1277,7 +1329,9 @@ bool launch_conv_halo(cv_denoiser* d, const cvd::ConvArgs& a, uint32_t nt) {
          try_halo<16, 2, true, true>(d, a, nt, ns) || try_halo<32, 2, true, true>(d, a, nt, ns) ||
          try_halo<64, 4, true, true>(d, a, nt, ns) || try_halo<128, 4, true, true>(d, a, nt, ns) ||
          try_halo<96, 2, false, false, true>(d, a, nt, ns) ||
-         try_halo<192, 4, false, false, true>(d, a, nt, ns);
+         try_halo<192, 4, false, false, true>(d, a, nt, ns) ||
+         try_halo<32, 2, false, false, false, true>(d, a, nt, ns) ||
+         try_halo<64, 4, false, false, false, true>(d, a, nt, ns);
 }
 
 cv_status launch_conv(cv_denoiser* d, const cvd::ConvArgs& a, uint32_t nt) {
@@ -1285,8 +1339,9 @@ cv_status launch_conv(cv_denoiser* d, const cvd::ConvArgs& a, uint32_t nt) {
     HIPTRY(hipGetLastError(), "conv");
     return CV_OK;
   }
-  // the general kernel reads its whole input from x
-  if (a.xup) return fail(CV_ERR_INVALID_ARGUMENT, "fused upsampling needs the halo-tile kernel");
+  // the general kernel reads its whole input from x and writes y only
+  if (a.xup || a.pool)
+    return fail(CV_ERR_INVALID_ARGUMENT, "fused upsampling / pooling needs the halo-tile kernel");
   const uint32_t blocks = (a.npix + cvd::kConvPix - 1) / cvd::kConvPix;
   hipStream_t s = d->stream;
   switch (nt) {
@@ -1305,10 +1360,13 @@ cv_status launch_conv(cv_denoiser* d, const cvd::ConvArgs& a, uint32_t nt) {
 // y = relu(conv(x) + b) at level size (h, w).
 cv_status conv_fwd(cv_denoiser* d, const Conv& L, const uint16_t* x, uint32_t xcs, uint16_t* y,
                    uint32_t ycs, float* yf, uint32_t yfcs, uint32_t h, uint32_t w,
-                   const uint16_t* xup = nullptr, uint32_t xupcs = 0, uint32_t up_chunks = 0) {
+                   const uint16_t* xup = nullptr, uint32_t xupcs = 0, uint32_t up_chunks = 0,
+                   uint16_t* pool = nullptr, uint32_t poolcs = 0) {
   cvd::ConvArgs a{};
   a.x = x;
   a.xcs = xcs;
+  a.pool = pool;
+  a.poolcs = poolcs;
   a.xup = xup;
   a.xupcs = xupcs;
   a.up_chunks = up_chunks;
@@ -1508,22 +1566,37 @@ cv_status pack_weights(cv_denoiser* d) {
 // Forward pass from d->x0 to d->out (f32 log1p radiance, [p][3]).  With
 // `materialize` (training: the backward reads them) the upsampled halves of
 // cat0 / cat1 are written out; otherwise the decoder convolutions read them
-// from the lower level directly, the same values.
-cv_status forward(cv_denoiser* d, bool materialize) {
+// from the lower level directly, the same values.  The training step passes
+// materialize_pool = false: the pooled tensors come from the encoder
+// convolutions' epilogues either way, except on the fully materialized path.
+cv_status forward(cv_denoiser* d, bool materialize, bool materialize_pool) {
   const uint32_t H = d->h, W = d->w, H1 = H / 2, W1 = W / 2, H2 = H / 4, W2 = W / 4;
   auto& L = d->conv;
   hipStream_t s = d->stream;
   cv_status cs;
 #define CVD_TRY(x) \
   if ((cs = (x)) != CV_OK) return cs
+  // the pooled encoder outputs come from the convolutions' epilogues (the
+  // same bits as k_pool, which the materialized path runs)
+  const bool fuse_pool = !materialize_pool && d->halo_convs && W % (2 * cvd::kHaloW) == 0;
   CVD_TRY(conv_fwd(d, L[0], d->x0.p, 16, d->e0a.p, 32, nullptr, 0, H, W));
-  CVD_TRY(conv_fwd(d, L[1], d->e0a.p, 32, d->cat0.p + 64, 96, nullptr, 0, H, W));
-  hipLaunchKernelGGL(cvd::k_pool, dim3(grid1((size_t)d->nb * H1 * W1 * 4)), dim3(256), 0, s,
-                     d->cat0.p + 64, 96u, d->p0.p, 32u, 32u, d->nb, H1, W1);
+  if (fuse_pool) {
+    CVD_TRY(conv_fwd(d, L[1], d->e0a.p, 32, d->cat0.p + 64, 96, nullptr, 0, H, W, nullptr, 0, 0,
+                     d->p0.p, 32));
+  } else {
+    CVD_TRY(conv_fwd(d, L[1], d->e0a.p, 32, d->cat0.p + 64, 96, nullptr, 0, H, W));
+    hipLaunchKernelGGL(cvd::k_pool, dim3(grid1((size_t)d->nb * H1 * W1 * 4)), dim3(256), 0, s,
+                       d->cat0.p + 64, 96u, d->p0.p, 32u, 32u, d->nb, H1, W1);
+  }
   CVD_TRY(conv_fwd(d, L[2], d->p0.p, 32, d->e1a.p, 64, nullptr, 0, H1, W1));
-  CVD_TRY(conv_fwd(d, L[3], d->e1a.p, 64, d->cat1.p + 128, 192, nullptr, 0, H1, W1));
-  hipLaunchKernelGGL(cvd::k_pool, dim3(grid1((size_t)d->nb * H2 * W2 * 8)), dim3(256), 0, s,
-                     d->cat1.p + 128, 192u, d->p1.p, 64u, 64u, d->nb, H2, W2);
+  if (fuse_pool) {
+    CVD_TRY(conv_fwd(d, L[3], d->e1a.p, 64, d->cat1.p + 128, 192, nullptr, 0, H1, W1, nullptr, 0,
+                     0, d->p1.p, 64));
+  } else {
+    CVD_TRY(conv_fwd(d, L[3], d->e1a.p, 64, d->cat1.p + 128, 192, nullptr, 0, H1, W1));
+    hipLaunchKernelGGL(cvd::k_pool, dim3(grid1((size_t)d->nb * H2 * W2 * 8)), dim3(256), 0, s,
+                       d->cat1.p + 128, 192u, d->p1.p, 64u, 64u, d->nb, H2, W2);
+  }
   CVD_TRY(conv_fwd(d, L[4], d->p1.p, 64, d->b2a.p, 128, nullptr, 0, H2, W2));
   CVD_TRY(conv_fwd(d, L[5], d->b2a.p, 128, d->b2b.p, 128, nullptr, 0, H2, W2));
   const bool fuse_up = !materialize && d->halo_convs && W1 % cvd::kHaloW == 0;
@@ -1641,7 +1714,7 @@ cv_status denoise_image(cv_denoiser* d, const float* hdr, const float* features,
                      height, spp, d->in_f32.p);
   hipLaunchKernelGGL(cvd::k_to_input, dim3(grid1(P)), dim3(256), 0, s, d->in_f32.p, d->x0.p,
                      (uint32_t)P);
-  if ((cs = forward(d, false)) != CV_OK) return cs;
+  if ((cs = forward(d, false, false)) != CV_OK) return cs;
   hipLaunchKernelGGL(cvd::k_output, dim3(grid1(P)), dim3(256), 0, s, d->out.p, width, height, spp,
                      d->tgt_f32.p);
   HIPTRY(hipGetLastError(), "denoise");
@@ -1771,7 +1844,8 @@ cv_status cv_denoiser_forward(cv_denoiser* d, const float* input, float* output,
   if ((cs = stage_in(d, d->in_f32, input, P * 10, dev, &in)) != CV_OK) return cs;
   hipLaunchKernelGGL(cvd::k_to_input, dim3(grid1(P)), dim3(256), 0, d->stream, in, d->x0.p,
                      (uint32_t)P);
-  if ((cs = forward(d, flags & CV_DENOISE_MATERIALIZE)) != CV_OK) return cs;
+  const bool mat = flags & CV_DENOISE_MATERIALIZE;
+  if ((cs = forward(d, mat, mat)) != CV_OK) return cs;
   HIPTRY(hipMemcpyAsync(output, d->out.p, P * 12, dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
                         d->stream),
          "download");
@@ -1795,7 +1869,7 @@ cv_status cv_denoiser_train_step(cv_denoiser* d, const float* input, const float
   hipStream_t s = d->stream;
   hipLaunchKernelGGL(cvd::k_to_input, dim3(grid1(P)), dim3(256), 0, s, in, d->x0.p,
                      (uint32_t)P);
-  if ((cs = forward(d, true)) != CV_OK) return cs;
+  if ((cs = forward(d, true, false)) != CV_OK) return cs;
   hipLaunchKernelGGL(cvd::k_fill, dim3(1), dim3(64), 0, s, d->loss.p, (size_t)1, 0.0f);
   hipLaunchKernelGGL(cvd::k_l1_loss, dim3(std::min<size_t>(grid1(P * 3), 1024)), dim3(256), 0, s,
                      d->out.p, tg, d->dout.p, (uint32_t)P, d->loss.p);

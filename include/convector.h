@@ -455,10 +455,11 @@ cv_status cv_denoiser_create(uint64_t seed, cv_denoiser** out);
 cv_status cv_denoiser_get_params(cv_denoiser* d, float* params, float* grads);
 /* Replaces the parameters and resets the Adam state. */
 cv_status cv_denoiser_set_params(cv_denoiser* d, const float* params);
-/* flags: CV_PTR_DEVICE; CV_DENOISE_MATERIALIZE writes the upsampled
- * decoder inputs out before their convolutions, as a training step does,
- * instead of reading them from the lower level inside the convolution (the
- * same values and outputs; slower). */
+/* flags: CV_PTR_DEVICE; CV_DENOISE_MATERIALIZE runs the separate pooling
+ * and upsampling kernels (the pooled encoder outputs and the upsampled
+ * decoder inputs written out by kernels of their own) instead of pooling in
+ * the encoder convolutions' epilogues and reading the upsampled channels
+ * inside the decoder convolutions: the same values and outputs, slower. */
 #define CV_DENOISE_MATERIALIZE 2u
 cv_status cv_denoiser_forward(cv_denoiser* d, const float* input, float* output, uint32_t nb,
                               uint32_t height, uint32_t width, uint32_t flags);
