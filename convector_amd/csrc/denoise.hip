@@ -412,111 +412,121 @@ __global__ __launch_bounds__(256, 2) void k_conv_halo(ConvArgs a) {
         __syncthreads();
       }
     }
-#pragma unroll
-    for (int m = 0; m < MF; m++) {
-      const int py = y0 + wrow + m / 2;
-      if (py >= (int)a.h) continue;
-      const size_t p = img_base + (size_t)py * a.w_ + (uint32_t)x0 + 16 * (m % 2) + col;
-#pragma unroll
-      for (int t = 0; t < NS; t++) {
-        const uint32_t j0 = row0 + t * 16 + 4 * kq;
-        if (j0 >= a.nout) continue;
-        float v[4];
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-          const uint32_t j = j0 + r;
-          if constexpr (PERSIST)
-            v[r] = acc[m][t][r] + bias_s[j - row0];
-          else
-            v[r] = acc[m][t][r] + ((a.bias && j < a.nout) ? a.bias[j] : 0.0f);
-          if (a.relu) v[r] = fmaxf(v[r], 0.0f);
-        }
-        const bool whole = j0 + 4 <= a.nout;
-        if constexpr (MASK) {
-          // the ReLU gradient of the layer below: 4 bf16 of its output at once
-          // (the host checks the mask's alignment and stride)
-          if (whole) {
-            const uint2 mk = *reinterpret_cast<const uint2*>(a.mask + p * a.mcs + j0);
-            const uint32_t mw[4] = {mk.x << 16, mk.x & 0xffff0000u, mk.y << 16,
-                                    mk.y & 0xffff0000u};
-#pragma unroll
-            for (int r = 0; r < 4; r++)
-              if (!(__uint_as_float(mw[r]) > 0.0f)) v[r] = 0.0f;
-          } else {
-            for (int r = 0; r < 4; r++)
-              if (j0 + r < a.nout && !(from_bf16(a.mask[p * a.mcs + j0 + r]) > 0.0f))
-                v[r] = 0.0f;
-          }
-        }
-        if (a.y) {
-          if (whole && vec_y) {
-            uint2 pk;
-            pk.x = pack_bf16x2(v[0], v[1]);
-            pk.y = pack_bf16x2(v[2], v[3]);
-            *reinterpret_cast<uint2*>(a.y + p * a.ycs + j0) = pk;
-          } else {
-            for (int r = 0; r < 4; r++)
-              if (j0 + r < a.nout) a.y[p * a.ycs + j0 + r] = to_bf16(v[r]);
-          }
-        }
-        if (a.yf) {
-          float* o = a.yf + p * a.yfcs + j0;
-          if (whole && vec_f) {
-            float4 w4 = make_float4(v[0], v[1], v[2], v[3]);
-            if (a.accumulate) {
-              const float4 old = *reinterpret_cast<const float4*>(o);
-              w4.x += old.x;
-              w4.y += old.y;
-              w4.z += old.z;
-              w4.w += old.w;
-            }
-            *reinterpret_cast<float4*>(o) = w4;
-          } else {
-            for (int r = 0; r < 4; r++)
-              if (j0 + r < a.nout) o[r] = a.accumulate ? o[r] + v[r] : v[r];
-          }
-        }
-      }
-    }
     if constexpr (POOL) {
-      // fragment m (upper row of the wave's pair) and m + 2 (lower row):
-      // the even lane of each column pair holds pixel (2Y, 2X) and reads
-      // its neighbour (2Y, 2X + 1) across the lane pair (DPP quad_perm
-      // [1, 0, 3, 2])
+      // Fragments m (the upper row of the wave's pair) and m + 2 (the lower
+      // row) together: both are stored, and the even lane of each column
+      // pair, which holds pixel (2Y, 2X), takes its neighbour (2Y, 2X + 1)
+      // by a DPP lane swap (quad_perm [1, 0, 3, 2]) and writes the 2 x 2
+      // average of the stored bf16 values, summed in the order of k_pool.
+      // (The host checks: y aligned for 8-byte stores, whole groups of 4.)
 #pragma unroll
       for (int m = 0; m < 2; m++) {
         const int py = y0 + wrow;
         if (py >= (int)a.h) continue;
         const uint32_t px = (uint32_t)x0 + 16 * m + col;
+        const size_t p = img_base + (size_t)py * a.w_ + px;
         const size_t q = img_base / 4 + (size_t)(py >> 1) * (a.w_ >> 1) + (px >> 1);
 #pragma unroll
         for (int t = 0; t < NS; t++) {
           const uint32_t j0 = row0 + t * 16 + 4 * kq;
-          if (j0 + 4 > a.nout) continue;
-          float s[4];
+          if (j0 >= a.nout) continue;
+          float u[4], l[4];
 #pragma unroll
           for (int r = 0; r < 4; r++) {
             const float b = PERSIST ? bias_s[t * 16 + 4 * kq + r]
                                     : (a.bias ? a.bias[j0 + r] : 0.0f);
-            float u = acc[m][t][r] + b, l = acc[m + 2][t][r] + b;
+            u[r] = acc[m][t][r] + b;
+            l[r] = acc[m + 2][t][r] + b;
             if (a.relu) {
-              u = fmaxf(u, 0.0f);
-              l = fmaxf(l, 0.0f);
+              u[r] = fmaxf(u[r], 0.0f);
+              l[r] = fmaxf(l[r], 0.0f);
             }
-            // the stored bf16 values
-            u = from_bf16(to_bf16(u));
-            l = from_bf16(to_bf16(l));
-            const float un = __int_as_float(
-                __builtin_amdgcn_mov_dpp(__float_as_int(u), 0xB1, 0xF, 0xF, false));
-            const float ln = __int_as_float(
-                __builtin_amdgcn_mov_dpp(__float_as_int(l), 0xB1, 0xF, 0xF, false));
-            s[r] = 0.25f * (((u + un) + l) + ln);
           }
-          if ((col & 1) == 0) {
-            uint2 pk;
-            pk.x = (uint32_t)to_bf16(s[0]) | ((uint32_t)to_bf16(s[1]) << 16);
-            pk.y = (uint32_t)to_bf16(s[2]) | ((uint32_t)to_bf16(s[3]) << 16);
-            *reinterpret_cast<uint2*>(a.pool + q * a.poolcs + j0) = pk;
+          const uint2 pu = make_uint2(pack_bf16x2(u[0], u[1]), pack_bf16x2(u[2], u[3]));
+          const uint2 pl = make_uint2(pack_bf16x2(l[0], l[1]), pack_bf16x2(l[2], l[3]));
+          *reinterpret_cast<uint2*>(a.y + p * a.ycs + j0) = pu;
+          *reinterpret_cast<uint2*>(a.y + (p + a.w_) * a.ycs + j0) = pl;
+          const uint32_t wu[4] = {pu.x << 16, pu.x & 0xffff0000u, pu.y << 16, pu.y & 0xffff0000u};
+          const uint32_t wl[4] = {pl.x << 16, pl.x & 0xffff0000u, pl.y << 16, pl.y & 0xffff0000u};
+          float s4[4];
+#pragma unroll
+          for (int r = 0; r < 4; r++) {
+            const float a0 = __uint_as_float(wu[r]), c0 = __uint_as_float(wl[r]);
+            const float b0 = __int_as_float(
+                __builtin_amdgcn_mov_dpp((int)wu[r], 0xB1, 0xF, 0xF, false));
+            const float d0 = __int_as_float(
+                __builtin_amdgcn_mov_dpp((int)wl[r], 0xB1, 0xF, 0xF, false));
+            s4[r] = 0.25f * (((a0 + b0) + c0) + d0);
+          }
+          if ((col & 1) == 0)
+            *reinterpret_cast<uint2*>(a.pool + q * a.poolcs + j0) =
+                make_uint2(pack_bf16x2(s4[0], s4[1]), pack_bf16x2(s4[2], s4[3]));
+        }
+      }
+    } else {
+#pragma unroll
+      for (int m = 0; m < MF; m++) {
+        const int py = y0 + wrow + m / 2;
+        if (py >= (int)a.h) continue;
+        const size_t p = img_base + (size_t)py * a.w_ + (uint32_t)x0 + 16 * (m % 2) + col;
+#pragma unroll
+        for (int t = 0; t < NS; t++) {
+          const uint32_t j0 = row0 + t * 16 + 4 * kq;
+          if (j0 >= a.nout) continue;
+          float v[4];
+#pragma unroll
+          for (int r = 0; r < 4; r++) {
+            const uint32_t j = j0 + r;
+            if constexpr (PERSIST)
+              v[r] = acc[m][t][r] + bias_s[j - row0];
+            else
+              v[r] = acc[m][t][r] + ((a.bias && j < a.nout) ? a.bias[j] : 0.0f);
+            if (a.relu) v[r] = fmaxf(v[r], 0.0f);
+          }
+          const bool whole = j0 + 4 <= a.nout;
+          if constexpr (MASK) {
+            // the ReLU gradient of the layer below: 4 bf16 of its output at once
+            // (the host checks the mask's alignment and stride)
+            if (whole) {
+              const uint2 mk = *reinterpret_cast<const uint2*>(a.mask + p * a.mcs + j0);
+              const uint32_t mw[4] = {mk.x << 16, mk.x & 0xffff0000u, mk.y << 16,
+                                      mk.y & 0xffff0000u};
+#pragma unroll
+              for (int r = 0; r < 4; r++)
+                if (!(__uint_as_float(mw[r]) > 0.0f)) v[r] = 0.0f;
+            } else {
+              for (int r = 0; r < 4; r++)
+                if (j0 + r < a.nout && !(from_bf16(a.mask[p * a.mcs + j0 + r]) > 0.0f))
+                  v[r] = 0.0f;
+            }
+          }
+          if (a.y) {
+            if (whole && vec_y) {
+              uint2 pk;
+              pk.x = pack_bf16x2(v[0], v[1]);
+              pk.y = pack_bf16x2(v[2], v[3]);
+              *reinterpret_cast<uint2*>(a.y + p * a.ycs + j0) = pk;
+            } else {
+              for (int r = 0; r < 4; r++)
+                if (j0 + r < a.nout) a.y[p * a.ycs + j0 + r] = to_bf16(v[r]);
+            }
+          }
+          if (a.yf) {
+            float* o = a.yf + p * a.yfcs + j0;
+            if (whole && vec_f) {
+              float4 w4 = make_float4(v[0], v[1], v[2], v[3]);
+              if (a.accumulate) {
+                const float4 old = *reinterpret_cast<const float4*>(o);
+                w4.x += old.x;
+                w4.y += old.y;
+                w4.z += old.z;
+                w4.w += old.w;
+              }
+              *reinterpret_cast<float4*>(o) = w4;
+            } else {
+              for (int r = 0; r < 4; r++)
+                if (j0 + r < a.nout) o[r] = a.accumulate ? o[r] + v[r] : v[r];
+            }
           }
         }
       }
@@ -1312,7 +1322,7 @@ bool launch_conv_halo(cv_denoiser* d, const cvd::ConvArgs& a, uint32_t nt) {
   if (a.mask && (a.mcs % 4 || (uintptr_t)a.mask % 8)) return false;
   // a fused pooling: even sizes, whole groups of 4 outputs, 8-byte stores
   if (a.pool && (a.h % 2 || a.w_ % 2 || a.nout % 4 || a.poolcs % 4 || (uintptr_t)a.pool % 8 ||
-                 !a.y))
+                 !a.y || a.ycs % 4 || (uintptr_t)a.y % 8 || a.yf || a.mask))
     return false;
   // a fused upsampling: even sizes, whole 16-byte pieces of xup
   if (a.xup && (a.h % 2 || a.w_ % 2 || a.xupcs % 8 || a.xupcs < 32 * a.up_chunks ||
