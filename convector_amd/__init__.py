@@ -468,15 +468,19 @@ class Denoiser:
         check(lib().cv_denoiser_forward(self._h, _p(inp), _p(out), nb, h, w, flags))
         return out
 
-    def train_step(self, inp: np.ndarray, target: np.ndarray, lr: float) -> float:
+    def train_step(self, inp: np.ndarray, target: np.ndarray, lr: float,
+                   materialize: bool = False) -> float:
         """One Adam step on L1(log1p) against linear target (nb, h, w, 3);
-        lr = 0 computes the gradients only.  Returns the loss."""
+        lr = 0 computes the gradients only.  Returns the loss.  materialize:
+        the separate pooling / upsampling-gradient kernels
+        (CV_DENOISE_MATERIALIZE; the same results)."""
         inp = np.ascontiguousarray(inp, np.float32)
         target = np.ascontiguousarray(target, np.float32)
         nb, h, w, _ = inp.shape
         loss = ctypes.c_float()
-        check(lib().cv_denoiser_train_step(self._h, _p(inp), _p(target), nb, h, w, float(lr), 0,
-                                           ctypes.byref(loss)))
+        flags = _native.CV_DENOISE_MATERIALIZE if materialize else 0
+        check(lib().cv_denoiser_train_step(self._h, _p(inp), _p(target), nb, h, w, float(lr),
+                                           flags, ctypes.byref(loss)))
         return loss.value
 
     def train_step_device(self, inp_ptr: int, target_ptr: int, nb, h, w, lr) -> float:

@@ -93,6 +93,15 @@ struct ConvArgs {
   // image), summed in the order of k_pool
   uint16_t* pool;
   uint32_t poolcs;
+  // non-null (k_conv_halo<..., UG>, an input gradient): outputs [0, ug_ch)
+  // are the gradient of a nearest 2x upsampling; their 2 x 2 sums (in the
+  // order of k_upsample_grad), masked by the ReLU output ug_mask of the
+  // layer below (pixel stride ug_mcs, h / 2 x w / 2), are written as that
+  // layer's bf16 dZ (pixel stride ug_dzcs); the other outputs go to yf
+  uint16_t* ug_dz;
+  uint32_t ug_dzcs;
+  const uint16_t* ug_mask;
+  uint32_t ug_mcs, ug_ch;
 };
 
 // The convolution with the operands staged in LDS: a block of 4 waves
@@ -243,7 +252,8 @@ __global__ __launch_bounds__(256) void k_conv_lds(ConvArgs a) {
 // ---------------------------------------------------------------------------
 constexpr uint32_t kHaloW = 32, kHaloH = 8;
 
-template <int KC, int NS, bool FLIP, bool MASK = false, bool UP = false, bool POOL = false>
+template <int KC, int NS, bool FLIP, bool MASK = false, bool UP = false, bool POOL = false,
+          bool UG = false>
 __global__ __launch_bounds__(256, 2) void k_conv_halo(ConvArgs a) {
   constexpr int MF = 4;
   constexpr int HH = (int)kHaloH + 2, HW = (int)kHaloW + 2;
@@ -461,6 +471,41 @@ __global__ __launch_bounds__(256, 2) void k_conv_halo(ConvArgs a) {
           if ((col & 1) == 0)
             *reinterpret_cast<uint2*>(a.pool + q * a.poolcs + j0) =
                 make_uint2(pack_bf16x2(s4[0], s4[1]), pack_bf16x2(s4[2], s4[3]));
+        }
+      }
+    } else if (UG && row0 < a.ug_ch) {
+      // the upsampling's gradient: the 2 x 2 sum of the f32 values (the
+      // even lane of each column pair reads its neighbour by DPP, as in
+      // the pooling above), masked by the layer below's ReLU output
+#pragma unroll
+      for (int m = 0; m < 2; m++) {
+        const int py = y0 + wrow;
+        if (py >= (int)a.h) continue;
+        const uint32_t px = (uint32_t)x0 + 16 * m + col;
+        const size_t q = img_base / 4 + (size_t)(py >> 1) * (a.w_ >> 1) + (px >> 1);
+#pragma unroll
+        for (int t = 0; t < NS; t++) {
+          const uint32_t j0 = row0 + t * 16 + 4 * kq;
+          float sv[4];
+#pragma unroll
+          for (int r = 0; r < 4; r++) {
+            const float u = acc[m][t][r], l = acc[m + 2][t][r];
+            const float un = __int_as_float(
+                __builtin_amdgcn_mov_dpp(__float_as_int(u), 0xB1, 0xF, 0xF, false));
+            const float ln = __int_as_float(
+                __builtin_amdgcn_mov_dpp(__float_as_int(l), 0xB1, 0xF, 0xF, false));
+            sv[r] = ((u + un) + l) + ln;
+          }
+          if ((col & 1) == 0) {
+            const uint2 mk = *reinterpret_cast<const uint2*>(a.ug_mask + q * a.ug_mcs + j0);
+            const uint32_t mw[4] = {mk.x << 16, mk.x & 0xffff0000u, mk.y << 16,
+                                    mk.y & 0xffff0000u};
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+              if (!(__uint_as_float(mw[r]) > 0.0f)) sv[r] = 0.0f;
+            *reinterpret_cast<uint2*>(a.ug_dz + q * a.ug_dzcs + j0) =
+                make_uint2(pack_bf16x2(sv[0], sv[1]), pack_bf16x2(sv[2], sv[3]));
+          }
         }
       }
     } else {
@@ -1277,14 +1322,14 @@ namespace {
 // so that a chunk's weights and tile fit in LDS beside other blocks.
 // Workgroups of a halo-tile kernel resident on the device at once (CUs x
 // blocks per CU), taken once per instantiation.
-template <int KC, int NS, bool FLIP, bool MASK, bool UP, bool POOL>
+template <int KC, int NS, bool FLIP, bool MASK, bool UP, bool POOL, bool UG>
 uint32_t halo_resident_blocks() {
   static const uint32_t n = [] {
     int dev = 0, cus = 0, per = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
         hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per, cvd::k_conv_halo<KC, NS, FLIP, MASK, UP, POOL>, 256, 0) != hipSuccess ||
+            &per, cvd::k_conv_halo<KC, NS, FLIP, MASK, UP, POOL, UG>, 256, 0) != hipSuccess ||
         cus <= 0 || per <= 0)
       return 512u;
     return (uint32_t)(cus * per);
@@ -1292,11 +1337,15 @@ uint32_t halo_resident_blocks() {
   return n;
 }
 
-template <int KC, int NS, bool FLIP, bool MASK = false, bool UP = false, bool POOL = false>
+template <int KC, int NS, bool FLIP, bool MASK = false, bool UP = false, bool POOL = false,
+          bool UG = false>
 bool try_halo(cv_denoiser* d, const cvd::ConvArgs& a, uint32_t nt, uint32_t ns) {
   if (a.kc != (uint32_t)KC || ns != (uint32_t)NS || (a.flip != 0) != FLIP ||
-      (a.mask != nullptr) != MASK || (a.xup != nullptr) != UP || (a.pool != nullptr) != POOL)
+      (a.mask != nullptr) != MASK || (a.xup != nullptr) != UP || (a.pool != nullptr) != POOL ||
+      (a.ug_dz != nullptr) != UG)
     return false;
+  // the upsampling's gradient covers whole slices of NS * 16 outputs
+  if (UG && a.ug_ch % (NS * 16)) return false;
   const uint32_t imgs = a.npix / (a.h * a.w_);
   const uint32_t tiles =
       imgs * (a.w_ / cvd::kHaloW) * ((a.h + cvd::kHaloH - 1) / cvd::kHaloH);
@@ -1305,11 +1354,11 @@ bool try_halo(cv_denoiser* d, const cvd::ConvArgs& a, uint32_t nt, uint32_t ns) 
     // persistent blocks (one input chunk): as many as are resident with
     // every slice's, each taking an equal share of the tiles
     const uint32_t cap =
-        std::max(1u, halo_resident_blocks<KC, NS, FLIP, MASK, UP, POOL>() / (nt / NS));
+        std::max(1u, halo_resident_blocks<KC, NS, FLIP, MASK, UP, POOL, UG>() / (nt / NS));
     const uint32_t per = (tiles + cap - 1) / cap;
     blocks = (tiles + per - 1) / per;
   }
-  hipLaunchKernelGGL((cvd::k_conv_halo<KC, NS, FLIP, MASK, UP, POOL>), dim3(blocks, nt / NS),
+  hipLaunchKernelGGL((cvd::k_conv_halo<KC, NS, FLIP, MASK, UP, POOL, UG>), dim3(blocks, nt / NS),
                      dim3(256), 0, d->stream, a);
   return true;
 }
@@ -1323,6 +1372,11 @@ bool launch_conv_halo(cv_denoiser* d, const cvd::ConvArgs& a, uint32_t nt) {
   // a fused pooling: even sizes, whole groups of 4 outputs, 8-byte stores
   if (a.pool && (a.h % 2 || a.w_ % 2 || a.nout % 4 || a.poolcs % 4 || (uintptr_t)a.pool % 8 ||
                  !a.y || a.ycs % 4 || (uintptr_t)a.y % 8 || a.yf || a.mask))
+    return false;
+  // a fused upsampling gradient: even sizes, 8-byte mask reads and dZ stores
+  if (a.ug_dz && (a.h % 2 || a.w_ % 2 || !a.ug_mask || a.ug_mcs % 4 || a.ug_dzcs % 4 ||
+                  (uintptr_t)a.ug_mask % 8 || (uintptr_t)a.ug_dz % 8 || a.y || a.mask ||
+                  a.ug_ch > a.nout))
     return false;
   // a fused upsampling: even sizes, whole 16-byte pieces of xup
   if (a.xup && (a.h % 2 || a.w_ % 2 || a.xupcs % 8 || a.xupcs < 32 * a.up_chunks ||
@@ -1341,7 +1395,9 @@ bool launch_conv_halo(cv_denoiser* d, const cvd::ConvArgs& a, uint32_t nt) {
          try_halo<96, 2, false, false, true>(d, a, nt, ns) ||
          try_halo<192, 4, false, false, true>(d, a, nt, ns) ||
          try_halo<32, 2, false, false, false, true>(d, a, nt, ns) ||
-         try_halo<64, 4, false, false, false, true>(d, a, nt, ns);
+         try_halo<64, 4, false, false, false, true>(d, a, nt, ns) ||
+         try_halo<32, 2, true, false, false, false, true>(d, a, nt, ns) ||
+         try_halo<64, 4, true, false, false, false, true>(d, a, nt, ns);
 }
 
 cv_status launch_conv(cv_denoiser* d, const cvd::ConvArgs& a, uint32_t nt) {
@@ -1350,7 +1406,7 @@ cv_status launch_conv(cv_denoiser* d, const cvd::ConvArgs& a, uint32_t nt) {
     return CV_OK;
   }
   // the general kernel reads its whole input from x and writes y only
-  if (a.xup || a.pool)
+  if (a.xup || a.pool || a.ug_dz)
     return fail(CV_ERR_INVALID_ARGUMENT, "fused upsampling / pooling needs the halo-tile kernel");
   const uint32_t blocks = (a.npix + cvd::kConvPix - 1) / cvd::kConvPix;
   hipStream_t s = d->stream;
@@ -1396,9 +1452,21 @@ cv_status conv_fwd(cv_denoiser* d, const Conv& L, const uint16_t* x, uint32_t xc
   return launch_conv(d, a, L.cout_pad / 16);
 }
 
+// The gradient of a nearest 2x upsampling fused into an input-gradient
+// convolution: its first `ch` outputs are summed 2 x 2, masked by the ReLU
+// output y (pixel stride ycs) of the layer below the upsampling and written
+// as that layer's bf16 dZ (pixel stride dzcs).
+struct UpGrad {
+  uint16_t* dz;
+  uint32_t dzcs;
+  const uint16_t* y;
+  uint32_t ycs, ch;
+};
+
 cv_status conv_bwd_input(cv_denoiser* d, const Conv& L, const uint16_t* dz, float* gx,
                          uint32_t gxcs, bool accumulate, const uint16_t* x, uint32_t xcs,
-                         uint16_t* dz_next, uint32_t npix, uint32_t h, uint32_t w);
+                         uint16_t* dz_next, uint32_t npix, uint32_t h, uint32_t w,
+                         const UpGrad* ug);
 
 // Backward of one convolution: dZ (dY at y's shape, f32, through the ReLU
 // mask; or, with dz_ready, already in dz), dW and db, and dX (+= into gx
@@ -1409,7 +1477,7 @@ cv_status conv_bwd_input(cv_denoiser* d, const Conv& L, const uint16_t* dz, floa
 cv_status conv_bwd(cv_denoiser* d, const Conv& L, const uint16_t* x, uint32_t xcs,
                    const uint16_t* y, uint32_t ycs, const float* gy, uint32_t gycs, uint16_t* dz,
                    bool dz_ready, float* gx, uint32_t gxcs, bool accumulate, uint16_t* dz_next,
-                   uint32_t h, uint32_t w) {
+                   uint32_t h, uint32_t w, const UpGrad* ug = nullptr) {
   const uint32_t npix = d->nb * h * w;
   hipStream_t s = d->stream;
   if (!dz_ready) {
@@ -1461,7 +1529,7 @@ cv_status conv_bwd(cv_denoiser* d, const Conv& L, const uint16_t* x, uint32_t xc
     hipLaunchKernelGGL(cvd::k_wgrad_reduce, dim3((nw + L.cout + 63) / 64), dim3(1024), 0, s,
                        d->wpart.p, splits, nw, L.cout, g.dw, g.db);
     HIPTRY(hipGetLastError(), "wgrad reduce");
-    return conv_bwd_input(d, L, dz, gx, gxcs, accumulate, x, xcs, dz_next, npix, h, w);
+    return conv_bwd_input(d, L, dz, gx, gxcs, accumulate, x, xcs, dz_next, npix, h, w, ug);
   }
   // about 2048 blocks: pixel splits of a multiple of 64 pixels
   const uint32_t col_tiles = (9 * L.cin + cvd::kWgCols - 1) / cvd::kWgCols;
@@ -1482,14 +1550,15 @@ cv_status conv_bwd(cv_denoiser* d, const Conv& L, const uint16_t* x, uint32_t xc
   hipLaunchKernelGGL(cvd::k_wgrad_reduce, dim3((nw + L.cout + 63) / 64), dim3(1024), 0, s,
                      d->wpart.p, splits, nw, L.cout, g.dw, g.db);
   HIPTRY(hipGetLastError(), "wgrad reduce");
-  return conv_bwd_input(d, L, dz, gx, gxcs, accumulate, x, xcs, dz_next, npix, h, w);
+  return conv_bwd_input(d, L, dz, gx, gxcs, accumulate, x, xcs, dz_next, npix, h, w, ug);
 }
 
 // dX of one convolution (+= into gx when accumulate): the convolution of
 // dZ with the flipped weights.
 cv_status conv_bwd_input(cv_denoiser* d, const Conv& L, const uint16_t* dz, float* gx,
                          uint32_t gxcs, bool accumulate, const uint16_t* x, uint32_t xcs,
-                         uint16_t* dz_next, uint32_t npix, uint32_t h, uint32_t w) {
+                         uint16_t* dz_next, uint32_t npix, uint32_t h, uint32_t w,
+                         const UpGrad* ug) {
   if (gx || dz_next) {
     cvd::ConvArgs a{};
     a.x = dz;
@@ -1509,6 +1578,14 @@ cv_status conv_bwd_input(cv_denoiser* d, const Conv& L, const uint16_t* dz, floa
     } else {
       a.yf = gx;
       a.yfcs = gxcs;
+    }
+    if (ug) {
+      if (dz_next) return fail(CV_ERR_INVALID_ARGUMENT, "fused upsampling gradient needs a plain dX");
+      a.ug_dz = ug->dz;
+      a.ug_dzcs = ug->dzcs;
+      a.ug_mask = ug->y;
+      a.ug_mcs = ug->ycs;
+      a.ug_ch = ug->ch;
     }
     a.accumulate = accumulate ? 1u : 0u;
     a.flip = 1;
@@ -1632,7 +1709,11 @@ cv_status forward(cv_denoiser* d, bool materialize, bool materialize_pool) {
 }
 
 // Backward pass from d->dout to d->grads (zeroed here first).
-cv_status backward(cv_denoiser* d) {
+// With `materialize` the upsampling gradients run as kernels of their own
+// (k_upsample_grad, then the ReLU gradient of the layer below); otherwise
+// the input-gradient convolutions of the two decoder layers above an
+// upsampling write the layer below's dZ directly (the same bits).
+cv_status backward(cv_denoiser* d, bool materialize) {
   const uint32_t H = d->h, W = d->w, H1 = H / 2, W1 = W / 2, H2 = H / 4, W2 = W / 4;
   auto& L = d->conv;
   hipStream_t s = d->stream;
@@ -1650,19 +1731,24 @@ cv_status backward(cv_denoiser* d) {
                    false, dzB, H, W));
   CVD_TRY(conv_bwd(d, L[9], d->d0a.p, 32, d->d0b.p, 32, nullptr, 0, dzB, true, nullptr, 0, false,
                    dzA, H, W));
+  const bool fuse = !materialize && d->halo_convs;
+  // cat0[0:64] = up(d1b): with fuse, L8's dX writes L7's dZ into dzB
+  const UpGrad ug0{dzB, 64, d->d1b.p, 64, 64};
   CVD_TRY(conv_bwd(d, L[8], d->cat0.p, 96, d->d0a.p, 32, nullptr, 0, dzA, true, d->g_cat0.p, 96,
-                   false, nullptr, H, W));
-  // cat0[0:64] = up(d1b)
-  hipLaunchKernelGGL(cvd::k_upsample_grad, dim3(grid1((size_t)nb * H1 * W1 * 16)), dim3(256), 0, s,
-                     d->g_cat0.p, 96u, d->g_d1b.p, 64u, 64u, nb, H1, W1);
-  CVD_TRY(conv_bwd(d, L[7], d->d1a.p, 64, d->d1b.p, 64, d->g_d1b.p, 64, dzB, false, nullptr, 0,
+                   false, nullptr, H, W, fuse ? &ug0 : nullptr));
+  if (!fuse)
+    hipLaunchKernelGGL(cvd::k_upsample_grad, dim3(grid1((size_t)nb * H1 * W1 * 16)), dim3(256), 0,
+                       s, d->g_cat0.p, 96u, d->g_d1b.p, 64u, 64u, nb, H1, W1);
+  CVD_TRY(conv_bwd(d, L[7], d->d1a.p, 64, d->d1b.p, 64, d->g_d1b.p, 64, dzB, fuse, nullptr, 0,
                    false, dzA, H1, W1));
+  // cat1[0:128] = up(b2b): with fuse, L6's dX writes L5's dZ into dzB
+  const UpGrad ug1{dzB, 128, d->b2b.p, 128, 128};
   CVD_TRY(conv_bwd(d, L[6], d->cat1.p, 192, d->d1a.p, 64, nullptr, 0, dzA, true, d->g_cat1.p, 192,
-                   false, nullptr, H1, W1));
-  // cat1[0:128] = up(b2b)
-  hipLaunchKernelGGL(cvd::k_upsample_grad, dim3(grid1((size_t)nb * H2 * W2 * 32)), dim3(256), 0,
-                     s, d->g_cat1.p, 192u, d->g_b2b.p, 128u, 128u, nb, H2, W2);
-  CVD_TRY(conv_bwd(d, L[5], d->b2a.p, 128, d->b2b.p, 128, d->g_b2b.p, 128, dzB, false, nullptr, 0,
+                   false, nullptr, H1, W1, fuse ? &ug1 : nullptr));
+  if (!fuse)
+    hipLaunchKernelGGL(cvd::k_upsample_grad, dim3(grid1((size_t)nb * H2 * W2 * 32)), dim3(256), 0,
+                       s, d->g_cat1.p, 192u, d->g_b2b.p, 128u, 128u, nb, H2, W2);
+  CVD_TRY(conv_bwd(d, L[5], d->b2a.p, 128, d->b2b.p, 128, d->g_b2b.p, 128, dzB, fuse, nullptr, 0,
                    false, dzA, H2, W2));
   CVD_TRY(conv_bwd(d, L[4], d->p1.p, 64, d->b2a.p, 128, nullptr, 0, dzA, true, d->g_p1.p, 64,
                    false, nullptr, H2, W2));
@@ -1879,11 +1965,12 @@ cv_status cv_denoiser_train_step(cv_denoiser* d, const float* input, const float
   hipStream_t s = d->stream;
   hipLaunchKernelGGL(cvd::k_to_input, dim3(grid1(P)), dim3(256), 0, s, in, d->x0.p,
                      (uint32_t)P);
-  if ((cs = forward(d, true, false)) != CV_OK) return cs;
+  const bool mat = flags & CV_DENOISE_MATERIALIZE;
+  if ((cs = forward(d, true, mat)) != CV_OK) return cs;
   hipLaunchKernelGGL(cvd::k_fill, dim3(1), dim3(64), 0, s, d->loss.p, (size_t)1, 0.0f);
   hipLaunchKernelGGL(cvd::k_l1_loss, dim3(std::min<size_t>(grid1(P * 3), 1024)), dim3(256), 0, s,
                      d->out.p, tg, d->dout.p, (uint32_t)P, d->loss.p);
-  if ((cs = backward(d)) != CV_OK) return cs;
+  if ((cs = backward(d, mat)) != CV_OK) return cs;
   if (lr > 0.0f) {
     d->step++;
     const float b1 = 0.9f, b2 = 0.999f;

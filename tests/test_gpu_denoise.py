@@ -60,6 +60,23 @@ def test_fused_upsampling_forward_is_exact(cv, nb, h, w):
     assert fused.tobytes() == ref.tobytes()
 
 
+@pytest.mark.parametrize("nb,h,w", [(2, 16, 64), (3, 36, 64), (2, 256, 256)])
+def test_fused_training_step_is_exact(cv, nb, h, w):
+    """Pooling in the encoder epilogues and the upsampling gradients in the
+    decoder's input-gradient convolutions (writing the layer below's dZ)
+    give the gradients, loss and Adam step of the separate kernels, bit for
+    bit."""
+    inp, target = _random_batch(14, nb=nb, h=h, w=w)
+    d1, d2 = cv.Denoiser(15), cv.Denoiser(15)
+    l1 = d1.train_step(inp, target, 0.0)
+    l2 = d2.train_step(inp, target, 0.0, materialize=True)
+    assert d1.grads().tobytes() == d2.grads().tobytes()
+    assert l1 == l2 or abs(l1 - l2) <= 1e-6 * abs(l2)  # the loss is a float atomic sum
+    d1.train_step(inp, target, 1e-3)
+    d2.train_step(inp, target, 1e-3, materialize=True)
+    assert d1.params().tobytes() == d2.params().tobytes()
+
+
 def test_gradients_match_torch(cv):
     d = cv.Denoiser(5)
     p = d.params()
