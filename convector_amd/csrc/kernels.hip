@@ -77,35 +77,49 @@ __device__ __forceinline__ float entry_tmin(entry_t e) { return u2f((uint32_t)(e
 // deeper entries in a global spill area (stride = all persistent lanes).
 // The pointers are address-space qualified so that LDS accesses compile to
 // ds_read_b64 / ds_write_b64 and spill accesses to global loads / stores.
+// The top entry (sp - 1) is held in a register and entries [0, sp - 1) in
+// memory: a pop returns the register at once and reloads the next top,
+// which is not needed before the next pop, so no step waits on the stack
+// read; a push stores the previous top.
 template <int S, bool PACKED>
 struct Stack {
   lds_u64* lds;
   glb_u64* spill;
   uint32_t stride;
   int sp;
+  entry_t top;
   __device__ __forceinline__ Stack(uint2* l, uint2* g, uint32_t str, int p)
-      : lds((lds_u64*)l), spill((glb_u64*)g), stride(str), sp(p) {}
-  // Pushes e when p is set.  Below S the slot at sp is written either way
-  // (it is above the top of the stack, so an unused write is harmless).
+      : lds((lds_u64*)l), spill((glb_u64*)g), stride(str), sp(p), top(0) {}
+  // Pushes e when p is set.  The previous top goes to slot sp - 1, which
+  // lies above the memory part of the stack, so when p is not set (or the
+  // stack is empty, slot 0) the LDS write is harmless and is done anyway.
   // The LDS-only case is tested for the whole wave, so the common path has
   // no divergent branch.
   __device__ __forceinline__ void push_if(bool p, entry_t e) {
-    if (__builtin_expect(!__any(sp >= S), 1)) {
-      lds[sp * 64] = e;
-      sp += (int)p;
-    } else if (sp < S) {
-      lds[sp * 64] = e;
-      sp += (int)p;
+    const int k = sp > 0 ? sp - 1 : 0;
+    if (__builtin_expect(!__any(k >= S), 1)) {
+      lds[k * 64] = top;
+    } else if (k < S) {
+      lds[k * 64] = top;
     } else if (p) {
-      spill[(size_t)(sp - S) * stride] = e;
+      spill[(size_t)(k - S) * stride] = top;
+    }
+    if (p) {
+      top = e;
       sp++;
     }
   }
   __device__ __forceinline__ entry_t pop() {
+    const entry_t e = top;
     sp--;
-    if (__builtin_expect(!__any(sp >= S), 1)) return lds[sp * 64];
-    if (sp < S) return lds[sp * 64];
-    return spill[(size_t)(sp - S) * stride];
+    const int k = sp > 0 ? sp - 1 : 0;  // the new top (slot 0 when empty: unused)
+    if (__builtin_expect(!__any(k >= S), 1))
+      top = lds[k * 64];
+    else if (k < S)
+      top = lds[k * 64];
+    else
+      top = spill[(size_t)(k - S) * stride];
+    return e;
   }
 };
 
