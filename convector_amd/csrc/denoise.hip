@@ -1093,6 +1093,35 @@ __global__ void k_pool_grad(const float* dy, uint32_t dycs, float* dx, uint32_t 
   }
 }
 
+// k_pool_grad followed by k_relu_grad8 in one pass: for each pixel of the
+// full-resolution gradient g (c channels, stride gcs) the pooled gradient's
+// share is added as k_pool_grad adds it (g + 0.25 * gp of the parent pixel,
+// stride gpcs, h2 x w2 per image), masked by the ReLU output y (stride ycs)
+// and written as bf16 dZ (stride c), 8 channels per thread.
+__global__ void k_pool_relu_grad(const float* g, uint32_t gcs, const float* gp, uint32_t gpcs,
+                                 const uint16_t* y, uint32_t ycs, uint16_t* dz, uint32_t c,
+                                 uint32_t nb, uint32_t h2, uint32_t w2) {
+  const uint32_t gr = c / 8, n = nb * (2 * h2) * (2 * w2) * gr;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t ch = (i % gr) * 8, p = i / gr;
+    const uint32_t X = p % (2 * w2), t = p / (2 * w2), Y = t % (2 * h2), B = t / (2 * h2);
+    const uint32_t q = (B * h2 + Y / 2) * w2 + X / 2;
+    const float4 a0 = *reinterpret_cast<const float4*>(g + (size_t)p * gcs + ch);
+    const float4 a1 = *reinterpret_cast<const float4*>(g + (size_t)p * gcs + ch + 4);
+    const float4 b0 = *reinterpret_cast<const float4*>(gp + (size_t)q * gpcs + ch);
+    const float4 b1 = *reinterpret_cast<const float4*>(gp + (size_t)q * gpcs + ch + 4);
+    float v[8] = {a0.x + 0.25f * b0.x, a0.y + 0.25f * b0.y, a0.z + 0.25f * b0.z,
+                  a0.w + 0.25f * b0.w, a1.x + 0.25f * b1.x, a1.y + 0.25f * b1.y,
+                  a1.z + 0.25f * b1.z, a1.w + 0.25f * b1.w};
+    float yv[8];
+    unpack8(*reinterpret_cast<const uint4*>(y + (size_t)p * ycs + ch), yv);
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+      if (!(yv[k] > 0.0f)) v[k] = 0.0f;
+    *reinterpret_cast<uint4*>(dz + (size_t)p * c + ch) = pack8(v);
+  }
+}
+
 // Nearest 2x upsampling, BF16.
 __global__ void k_upsample(const uint16_t* x, uint32_t xcs, uint16_t* y, uint32_t ycs, uint32_t c,
                            uint32_t nb, uint32_t h, uint32_t w) {
@@ -1753,15 +1782,26 @@ cv_status backward(cv_denoiser* d, bool materialize) {
   CVD_TRY(conv_bwd(d, L[4], d->p1.p, 64, d->b2a.p, 128, nullptr, 0, dzA, true, d->g_p1.p, 64,
                    false, nullptr, H2, W2));
   // p1 = pool(cat1[128:192]): its gradient joins the skip gradient in g_cat1
-  hipLaunchKernelGGL(cvd::k_pool_grad, dim3(grid1((size_t)nb * H2 * W2 * 16)), dim3(256), 0, s,
-                     d->g_p1.p, 64u, d->g_cat1.p + 128, 192u, 64u, nb, H2, W2);
+  // (with fuse, in the same pass as L3's ReLU gradient, into dzB)
+  if (fuse)
+    hipLaunchKernelGGL(cvd::k_pool_relu_grad, dim3(grid1((size_t)nb * H1 * W1 * 8)), dim3(256), 0,
+                       s, d->g_cat1.p + 128, 192u, d->g_p1.p, 64u, d->cat1.p + 128, 192u, dzB,
+                       64u, nb, H2, W2);
+  else
+    hipLaunchKernelGGL(cvd::k_pool_grad, dim3(grid1((size_t)nb * H2 * W2 * 16)), dim3(256), 0, s,
+                       d->g_p1.p, 64u, d->g_cat1.p + 128, 192u, 64u, nb, H2, W2);
   CVD_TRY(conv_bwd(d, L[3], d->e1a.p, 64, d->cat1.p + 128, 192, d->g_cat1.p + 128, 192, dzB,
-                   false, nullptr, 0, false, dzA, H1, W1));
+                   fuse, nullptr, 0, false, dzA, H1, W1));
   CVD_TRY(conv_bwd(d, L[2], d->p0.p, 32, d->e1a.p, 64, nullptr, 0, dzA, true, d->g_p0.p, 32,
                    false, nullptr, H1, W1));
-  hipLaunchKernelGGL(cvd::k_pool_grad, dim3(grid1((size_t)nb * H1 * W1 * 8)), dim3(256), 0, s,
-                     d->g_p0.p, 32u, d->g_cat0.p + 64, 96u, 32u, nb, H1, W1);
-  CVD_TRY(conv_bwd(d, L[1], d->e0a.p, 32, d->cat0.p + 64, 96, d->g_cat0.p + 64, 96, dzB, false,
+  if (fuse)
+    hipLaunchKernelGGL(cvd::k_pool_relu_grad, dim3(grid1((size_t)nb * H * W * 4)), dim3(256), 0, s,
+                       d->g_cat0.p + 64, 96u, d->g_p0.p, 32u, d->cat0.p + 64, 96u, dzB, 32u, nb,
+                       H1, W1);
+  else
+    hipLaunchKernelGGL(cvd::k_pool_grad, dim3(grid1((size_t)nb * H1 * W1 * 8)), dim3(256), 0, s,
+                       d->g_p0.p, 32u, d->g_cat0.p + 64, 96u, 32u, nb, H1, W1);
+  CVD_TRY(conv_bwd(d, L[1], d->e0a.p, 32, d->cat0.p + 64, 96, d->g_cat0.p + 64, 96, dzB, fuse,
                    nullptr, 0, false, dzA, H, W));
   CVD_TRY(conv_bwd(d, L[0], d->x0.p, 16, d->e0a.p, 32, nullptr, 0, dzA, true, nullptr, 0, false,
                    nullptr, H, W));
