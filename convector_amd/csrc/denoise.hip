@@ -607,6 +607,11 @@ struct WGradArgs {
   uint32_t pix_per_block;
   float* part;         // k_wgrad_t: per pixel split [z][cout * 9 * cin + cout]
   uint32_t tiles_per_split;  // k_wgrad_halo: 8 x 32 pixel tiles per split
+  // non-null (k_wgrad_halo<..., UP>): input channels [0, up_ch) are the
+  // nearest 2x upsampling of xup (pixel stride xupcs, h / 2 x w / 2), read
+  // from it directly (a multiple of the block's input channels)
+  const uint16_t* xup;
+  uint32_t xupcs, up_ch;
 };
 
 // 8 pixels x 8 bf16 channels (in[i] = pixel i) to 8 channels x 8 pixels
@@ -810,7 +815,7 @@ constexpr uint32_t kWhCols = 48;                    // X^T row: pixels x0 - 8 ..
 constexpr uint32_t kWhCi = 10 * kWhCols + 8;        // X^T stride per input channel (bf16)
 constexpr uint32_t kWhCo = 8 * 32 + 8;              // dZ^T stride per output channel
 
-template <int NS, int CT>
+template <int NS, int CT, bool UP = false>
 __global__ __launch_bounds__(64 * NS * CT) void k_wgrad_halo(WGradArgs a) {
   constexpr int BS = 64 * NS * CT;
   constexpr int CC = 16 * CT, CO = 16 * NS;
@@ -833,6 +838,8 @@ __global__ __launch_bounds__(64 * NS * CT) void k_wgrad_halo(WGradArgs a) {
   // the tile's pieces into registers: a piece is 8 channels of two
   // neighbouring pixels (X: columns 6 + 2 k of the halo row, i.e. pixels
   // x0 - 2 + 2 k; dZ: pixels x0 + 2 k)
+  // UP: this block's input channels are upsampled ones (block-uniform)
+  const bool from_up = UP && ci0 < a.up_ch;
   auto load_tile = [&](uint32_t t) {
     const uint32_t tx = t % ntx, ty = (t / ntx) % nty, img = t / (ntx * nty);
     const int x0 = (int)(tx * 32), y0 = (int)(ty * 8);
@@ -845,10 +852,19 @@ __global__ __launch_bounds__(64 * NS * CT) void k_wgrad_halo(WGradArgs a) {
 #pragma unroll
       for (int e = 0; e < 2; e++) {
         const int px = x0 - 2 + 2 * pr + e;
-        xr[i][e] = (q < XP && sy >= 0 && sy < (int)a.h && px >= 0 && px < (int)a.w_)
-                       ? *reinterpret_cast<const u32x4*>(
-                             a.x + (base + (size_t)sy * a.w_ + (uint32_t)px) * a.xcs + ci0 + 8 * g8)
-                       : u32x4{0u, 0u, 0u, 0u};
+        const bool in = q < XP && sy >= 0 && sy < (int)a.h && px >= 0 && px < (int)a.w_;
+        if (from_up)
+          xr[i][e] = in ? *reinterpret_cast<const u32x4*>(
+                              a.xup +
+                              (base / 4 + (size_t)(sy >> 1) * (a.w_ >> 1) + ((uint32_t)px >> 1)) *
+                                  a.xupcs +
+                              ci0 + 8 * g8)
+                        : u32x4{0u, 0u, 0u, 0u};
+        else
+          xr[i][e] = in ? *reinterpret_cast<const u32x4*>(
+                              a.x + (base + (size_t)sy * a.w_ + (uint32_t)px) * a.xcs + ci0 +
+                              8 * g8)
+                        : u32x4{0u, 0u, 0u, 0u};
       }
     }
 #pragma unroll
@@ -1492,6 +1508,14 @@ struct UpGrad {
   uint32_t ycs, ch;
 };
 
+// A layer input whose first `ch` channels are the nearest 2x upsampling of
+// x (pixel stride cs) and not written out: the weight gradient reads them
+// from x.
+struct UpIn {
+  const uint16_t* x;
+  uint32_t cs, ch;
+};
+
 cv_status conv_bwd_input(cv_denoiser* d, const Conv& L, const uint16_t* dz, float* gx,
                          uint32_t gxcs, bool accumulate, const uint16_t* x, uint32_t xcs,
                          uint16_t* dz_next, uint32_t npix, uint32_t h, uint32_t w,
@@ -1506,7 +1530,8 @@ cv_status conv_bwd_input(cv_denoiser* d, const Conv& L, const uint16_t* dz, floa
 cv_status conv_bwd(cv_denoiser* d, const Conv& L, const uint16_t* x, uint32_t xcs,
                    const uint16_t* y, uint32_t ycs, const float* gy, uint32_t gycs, uint16_t* dz,
                    bool dz_ready, float* gx, uint32_t gxcs, bool accumulate, uint16_t* dz_next,
-                   uint32_t h, uint32_t w, const UpGrad* ug = nullptr) {
+                   uint32_t h, uint32_t w, const UpGrad* ug = nullptr,
+                   const UpIn* up = nullptr) {
   const uint32_t npix = d->nb * h * w;
   hipStream_t s = d->stream;
   if (!dz_ready) {
@@ -1532,6 +1557,11 @@ cv_status conv_bwd(cv_denoiser* d, const Conv& L, const uint16_t* x, uint32_t xc
   g.npix = npix;
   g.h = h;
   g.w_ = w;
+  if (up) {
+    g.xup = up->x;
+    g.xupcs = up->cs;
+    g.up_ch = up->ch;
+  }
   const uint32_t nw = L.cout * 9 * L.cin;
   if (w % 32 == 0 && npix % (h * w) == 0 && L.cin % 16 == 0 && L.cout_pad % 16 == 0 &&
       xcs % 8 == 0 && (uintptr_t)x % 16 == 0) {
@@ -1549,7 +1579,13 @@ cv_status conv_bwd(cv_denoiser* d, const Conv& L, const uint16_t* x, uint32_t xc
     HIPTRY(d->wpart.reserve((size_t)splits * (nw + L.cout)), "alloc");
     g.part = d->wpart.p;
     const dim3 grid(L.cin / (16 * CT), L.cout_pad / (16 * NS), splits), block(64 * NS * CT);
-    if (NS == 2 && CT == 2)
+    if (up) {
+      // whole blocks of 32 input channels upsampled or not, 16-byte pieces
+      if (NS != 2 || CT != 2 || up->ch % 32 || up->cs % 8 || (uintptr_t)up->x % 16 || h % 2 ||
+          w % 2)
+        return fail(CV_ERR_INVALID_ARGUMENT, "upsampled input needs the halo weight gradient");
+      hipLaunchKernelGGL((cvd::k_wgrad_halo<2, 2, true>), grid, block, 0, s, g);
+    } else if (NS == 2 && CT == 2)
       hipLaunchKernelGGL((cvd::k_wgrad_halo<2, 2>), grid, block, 0, s, g);
     else if (NS == 2)
       hipLaunchKernelGGL((cvd::k_wgrad_halo<2, 1>), grid, block, 0, s, g);
@@ -1563,6 +1599,7 @@ cv_status conv_bwd(cv_denoiser* d, const Conv& L, const uint16_t* x, uint32_t xc
     HIPTRY(hipGetLastError(), "wgrad reduce");
     return conv_bwd_input(d, L, dz, gx, gxcs, accumulate, x, xcs, dz_next, npix, h, w, ug);
   }
+  if (up) return fail(CV_ERR_INVALID_ARGUMENT, "upsampled input needs the halo weight gradient");
   // about 2048 blocks: pixel splits of a multiple of 64 pixels
   const uint32_t col_tiles = (9 * L.cin + cvd::kWgCols - 1) / cvd::kWgCols;
   uint32_t splits = std::max(1u, std::min(npix / 2048u, 2048u / col_tiles));
@@ -1683,11 +1720,11 @@ cv_status pack_weights(cv_denoiser* d) {
 }
 
 // Forward pass from d->x0 to d->out (f32 log1p radiance, [p][3]).  With
-// `materialize` (training: the backward reads them) the upsampled halves of
-// cat0 / cat1 are written out; otherwise the decoder convolutions read them
-// from the lower level directly, the same values.  The training step passes
-// materialize_pool = false: the pooled tensors come from the encoder
-// convolutions' epilogues either way, except on the fully materialized path.
+// `materialize` the upsampled halves of cat0 / cat1 are written out by
+// k_upsample; otherwise the decoder convolutions read them from the lower
+// level directly (the same values; a training step's weight gradients then
+// read them there too).  With materialize_pool k_pool writes the pooled
+// encoder outputs; otherwise the encoder convolutions' epilogues do.
 cv_status forward(cv_denoiser* d, bool materialize, bool materialize_pool) {
   const uint32_t H = d->h, W = d->w, H1 = H / 2, W1 = W / 2, H2 = H / 4, W2 = W / 4;
   auto& L = d->conv;
@@ -1766,8 +1803,11 @@ cv_status backward(cv_denoiser* d, bool materialize) {
   const bool fuse = !materialize && d->halo_convs;
   // cat0[0:64] = up(d1b): with fuse, L8's dX writes L7's dZ into dzB
   const UpGrad ug0{dzB, 64, d->d1b.p, 64, 64};
+  // with fuse the forward did not write cat0[0:64]: the weight gradient
+  // reads the upsampled channels from d1b
+  const UpIn up0{d->d1b.p, 64, 64};
   CVD_TRY(conv_bwd(d, L[8], d->cat0.p, 96, d->d0a.p, 32, nullptr, 0, dzA, true, d->g_cat0.p, 96,
-                   false, nullptr, H, W, fuse ? &ug0 : nullptr));
+                   false, nullptr, H, W, fuse ? &ug0 : nullptr, fuse ? &up0 : nullptr));
   if (!fuse)
     hipLaunchKernelGGL(cvd::k_upsample_grad, dim3(grid1((size_t)nb * H1 * W1 * 16)), dim3(256), 0,
                        s, d->g_cat0.p, 96u, d->g_d1b.p, 64u, 64u, nb, H1, W1);
@@ -1775,8 +1815,9 @@ cv_status backward(cv_denoiser* d, bool materialize) {
                    false, dzA, H1, W1));
   // cat1[0:128] = up(b2b): with fuse, L6's dX writes L5's dZ into dzB
   const UpGrad ug1{dzB, 128, d->b2b.p, 128, 128};
+  const UpIn up1{d->b2b.p, 128, 128};
   CVD_TRY(conv_bwd(d, L[6], d->cat1.p, 192, d->d1a.p, 64, nullptr, 0, dzA, true, d->g_cat1.p, 192,
-                   false, nullptr, H1, W1, fuse ? &ug1 : nullptr));
+                   false, nullptr, H1, W1, fuse ? &ug1 : nullptr, fuse ? &up1 : nullptr));
   if (!fuse)
     hipLaunchKernelGGL(cvd::k_upsample_grad, dim3(grid1((size_t)nb * H2 * W2 * 32)), dim3(256), 0,
                        s, d->g_cat1.p, 192u, d->g_b2b.p, 128u, 128u, nb, H2, W2);
@@ -2009,7 +2050,7 @@ cv_status cv_denoiser_train_step(cv_denoiser* d, const float* input, const float
   hipLaunchKernelGGL(cvd::k_to_input, dim3(grid1(P)), dim3(256), 0, s, in, d->x0.p,
                      (uint32_t)P);
   const bool mat = flags & CV_DENOISE_MATERIALIZE;
-  if ((cs = forward(d, true, mat)) != CV_OK) return cs;
+  if ((cs = forward(d, mat, mat)) != CV_OK) return cs;
   hipLaunchKernelGGL(cvd::k_fill, dim3(1), dim3(64), 0, s, d->loss.p, (size_t)1, 0.0f);
   hipLaunchKernelGGL(cvd::k_l1_loss, dim3(std::min<size_t>(grid1(P * 3), 1024)), dim3(256), 0, s,
                      d->out.p, tg, d->dout.p, (uint32_t)P, d->loss.p);
