@@ -205,6 +205,10 @@ hipError_t launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t s);
 hipError_t launch_fold_hdr(float* hdr, float* slots, size_t slot_stride, uint32_t count,
                            uint32_t width, uint32_t x, uint32_t y, uint32_t w, uint32_t h,
                            hipStream_t s);
+// features += slots[0] + ... + slots[count-1] (8 floats per pixel, in that
+// order), clearing the slots.
+hipError_t launch_fold_features(float* feat, float* slots, size_t slot_stride, uint32_t count,
+                                uint32_t npix, hipStream_t s);
 
 // ---------------------------------------------------------------- builder
 

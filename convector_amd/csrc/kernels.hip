@@ -1297,6 +1297,22 @@ __global__ void k_fold_hdr(float* __restrict__ hdr, float* __restrict__ slots, s
   }
 }
 
+// The same for the denoiser's first-hit features (8 floats per pixel,
+// row-major over n pixels): features += slots[0] + ... in frame order, the
+// slots cleared.  One thread per float.
+__global__ void k_fold_features(float* __restrict__ feat, float* __restrict__ slots,
+                                size_t slot_stride, uint32_t count, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    float v = feat[i];
+    for (uint32_t k = 0; k < count; k++) {
+      v = v + slots[k * slot_stride + i];
+      slots[k * slot_stride + i] = 0.0f;
+    }
+    feat[i] = v;
+  }
+}
+
 __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0u;
@@ -1731,6 +1747,15 @@ hipError_t launch_fold_hdr(float* hdr, float* slots, size_t slot_stride, uint32_
   if (count == 0 || npix == 0) return hipSuccess;
   hipLaunchKernelGGL(k_fold_hdr, dim3(std::min<uint32_t>((npix + 255) / 256, 8192)), dim3(256), 0,
                      s, hdr, slots, slot_stride, count, width, x, y, w, npix);
+  return hipGetLastError();
+}
+
+hipError_t launch_fold_features(float* feat, float* slots, size_t slot_stride, uint32_t count,
+                                uint32_t npix, hipStream_t s) {
+  const size_t n = (size_t)npix * 8;
+  if (count == 0 || n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fold_features, dim3((uint32_t)std::min<size_t>((n + 255) / 256, 8192)),
+                     dim3(256), 0, s, feat, slots, slot_stride, count, n);
   return hipGetLastError();
 }
 

@@ -43,6 +43,11 @@ struct FoldDesc {
   float* slots = nullptr;
   size_t stride = 0;
   uint32_t count = 0, width = 0, rect[4] = {0, 0, 0, 0};
+  // non-null: the features of the same frames, folded likewise
+  float* feat = nullptr;
+  float* fslots = nullptr;
+  size_t fstride = 0;
+  uint32_t fnpix = 0;
 };
 
 struct StepPlan {
@@ -488,6 +493,8 @@ cv_status pipeline_run(Pipeline& p, const std::vector<StepPlan>& steps, const Ru
     for (const FoldDesc& fd : s.folds) {
       hipError_t fe = cvk::launch_fold_hdr(fd.hdr, fd.slots, fd.stride, fd.count, fd.width,
                                            fd.rect[0], fd.rect[1], fd.rect[2], fd.rect[3], st);
+      if (fe == hipSuccess && fd.feat)
+        fe = cvk::launch_fold_features(fd.feat, fd.fslots, fd.fstride, fd.count, fd.fnpix, st);
       if (fe != hipSuccess) {
         status = hip_fail(fe, "fold");
         break;
@@ -1500,9 +1507,9 @@ void serve_loop(cv_server* sv) {
     // one step: the next frame of every active request, then further
     // frames (frame-major, in admission order) while the step's path
     // capacity lasts, so that few requests still fill the GPU.  A further
-    // frame accumulates into the request's own zeroed buffer and is folded
-    // into its image in frame order after the step; denoised requests sum
-    // their features in place and take one frame per step.
+    // frame accumulates into the request's own zeroed buffer (and, for a
+    // denoised request, its first-hit features into another) and is folded
+    // into its image (features) in frame order after the step.
     std::vector<uint32_t> take(act.size(), 1);
     {
       size_t paths = 0;
@@ -1512,7 +1519,7 @@ void serve_loop(cv_server* sv) {
         for (size_t k = 0; k < act.size(); k++) {
           const ServeReq& r = *act[k];
           const size_t npix = (size_t)r.req.width * r.req.height;
-          if (r.req.denoise || r.frames_done + take[k] >= r.req.spp || take[k] >= 64 ||
+          if (r.frames_done + take[k] >= r.req.spp || take[k] >= 64 ||
               paths + npix > sv->max_paths)
             continue;
           take[k]++;
@@ -1524,7 +1531,9 @@ void serve_loop(cv_server* sv) {
       std::lock_guard<std::mutex> lk(sv->mu);
       for (size_t k = 0; k < act.size(); k++) {
         ServeReq& r = *act[k];
-        const size_t frame_floats = (size_t)r.req.width * r.req.height * 3;
+        // slots: the further frames' HDR buffers, then their features
+        const size_t npix = (size_t)r.req.width * r.req.height;
+        const size_t frame_floats = npix * 3 + (r.req.denoise ? npix * 8 : 0);
         const size_t need = (size_t)(take[k] - 1) * frame_floats;
         if (need <= r.slots.n) continue;
         const size_t extra = (need - r.slots.n) * sizeof(float);
@@ -1567,11 +1576,16 @@ void serve_loop(cv_server* sv) {
       d.height = q.height;
       d.rect_w = q.width;
       d.rect_h = q.height;
-      d.features = q.denoise ? r.feat.p : nullptr;
-      const size_t frame_floats = (size_t)q.width * q.height * 3;
+      const size_t npix = (size_t)q.width * q.height;
+      const size_t frame_floats = npix * 3, feat_floats = npix * 8;
+      // the feature slots follow the HDR slots of this step's frames
+      float* const fslots = r.slots.p + (size_t)(take[k] - 1) * frame_floats;
       for (uint32_t f = 0; f < take[k]; f++) {
         d.frame = 1 + r.frames_done + f;
         d.hdr = f == 0 ? r.hdr.p : r.slots.p + (size_t)(f - 1) * frame_floats;
+        d.features = !q.denoise ? nullptr
+                     : f == 0   ? r.feat.p
+                                : fslots + (size_t)(f - 1) * feat_floats;
         d.path_base = step.paths;
         step.jobs.push_back(d);
         step.paths += q.width * q.height;
@@ -1585,6 +1599,12 @@ void serve_loop(cv_server* sv) {
         fd.width = q.width;
         fd.rect[2] = q.width;
         fd.rect[3] = q.height;
+        if (q.denoise) {
+          fd.feat = r.feat.p;
+          fd.fslots = fslots;
+          fd.fstride = feat_floats;
+          fd.fnpix = (uint32_t)npix;
+        }
         step.folds.push_back(fd);
       }
       max_depth = std::max(max_depth, cvh::stack_depth(sc, width));
