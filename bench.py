@@ -666,6 +666,10 @@ def bench_denoise_kernels(args):
     time a denoised frame pays), then --steps Adam steps at batch 16 of
     256x256 (forward, backward, update).  No rendering, so a kernel trace of
     this run holds the network's launches only."""
+    print(json.dumps(run_denoise_kernels(args)), flush=True)
+
+
+def run_denoise_kernels(args):
     import convector_amd as cv
 
     W, H = 1280, 736
@@ -730,7 +734,7 @@ def bench_denoise_kernels(args):
                      "note": "whole network (%.3g FLOP of 3x3 convolutions) over its wall "
                              "time, pooling and upsampling included" % flops},
     }
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def _hip_copy(hip, dst, src, nbytes, kind):
@@ -1081,6 +1085,8 @@ def main():
                     help="frame workload: leave out the nested 64-job batch result")
     ap.add_argument("--no-wide", action="store_true",
                     help="frame workload: leave out the width-4 (tolerance mode) step")
+    ap.add_argument("--no-denoiser", action="store_true",
+                    help="frame workload: leave out the nested denoiser network timings")
     ap.add_argument("--batch-bins", type=int, default=32, choices=(16, 32, 64),
                     help="batch workload: centroid bins of the GPU BVH builds (64: the "
                          "reference's)")
@@ -1339,6 +1345,23 @@ def main():
         except Exception as e:
             log("batch workload failed: %r" % (e,))
             out["batch"] = None
+    if not args.no_denoiser:
+        # the learned denoiser's kernels (config 3's network): one 1280x736
+        # forward and a batch-16 training step, timed in the same run
+        dargs = argparse.Namespace(**vars(args))
+        dargs.steps, dargs.warmup = 20, 3
+        try:
+            do = run_denoise_kernels(dargs)
+            out["denoiser"] = {
+                "workload": do["config"]["workload"],
+                "forward_ms_1280x736": do["value"],
+                "train_ms_batch16_256": do["train_ms_per_step"],
+                "dtype": "bf16",
+                "roofline": do["roofline"],
+            }
+        except Exception as e:
+            log("denoiser kernels failed: %r" % (e,))
+            out["denoiser"] = None
     print(json.dumps(out), flush=True)
 
 

@@ -826,7 +826,8 @@ __global__ __launch_bounds__(256) void k_extend_refill(ExtendArgs a) {
     return (c << glog) + ((full & (parts - 1)) == x ? (n & gmask) : 0u);
   };
   auto part_cursor = [&](uint32_t x) -> uint32_t* {
-    return parts == 1 ? &a.ctrl->ext_cursor[a.bounce] : &a.ctrl->part_cursor[a.bounce][x * kPartStride];
+    return parts == 1 ? &a.ctrl->ext_cursor[a.bounce]
+                      : &a.ctrl->part_cursor[a.bounce][x * kPartStride];
   };
   uint32_t part = 0;
   if (parts > 1)
