@@ -889,7 +889,8 @@ __global__ __launch_bounds__(256) void k_extend_refill(ExtendArgs a) {
         if (old || l < end1) {
           const uint32_t x = old ? part0 : pool_part;
           const uint32_t k = parts == 1 ? l : (((l >> glog) * parts + x) << glog) | (l & gmask);
-          const uint32_t i = a.order ? a.order[k] : k;
+          // hits-only queries are never sorted
+          const uint32_t i = !QUERY && a.order ? a.order[k] : k;
           if (MULTI) {
             const float4 r0 = a.r0[i], r1 = a.r1[i];
             o = mk3(r0.x, r0.y, r0.z);
