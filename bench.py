@@ -831,8 +831,9 @@ def bench_rays(args):
             scene = cv.Scene.from_meshes([mesh])
             info = scene.bvh_info()
             tun = scene.tuning()
-            kernel = "k_extend_refill<false, %d, %s, false, true>" % (
-                tun["stack_entries"], "true" if info["max_leaf"] <= 31 else "false")
+            kernel = "k_extend_refill<false, %d, %s, false, true, %d>" % (
+                tun["stack_entries"], "true" if info["max_leaf"] <= 31 else "false",
+                tun["bvh_width"] or 2)
             for coh in (False, True):
                 label = "%s %s" % (name, "coherent" if coh else "decoherent")
                 # counting pass: full cv_isect output with per-ray box and
