@@ -320,31 +320,37 @@ __global__ __launch_bounds__(256, 2) void k_conv_halo(ConvArgs a) {
     woff[i] = (row0 + (uint32_t)r) * a.kpad + (uint32_t)k;
   }
   u32x4 xr[XPT], wr[WPT];
+  // Input pieces outside the image are loaded from the tensor's first
+  // pixel (every load is unconditional, so that the compiler does not wait
+  // for each one to merge it with a zero inside the load sequence) and
+  // zeroed when stored to LDS.
+  // (UP: the chunk's source, the lower level or the skip tensor, is chosen
+  // by selects rather than a branch, for the same reason.)
 #define CV_HALO_LOAD(c)                                                                   \
   {                                                                                       \
-    if (UP && (uint32_t)(c) < a.up_chunks) {                                              \
-      _Pragma("unroll") for (int i = 0; i < XPT; i++) xr[i] =                             \
-          xoff[i] != ~0u ? *reinterpret_cast<const u32x4*>(a.xup + up_offset(i) + (c) * CC) \
-                         : u32x4{0u, 0u, 0u, 0u};                                         \
-    } else {                                                                              \
-      _Pragma("unroll") for (int i = 0; i < XPT; i++) xr[i] =                             \
-          xoff[i] != ~0u ? *reinterpret_cast<const u32x4*>(a.x + xoff[i] + (c) * CC)       \
-                         : u32x4{0u, 0u, 0u, 0u};                                         \
-    }                                                                                     \
+    const bool from_up = UP && (uint32_t)(c) < a.up_chunks;                               \
+    const uint16_t* src = from_up ? a.xup : a.x;                                          \
+    _Pragma("unroll") for (int i = 0; i < XPT; i++) xr[i] =                               \
+        *reinterpret_cast<const u32x4*>(                                                  \
+            src + (xoff[i] == ~0u ? 0u : from_up ? up_offset(i) : xoff[i]) + (c) * CC);   \
     _Pragma("unroll") for (int i = 0; i < WPT; i++) wr[i] =                               \
         *reinterpret_cast<const u32x4*>(a.w + woff[i] + (c) * 32);                        \
   }
 #define CV_HALO_LOAD_X()                                                                  \
   {                                                                                       \
     _Pragma("unroll") for (int i = 0; i < XPT; i++) xr[i] =                               \
-        xoff[i] != ~0u ? *reinterpret_cast<const u32x4*>(a.x + xoff[i])                    \
-                       : u32x4{0u, 0u, 0u, 0u};                                           \
+        *reinterpret_cast<const u32x4*>(a.x + (xoff[i] != ~0u ? xoff[i] : 0u));            \
   }
+  // The scheduling barrier keeps the zeroing selects, which wait for the
+  // loads, from being hoisted into the MFMAs that the loads overlap.
 #define CV_HALO_STORE_X()                                                                 \
   {                                                                                       \
+    __builtin_amdgcn_sched_barrier(0);                                                    \
     _Pragma("unroll") for (int i = 0; i < XPT; i++) {                                     \
       const int q = (int)tid + 256 * i;                                                   \
-      if (q < XP) *reinterpret_cast<u32x4*>(&xs[(q / PP) * RS + (q % PP) * 8]) = xr[i];   \
+      if (q < XP)                                                                         \
+        *reinterpret_cast<u32x4*>(&xs[(q / PP) * RS + (q % PP) * 8]) =                    \
+            xoff[i] != ~0u ? xr[i] : u32x4{0u, 0u, 0u, 0u};                               \
     }                                                                                     \
   }
 #define CV_HALO_STORE()                                                                   \
@@ -386,9 +392,8 @@ __global__ __launch_bounds__(256, 2) void k_conv_halo(ConvArgs a) {
       tile_offsets(next, nx0, ny0, nbase);
       CV_HALO_LOAD_X();
     }
-#pragma unroll 1
-    for (int c = 0; c < NCH; c++) {
-      if (c + 1 < NCH) CV_HALO_LOAD(c + 1);
+    // the MFMAs of the chunk in LDS
+    auto chunk_mfma = [&]() {
 #pragma unroll
       for (int st = 0; st < KS; st++) {
         // the lane's tap: with KC == 16 the upper 32 lanes take the odd one
@@ -416,12 +421,23 @@ __global__ __launch_bounds__(256, 2) void k_conv_halo(ConvArgs a) {
             acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Wf[t], X, acc[m][t], 0, 0, 0);
         }
       }
-      if (c + 1 < NCH) {
-        __syncthreads();
-        CV_HALO_STORE();
-        __syncthreads();
-      }
+    };
+    // Every chunk but the last loads the next one during its MFMAs.  The
+    // last is peeled off so that every path through the loop stores the
+    // loaded registers: with the store under a condition, the compiler
+    // takes the loads as possibly pending at the loop's head and waits
+    // for all of them there, before the MFMAs they were meant to overlap.
+#pragma unroll 1
+    for (int c = 0; c + 1 < NCH; c++) {
+      CV_HALO_LOAD(c + 1);
+      // (the loads are issued before the MFMAs, not scheduled among them)
+      __builtin_amdgcn_sched_barrier(0);
+      chunk_mfma();
+      __syncthreads();
+      CV_HALO_STORE();
+      __syncthreads();
     }
+    chunk_mfma();
     if constexpr (POOL) {
       // Fragments m (the upper row of the wave's pair) and m + 2 (the lower
       // row) together: both are stored, and the even lane of each column
