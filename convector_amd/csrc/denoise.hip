@@ -1046,23 +1046,28 @@ __device__ __forceinline__ uint4 pack8(const float* f) {
 }
 
 // dZ = dY * (Y > 0) for ReLU layers (or dY for linear ones), as BF16 with
-// the padded channels [c, zcs) zeroed.
+// the padded channels [c, zcs) zeroed (zcs a multiple of 8, dz 16-B
+// aligned): one thread per pixel, 8 channels per store.
 __global__ void k_relu_grad(const float* dy, uint32_t dycs, const uint16_t* y, uint32_t ycs,
                             uint32_t c, uint16_t* dz, uint32_t zcs, uint32_t npix, uint32_t relu) {
-  const uint32_t n = npix * zcs;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const uint32_t p = i / zcs;
-    const uint32_t ch = i % zcs;
-    float v = 0.0f;
-    if (ch < c) {
-      v = dy[(size_t)p * dycs + ch];
-      if (relu && !(from_bf16(y[(size_t)p * ycs + ch]) > 0.0f)) v = 0.0f;
+  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < npix;
+       p += gridDim.x * blockDim.x) {
+    for (uint32_t g = 0; g < zcs; g += 8) {
+      float v[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        const uint32_t ch = g + k;
+        float x = 0.0f;
+        if (ch < c) {
+          x = dy[(size_t)p * dycs + ch];
+          if (relu && !(from_bf16(y[(size_t)p * ycs + ch]) > 0.0f)) x = 0.0f;
+        }
+        v[k] = x;
+      }
+      *reinterpret_cast<uint4*>(dz + (size_t)p * zcs + g) = pack8(v);
     }
-    dz[i] = to_bf16(v);
   }
 }
-
-// The same for c == zcs, all multiples of 8: 8 channels per thread.
 __global__ void k_relu_grad8(const float* dy, uint32_t dycs, const uint16_t* y, uint32_t ycs,
                              uint32_t c, uint16_t* dz, uint32_t npix, uint32_t relu) {
   const uint32_t g = c / 8, n = npix * g;
@@ -1201,10 +1206,11 @@ __global__ void k_to_input(const float* in, uint16_t* x, uint32_t npix) {
 }
 
 // L1 loss in log space: out (predicted log1p radiance, [p][3]) against
-// log1p(target); writes dL/dout / npix*3 into dout ([p][3]) and adds the
-// loss sum to *loss.
+// log1p(target); writes dL/dout / npix*3 into dout ([p][3]) and block b's
+// share of the loss to part[b] (k_sum_parts adds them in a fixed order: no
+// atomics, which on one address serialise, and the same sum every run).
 __global__ void k_l1_loss(const float* out, const float* target, float* dout, uint32_t npix,
-                          float* loss) {
+                          float* part_out) {
   __shared__ float part[256];
   float s = 0.0f;
   const float scale = 1.0f / (3.0f * (float)npix);
@@ -1221,7 +1227,22 @@ __global__ void k_l1_loss(const float* out, const float* target, float* dout, ui
     if (threadIdx.x < o) part[threadIdx.x] += part[threadIdx.x + o];
     __syncthreads();
   }
-  if (threadIdx.x == 0) atomicAdd(loss, part[0] * scale);
+  if (threadIdx.x == 0) part_out[blockIdx.x] = part[0] * scale;
+}
+
+// *sum = part[0] + ... + part[n - 1] (n <= 1024) by one 256-thread block in
+// a fixed order.
+__global__ __launch_bounds__(256) void k_sum_parts(const float* part, uint32_t n, float* sum) {
+  __shared__ float red[256];
+  float s = 0.0f;
+  for (uint32_t i = threadIdx.x; i < n; i += 256) s += part[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (uint32_t o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *sum = red[0];
 }
 
 // Adam (Kingma & Ba) on the flat FP32 parameters.
@@ -1239,8 +1260,24 @@ __global__ void k_adam(float* w, const float* g, float* m, float* v, uint32_t n,
 
 // FP32 master weights [cout][9][cin] -> BF16 W [cout_pad][kpad] (k = tap *
 // cin + c) and Wt [cin][kpad_t] (k = tap * cout_pad + co), zero padded.
-__global__ void k_pack(const float* w, uint32_t cout, uint32_t cin, uint32_t cout_pad,
-                       uint32_t kpad, uint32_t kpad_t, uint16_t* W, uint16_t* Wt) {
+// One layer of k_pack: its weights in the parameter vector and its two
+// packed copies.
+struct PackLayer {
+  const float* w;
+  uint16_t* W;
+  uint16_t* Wt;
+  uint32_t cout, cin, cout_pad, kpad, kpad_t;
+};
+struct PackArgs {
+  PackLayer l[11];
+};
+// blockIdx.y: the layer
+__global__ void k_pack(PackArgs pa) {
+  const PackLayer& P = pa.l[blockIdx.y];
+  const float* w = P.w;
+  uint16_t *W = P.W, *Wt = P.Wt;
+  const uint32_t cout = P.cout, cin = P.cin, cout_pad = P.cout_pad, kpad = P.kpad,
+                 kpad_t = P.kpad_t;
   const uint32_t n1 = cout_pad * kpad, n2 = cin * kpad_t;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n1 + n2;
        i += gridDim.x * blockDim.x) {
@@ -1350,6 +1387,8 @@ struct Act {
 const uint32_t kLayers[11][3] = {
     {16, 32, 1},  {32, 32, 1},   {32, 64, 1},  {64, 64, 1},  {64, 128, 1}, {128, 128, 1},
     {192, 64, 1}, {64, 64, 1},   {96, 32, 1},  {32, 32, 1},  {32, 3, 0}};
+// blocks of the loss kernel (its partials follow the loss word)
+constexpr uint32_t kLossParts = 1024;
 
 }  // namespace
 
@@ -1555,7 +1594,7 @@ cv_status conv_bwd(cv_denoiser* d, const Conv& L, const uint16_t* x, uint32_t xc
       hipLaunchKernelGGL(cvd::k_relu_grad8, dim3(grid1((size_t)npix * L.cout / 8)), dim3(256), 0,
                          s, gy, gycs, y, ycs, L.cout, dz, npix, L.relu ? 1u : 0u);
     else
-      hipLaunchKernelGGL(cvd::k_relu_grad, dim3(grid1((size_t)npix * L.cout_pad)), dim3(256), 0, s,
+      hipLaunchKernelGGL(cvd::k_relu_grad, dim3(grid1(npix)), dim3(256), 0, s,
                          gy, gycs, y, ycs, L.cout, dz, L.cout_pad, npix, L.relu ? 1u : 0u);
     HIPTRY(hipGetLastError(), "relu grad");
   }
@@ -1717,7 +1756,7 @@ cv_status reserve_acts(cv_denoiser* d, uint32_t nb, uint32_t h, uint32_t w) {
   HIPTRY(d->g_p1.alloc(P2 * 64), "alloc");
   HIPTRY(d->g_b2b.alloc(P2 * 128), "alloc");
   HIPTRY(d->g_d1b.alloc(P1 * 64), "alloc");
-  HIPTRY(d->loss.alloc(1), "alloc");
+  HIPTRY(d->loss.alloc(1 + kLossParts), "alloc");
   d->cap_pixels = need;
   d->nb = nb;
   d->h = h;
@@ -1726,12 +1765,17 @@ cv_status reserve_acts(cv_denoiser* d, uint32_t nb, uint32_t h, uint32_t w) {
 }
 
 cv_status pack_weights(cv_denoiser* d) {
-  for (Conv& L : d->conv) {
-    const uint32_t n = L.cout_pad * L.kpad + L.cin * L.kpad_t;
-    hipLaunchKernelGGL(cvd::k_pack, dim3(grid1(n)), dim3(256), 0, d->stream, d->params.p + L.w_off,
-                       L.cout, L.cin, L.cout_pad, L.kpad, L.kpad_t, L.W.p, L.Wt.p);
-    HIPTRY(hipGetLastError(), "pack");
+  cvd::PackArgs pa{};
+  uint32_t most = 0;
+  for (int i = 0; i < 11; i++) {
+    const Conv& L = d->conv[i];
+    pa.l[i] = {d->params.p + L.w_off, L.W.p, L.Wt.p, L.cout, L.cin, L.cout_pad, L.kpad, L.kpad_t};
+    most = std::max(most, L.cout_pad * L.kpad + L.cin * L.kpad_t);
   }
+  // every layer in one launch: a row of blocks per layer
+  hipLaunchKernelGGL(cvd::k_pack, dim3(std::min<uint32_t>((most + 1023) / 1024, 128u), 11),
+                     dim3(256), 0, d->stream, pa);
+  HIPTRY(hipGetLastError(), "pack");
   return CV_OK;
 }
 
@@ -2067,9 +2111,12 @@ cv_status cv_denoiser_train_step(cv_denoiser* d, const float* input, const float
                      (uint32_t)P);
   const bool mat = flags & CV_DENOISE_MATERIALIZE;
   if ((cs = forward(d, mat, mat)) != CV_OK) return cs;
-  hipLaunchKernelGGL(cvd::k_fill, dim3(1), dim3(64), 0, s, d->loss.p, (size_t)1, 0.0f);
-  hipLaunchKernelGGL(cvd::k_l1_loss, dim3(std::min<size_t>(grid1(P * 3), 1024)), dim3(256), 0, s,
-                     d->out.p, tg, d->dout.p, (uint32_t)P, d->loss.p);
+  // the loss: block partials after the sum's word, then their sum
+  const uint32_t lblocks = (uint32_t)std::min<size_t>(grid1(P * 3), kLossParts);
+  hipLaunchKernelGGL(cvd::k_l1_loss, dim3(lblocks), dim3(256), 0, s, d->out.p, tg, d->dout.p,
+                     (uint32_t)P, d->loss.p + 1);
+  hipLaunchKernelGGL(cvd::k_sum_parts, dim3(1), dim3(256), 0, s, d->loss.p + 1, lblocks,
+                     d->loss.p);
   if ((cs = backward(d, mat)) != CV_OK) return cs;
   if (lr > 0.0f) {
     d->step++;
