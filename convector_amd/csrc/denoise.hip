@@ -1779,13 +1779,14 @@ cv_status pack_weights(cv_denoiser* d) {
   return CV_OK;
 }
 
-// Forward pass from d->x0 to d->out (f32 log1p radiance, [p][3]).  With
-// `materialize` the upsampled halves of cat0 / cat1 are written out by
-// k_upsample; otherwise the decoder convolutions read them from the lower
-// level directly (the same values; a training step's weight gradients then
-// read them there too).  With materialize_pool k_pool writes the pooled
-// encoder outputs; otherwise the encoder convolutions' epilogues do.
-cv_status forward(cv_denoiser* d, bool materialize, bool materialize_pool) {
+// Forward pass from d->x0 to `out` (f32 log1p radiance, [p][3]), d->out
+// when null.  With `materialize` the upsampled halves of cat0 / cat1 are
+// written out by k_upsample; otherwise the decoder convolutions read them
+// from the lower level directly (the same values; a training step's weight
+// gradients then read them there too).  With materialize_pool k_pool
+// writes the pooled encoder outputs; otherwise the encoder convolutions'
+// epilogues do.
+cv_status forward(cv_denoiser* d, bool materialize, bool materialize_pool, float* out = nullptr) {
   const uint32_t H = d->h, W = d->w, H1 = H / 2, W1 = W / 2, H2 = H / 4, W2 = W / 4;
   auto& L = d->conv;
   hipStream_t s = d->stream;
@@ -1832,7 +1833,7 @@ cv_status forward(cv_denoiser* d, bool materialize, bool materialize_pool) {
     CVD_TRY(conv_fwd(d, L[8], d->cat0.p, 96, d->d0a.p, 32, nullptr, 0, H, W));
   }
   CVD_TRY(conv_fwd(d, L[9], d->d0a.p, 32, d->d0b.p, 32, nullptr, 0, H, W));
-  CVD_TRY(conv_fwd(d, L[10], d->d0b.p, 32, nullptr, 0, d->out.p, 3, H, W));
+  CVD_TRY(conv_fwd(d, L[10], d->d0b.p, 32, nullptr, 0, out ? out : d->out.p, 3, H, W));
   HIPTRY(hipGetLastError(), "forward");
   return CV_OK;
 }
@@ -2085,10 +2086,11 @@ cv_status cv_denoiser_forward(cv_denoiser* d, const float* input, float* output,
   hipLaunchKernelGGL(cvd::k_to_input, dim3(grid1(P)), dim3(256), 0, d->stream, in, d->x0.p,
                      (uint32_t)P);
   const bool mat = flags & CV_DENOISE_MATERIALIZE;
-  if ((cs = forward(d, mat, mat)) != CV_OK) return cs;
-  HIPTRY(hipMemcpyAsync(output, d->out.p, P * 12, dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
-                        d->stream),
-         "download");
+  // device output: the last layer writes it directly
+  if ((cs = forward(d, mat, mat, dev ? output : nullptr)) != CV_OK) return cs;
+  if (!dev)
+    HIPTRY(hipMemcpyAsync(output, d->out.p, P * 12, hipMemcpyDeviceToHost, d->stream),
+           "download");
   HIPTRY(hipStreamSynchronize(d->stream), "sync");
   return CV_OK;
 }

@@ -47,6 +47,34 @@ def test_forward_matches_torch(cv):
     assert np.abs(ref).mean() > 1e-3
 
 
+def test_device_forward_is_exact(cv):
+    """Caller device buffers (the input also at an offset of one float, the
+    output written by the last layer in place) give the bits of the
+    host-pointer call."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    nb, h, w = 2, 36, 128
+    d = cv.Denoiser(17)
+    inp, _ = _random_batch(18, nb=nb, h=h, w=w)
+    ref = d.forward(inp)
+    n_in, n_out = inp.nbytes, nb * h * w * 12
+    di, do = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(di), ctypes.c_size_t(n_in + 16)) == 0
+    assert hip.hipMalloc(ctypes.byref(do), ctypes.c_size_t(n_out)) == 0
+    try:
+        for off in (0, 4):
+            assert hip.hipMemcpy(ctypes.c_void_p(di.value + off), inp.ctypes.data_as(ctypes.c_void_p),
+                                 ctypes.c_size_t(n_in), 1) == 0
+            d.forward_device(di.value + off, do.value, nb, h, w)
+            out = np.empty_like(ref)
+            assert hip.hipMemcpy(out.ctypes.data_as(ctypes.c_void_p), do, ctypes.c_size_t(n_out),
+                                 2) == 0
+            assert out.tobytes() == ref.tobytes(), off
+    finally:
+        hip.hipFree(di)
+        hip.hipFree(do)
+
+
 @pytest.mark.parametrize("nb,h,w", [(2, 16, 64), (1, 36, 128), (1, 736, 1280)])
 def test_fused_upsampling_forward_is_exact(cv, nb, h, w):
     """The decoder convolutions reading the upsampled halves of their inputs
