@@ -1085,9 +1085,13 @@ def main():
     ap.add_argument("--no-batch", action="store_true",
                     help="frame workload: leave out the nested 64-job batch result")
     ap.add_argument("--no-wide", action="store_true",
-                    help="frame workload: leave out the width-4 (tolerance mode) step")
+                    help="frame workload: only the exact width-2 steps (leave out the width-4 "
+                         "and ray-sort steps, the end-to-end run and the nested denoiser)")
     ap.add_argument("--no-denoiser", action="store_true",
                     help="frame workload: leave out the nested denoiser network timings")
+    ap.add_argument("--no-e2e", action="store_true",
+                    help="frame workload: leave out the end-to-end figure (a second scene "
+                         "upload and build, one step, the HDR download)")
     ap.add_argument("--batch-bins", type=int, default=32, choices=(16, 32, 64),
                     help="batch workload: centroid bins of the GPU BVH builds (64: the "
                          "reference's)")
@@ -1187,11 +1191,12 @@ def main():
         step(cv.RENDER_TIMING)
     cv.device_synchronize()
     ext_ms = ext_launch = rays = 0
-    tot_gpu_ms = 0.0
+    tot_gpu_ms = shade_ms = 0.0
     t0 = time.perf_counter()
     for k in range(args.steps):
         s = step(cv.RENDER_TIMING)
         ext_ms += s["extend_ms"]
+        shade_ms += s["shade_ms"]
         ext_launch += s["extend_launches"]
         rays += s["extend_rays"]
         tot_gpu_ms += s["total_ms"]
@@ -1266,6 +1271,37 @@ def main():
                     "node + triangle bytes (32 B per box, 36 B per triangle test) exceed what "
                     "reaches HBM because L2 and MALL serve the tree",
         },
+        "queries": {
+            "box_tests_per_query": round(sum_b / rays_per_step, 3),
+            "tri_tests_per_query": round(sum_t / rays_per_step, 3),
+            "note": "the GPU's own counts (a counting pass of the same step); the traversal "
+                    "is bit-identical to the oracle's, so these are the oracle's counts too",
+        },
+    }
+    # the shade kernels against the per-path model of a shade step: the
+    # light list's triangles read by intersect_direct (num_direct x 36 B, 40
+    # ops each) plus about 64 B and 150 ops of sampling, MIS and BRDF
+    nd = info["num_direct"]
+    shade_launch_ms = shade_ms / ext_launch
+    shade_paths = rays / ext_launch
+    shade_gbs = shade_paths * (36.0 * nd + 64.0) / (shade_launch_ms * 1e-3) / 1e9
+    shade_tops = shade_paths * (40.0 * nd + 150.0) / (shade_launch_ms * 1e-3) / 1e12
+    out["shade"] = {
+        "bound": "valu",
+        "achieved": round(shade_tops, 3),
+        "peak": VALU_PEAK_TOPS,
+        "unit": "T lane-ops/s",
+        "frac": round(shade_tops / VALU_PEAK_TOPS, 4),
+        "avg_launch_ms": round(shade_launch_ms, 4),
+        "paths_per_launch": round(shade_paths),
+        "ops_per_path": 40 * nd + 150,
+        "bytes_per_path": 36 * nd + 64,
+        "algorithmic_gb_per_s": round(shade_gbs, 1),
+        "note": "HIP-event time of k_shade per bounce against the per-path model of a shade "
+                "step (intersect_direct over the light list, 40 ops and 36 B per light "
+                "triangle, plus about 150 ops and 64 B of sampling, MIS and BRDF); every path "
+                "reads the same light triangles, which L2 serves, so the algorithmic bytes "
+                "exceed what reaches HBM and are not a fraction of its peak",
     }
     # the same step with the tree collapsed to 4 children (a tolerance mode:
     # hits as the reference's except at exact ties), reported beside
@@ -1309,6 +1345,31 @@ def main():
         except Exception as e:
             log("ray-sort steps failed: %r" % (e,))
         scene.set_tuning(ray_sort_bits=cv.RAY_SORT_NONE)
+    if not (args.no_e2e or args.no_wide):
+        # end to end: the scene uploaded and built again, one step into a
+        # fresh renderer (its buffers allocated on first use), the HDR
+        # image downloaded to the host
+        try:
+            cv.device_synchronize()
+            t1 = time.perf_counter()
+            scene2 = cv.Scene.from_scene_data(sd)
+            r2 = cv.Renderer(scene2, W, H)
+            hdr2 = DeviceBuffer(W * H * 3 * 4)
+            hdr2.zero()
+            r2.accumulate_frames_device(hdr2.ptr.value, 1, SPP, 0)
+            host = np.empty(W * H * 3, np.float32)
+            _hip_copy(hdr2.hip, host.ctypes.data_as(ctypes.c_void_p), hdr2.ptr, host.nbytes, 2)
+            cv.device_synchronize()
+            e2e_ms = (time.perf_counter() - t1) * 1e3
+            out["end_to_end"] = {
+                "ms": round(e2e_ms, 1),
+                "bvh_build_ms": round(scene2.bvh_info()["build_ms"], 2),
+                "finite": bool(np.isfinite(host).all()),
+                "note": "wall time of: triangles uploaded and the BVH built on the GPU, a "
+                        "renderer created, one %d-spp step, the HDR image downloaded" % SPP}
+            del r2, hdr2, scene2
+        except Exception as e:
+            log("end-to-end run failed: %r" % (e,))
     if not args.no_cpu:
         try:
             cb = cpu_baseline(sd, scene, args)
@@ -1346,7 +1407,7 @@ def main():
         except Exception as e:
             log("batch workload failed: %r" % (e,))
             out["batch"] = None
-    if not args.no_denoiser:
+    if not (args.no_denoiser or args.no_wide):
         # the learned denoiser's kernels (config 3's network): one 1280x736
         # forward and a batch-16 training step, timed in the same run
         dargs = argparse.Namespace(**vars(args))
