@@ -1284,14 +1284,34 @@ __global__ void k_fold_hdr(float* __restrict__ hdr, float* __restrict__ slots, s
     const uint32_t px = x0 + p % w, py = y0 + p / w;
     const size_t off = hdr_offset(px, py, width);
     float r = hdr[off], g = hdr[off + 8], b = hdr[off + 16];
-    for (uint32_t k = 0; k < count; k++) {
-      float* s = slots + k * slot_stride + off;
-      r = r + s[0];
-      g = g + s[8];
-      b = b + s[16];
-      s[0] = 0.0f;
-      s[8] = 0.0f;
-      s[16] = 0.0f;
+    // 8 buffers at a time: their loads first (issued together rather than
+    // each behind the previous buffer's clearing stores), then the sums in
+    // frame order, then the clearing
+    for (uint32_t k0 = 0; k0 < count; k0 += 8) {
+      float v[8][3];
+#pragma unroll
+      for (uint32_t j = 0; j < 8; j++) {
+        if (k0 + j >= count) break;
+        const float* s = slots + (k0 + j) * slot_stride + off;
+        v[j][0] = s[0];
+        v[j][1] = s[8];
+        v[j][2] = s[16];
+      }
+#pragma unroll
+      for (uint32_t j = 0; j < 8; j++) {
+        if (k0 + j >= count) break;
+        r = r + v[j][0];
+        g = g + v[j][1];
+        b = b + v[j][2];
+      }
+#pragma unroll
+      for (uint32_t j = 0; j < 8; j++) {
+        if (k0 + j >= count) break;
+        float* s = slots + (k0 + j) * slot_stride + off;
+        s[0] = 0.0f;
+        s[8] = 0.0f;
+        s[16] = 0.0f;
+      }
     }
     hdr[off] = r;
     hdr[off + 8] = g;
@@ -1307,9 +1327,18 @@ __global__ void k_fold_features(float* __restrict__ feat, float* __restrict__ sl
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x) {
     float v = feat[i];
-    for (uint32_t k = 0; k < count; k++) {
-      v = v + slots[k * slot_stride + i];
-      slots[k * slot_stride + i] = 0.0f;
+    // 8 buffers at a time, loads first (as in k_fold_hdr)
+    for (uint32_t k0 = 0; k0 < count; k0 += 8) {
+      float t[8];
+#pragma unroll
+      for (uint32_t j = 0; j < 8; j++)
+        if (k0 + j < count) t[j] = slots[(k0 + j) * slot_stride + i];
+#pragma unroll
+      for (uint32_t j = 0; j < 8; j++)
+        if (k0 + j < count) v = v + t[j];
+#pragma unroll
+      for (uint32_t j = 0; j < 8; j++)
+        if (k0 + j < count) slots[(k0 + j) * slot_stride + i] = 0.0f;
     }
     feat[i] = v;
   }
