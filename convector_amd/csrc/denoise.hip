@@ -372,6 +372,21 @@ __global__ __launch_bounds__(256, 2) void k_conv_halo(ConvArgs a) {
       bias_s[tid] = (a.bias && row0 + tid < a.nout) ? a.bias[row0 + tid] : 0.0f;
   }
   __syncthreads();
+  // Persistent blocks: a tile's halo is loaded two tiles ahead.  The loads
+  // of tile t + 1 are issued before tile t - 1's epilogue and overlap it
+  // and tile t's MFMAs; after those MFMAs they are stored to LDS and the
+  // loads of tile t + 2 are issued, before tile t's epilogue (whose stores
+  // they then do not wait behind).
+  uint32_t next = tile + gridDim.x;
+  bool more = PERSIST && next < ntiles;
+  int nx0 = 0, ny0 = 0;
+  size_t nbase = 0;
+  if constexpr (PERSIST) {
+    if (more) {
+      tile_offsets(next, nx0, ny0, nbase);
+      CV_HALO_LOAD_X();
+    }
+  }
   // lane: output channels row0 + t * 16 + 4 * kq + r (r = 0..3) of pixel
   // col of each fragment
   const bool vec_y = a.y && ((uintptr_t)a.y % 8 == 0) && (a.ycs % 4 == 0);
@@ -383,15 +398,6 @@ __global__ __launch_bounds__(256, 2) void k_conv_halo(ConvArgs a) {
     for (int m = 0; m < MF; m++)
 #pragma unroll
       for (int t = 0; t < NS; t++) acc[m][t] = f4{0.0f, 0.0f, 0.0f, 0.0f};
-    // persistent blocks: the next tile's halo is loaded during this one's MFMAs
-    const uint32_t next = tile + gridDim.x;
-    const bool more = PERSIST && next < ntiles;
-    int nx0 = 0, ny0 = 0;
-    size_t nbase = 0;
-    if (more) {
-      tile_offsets(next, nx0, ny0, nbase);
-      CV_HALO_LOAD_X();
-    }
     // the MFMAs of the chunk in LDS
     auto chunk_mfma = [&]() {
 #pragma unroll
@@ -438,6 +444,21 @@ __global__ __launch_bounds__(256, 2) void k_conv_halo(ConvArgs a) {
       __syncthreads();
     }
     chunk_mfma();
+    uint32_t next2 = 0;
+    bool more2 = false;
+    int n2x0 = 0, n2y0 = 0;
+    size_t n2base = 0;
+    if constexpr (PERSIST) {
+      __syncthreads();
+      if (more) CV_HALO_STORE_X();
+      __syncthreads();
+      next2 = next + gridDim.x;
+      more2 = more && next2 < ntiles;
+      if (more2) {
+        tile_offsets(next2, n2x0, n2y0, n2base);
+        CV_HALO_LOAD_X();
+      }
+    }
     if constexpr (POOL) {
       // Fragments m (the upper row of the wave's pair) and m + 2 (the lower
       // row) together: both are stored, and the even lane of each column
@@ -593,13 +614,15 @@ __global__ __launch_bounds__(256, 2) void k_conv_halo(ConvArgs a) {
       }
     }
     if (!more) break;
-    __syncthreads();
-    CV_HALO_STORE_X();
-    __syncthreads();
     tile = next;
     x0 = nx0;
     y0 = ny0;
     img_base = nbase;
+    next = next2;
+    nx0 = n2x0;
+    ny0 = n2y0;
+    nbase = n2base;
+    more = more2;
   }
 #undef CV_HALO_LOAD
 #undef CV_HALO_LOAD_X
