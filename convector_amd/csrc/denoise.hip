@@ -862,8 +862,14 @@ __global__ __launch_bounds__(64 * NS * CT) void k_wgrad_halo(WGradArgs a) {
   constexpr int XP = 10 * 18 * XG;         // X pixel-pair pieces per tile
   constexpr int ZP = 8 * 16 * ZG;          // dZ pixel-pair pieces per tile
   constexpr int XPT = (XP + BS - 1) / BS, ZPT = (ZP + BS - 1) / BS;
-  __shared__ __attribute__((aligned(16))) uint16_t xs[CC * kWhCi];
-  __shared__ __attribute__((aligned(16))) uint16_t zs[CO * kWhCo];
+  // Each 8-channel group's rows start 32 / groups dwords further on (a
+  // multiple of 16 bytes, as the 16-byte reads need): the transposing
+  // stores write one channel row per group at a time, and with the row
+  // strides alone (multiples of 32 dwords apart for 8 channels) every
+  // group would hit the same LDS banks.
+  constexpr int XSH = 64 / XG, ZSH = 64 / ZG;  // bf16 elements
+  __shared__ __attribute__((aligned(16))) uint16_t xs[CC * kWhCi + (XG - 1) * XSH];
+  __shared__ __attribute__((aligned(16))) uint16_t zs[CO * kWhCo + (ZG - 1) * ZSH];
   const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const uint32_t ct = wave / CT, it = wave % CT;
   const uint32_t ln = lane & 15, g = lane >> 4;
@@ -929,7 +935,8 @@ __global__ __launch_bounds__(64 * NS * CT) void k_wgrad_halo(WGradArgs a) {
       const int q = (int)tid + BS * i;
       if (q >= XP) continue;
       const int g8 = q % XG, pr = (q / XG) % 18, row = q / XG / 18;
-      uint32_t* d = reinterpret_cast<uint32_t*>(xs + (8 * g8) * kWhCi + row * kWhCols + 6 + 2 * pr);
+      uint32_t* d =
+          reinterpret_cast<uint32_t*>(xs + (8 * g8) * kWhCi + g8 * XSH + row * kWhCols + 6 + 2 * pr);
 #pragma unroll
       for (int e = 0; e < 8; e++) {
         const uint32_t lo = xr[i][0][e / 2], hi = xr[i][1][e / 2];
@@ -941,7 +948,7 @@ __global__ __launch_bounds__(64 * NS * CT) void k_wgrad_halo(WGradArgs a) {
       const int q = (int)tid + BS * i;
       if (q >= ZP) continue;
       const int g8 = q % ZG, pr = (q / ZG) % 16, row = q / ZG / 16;
-      uint32_t* d = reinterpret_cast<uint32_t*>(zs + (8 * g8) * kWhCo + row * 32 + 2 * pr);
+      uint32_t* d = reinterpret_cast<uint32_t*>(zs + (8 * g8) * kWhCo + g8 * ZSH + row * 32 + 2 * pr);
 #pragma unroll
       for (int e = 0; e < 8; e++) {
         const uint32_t lo = zr[i][0][e / 2], hi = zr[i][1][e / 2];
@@ -964,12 +971,13 @@ __global__ __launch_bounds__(64 * NS * CT) void k_wgrad_halo(WGradArgs a) {
     bf8 A[8];
 #pragma unroll
     for (int r = 0; r < 8; r++)
-      A[r] = *reinterpret_cast<const bf8*>(&zs[(ct * 16 + ln) * kWhCo + r * 32 + 8 * g]);
+      A[r] = *reinterpret_cast<const bf8*>(
+          &zs[(ct * 16 + ln) * kWhCo + ((ct * 16 + ln) / 8) * ZSH + r * 32 + 8 * g]);
     if (it == 0) {
 #pragma unroll
       for (int r = 0; r < 8; r++) accb = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[r], ones, accb, 0, 0, 0);
     }
-    const uint16_t* xrow0 = &xs[(it * 16 + ln) * kWhCi + 8 * g];
+    const uint16_t* xrow0 = &xs[(it * 16 + ln) * kWhCi + ((it * 16 + ln) / 8) * XSH + 8 * g];
 #pragma unroll
     for (int hr = 0; hr < 10; hr++) {
       // windows of pixels x0 + 8 g - 8, x0 + 8 g, x0 + 8 g + 8 (8 each)
