@@ -466,7 +466,8 @@ cv_status cv_denoiser_forward(cv_denoiser* d, const float* input, float* output,
 /* Forward, L1 loss against log1p(target) (3 f32 per pixel, linear
  * radiance), backward into the gradients, then one Adam step with learning
  * rate lr (lr = 0: gradients only).  loss (may be NULL) receives the mean
- * absolute error before the step.  flags: CV_PTR_DEVICE;
+ * absolute error before the step, summed in a fixed order (the same value
+ * for the same inputs and weights on every call).  flags: CV_PTR_DEVICE;
  * CV_DENOISE_MATERIALIZE runs the separate pooling, upsampling-gradient and
  * ReLU-gradient kernels instead of their fused forms (the same bits). */
 cv_status cv_denoiser_train_step(cv_denoiser* d, const float* input, const float* target,
