@@ -47,6 +47,21 @@ def test_forward_matches_torch(cv):
     assert np.abs(ref).mean() > 1e-3
 
 
+def test_loss_is_deterministic(cv):
+    """The loss is summed from block partials in a fixed order: with lr = 0
+    (gradients only, the weights unchanged) the same batch gives the same
+    loss bits on every call, and it equals the mean absolute error of the
+    forward's output in log space to float precision."""
+    d = cv.Denoiser(23)
+    inp, tgt = _random_batch(24, nb=2, h=36, w=128)
+    losses = [d.train_step(inp, tgt, 0.0) for _ in range(3)]
+    assert np.isfinite(losses[0])
+    assert losses[0] == losses[1] == losses[2]
+    out = d.forward(inp, materialize=True)
+    ref = float(np.mean(np.abs(out.astype(np.float64) - np.log1p(tgt.astype(np.float64)))))
+    assert abs(losses[0] - ref) <= 1e-5 * max(ref, 1e-30)
+
+
 def test_device_forward_is_exact(cv):
     """Caller device buffers (the input also at an offset of one float, the
     output written by the last layer in place) give the bits of the
