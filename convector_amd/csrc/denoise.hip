@@ -1339,16 +1339,17 @@ __global__ void k_fill(float* p, size_t n, float v) {
 // Renderer outputs -> network input: log1p of the mean radiance (16x4-block
 // HDR layout / spp), albedo, octahedral normal, log1p depth, spp -- the
 // features summed over spp frames, 8 f32 per pixel (albedo rgb, normal xyz,
-// depth, 0) -- as 10 f32 per pixel, rows y from 0.
+// depth, 0) -- as 10 f32 per pixel into `out` (rows y from 0) and/or as the
+// network's 16-channel bf16 input into `x` (either may be null).
 __global__ void k_features(const float* hdr, const float* feat, uint32_t width, uint32_t height,
-                           uint32_t spp, float* out) {
+                           uint32_t spp, float* out, uint16_t* x) {
   const uint32_t npix = width * height;
   for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < npix; p += gridDim.x * blockDim.x) {
     const uint32_t px = p % width, py = p / width;
     const float* h = hdr + cvk::hdr_offset(px, py, width);
     const float inv = 1.0f / (float)spp;
     const float* f = feat + (size_t)p * 8;
-    float* o = out + (size_t)p * 10;
+    float o[10];
     o[0] = log1pf(fmaxf(h[0] * inv, 0.0f));
     o[1] = log1pf(fmaxf(h[8] * inv, 0.0f));
     o[2] = log1pf(fmaxf(h[16] * inv, 0.0f));
@@ -1372,6 +1373,19 @@ __global__ void k_features(const float* hdr, const float* feat, uint32_t width, 
     o[7] = oy;
     o[8] = log1pf(fmaxf(f[6] * inv, 0.0f));
     o[9] = (float)spp;
+    if (out) {
+#pragma unroll
+      for (int k = 0; k < 10; k++) out[(size_t)p * 10 + k] = o[k];
+    }
+    if (x) {
+      // the network's bf16 input directly (what k_to_input makes of `out`)
+      float f[16];
+#pragma unroll
+      for (int k = 0; k < 16; k++) f[k] = k < 10 ? o[k] : 0.0f;
+      uint4* xo = reinterpret_cast<uint4*>(x + (size_t)p * 16);
+      xo[0] = pack8(f);
+      xo[1] = pack8(f + 8);
+    }
   }
 }
 
@@ -1979,20 +1993,18 @@ cv_status denoise_image(cv_denoiser* d, const float* hdr, const float* features,
   std::lock_guard<std::mutex> g(d->mu);
   if ((cs = reserve_acts(d, 1, height, width)) != CV_OK) return cs;
   const size_t P = (size_t)width * height;
-  HIPTRY(d->in_f32.reserve(P * 10), "alloc");
-  HIPTRY(d->tgt_f32.reserve(P * 3), "alloc");
+  if (!dst_device) HIPTRY(d->tgt_f32.reserve(P * 3), "alloc");
   hipStream_t s = d->stream;
+  // the features straight into the network's bf16 input
   hipLaunchKernelGGL(cvd::k_features, dim3(grid1(P)), dim3(256), 0, s, hdr, features, width,
-                     height, spp, d->in_f32.p);
-  hipLaunchKernelGGL(cvd::k_to_input, dim3(grid1(P)), dim3(256), 0, s, d->in_f32.p, d->x0.p,
-                     (uint32_t)P);
+                     height, spp, nullptr, d->x0.p);
   if ((cs = forward(d, false, false)) != CV_OK) return cs;
+  // a device destination receives the HDR image directly
   hipLaunchKernelGGL(cvd::k_output, dim3(grid1(P)), dim3(256), 0, s, d->out.p, width, height, spp,
-                     d->tgt_f32.p);
+                     dst_device ? dst : d->tgt_f32.p);
   HIPTRY(hipGetLastError(), "denoise");
-  HIPTRY(hipMemcpyAsync(dst, d->tgt_f32.p, P * 12,
-                        dst_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s),
-         "download");
+  if (!dst_device)
+    HIPTRY(hipMemcpyAsync(dst, d->tgt_f32.p, P * 12, hipMemcpyDeviceToHost, s), "download");
   HIPTRY(hipStreamSynchronize(s), "sync");
   return CV_OK;
 }
@@ -2189,7 +2201,7 @@ cv_status cv_denoiser_prepare_input(const float* hdr, const float* features, uin
     op = o.p;
   }
   hipLaunchKernelGGL(cvd::k_features, dim3(grid1(P)), dim3(256), 0, 0, hp, fp, width, height, spp,
-                     op);
+                     op, nullptr);
   HIPTRY(hipGetLastError(), "features");
   if (!dev) HIPTRY(hipMemcpy(input, op, P * 40, hipMemcpyDeviceToHost), "download");
   HIPTRY(hipDeviceSynchronize(), "sync");
