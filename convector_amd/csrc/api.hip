@@ -778,7 +778,11 @@ cv_status cv_bvh_intersect_nearest(const cv_scene* sc, const cv_ray* rays, size_
     // 968 triangles).  Rendering keeps the tuned value (1 is fastest there).
     x.batches = s->cfg.fetch_batches > 4 ? s->cfg.fetch_batches : 4u;
     x.refill = s->cfg.refill <= 64 ? s->cfg.refill : 0u;
-    x.interval = s->cfg.refill_interval;
+    // Refill checks 4 times as often as the renderer's: a query ends after
+    // a few dozen box tests, so a finished lane would otherwise idle for a
+    // large share of a ray's steps (2^24 rays on suzanne: 1.79 -> 1.65 ms
+    // with 4 steps between checks instead of 16; the bunny's within 1 %).
+    x.interval = s->cfg.refill_interval > 4 ? s->cfg.refill_interval / 4 : 1u;
     x.leaf_batch = s->cfg.leaf_batch <= 64 ? s->cfg.leaf_batch : 0u;
     x.parts = s->cfg.parts;
     HIPTRY(cvk::launch_extend(x, false, s->packed, false, s->cfg.stack, s->cfg.blocks,
