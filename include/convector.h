@@ -222,7 +222,8 @@ typedef struct cv_isect {
  * ray is traversed as the reference traverses a packet holding that ray in
  * every lane.  hits and isects may each be NULL.  A call with hits and no
  * isects traverses with the scene's tuning, taking at least 4 batches of 64
- * rays per fetch.  hits must be 16-byte aligned.  Returns after the results
+ * rays per fetch and checking for refills every refill_interval / 4 steps
+ * (at least 1).  hits must be 16-byte aligned.  Returns after the results
  * are written. */
 cv_status cv_bvh_intersect_nearest(const cv_scene* scene, const cv_ray* rays, size_t n,
                                    cv_hit* hits, cv_isect* isects, uint32_t flags);
